@@ -1,0 +1,166 @@
+"""Benchmark: PBF particle-substeps/s on the 1,048,576-particle dam break.
+
+Workload (one "step" = pbf_step(dt = 1/60, 2 substeps, 4 Jacobi iterations)):
+128 x 128 x 64 particles, r = 0.025, h = 0.1, in a 12.8^3 box with a 128^3
+trilinear box volume map, XSPH 0.01, vorticity 0.01, surface tension 0.1.
+Inputs are resident in HBM before the timed region; the timed region is
+bracketed by stream synchronisation.  Per-pass GPU time comes from HIP events
+recorded on the engine's stream around every launch of the timed region.
+
+Prints one JSON line (see keys below); diagnostics go to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "webgpu-fluid-simulation_amd"), os.path.join(ROOT, "oracle")]
+
+import numpy as np  # noqa: E402
+
+import scenes  # noqa: E402
+from pbf_host import PBF_FLAG_TIMING  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+# Algorithmic bytes per particle (per cell for the scan) of each pass; B = 1
+# when a boundary volume map is active.  Stale-neighbour-faithful accounting:
+# one Jacobi iteration moves 68 + 32 B bytes per particle (lambda 28 + 16 B,
+# dp 40 + 16 B).
+def pass_bytes(B):
+    return {"predict_bin": 56, "scatter": 24, "settle": 68, "build_list": 16, "lambda": 28 + 16 * B,
+            "dp": 40 + 16 * B, "density": 64, "vorticity": 64, "xsph": 76, "diag": 16 + 16 * B}
+
+
+def substep_bytes(N, C, I, B):
+    return N * (304 + 68 * I + 32 * B * I) + 12 * C
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sc, budget_s: float):
+    """The oracle (CPU restatement, OpenMP) on a bounded sample of the same workload."""
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    o = oracle.Oracle(sc.positions, sc.particle_radius, sc.kernel_radius, 1000.0, sc.domain_lo, sc.domain_hi)
+    c = sc.config
+    o.set_config(c["XSPH"], c["vorticity"], c["surfaceTension"], c["gravity"])
+    o.set_trilinear_field(sc.field.res, sc.field.lo, sc.field.hi, sc.field.sdf_vol)
+    dt = sc.substep_dt()
+    o.substep(dt, sc.iterations)  # warm-up
+    k, t0 = 0, time.perf_counter()
+    while True:
+        o.substep(dt, sc.iterations)
+        k += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or k >= 50:
+            break
+    return {"value": sc.n * k / el, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle/pbf_oracle.c on the same {sc.n}-particle scene: {k} substeps after 1 warm-up, "
+                      f"{el:.1f} s, OpenMP {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timing", action="store_true", help="no per-pass events (graph replay)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    sc = scenes.cfg1()
+    N = sc.n
+    B = 1 if sc.field is not None else 0
+    C = int(np.prod([int(np.ceil((sc.domain_hi[a] - sc.domain_lo[a]) / sc.kernel_radius - 1e-7))
+                     for a in range(3)]))
+    log(f"scene {sc.name}: N={N} cells={C} I={sc.iterations} substeps/step={sc.substeps}")
+    flags = 0 if args.no_timing else PBF_FLAG_TIMING
+    sim = scenes.make_engine(sc, flags=flags)
+    for _ in range(args.warmup):
+        sim.step(sc.dt, sc.substeps, sc.iterations)
+    sim.sync()
+    sim.pass_times()  # discard warm-up events
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.step(sc.dt, sc.substeps, sc.iterations)
+    sim.sync()
+    el = time.perf_counter() - t0
+    passes = sim.pass_times()
+    mean_err, max_err = sim.density_error()
+    pos = sim.read_positions()
+    assert np.isfinite(pos).all(), "non-finite positions"
+    substeps = args.steps * sc.substeps
+    value = N * substeps / el
+    log(f"timed {args.steps} steps in {el:.3f} s -> {value / 1e6:.1f} M particle-substeps/s; "
+        f"density error mean {mean_err:.3e} max {max_err:.3e}")
+
+    roof = None
+    pb = pass_bytes(B)
+    per_pass = {}
+    if passes and sum(v[1] for v in passes.values()) > 0:
+        for name, (ms, cnt) in passes.items():
+            if cnt:
+                per_pass[name] = {"ms_total": round(ms, 3), "launches": int(cnt), "us_avg": round(1e3 * ms / cnt, 2)}
+        dom = max((k for k in per_pass if k in pb), key=lambda k: per_pass[k]["ms_total"])
+        avg_s = per_pass[dom]["ms_total"] / per_pass[dom]["launches"] / 1e3
+        bytes_per_launch = pb[dom] * N
+        ach = bytes_per_launch / avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": round(avg_s * 1e6, 2)}
+        for name in per_pass:
+            if name in pb:
+                a = pb[name] * N / (per_pass[name]["us_avg"] * 1e-6) / 1e9
+                per_pass[name]["algorithmic_GBs"] = round(a, 1)
+        gpu_ms = sum(v["ms_total"] for v in per_pass.values())
+        log("per-pass:", json.dumps(per_pass))
+        log(f"sum of pass times {gpu_ms:.1f} ms of {el * 1e3:.1f} ms wall")
+
+    whole = substep_bytes(N, C, sc.iterations, B) * substeps / el / 1e9
+    cpu = None
+    if not args.no_cpu:
+        log(f"cpu baseline: oracle for ~{args.cpu_seconds:.0f} s ...")
+        cpu = cpu_baseline(sc, args.cpu_seconds)
+        log(f"cpu baseline {cpu['value'] / 1e6:.3f} M particle-substeps/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": "particle-substeps/s at 1M (PBF dam break)",
+        "value": round(value, 1),
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded lattice dam break)",
+        "config": {"workload": "cfg1 dam break: 1,048,576 particles (128x128x64), r=0.025, h=0.1, "
+                               "12.8^3 box, 128^3 trilinear box volume map, dt=1/60 x 2 substeps, 4 Jacobi "
+                               "iterations, XSPH 0.01, vorticity 0.01, surface tension 0.1",
+                   "particles": N, "substeps_per_step": sc.substeps, "iterations": sc.iterations,
+                   "grid_cells": C, "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "whole_substep_algorithmic_GBs": round(whole, 1),
+        "density_error_after_run": {"mean": mean_err, "max": max_err},
+        "passes": per_pass,
+    }
+    sim.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

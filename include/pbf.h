@@ -1,0 +1,209 @@
+/*
+ * pbf.h -- C ABI of the MI355X Position Based Fluids engine (libpbf.so).
+ *
+ * Plain C: pointers, sizes and status codes only.  Every entry point stands
+ * in for a member of the reference simulator class (LinzhouLi/WebGPU-Fluid-
+ * Simulation, src/simulator/SPH.ts, PBF/PBFConfig.ts, PBF/PBF.ts); the
+ * citation above each declaration names the member it replaces.  The
+ * Node-API addon (node/pbf_addon.cc) and the Python host mirror (pbf_host.py)
+ * bind exactly these symbols.
+ *
+ * Conventions
+ *   - return value: PBF_OK (0) or a negative PBF_E_* code; pbf_last_error()
+ *     returns the message (the reference's own text where it has one).
+ *   - host pointers are borrowed for the duration of the call only.
+ *   - a context is single-threaded and owns one HIP stream; pbf_step is
+ *     asynchronous on that stream, every pbf_read_* synchronizes.
+ *   - particle arrays cross the boundary in the caller's (original) order as
+ *     float4 rows (x, y, z, w) like the reference's 16-byte-stride buffers.
+ */
+#ifndef PBF_H
+#define PBF_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBF_ABI_VERSION 1
+
+enum {
+  PBF_OK = 0,
+  PBF_E_INVALID = -1,   /* bad argument */
+  PBF_E_CAPACITY = -2,  /* "To Many Particles!" and other capacity limits */
+  PBF_E_LENGTH = -3,    /* "Illegal length of Particle Position Buffer!" */
+  PBF_E_FORMAT = -4,    /* "Invalid Boundary Model File!" */
+  PBF_E_DEVICE = -5,    /* HIP runtime error */
+  PBF_E_OVERFLOW = -6,  /* a particle had more neighbours than max_neighbors */
+  PBF_E_STATE = -7      /* call out of order (e.g. no particles yet) */
+};
+
+enum { PBF_GRID_DENSE = 0, PBF_GRID_HASH = 1 };
+enum { PBF_FIELD_NONE = 0, PBF_FIELD_CUBIC = 1, PBF_FIELD_TRILINEAR = 2 };
+
+typedef struct pbf_ctx pbf_ctx;
+
+/* Construction parameters.  Reference: the constants of SPH.ts:24-27,33 and
+ * PBFConfig.ts:21 (r = 0.006, h = 0.025, rho0 = 1000, dt = 1/300,
+ * MAX_PARTICLE_NUM, MAX_NEIGHBOR_COUNT = 60) and the fixed unit-box domain of
+ * PBF/shader/common.ts:43-68, all made runtime values. */
+typedef struct {
+  uint32_t capacity;        /* max particles over all scenes (SPH.MAX_PARTICLE_NUM) */
+  uint32_t max_neighbors;   /* per-particle neighbour capacity (SPH.MAX_NEIGHBOR_COUNT); 0 -> 96 */
+  double particle_radius;   /* r */
+  double kernel_radius;     /* h; 0 -> 4 r */
+  double rest_density;      /* rho0; 0 -> 1000 */
+  double time_step;         /* dt used by pbf_run (SPH.TIME_STEP); 0 -> 1/300 */
+  double domain_lo[3];      /* wall-clamp box; reference: (0,0,0) */
+  double domain_hi[3];      /* reference: (1,1,1) */
+  uint32_t grid_mode;       /* PBF_GRID_DENSE (cell = h) or PBF_GRID_HASH */
+  uint32_t hash_log2;       /* hash table size 2^hash_log2 (PBF_GRID_HASH) */
+  uint32_t flags;           /* PBF_FLAG_* */
+  uint32_t reserved[5];
+} pbf_params;
+
+enum {
+  PBF_FLAG_NO_GRAPH = 1u << 0,      /* launch passes directly instead of replaying a hipGraph */
+  PBF_FLAG_CUBIC_FIXED = 1u << 1,   /* cubic volume map: read the corrected edge nodes */
+  PBF_FLAG_TIMING = 1u << 2         /* record per-pass HIP events (pbf_pass_times) */
+};
+
+/* Simulation options.  Reference: PBFConfig.setConfig (PBFConfig.ts:31-46)
+ * writing the SimulationOptions uniform (src/common/shader.ts:50-57). */
+typedef struct {
+  uint32_t iteration;       /* Jacobi iterations per substep */
+  float xsph;               /* XSPH coefficient */
+  float vorticity;          /* vorticity confinement epsilon */
+  float surface_tension;    /* Akinci surface-tension gamma */
+  float gravity;            /* magnitude; applied as (0, -gravity, 0) */
+} pbf_config;
+
+/* One scene of a batched context (no reference counterpart: the reference
+ * runs a single scene in a fixed unit box). */
+typedef struct {
+  double domain_lo[3];
+  double domain_hi[3];
+  uint32_t particle_count;  /* particles of this scene, in scene order */
+  uint32_t reserved[3];
+} pbf_scene_desc;
+
+/* Reference: new PBF() + PBF.initResource (PBF.ts:67-69,262-279). */
+int pbf_create(const pbf_params *params, pbf_ctx **out);
+/* Batched variant: `nscenes` independent scenes share every launch. */
+int pbf_batch_create(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc *scenes, pbf_ctx **out);
+int pbf_destroy(pbf_ctx *ctx);
+/* Reference: PBF.reset (PBF.ts:71-89) -> SPH.baseReset (SPH.ts:57-66). */
+int pbf_reset(pbf_ctx *ctx);
+/* Reference: PBFConfig.setConfig (PBFConfig.ts:31-46). */
+int pbf_set_config(pbf_ctx *ctx, const pbf_config *cfg);
+
+/* Reference: SPH.voxelizeCube (SPH.ts:99-121): appends a lattice of spacing
+ * 2r to the context's staged particle array (f64 arithmetic, then f32).
+ * jitter adds a seeded uniform offset in [-jitter, +jitter] per axis (0 = off). */
+int pbf_voxelize_cube(pbf_ctx *ctx, const double min[3], const double max[3], double jitter, uint64_t seed,
+                      uint32_t *added);
+/* Reference: SPH.voxelizeSphere (SPH.ts:123-149). */
+int pbf_voxelize_sphere(pbf_ctx *ctx, const double center[3], double radius, double jitter, uint64_t seed,
+                        uint32_t *added);
+/* GPU-free forms of the two voxelizers: write float4 rows (w = 0) for the
+ * particles serial0, serial0+1, ... (the jitter stream index) into `out`
+ * (NULL: count only). */
+int pbf_voxelize_cube_host(double particle_radius, const double min[3], const double max[3], double jitter,
+                           uint64_t seed, uint64_t serial0, float *out, uint64_t capacity, uint32_t *count);
+int pbf_voxelize_sphere_host(double particle_radius, const double center[3], double radius, double jitter,
+                             uint64_t seed, uint64_t serial0, float *out, uint64_t capacity, uint32_t *count);
+/* Number of staged particles (SPH.particleCount before upload). */
+uint32_t pbf_staged_count(const pbf_ctx *ctx);
+/* Reference: PBFConfig.setParticlePosition (PBFConfig.ts:48-62): uploads the
+ * staged particles.  Errors: 'To Many Particles!'. */
+int pbf_set_particle_position(pbf_ctx *ctx);
+/* Upload an explicit float4 array (xyzw, w ignored) of `count` particles;
+ * `length` is the number of floats.  Errors: 'Illegal length of Particle
+ * Position Buffer!' (length != 4*count), 'To Many Particles!'. */
+int pbf_set_positions(pbf_ctx *ctx, const float *xyzw, size_t length, uint32_t count);
+
+/* Reference: PBF.setBoundaryData (PBF.ts:91-94) -> BoundaryModel.setData
+ * (volumeMap.ts:23-39): parse .cdm text (SDF line, volume line) of a cubic
+ * 32-node field over the unit box.  Error: 'Invalid Boundary Model File!'. */
+int pbf_set_boundary_data(pbf_ctx *ctx, const char *text, size_t length);
+/* GPU-free parse of .cdm text: resolution and field length first (sdf/vol
+ * may be NULL), then the values when capacity >= length. */
+int pbf_parse_cdm(const char *text, size_t length, int32_t res[3], float *sdf, float *vol, size_t capacity,
+                  size_t *field_length);
+/* Cubic 32-node field over [lo, hi] from arrays in the .cdm layout. */
+int pbf_set_cubic_field(pbf_ctx *ctx, uint32_t scene, const int32_t res[3], const double lo[3], const double hi[3],
+                        const float *sdf, const float *vol, size_t length);
+/* Trilinear volume map: (res+1)^3 nodes over [lo, hi], x fastest, (sdf, vol)
+ * interleaved.  No reference counterpart (the reference ships 20^3 cubic maps). */
+int pbf_set_volume_map(pbf_ctx *ctx, uint32_t scene, const int32_t res[3], const double lo[3], const double hi[3],
+                       const float *sdf_vol);
+int pbf_clear_boundary(pbf_ctx *ctx);
+
+/* Host-side map builder for analytic solids: kind 0 = inside of the box
+ * [lo, hi] is fluid (walls outside; shape, if given, = box lo[3], hi[3]),
+ * kind 1 = that box plus a sphere and a torus (shape[0..2] sphere centre,
+ * shape[3] radius, shape[4..6] torus centre, shape[7] major R, shape[8]
+ * minor r; torus axis = y).  Writes (res+1)^3 (sdf, vol) pairs, sdf > 0 in
+ * the fluid.  vol: with particle_radius > 0 the boundary volume whose mirrored
+ * sample at max(d + r/2, 2r) reproduces 0.9 x the poly6-weighted solid part
+ * of the kernel ball; with particle_radius == 0 the solid volume in the ball.
+ * No reference counterpart: the reference ships precomputed .cdm maps. */
+int pbf_build_volume_map(uint32_t kind, const double *shape, const int32_t res[3], const double lo[3],
+                         const double hi[3], double kernel_radius, double particle_radius, float *sdf_vol);
+
+/* Reference: PBF.run (PBF.ts:398-448) encodes one substep with the
+ * compile-time dt and the configured iteration count.  pbf_step runs
+ * `substeps` substeps of dt/substeps with `iterations` Jacobi iterations each
+ * (0 = the configured count), asynchronously on the context stream. */
+int pbf_step(pbf_ctx *ctx, float dt, uint32_t substeps, uint32_t iterations);
+/* Exactly PBF.run: one substep of params.time_step, configured iterations. */
+int pbf_run(pbf_ctx *ctx);
+int pbf_sync(pbf_ctx *ctx);
+
+/* Reference: the public `position` buffer (SPH.ts:39) read by the renderer:
+ * float4 (x, y, z, density) in the original particle order. */
+int pbf_read_positions(pbf_ctx *ctx, float *xyzw, uint32_t count);
+/* Reference: the protected `velocity` buffer (SPH.ts:40); float4, w = 0. */
+int pbf_read_velocities(pbf_ctx *ctx, float *xyzw, uint32_t count);
+
+enum {
+  PBF_FIELD_POSITION = 0, PBF_FIELD_VELOCITY = 1, PBF_FIELD_ACCEL = 2, PBF_FIELD_LAMBDA = 3,
+  PBF_FIELD_OMEGA = 4, PBF_FIELD_NORMAL = 5, PBF_FIELD_BOUNDARY = 6, PBF_FIELD_PREDICTED = 7
+};
+/* Any per-particle float4 state in original order (debug readback of
+ * PBF.debug, PBF.ts:514-573). LAMBDA is in .x. */
+int pbf_read_field(pbf_ctx *ctx, uint32_t field, float *out4, uint32_t count);
+
+/* Neighbour search results of the last substep (NeighborSearch.debug,
+ * neighborSearch.ts:303-378).  Cells are indexed globally over all scenes.
+ * counts/offsets may be NULL.  Neighbour lists are in original particle ids,
+ * ordered as walked (27 cells x-major, ascending id inside a cell). */
+uint32_t pbf_num_cells(const pbf_ctx *ctx);
+int pbf_read_cells(pbf_ctx *ctx, uint32_t *counts, uint32_t *offsets);
+int pbf_read_neighbors(pbf_ctx *ctx, uint32_t *nbr_off, uint32_t *nbr_ids, uint64_t capacity, uint64_t *pairs);
+/* Predict + neighbour search only; the state is not advanced. */
+int pbf_search(pbf_ctx *ctx, float dt);
+
+/* FP64 density-constraint error of the last substep: mean and max over
+ * particles of max(0, rho~ - 1), rho~ with the boundary term; computed and
+ * reduced in FP64 on the GPU (no reference counterpart; nearest is K9's C,
+ * pressureConstrain.ts:80). Per-scene values when scene < nscenes, else all. */
+int pbf_density_error(pbf_ctx *ctx, uint32_t scene, double *mean, double *maxv);
+/* Record the error after every substep into a ring of `capacity` entries. */
+int pbf_record_density_error(pbf_ctx *ctx, uint32_t enable);
+int pbf_read_density_history(pbf_ctx *ctx, double *mean_max, uint32_t capacity, uint32_t *count);
+
+uint32_t pbf_particle_count(const pbf_ctx *ctx);
+uint32_t pbf_scene_count(const pbf_ctx *ctx);
+const char *pbf_last_error(const pbf_ctx *ctx);
+const char *pbf_version(void);
+
+/* Per-pass GPU time in ms accumulated since the last call (PBF_FLAG_TIMING):
+ * names are returned as one '\n'-separated string. */
+int pbf_pass_times(pbf_ctx *ctx, double *ms, uint32_t *launches, uint32_t capacity, uint32_t *count,
+                   const char **names);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,127 @@
+"""CPU checks of the drop-in boundary: libpbf.so loads, exports every entry
+point declared in include/pbf.h, and its GPU-free host code (voxelizers,
+.cdm parser, volume-map builder) matches independent restatements."""
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import pbf_host
+import scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pbf.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pbf_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = pbf_host.load_library()
+    out = subprocess.run(["nm", "-D", "--defined-only", pbf_host.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\sT\s(pbf_\w+)", out))
+    missing = [s for s in declared_symbols() if s not in exported]
+    assert not missing, missing
+    assert len(declared_symbols()) >= 30
+    assert lib.pbf_version().decode().startswith("pbf-mi355x")
+
+
+def test_voxelize_cube_matches_reference_loop():
+    # SPH.voxelizeCube (SPH.ts:99-121): floor((max-min)/2r) per axis, i outer.
+    r, mn, mx = 0.006, (0.15, 0.35, 0.15), (0.65, 0.85, 0.65)
+    p = scenes.lattice_cube(r, mn, mx)
+    d = 2 * r
+    n = [math.floor((mx[a] - mn[a]) / d) for a in range(3)]
+    assert p.shape == (n[0] * n[1] * n[2], 4) == (41 ** 3, 4)
+    I, J, K = np.meshgrid(*[np.arange(k) for k in n], indexing="ij")
+    want = np.stack([I.ravel() * d + mn[0], J.ravel() * d + mn[1], K.ravel() * d + mn[2]], -1).astype(np.float32)
+    assert np.array_equal(p[:, :3], want) and np.all(p[:, 3] == 0)
+
+
+def test_voxelize_sphere_matches_reference_loop():
+    r, c, R = 0.006, (0.5, 0.7, 0.5), 0.12
+    p = scenes.lattice_sphere(r, c, R)
+    n = math.floor(R / r / 2)
+    g = np.arange(-n, n + 1) * 2 * r
+    I, J, K = np.meshgrid(g, g, g, indexing="ij")
+    q = np.stack([I.ravel(), J.ravel(), K.ravel()], -1)
+    keep = (q[:, 0] ** 2 + q[:, 1] ** 2) + q[:, 2] ** 2 <= R * R
+    assert np.array_equal(p[:, :3], (q[keep] + np.array(c)).astype(np.float32))
+
+
+def test_jitter_is_seeded_and_bounded():
+    r = 0.025
+    a = scenes.lattice_cube(r, (0, 0, 0), (0.5, 0.5, 0.5), 0.01 * r, 7)
+    b = scenes.lattice_cube(r, (0, 0, 0), (0.5, 0.5, 0.5), 0.01 * r, 7)
+    c = scenes.lattice_cube(r, (0, 0, 0), (0.5, 0.5, 0.5), 0.01 * r, 8)
+    z = scenes.lattice_cube(r, (0, 0, 0), (0.5, 0.5, 0.5))
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    dev = np.abs(a[:, :3].astype(np.float64) - z[:, :3])
+    assert dev.max() <= 0.01 * r * (1 + 1e-5) and dev.max() > 0.005 * r
+    assert abs(np.mean(a[:, :3] - z[:, :3])) < 0.001 * r
+
+
+def test_cdm_round_trip_and_errors():
+    fld = scenes.unit_box_cdm_field()
+    res, sdf, vol = pbf_host.parse_cdm(scenes.cdm_text(fld))
+    assert res == (20, 20, 20) and sdf.size == 62181
+    assert np.array_equal(sdf, fld.sdf) and np.array_equal(vol, fld.vol)
+    for bad in ["", "20 20 20\n1\n0.5 \n1\n0.5 \n", "x y z\n" + "\n".join(scenes.cdm_text(fld).split("\n")[1:])]:
+        with pytest.raises(pbf_host.PBFError, match="Invalid Boundary Model File!"):
+            pbf_host.parse_cdm(bad)
+
+
+CDM = "/root/reference/public/model/domain_boundary.cdm"
+
+
+@pytest.mark.skipif(not os.path.exists(CDM), reason="reference data not mounted")
+def test_parser_on_reference_domain_boundary_file():
+    # The reference's own unit-box field: SDF = min distance to the faces.
+    res, sdf, vol = pbf_host.parse_cdm(open(CDM, "rb").read())
+    assert res == (20, 20, 20) and sdf.size == 62181 == vol.size
+    pts = scenes.cubic_nodes(res)
+    box = np.concatenate([np.minimum.reduce([x, 1 - x, y, 1 - y, z, 1 - z]).ravel() for (x, y, z) in pts])
+    assert np.abs(sdf - box).max() < 4e-7
+
+
+def test_volume_map_builder_box():
+    h, r = 0.1, 0.025
+    res, lo, hi = (32, 32, 32), (0.0, 0.0, 0.0), (1.6, 1.6, 1.6)
+    sv = pbf_host.build_volume_map(0, res, lo, hi, h, None, r)
+    nodes = np.stack(np.meshgrid(*[np.linspace(lo[a], hi[a], res[a] + 1) for a in range(3)], indexing="ij"), -1)
+    nodes = nodes.transpose(2, 1, 0, 3).reshape(-1, 3)
+    box = np.minimum.reduce([nodes[:, 0], 1.6 - nodes[:, 0], nodes[:, 1], 1.6 - nodes[:, 1], nodes[:, 2],
+                             1.6 - nodes[:, 2]])
+    np.testing.assert_allclose(sv[:, 0], box, atol=1e-6)
+    assert np.all(sv[box >= h, 1] == 0)
+    # planar wall away from edges: V_b = 0.9 int_{z > d} W / W(max(d + r/2, 2r)),
+    # with the poly6 mass of a slab: int over the disc = pi kp (h^2 - z^2)^4 / 4.
+    kp = 315 / (64 * math.pi * h ** 9)
+    for d in (0.0, 0.05):
+        z = np.linspace(d, h, 20001)
+        slab = np.trapz(math.pi * kp * (h * h - z * z) ** 4 / 4, z)
+        s = max(d + r / 2, 2 * r)
+        want = 0.9 * slab / (kp * (h * h - s * s) ** 3)
+        sel = (np.abs(box - d) < 1e-9) & (np.abs(nodes[:, 0] - 0.8) < 1e-9) & (np.abs(nodes[:, 2] - 0.8) < 1e-9)
+        got = sv[sel & (nodes[:, 1] < 0.8), 1]
+        assert got.size == 1
+        assert got[0] == pytest.approx(want, rel=0.06)  # 16^3 midpoint quadrature
+
+
+def test_volume_map_builder_obstacle_sdf():
+    res, lo, hi = (8, 8, 8), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
+    shape = [0.5, 0.5, 0.5, 0.2, 0.5, 0.5, 0.5, 0.3, 0.08]
+    sv = pbf_host.build_volume_map(1, res, lo, hi, 0.05, shape, 0.0125)
+    nodes = np.stack(np.meshgrid(*[np.linspace(0, 1, 9)] * 3, indexing="ij"), -1).transpose(2, 1, 0, 3).reshape(-1, 3)
+    box = np.minimum.reduce([nodes[:, 0], 1 - nodes[:, 0], nodes[:, 1], 1 - nodes[:, 1], nodes[:, 2], 1 - nodes[:, 2]])
+    sph = np.linalg.norm(nodes - 0.5, axis=1) - 0.2
+    q = np.hypot(nodes[:, 0] - 0.5, nodes[:, 2] - 0.5) - 0.3
+    tor = np.hypot(q, nodes[:, 1] - 0.5) - 0.08
+    np.testing.assert_allclose(sv[:, 0], np.minimum.reduce([box, sph, tor]), atol=1e-6)

@@ -1,0 +1,207 @@
+"""GPU parity: libpbf's HIP path (through the C ABI) against the CPU oracle.
+
+Bars: cell counts, scan offsets and neighbour sets bit-exact; one-step
+positions and velocities within 1e-5 (norm-relative, float32); the density
+error statistics of a 200-step run within 2 %.  The engine and the oracle
+share one numerics specification (same summation order, no contraction), so
+most comparisons below are in fact exact; they are asserted as such.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import scenes
+from pairs import gpu_fields, make_oracle, oracle_fields, rel_err, ulp_diff
+from pbf_host import PBF_FLAG_NO_GRAPH, PBFError
+
+pytestmark = pytest.mark.gpu
+
+
+def small_scenes():
+    return {
+        "cfg0_small": scenes.dam_break(16, 16, 8, map_res=32, name="cfg0_small"),
+        "cfg1_physics": scenes.cfg1_small(16, 16, 8, map_res=32),
+        "ref_cube_drop": scenes.ref_cube_drop(),
+        "ref_boundary_cubic": scenes.ref_boundary(scenes.unit_box_cdm_field()),
+        "hash_obstacle": scenes.obstacle_drop(12, 16, 12, map_res=48, hash_log2=12, name="hash_obstacle"),
+    }
+
+
+SCENES = small_scenes()
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_search_bit_exact(name):
+    sc = SCENES[name]
+    print(f"[search] {name} n={sc.n}", flush=True)
+    sim = scenes.make_engine(sc)
+    o = make_oracle(sc)
+    dt = sc.substep_dt()
+    sim.search(dt)
+    o.search(dt)
+    gc, go = sim.read_cells()
+    oc, oo = o.cells()
+    assert gc.sum() == sc.n
+    assert np.array_equal(gc, oc), "cell counts differ"
+    assert np.array_equal(go, oo), "scan offsets differ"
+    assert np.array_equal(sim.read_field("predicted")[:, :3], o.get(oracle.PREDICTED)[:, :3])
+    goff, gids = sim.read_neighbors()
+    ooff, oids = o.neighbors()
+    assert np.array_equal(goff, ooff), "neighbour counts differ"
+    assert np.array_equal(gids, oids), "neighbour lists differ (set or walk order)"
+    sim.close()
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_one_step_parity(name):
+    sc = SCENES[name]
+    print(f"[one step] {name} n={sc.n}", flush=True)
+    sim = scenes.make_engine(sc)
+    o = make_oracle(sc)
+    sim.step(sc.dt, sc.substeps, sc.iterations)
+    o.step(sc.dt, sc.substeps, sc.iterations)
+    g, c = gpu_fields(sim), oracle_fields(o)
+    assert rel_err(g["position"][:, :3], c["position"][:, :3]) <= 1e-5
+    assert rel_err(g["velocity"][:, :3], c["velocity"][:, :3]) <= 1e-5
+    assert rel_err(g["position"][:, 3], c["position"][:, 3]) <= 1e-5
+    for k in ("accel", "lambda", "omega", "normal"):
+        assert rel_err(g[k], c[k]) <= 1e-5, k
+    sim.close()
+
+
+@pytest.mark.parametrize("name", ["cfg0_small", "ref_cube_drop", "ref_boundary_cubic", "hash_obstacle"])
+def test_one_step_bit_exact(name):
+    sc = SCENES[name]
+    sim = scenes.make_engine(sc)
+    o = make_oracle(sc)
+    sim.step(sc.dt, sc.substeps, sc.iterations)
+    o.step(sc.dt, sc.substeps, sc.iterations)
+    g, c = gpu_fields(sim), oracle_fields(o)
+    for k in g:
+        assert ulp_diff(g[k], c[k]) == 0, (k, ulp_diff(g[k], c[k]))
+    sim.close()
+
+
+def test_many_steps_stay_exact():
+    sc = scenes.cfg1_small(16, 16, 8, map_res=32)
+    sim = scenes.make_engine(sc)
+    o = make_oracle(sc)
+    for s in range(30):
+        sim.step(sc.dt, sc.substeps, sc.iterations)
+        o.step(sc.dt, sc.substeps, sc.iterations)
+    g, c = gpu_fields(sim), oracle_fields(o)
+    assert rel_err(g["position"][:, :3], c["position"][:, :3]) <= 1e-5
+    assert rel_err(g["velocity"][:, :3], c["velocity"][:, :3]) <= 1e-5
+    print("ulp after 30 steps:", {k: ulp_diff(g[k], c[k]) for k in g})
+    sim.close()
+
+
+def test_density_error_fp64_matches_oracle():
+    sc = scenes.dam_break(16, 16, 8, map_res=32, name="diag")
+    sim = scenes.make_engine(sc)
+    o = make_oracle(sc)
+    for _ in range(5):
+        sim.step(sc.dt, sc.substeps, sc.iterations)
+        o.step(sc.dt, sc.substeps, sc.iterations)
+    gm, gx = sim.density_error()
+    om, ox = o.density_error()
+    assert gm == pytest.approx(om, rel=1e-12, abs=1e-18)
+    assert gx == pytest.approx(ox, rel=1e-12, abs=1e-18)
+
+
+def test_200_step_density_statistics_cfg0():
+    sc = scenes.cfg0()
+    print(f"[200 steps] n={sc.n}", flush=True)
+    sim = scenes.make_engine(sc)
+    sim.record_density_error(True)
+    o = make_oracle(sc)
+    ohist = []
+    for s in range(sc.steps):
+        sim.step(sc.dt, sc.substeps, sc.iterations)
+        for _ in range(sc.substeps):
+            o.substep(sc.substep_dt(), sc.iterations)
+            ohist.append(o.density_error())
+    ghist = sim.density_history()
+    ohist = np.array(ohist)
+    assert ghist.shape == ohist.shape
+    gmean, omean = ghist[:, 0].mean(), ohist[:, 0].mean()
+    gmax, omax = ghist[:, 1].max(), ohist[:, 1].max()
+    print(f"mean of mean error gpu {gmean:.6g} cpu {omean:.6g}; max gpu {gmax:.6g} cpu {omax:.6g}")
+    assert gmean == pytest.approx(omean, rel=0.02)
+    assert gmax == pytest.approx(omax, rel=0.02)
+
+
+def test_batched_scenes_match_independent_runs():
+    specs = scenes.batch_specs(6, 300, 3000, seed=99)
+    scs = [scenes.dam_break(*d, map_res=24, seed=1234 + k, name=f"b{k}") for k, d in enumerate(specs)]
+    print("[batch] sizes", [s.n for s in scs], flush=True)
+    sim = scenes.make_batch_engine(scs)
+    for _ in range(3):
+        sim.step(scs[0].dt, scs[0].substeps, scs[0].iterations)
+    gp, gv = sim.read_positions(), sim.read_velocities()
+    base = 0
+    for s in scs:
+        o = make_oracle(s)
+        for _ in range(3):
+            o.step(s.dt, s.substeps, s.iterations)
+        assert np.array_equal(gp[base:base + s.n], o.get(oracle.POSITION)), s.name
+        assert np.array_equal(gv[base:base + s.n, :3], o.get(oracle.VELOCITY)[:, :3]), s.name
+        base += s.n
+    per = [sim.density_error(k) for k in range(len(scs))]
+    assert all(np.isfinite(m) for m, _ in per)
+    sim.close()
+
+
+def test_graph_replay_equals_direct_launches():
+    sc = scenes.cfg1_small(16, 16, 8, map_res=32)
+    a = scenes.make_engine(sc)
+    b = scenes.make_engine(sc, flags=PBF_FLAG_NO_GRAPH)
+    for s in range(6):
+        a.step(sc.dt, sc.substeps, sc.iterations)
+        b.step(sc.dt, sc.substeps, sc.iterations)
+        # eager work and a synchronisation between graph replays
+        pa, pb = a.read_positions(), b.read_positions()
+        assert np.array_equal(pa, pb), s
+    assert np.array_equal(a.read_velocities(), b.read_velocities())
+    a.close()
+    b.close()
+
+
+def test_reference_error_messages():
+    sc = SCENES["cfg0_small"]
+    sim = scenes.make_engine(sc)
+    with pytest.raises(PBFError, match="Illegal length of Particle Position Buffer!"):
+        sim.setParticlePosition(np.zeros(10, np.float32), count=3)
+    with pytest.raises(PBFError, match="To Many Particles!"):
+        sim.setParticlePosition(np.zeros((sc.n + 1, 4), np.float32))
+    with pytest.raises(PBFError, match="Invalid Boundary Model File!"):
+        sim.setBoundaryData("20 20 20\n1\n0.5 \n1\n0.5 \n")
+    sim.close()
+
+
+def test_reference_api_flow_and_cdm_text():
+    # Controller.setSceneConfig flow (controller.ts:194-248): reset, voxelize,
+    # boundary text, setParticlePosition, run.
+    from pbf_host import PBF
+    sim = PBF()
+    sim.reset()
+    n = sim.voxelizeCube((0.15, 0.35, 0.15), (0.65, 0.85, 0.65))
+    assert n == 41 ** 3
+    sim.setBoundaryData(scenes.cdm_text(scenes.unit_box_cdm_field()))
+    sim.setConfig(scenes.REF_CONFIG)
+    sim.setParticlePosition()
+    sim.start()
+    sim.run()
+    sim.sync()
+    p = sim.read_positions()
+    assert p.shape == (n, 4) and np.isfinite(p).all() and (p[:, 3] > 0).all()
+    sim.close()
+
+
+def test_neighbour_overflow_is_reported():
+    sc = scenes.dam_break(8, 8, 8, map_res=16, name="overflow")
+    sim = scenes.make_engine(sc, max_neighbors=8)
+    sim.step(sc.dt, 1, 1)
+    with pytest.raises(PBFError, match="neighbour overflow"):
+        sim.sync()
+    sim.close()

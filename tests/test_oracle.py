@@ -1,0 +1,390 @@
+"""Pins of the CPU oracle (oracle/pbf_oracle.c) by closed forms, brute force,
+invariants and published formulas -- no GPU needed.
+
+Each test names the reference code the oracle restates and the independent
+fact that pins it.
+"""
+import math
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import scenes
+
+F32 = np.float32
+
+
+def make(points, r=0.025, h=None, lo=(0, 0, 0), hi=(1, 1, 1), grid_mode=0, hash_log2=22):
+    h = 4 * r if h is None else h
+    p = np.zeros((len(points), 4), np.float32)
+    p[:, :3] = points
+    return oracle.Oracle(p, r, h, 1000.0, lo, hi, grid_mode, hash_log2)
+
+
+# --------------------------------------------------------------------------
+# SPH kernels (src/simulator/PBF/shader/common.ts:1-40)
+def test_poly6_closed_forms():
+    h = 0.1
+    o = make(np.zeros((1, 3)), r=0.025, h=h)
+    p6, sp, co = o.kernels(np.array([0.0], np.float32))
+    assert p6[0] == pytest.approx(315 / (64 * math.pi * h ** 3), rel=2e-7)
+    # normalisation: 4 pi int_0^h r^2 W(r) dr = 1, W evaluated by the oracle
+    r = (np.arange(200000) + 0.5) / 200000 * h
+    W, _, _ = o.kernels((r * r).astype(np.float32))
+    integral = float(np.sum(4 * math.pi * r ** 2 * W.astype(np.float64)) * (h / 200000))
+    assert integral == pytest.approx(1.0, abs=2e-5)
+    # zero at and beyond h
+    W, _, _ = o.kernels(np.array([h * h * 1.0001, 4 * h * h], np.float32))
+    assert np.all(W == 0)
+
+
+def test_spiky_gradient_matches_derivative_of_spiky_kernel():
+    # W_spiky(r) = 15/(pi h^6) (h-r)^3 (Mueller et al. 2003); grad = W'(r) r_hat.
+    h = 0.025
+    o = make(np.zeros((1, 3)), r=0.006, h=h)
+    r = np.linspace(0.05, 0.999, 400) * h
+    _, f, _ = o.kernels((r * r).astype(np.float32))
+    Ws = lambda x: 15 / (math.pi * h ** 6) * (h - x) ** 3
+    eps = 1e-7 * h
+    dW = (Ws(r + eps) - Ws(r - eps)) / (2 * eps)
+    np.testing.assert_allclose(f.astype(np.float64) * r, dW, rtol=2e-4)
+    _, f0, _ = o.kernels(np.array([0.0, h * h, 2 * h * h], np.float32))
+    assert f0[0] == 0.0 and abs(f0[1]) < 1e-3 * abs(dW[0]) and f0[2] == 0.0
+
+
+def test_cohesion_akinci_2013_and_continuity():
+    # Akinci et al. 2013, eq. (2): C(r) = 32/(pi h^9) {(h-r)^3 r^3 if 2r > h; 2(h-r)^3 r^3 - h^6/64 if 0 < 2r <= h}
+    h = 0.1
+    o = make(np.zeros((1, 3)), r=0.025, h=h)
+    r = np.linspace(0.01, 0.99, 300) * h
+    _, _, f = o.kernels((r * r).astype(np.float32))
+    C = np.where(2 * r > h, (h - r) ** 3 * r ** 3, 2 * (h - r) ** 3 * r ** 3 - h ** 6 / 64) * 32 / (math.pi * h ** 9)
+    np.testing.assert_allclose(f.astype(np.float64) * r, C, rtol=2e-4, atol=1e-6 * np.abs(C).max())
+    lo, hi = np.float32(0.5 * h * (1 - 1e-5)), np.float32(0.5 * h * (1 + 1e-5))
+    _, _, fc = o.kernels(np.array([lo * lo, hi * hi], np.float32))
+    assert fc[0] * lo == pytest.approx(fc[1] * hi, rel=1e-3)
+
+
+# --------------------------------------------------------------------------
+# wall clamp (common.ts:43-68) and its sin
+def test_cephes_sinf_accuracy():
+    xs = np.concatenate([np.linspace(-8000, 8000, 40001), np.linspace(-3.5, 3.5, 7001)]).astype(np.float32)
+    err = max(abs(oracle.sinf(float(x)) - math.sin(float(x))) for x in xs)
+    assert err < 3e-7
+
+
+def test_clamp_box_and_inset():
+    o = make(np.zeros((1, 3)), lo=(0, 0, 0), hi=(2, 1, 1))
+    pts = np.array([[0.5, 0.5, 0.5], [-1.0, 0.3, 0.3], [2.5, 0.3, 1.5], [1.0, -0.1, 0.99995]], np.float32)
+    out = o.clamp(pts)
+    assert np.array_equal(out[0], pts[0])
+    assert 0.0 <= out[1, 0] <= 1e-4 and out[1, 1] == pts[1, 1]
+    assert 2.0 - 1e-4 <= out[2, 0] <= 2.0 and 1.0 - 1e-4 <= out[2, 2] <= 1.0
+    assert 0.0 <= out[3, 1] <= 1e-4 and out[3, 2] <= 1.0
+    assert np.array_equal(o.clamp(pts), out)
+
+
+# --------------------------------------------------------------------------
+# neighbour search (neighborShader.ts, neighborSearch.ts:270-301)
+def brute_force_lists(x, h2):
+    n = x.shape[0]
+    lists = []
+    for i in range(n):
+        d = x - x[i]  # x_j - x_i, float32
+        d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        lists.append(set(np.nonzero(d2 <= h2)[0].tolist()))
+    return lists
+
+
+@pytest.mark.parametrize("grid_mode,hash_log2", [(0, 22), (1, 22), (1, 6)])
+def test_neighbor_sets_vs_brute_force(grid_mode, hash_log2):
+    rng = np.random.default_rng(7)
+    r, h = 0.025, 0.1
+    x = rng.uniform(0.0, 0.8, (1500, 3)).astype(np.float32)
+    # pairs at exactly h and just beyond along an axis
+    x[:10] = np.array([[0.3, 0.3, 0.3]] * 10, np.float32)
+    x[1, 0] = np.float32(0.3) + np.float32(h)
+    x[2, 0] = np.nextafter(x[1, 0], np.float32(1))
+    x[3, 1] = np.float32(0.3) - np.float32(h)
+    o = make(x, r=r, h=h, hi=(1, 1, 1), grid_mode=grid_mode, hash_log2=hash_log2)
+    o.search(1e-9)
+    xs = o.get(oracle.PREDICTED)[:, :3]
+    h2 = np.float32(h * h)
+    want = brute_force_lists(xs, h2)
+    off, ids = o.neighbors()
+    for i in range(x.shape[0]):
+        got = ids[off[i]:off[i + 1]].tolist()
+        assert len(got) == len(set(got)), "duplicate neighbour (hash slot visited twice)"
+        assert set(got) == want[i], i
+
+
+def test_neighbor_order_and_cells():
+    rng = np.random.default_rng(3)
+    r, h = 0.025, 0.1
+    x = rng.uniform(0.0, 1.0, (3000, 3)).astype(np.float32)
+    o = make(x, r=r, h=h)
+    o.search(1e-9)
+    xs = o.get(oracle.PREDICTED)[:, :3]
+    D = 10
+    c = np.clip(np.floor(xs * np.float32(1 / h)).astype(np.int64), 0, D - 1)
+    key = (c[:, 0] * D + c[:, 1]) * D + c[:, 2]
+    counts, offsets = o.cells()
+    assert np.array_equal(counts, np.bincount(key, minlength=D ** 3).astype(np.uint32))
+    assert np.array_equal(offsets, np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32))
+    off, ids = o.neighbors()
+    for i in range(0, 3000, 37):
+        lst = ids[off[i]:off[i + 1]]
+        order = sorted(lst.tolist(), key=lambda j: (key[j], j))
+        assert lst.tolist() == order
+
+
+# --------------------------------------------------------------------------
+# boundary samplers (discreteFieldShader.ts:15-191, boundaryVolume.ts:26-57)
+def cubic_poly(coef):
+    mons = [(a, b, c) for a in range(4) for b in range(4) for c in range(4) if a + b + c <= 3]
+
+    def f(x, y, z):
+        return sum(k * x ** a * y ** b * z ** c for k, (a, b, c) in zip(coef, mons))
+
+    def g(x, y, z):
+        gx = sum(k * a * x ** max(a - 1, 0) * y ** b * z ** c for k, (a, b, c) in zip(coef, mons))
+        gy = sum(k * b * x ** a * y ** max(b - 1, 0) * z ** c for k, (a, b, c) in zip(coef, mons))
+        gz = sum(k * c * x ** a * y ** b * z ** max(c - 1, 0) for k, (a, b, c) in zip(coef, mons))
+        return np.stack([gx, gy, gz], -1)
+    return f, g, len(mons)
+
+
+def test_cubic_field_reproduces_cubic_polynomials():
+    rng = np.random.default_rng(11)
+    f, g, nm = cubic_poly([0] * 20)
+    coef = rng.uniform(-1, 1, nm)
+    f, g, _ = cubic_poly(coef)
+    fld = scenes.cubic_field_from(f, f, (6, 5, 7))
+    o = make(np.zeros((1, 3)))
+    o.set_cubic_field(fld.res, fld.lo, fld.hi, fld.sdf, fld.vol, quirk=0)
+    pts = rng.uniform(0.0, 0.999, (400, 3)).astype(np.float32)
+    out = o.field_eval(pts)
+    P = pts.astype(np.float64)
+    want = f(P[:, 0], P[:, 1], P[:, 2])
+    np.testing.assert_allclose(out[:, 3], want, atol=2e-5)
+    np.testing.assert_allclose(out[:, 4], want, atol=2e-5)
+    np.testing.assert_allclose(out[:, :3], g(P[:, 0], P[:, 1], P[:, 2]), atol=5e-4)
+
+
+def serendipity_numpy(res, sdf, pts, quirk):
+    """Independent float64 evaluation of the 32-node element with explicit
+    node coordinates; quirk swaps the edge nodes the reference reads twice."""
+    R = np.array(res)
+    nx, ny, nz = R + 1
+    nc = nx * ny * nz
+    nex = nz * ny * 2 * R[0]
+    ney = nx * nz * 2 * R[1]
+    out = []
+    for p in pts.astype(np.float64):
+        u = p * R
+        c = np.minimum(np.floor(u).astype(int), R - 1)
+        s = 2 * (u - c) - 1
+        i, j, k = c
+        val = 0.0
+        for ex in (0, 1):
+            for ey in (0, 1):
+                for ez in (0, 1):
+                    sx, sy, sz = 2 * ex - 1, 2 * ey - 1, 2 * ez - 1
+                    N = (1 + sx * s[0]) * (1 + sy * s[1]) * (1 + sz * s[2]) * (9 * (s @ s) - 19) / 64
+                    val += N * sdf[((k + ez) * ny + (j + ey)) * nx + (i + ex)]
+        for ax in range(3):
+            o1, o2 = [a for a in range(3) if a != ax]
+            for t, tsgn in ((0, -1), (1, 1)):
+                for e1 in (0, 1):
+                    for e2 in (0, 1):
+                        N = 9 / 64 * (1 - s[ax] ** 2) * (1 + 3 * tsgn * s[ax]) * (1 + (2 * e1 - 1) * s[o1]) * \
+                            (1 + (2 * e2 - 1) * s[o2])
+                        a2 = 2 * c[ax] + t
+                        if ax == 0:
+                            a2q = 2 * c[0] if (quirk and e2 == 1 and t == 1) else a2
+                            idx = nc + ((k + e2) * ny + (j + e1)) * 2 * R[0] + a2q
+                        elif ax == 1:
+                            idx = nc + nex + ((i + e1) * nz + (k + e2)) * 2 * R[1] + a2
+                        else:
+                            a2q = 2 * c[2] if (quirk and t == 1) else a2
+                            idx = nc + nex + ney + ((j + e2) * nx + (i + e1)) * 2 * R[2] + a2q
+                        val += N * sdf[idx]
+        out.append(val)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("quirk", [0, 1])
+def test_cubic_volume_path_index_quirk(quirk):
+    # A field linear in x and z exposes the reference's repeated edge reads
+    # (discreteFieldShader.ts:146-147,152-153) on the volume path only.
+    fld = scenes.cubic_field_from(lambda x, y, z: 0.3 + x - 2 * z + 0.5 * y,
+                                  lambda x, y, z: 0.3 + x - 2 * z + 0.5 * y, (4, 4, 4))
+    o = make(np.zeros((1, 3)))
+    o.set_cubic_field(fld.res, fld.lo, fld.hi, fld.sdf, fld.vol, quirk=quirk)
+    rng = np.random.default_rng(5)
+    pts = rng.uniform(0, 0.999, (200, 3)).astype(np.float32)
+    out = o.field_eval(pts)
+    np.testing.assert_allclose(out[:, 3], serendipity_numpy(fld.res, fld.sdf.astype(np.float64), pts, 0), atol=2e-6)
+    want_vol = serendipity_numpy(fld.res, fld.vol.astype(np.float64), pts, quirk)
+    np.testing.assert_allclose(out[:, 4], want_vol, atol=2e-6)
+    exact = 0.3 + pts[:, 0] - 2 * pts[:, 2] + 0.5 * pts[:, 1]
+    dev = np.abs(want_vol - exact).max()
+    assert (dev > 1e-3) if quirk else (dev < 1e-5)
+
+
+def test_trilinear_field_linear_exact():
+    res, lo, hi = (8, 6, 5), (0.0, -1.0, 0.5), (2.0, 1.0, 1.5)
+    nodes = np.stack(np.meshgrid(*[np.linspace(lo[a], hi[a], res[a] + 1) for a in range(3)], indexing="ij"), -1)
+    nodes = nodes.transpose(2, 1, 0, 3).reshape(-1, 3)  # x fastest
+    sv = np.zeros((nodes.shape[0], 2), np.float32)
+    sv[:, 0] = 0.25 + 0.5 * nodes[:, 0] - 0.25 * nodes[:, 1] + 0.75 * nodes[:, 2]
+    sv[:, 1] = 1.0 + nodes[:, 2]
+    o = make(np.zeros((1, 3)), lo=lo, hi=hi)
+    o.set_trilinear_field(res, lo, hi, sv)
+    rng = np.random.default_rng(2)
+    pts = np.stack([rng.uniform(lo[a], hi[a], 300) for a in range(3)], -1).astype(np.float32)
+    out = o.field_eval(pts)
+    P = pts.astype(np.float64)
+    np.testing.assert_allclose(out[:, 3], 0.25 + 0.5 * P[:, 0] - 0.25 * P[:, 1] + 0.75 * P[:, 2], atol=3e-6)
+    np.testing.assert_allclose(out[:, 4], 1.0 + P[:, 2], atol=3e-6)
+    np.testing.assert_allclose(out[:, :3], np.tile([0.5, -0.25, 0.75], (300, 1)), atol=3e-5)
+
+
+def test_boundary_debug_query_known_answer():
+    # DebugShader (discreteFieldShader.ts:194-237): point (0.5, 0.5, 0.02),
+    # ParticleRadius 0.007353, unit-box field: d = 0.02, n = +z,
+    # x_b = x - max(d + r/2, 2r) n.
+    fld = scenes.unit_box_cdm_field()
+    o = make(np.zeros((1, 3)), r=0.007353, h=0.025)
+    o.set_cubic_field(fld.res, fld.lo, fld.hi, fld.sdf, fld.vol, quirk=1)
+    out = o.boundary_sample(np.array([[0.5, 0.5, 0.02]], np.float32))[0]
+    zb = 0.02 - max(0.02 + 0.5 * 0.007353, 2 * 0.007353)
+    np.testing.assert_allclose(out[:3], [0.5, 0.5, zb], atol=2e-6)
+    assert out[3] > 0
+    # beyond h: no boundary sample
+    assert np.all(o.boundary_sample(np.array([[0.5, 0.5, 0.03]], np.float32)) == 0)
+
+
+# --------------------------------------------------------------------------
+# physics invariants and closed forms (pressureConstrain.ts, attributeUpdate.ts)
+def lattice(n, r, origin=0.5):
+    g = np.arange(n) * 2 * r
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], -1) + origin
+
+
+@pytest.mark.parametrize("r,h,expect", [(0.025, 0.1, 0.908798), (0.006, 0.025, 0.912940)])
+def test_rest_lattice_density(r, h, expect):
+    # V sum_j W_ij over a perfect 2r lattice, computed here in float64.
+    m = int(math.ceil(h / (2 * r)))
+    g = np.arange(-m, m + 1) * 2 * r
+    d2 = (g[:, None, None] ** 2 + g[None, :, None] ** 2 + g[None, None, :] ** 2).ravel()
+    W = np.where(d2 <= h * h, 315 / (64 * math.pi * h ** 9) * (h * h - d2) ** 3, 0)
+    rho = 0.9 * (2 * r) ** 3 * W.sum()
+    assert rho == pytest.approx(expect, abs=5e-7)
+    # the oracle's K12 density of an interior particle (gravity off)
+    x = lattice(11, r, 0.3)
+    o = make(x, r=r, h=h, hi=(2, 2, 2))
+    o.set_config(0.0, 0.0, 0.0, 0.0)
+    o.substep(1e-4, 3)
+    p = o.get(oracle.POSITION)
+    centre = np.argmin(np.linalg.norm(x - x.mean(0), axis=1))
+    assert p[centre, 3] == pytest.approx(1000 * rho, rel=1e-5)  # f32 sum of ~40 poly6 terms
+
+
+def test_lattice_block_is_stationary_without_gravity():
+    # rho~ < 1 everywhere -> C = 0 -> lambda = 0, dp = 0, v = 0; with tension
+    # off nothing moves, bit for bit.
+    x = lattice(8, 0.025, 0.4)
+    o = make(x, hi=(2, 2, 2))
+    o.set_config(0.01, 0.01, 0.0, 0.0)
+    for _ in range(5):
+        o.substep(1 / 120, 4)
+    p = o.get(oracle.POSITION)
+    assert np.array_equal(p[:, :3], x.astype(np.float32))
+    assert np.all(o.get(oracle.VELOCITY) == 0) and np.all(o.get(oracle.LAMBDA)[:, 0] == 0)
+
+
+def test_two_particle_lambda_and_correction_closed_form():
+    # h = 2r makes V W(0) > 1 so a pair is over-dense; lambda and dp in closed form.
+    r, h, d = 0.025, 0.05, 0.03
+    x = np.array([[0.5, 0.5, 0.5], [0.5 + d, 0.5, 0.5]])
+    o = make(x, r=r, h=h)
+    o.set_config(0.0, 0.0, 0.0, 0.0)
+    o.substep(1e-3, 1)
+    V = 0.9 * (2 * r) ** 3
+    W = lambda q: 315 / (64 * math.pi * h ** 9) * (h * h - q * q) ** 3
+    dW = -45 / (math.pi * h ** 6) * (h - d) ** 2
+    rho = V * (W(0) + W(d))
+    lam = -(rho - 1) / (2 * V * V * dW * dW + 1e-6)
+    assert o.get(oracle.LAMBDA)[0, 0] == pytest.approx(lam, rel=1e-5)
+    # grad W(x_0 - x_1) = dW * (-x_hat); dp_0 = 0.5 V (lam + lam) grad W
+    dp0 = -V * lam * dW
+    p = o.get(oracle.POSITION)
+    assert dp0 < 0  # the over-dense pair pushes apart
+    assert p[0, 0] == pytest.approx(0.5 + dp0, rel=1e-6)
+    assert p[1, 0] == pytest.approx(0.5 + d - dp0, rel=1e-6)
+
+
+def test_free_fall_recursion():
+    x = np.array([[0.5, 0.9, 0.5]])
+    o = make(x, hi=(1, 1, 1))
+    o.set_config(0.0, 0.0, 0.0, 9.8)
+    dt = np.float32(1 / 120)
+    y, v = 0.9, 0.0
+    for _ in range(20):
+        o.substep(float(dt), 2)
+        vn = v + float(dt) * -9.8
+        yn = y + float(dt) * vn
+        v, y = (yn - y) / float(dt), yn
+    p = o.get(oracle.POSITION)
+    assert p[0, 1] == pytest.approx(y, rel=1e-5)
+    assert o.get(oracle.VELOCITY)[0, 1] == pytest.approx(v, rel=1e-4)
+
+
+def test_density_error_fp64_brute_force():
+    sc = scenes.dam_break(6, 6, 6, map_res=16, name="tiny")
+    o = oracle.Oracle(sc.positions, sc.particle_radius, sc.kernel_radius, 1000.0, sc.domain_lo, sc.domain_hi)
+    o.set_trilinear_field(sc.field.res, sc.field.lo, sc.field.hi, sc.field.sdf_vol)
+    o.set_config(0.01, 0, 0, 9.8)
+    for _ in range(3):
+        o.substep(sc.substep_dt(), 4)
+    mean, mx = o.density_error()
+    p = o.get(oracle.POSITION)[:, :3].astype(np.float64)
+    off, ids = o.neighbors()
+    h, r = sc.kernel_radius, sc.particle_radius
+    V = 0.9 * (2 * r) ** 3
+    kp = 315 / (64 * math.pi * h ** 9)
+    b = o.boundary_sample(o.get(oracle.POSITION)[:, :3])
+    C = []
+    for i in range(p.shape[0]):
+        d2 = ((p[ids[off[i]:off[i + 1]]] - p[i]) ** 2).sum(1)
+        rho = V * np.sum(np.where(d2 <= h * h, kp * (h * h - d2) ** 3, 0))
+        if b[i, 3] > 0:
+            q = ((p[i] - b[i, :3].astype(np.float64)) ** 2).sum()
+            rho += b[i, 3] * (kp * (h * h - q) ** 3 if q <= h * h else 0)
+        C.append(max(rho - 1, 0))
+    assert mean == pytest.approx(np.mean(C), rel=1e-9, abs=1e-15)
+    assert mx == pytest.approx(np.max(C), rel=1e-9, abs=1e-15)
+
+
+def test_oracle_is_deterministic_across_thread_counts(tmp_path):
+    code = ("import sys; sys.path[:0]=%r\n"
+            "import numpy as np, oracle, scenes\n"
+            "sc=scenes.dam_break(12,12,8,map_res=16)\n"
+            "o=oracle.Oracle(sc.positions,sc.particle_radius,sc.kernel_radius,1000.0,sc.domain_lo,sc.domain_hi)\n"
+            "o.set_trilinear_field(sc.field.res,sc.field.lo,sc.field.hi,sc.field.sdf_vol)\n"
+            "o.set_config(0.01,0.01,0.1,9.8)\n"
+            "[o.substep(sc.substep_dt(),4) for _ in range(4)]\n"
+            "np.save(sys.argv[1], o.get(oracle.POSITION))\n") % (sys.path[:3],)
+    outs = []
+    for t in ("1", "8"):
+        f = tmp_path / f"p{t}.npy"
+        env = dict(os.environ, OMP_NUM_THREADS=t)
+        subprocess.run([sys.executable, "-c", code, str(f)], check=True, env=env)
+        outs.append(np.load(f))
+    assert np.array_equal(outs[0], outs[1])

@@ -1,0 +1,934 @@
+// pbf_engine.hip -- the engine behind the C ABI of include/pbf.h.
+//
+// One context = one HIP stream, device buffers sized at creation for
+// `capacity` particles and every scene's cells, and a cache of hipGraphs,
+// one per (dt, iterations, diagnostics) shape, each holding the launch
+// sequence of one substep.  All particle state is kept in cell-sorted slot
+// order; `ids` maps a slot to the caller's particle index.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pbf.h"
+#include "pbf_device.h"
+#include "pbf_internal.h"
+
+using namespace pbf;
+
+namespace {
+
+enum Pass {
+  P_PREDICT, P_SCAN, P_SCATTER, P_SETTLE, P_LIST, P_LAMBDA, P_DP, P_DENSITY, P_VORT, P_XSPH, P_DIAG, P_NPASS
+};
+const char *kPassNames = "predict_bin\nscan\nscatter\nsettle\nbuild_list\nlambda\ndp\ndensity\nvorticity\nxsph\ndiag";
+
+struct SceneHost {
+  double lo[3], hi[3];
+  uint32_t count;
+  int dims[3];
+  uint32_t ncells, cell_base, pbase;
+  int field_kind = PBF_FIELD_NONE;
+  int fres[3] = {0, 0, 0};
+  double flo[3] = {0, 0, 0}, fhi[3] = {0, 0, 0};
+  std::vector<float> field;  // cubic: sdf then vol; trilinear: interleaved
+};
+
+struct GraphEntry {
+  uint32_t dt_bits, iters, diag;
+  hipGraphExec_t exec;
+};
+
+struct PendingEvent {
+  int pass;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct pbf_ctx {
+  pbf_params params;
+  pbf_config cfg;
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<SceneHost> scenes;
+  bool batch = false;
+  uint32_t n = 0, ncap = 0, ld = 0, maxn = 0, ncells = 0;
+  std::vector<float> staged;
+  uint64_t staged_serial = 0;
+  // device state (slot order)
+  float4 *pos = nullptr, *vel = nullptr, *acc = nullptr, *om = nullptr, *nrm = nullptr;
+  float4 *XA = nullptr, *XB = nullptr, *Xold = nullptr, *B = nullptr, *xs = nullptr, *stage4 = nullptr;
+  uint32_t *ids = nullptr, *cellkey = nullptr, *rank = nullptr, *tmp_key = nullptr, *tmp_src = nullptr;
+  uint32_t *ncount = nullptr, *nbr = nullptr;
+  float *lam = nullptr, *lam_sorted = nullptr;
+  uint32_t *counts = nullptr, *offsets = nullptr;
+  std::vector<uint32_t *> scan_sums, scan_prefix;
+  uint32_t *flags = nullptr;  // [0] overflow, [1] max neighbour count
+  // scenes on device
+  Scene *d_scenes = nullptr;
+  uint32_t *sid = nullptr, *ranges = nullptr;
+  std::vector<Scene> hscenes;
+  float *pool = nullptr;
+  size_t pool_floats = 0;
+  bool any_field = false;
+  // diagnostics
+  double *ci = nullptr, *per_scene = nullptr, *hist = nullptr;
+  uint32_t *hist_ctr = nullptr;
+  uint32_t hist_cap = 4096;
+  bool record_diag = false;
+  // debug search
+  uint32_t *dbg_counts = nullptr, *dbg_ids = nullptr, *ids_save = nullptr;
+  bool last_was_search = false, have_dbg_counts = false, have_lists = false;
+  // graphs and timing
+  std::vector<GraphEntry> graphs;
+  std::vector<hipEvent_t> event_pool;
+  std::vector<PendingEvent> pending;
+  double pass_ms[P_NPASS] = {0};
+  uint32_t pass_count[P_NPASS] = {0};
+};
+
+namespace {
+
+int fail(pbf_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                              \
+  do {                                                                                            \
+    hipError_t e_ = (expr);                                                                       \
+    if (e_ != hipSuccess) return fail(c, PBF_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc((void **)p, count * sizeof(T));
+}
+
+Phys make_phys(const pbf_ctx *c, float dt) {
+  const double r = c->params.particle_radius, h = c->params.kernel_radius, rho0 = c->params.rest_density;
+  const double PI = 3.141592653589793;
+  const double d = 2.0 * r;
+  const double V64 = ((0.9 * d) * d) * d;  // SPH.ts:48-50
+  Phys P;
+  P.h = (float)h;
+  P.h2 = (float)(h * h);
+  P.half_h = (float)(0.5 * h);
+  const float rf = (float)r;
+  P.half_r = 0.5f * rf;
+  P.two_r = 2.0f * rf;
+  P.V = (float)V64;
+  P.V2 = (float)(V64 * V64);
+  P.m = (float)(V64 * rho0);
+  P.Kp = (float)(315.0 / (64.0 * PI * std::pow(h, 9)));
+  P.Ks = (float)(-45.0 / (PI * std::pow(h, 6)));
+  P.Kc = (float)(32.0 / (PI * std::pow(h, 9)));
+  P.bias = (float)(-std::pow(h, 6) / 64.0);
+  P.two_rho0 = (float)(2.0 * rho0);
+  P.eps_lambda = 1e-6f;
+  P.dt = dt;
+  P.inv_dt = (float)(1.0 / (double)dt);
+  P.gx = 0.0f;
+  P.gy = -c->cfg.gravity;
+  P.gz = 0.0f;
+  P.xsph_k = (c->cfg.xsph * P.inv_dt) * P.m;
+  P.vort = c->cfg.vorticity;
+  P.tension = c->cfg.surface_tension;
+  P.ld = c->ld;
+  P.maxn = c->maxn;
+  P.Kp64 = 315.0 / (64.0 * PI * std::pow(h, 9));
+  P.h2_64 = h * h;
+  P.V64 = V64;
+  return P;
+}
+
+void clear_graphs(pbf_ctx *c) {
+  for (auto &g : c->graphs) hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
+}
+
+LaunchCtx make_launch(pbf_ctx *c, float dt) {
+  LaunchCtx L;
+  L.stream = c->stream;
+  L.n = c->n;
+  L.multi = c->scenes.size() > 1;
+  L.hash = c->params.grid_mode == PBF_GRID_HASH;
+  L.field = c->any_field;
+  L.s0 = c->hscenes.empty() ? Scene{} : c->hscenes[0];
+  L.scenes = c->d_scenes;
+  L.sid = c->sid;
+  L.pool = c->pool;
+  L.P = make_phys(c, dt);
+  return L;
+}
+
+hipEvent_t take_event(pbf_ctx *c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+
+struct PassTimer {
+  pbf_ctx *c;
+  bool on;
+  int pass = -1;
+  hipEvent_t a = nullptr;
+  void begin(int p) {
+    if (!on) return;
+    pass = p;
+    a = take_event(c);
+    hipEventRecord(a, c->stream);
+  }
+  void end() {
+    if (!on || pass < 0) return;
+    hipEvent_t b = take_event(c);
+    hipEventRecord(b, c->stream);
+    c->pending.push_back({pass, a, b});
+    pass = -1;
+  }
+};
+
+void drain_events(pbf_ctx *c) {
+  for (auto &pe : c->pending) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, pe.a, pe.b) == hipSuccess) {
+      c->pass_ms[pe.pass] += ms;
+      c->pass_count[pe.pass] += 1;
+    }
+    c->event_pool.push_back(pe.a);
+    c->event_pool.push_back(pe.b);
+  }
+  c->pending.clear();
+}
+
+// Predict, bin, scan, reorder and neighbour lists.  dbg: keep cell counts.
+void enqueue_search(pbf_ctx *c, const LaunchCtx &L, PassTimer &T, bool dbg) {
+  T.begin(P_PREDICT);
+  launch_predict_bin(L, c->pos, c->vel, c->acc, c->XB, c->cellkey, c->rank, c->counts);
+  T.end();
+  if (dbg) hipMemcpyAsync(c->dbg_counts, c->counts, (size_t)c->ncells * 4, hipMemcpyDeviceToDevice, c->stream);
+  T.begin(P_SCAN);
+  launch_scan(c->stream, c->counts, c->ncells, c->offsets, 1, c->scan_sums.data(), c->scan_prefix.data(), 0);
+  T.end();
+  T.begin(P_SCATTER);
+  launch_scatter(L, c->cellkey, c->rank, c->offsets, c->ids, c->tmp_key, c->tmp_src);
+  T.end();
+  T.begin(P_SETTLE);
+  launch_settle(L, c->tmp_key, c->tmp_src, c->cellkey, c->offsets, c->XB, c->pos, c->lam, c->XA, c->Xold, c->ids,
+                c->lam_sorted);
+  T.end();
+  T.begin(P_LIST);
+  launch_build_list(L, c->XA, c->offsets, c->nbr, c->ncount, c->flags);
+  T.end();
+}
+
+// The launch sequence of one substep (PBF.ts:398-448).
+void enqueue_substep(pbf_ctx *c, const LaunchCtx &L, uint32_t iters, bool diag, PassTimer &T) {
+  enqueue_search(c, L, T, false);
+  float4 *X = c->XA, *Xn = c->XB;
+  for (uint32_t it = 0; it < iters; ++it) {
+    T.begin(P_LAMBDA);
+    launch_lambda(L, X, c->B, c->nbr, c->ncount);
+    T.end();
+    T.begin(P_DP);
+    launch_dp(L, X, c->B, c->nbr, c->ncount, Xn);
+    T.end();
+    std::swap(X, Xn);
+  }
+  T.begin(P_DENSITY);
+  launch_density(L, X, c->Xold, c->nbr, c->ncount, c->pos, c->vel, c->lam);
+  T.end();
+  T.begin(P_VORT);
+  launch_vorticity(L, c->pos, c->vel, c->nbr, c->ncount, c->om, c->nrm);
+  T.end();
+  T.begin(P_XSPH);
+  launch_xsph(L, c->pos, c->vel, c->om, c->nrm, c->nbr, c->ncount, c->acc);
+  T.end();
+  if (diag) {
+    T.begin(P_DIAG);
+    launch_diag(L, c->pos, c->nbr, c->ncount, c->ci, c->ranges, (uint32_t)c->scenes.size(), c->per_scene, c->hist,
+                c->hist_cap, c->hist_ctr);
+    T.end();
+  }
+}
+
+int run_substep(pbf_ctx *c, float dt, uint32_t iters) {
+  LaunchCtx L = make_launch(c, dt);
+  const bool timing = (c->params.flags & PBF_FLAG_TIMING) != 0;
+  const bool use_graph = !(c->params.flags & PBF_FLAG_NO_GRAPH) && !timing;
+  PassTimer T{c, timing};
+  c->last_was_search = false;
+  c->have_dbg_counts = false;
+  c->have_lists = true;
+  if (!use_graph) {
+    enqueue_substep(c, L, iters, c->record_diag, T);
+    HIPCHK(hipGetLastError());
+    return PBF_OK;
+  }
+  uint32_t bits;
+  std::memcpy(&bits, &dt, 4);
+  hipGraphExec_t exec = nullptr;
+  for (auto &g : c->graphs)
+    if (g.dt_bits == bits && g.iters == iters && g.diag == (uint32_t)c->record_diag) exec = g.exec;
+  if (!exec) {
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_substep(c, L, iters, c->record_diag, T);
+    hipError_t le = hipGetLastError();
+    hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+    if (le != hipSuccess) return fail(c, PBF_E_DEVICE, std::string("launch during capture: ") + hipGetErrorString(le));
+    if (ee != hipSuccess) return fail(c, PBF_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ie != hipSuccess) return fail(c, PBF_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    if (c->graphs.size() >= 16) clear_graphs(c);
+    c->graphs.push_back({bits, iters, (uint32_t)c->record_diag, exec});
+  }
+  HIPCHK(hipGraphLaunch(exec, c->stream));
+  return PBF_OK;
+}
+
+int check_flags(pbf_ctx *c) {
+  uint32_t f[2] = {0, 0};
+  HIPCHK(hipMemcpy(f, c->flags, 8, hipMemcpyDeviceToHost));
+  if (f[0]) {
+    HIPCHK(hipMemset(c->flags, 0, 4));
+    char buf[160];
+    std::snprintf(buf, sizeof(buf), "neighbour overflow: a particle had %u neighbours, max_neighbors is %u", f[1],
+                  c->maxn);
+    return fail(c, PBF_E_OVERFLOW, buf);
+  }
+  return PBF_OK;
+}
+
+int sync_ctx(pbf_ctx *c) {
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drain_events(c);
+  return check_flags(c);
+}
+
+void compute_grid(const pbf_params &p, SceneHost &s) {
+  for (int a = 0; a < 3; ++a) {
+    double L = s.hi[a] - s.lo[a];
+    int D = (int)std::ceil(L / p.kernel_radius - 1e-7);
+    s.dims[a] = D < 1 ? 1 : D;
+  }
+  if (p.grid_mode == PBF_GRID_HASH) s.ncells = 1u << p.hash_log2;
+  else s.ncells = (uint32_t)s.dims[0] * (uint32_t)s.dims[1] * (uint32_t)s.dims[2];
+}
+
+Scene to_device_scene(const pbf_ctx *c, const SceneHost &s, uint64_t field_off) {
+  Scene d;
+  std::memset(&d, 0, sizeof(d));
+  for (int a = 0; a < 3; ++a) {
+    d.lo[a] = (float)s.lo[a];
+    d.hi[a] = (float)s.hi[a];
+    d.dims[a] = s.dims[a];
+    d.fres[a] = s.fres[a];
+    d.flo[a] = (float)s.flo[a];
+    double L = s.fhi[a] - s.flo[a];
+    d.finv[a] = s.field_kind ? (float)(s.fres[a] / L) : 0.0f;
+    d.fscale[a] = s.field_kind == PBF_FIELD_CUBIC ? (float)(2.0 * s.fres[a] / L) : d.finv[a];
+  }
+  d.inv_cell = (float)(1.0 / c->params.kernel_radius);
+  d.cell_base = s.cell_base;
+  d.ncells = s.ncells;
+  d.hash_mask = c->params.grid_mode == PBF_GRID_HASH ? s.ncells - 1u : 0u;
+  d.pbase = s.pbase;
+  d.pcount = s.count;
+  d.field_kind = s.field_kind;
+  d.quirk = (c->params.flags & PBF_FLAG_CUBIC_FIXED) ? 0u : 1u;
+  d.field_off = field_off;
+  return d;
+}
+
+// Rebuild the device scene table and the boundary-field pool.
+int upload_scenes(pbf_ctx *c) {
+  size_t total = 0;
+  for (auto &s : c->scenes) total += s.field.size();
+  std::vector<uint64_t> offs;
+  c->hscenes.clear();
+  c->any_field = false;
+  uint64_t o = 0;
+  for (auto &s : c->scenes) {
+    offs.push_back(o);
+    o += s.field.size();
+    // keep float2 loads of trilinear nodes 8-byte aligned
+    if (o & 1) o += 1;
+    if (s.field_kind) c->any_field = true;
+  }
+  total = o;
+  if (total > c->pool_floats || !c->pool) {
+    if (c->pool) HIPCHK(hipFree(c->pool));
+    c->pool = nullptr;
+    HIPCHK(dalloc(&c->pool, total ? total : 2));
+    c->pool_floats = total;
+  }
+  for (size_t k = 0; k < c->scenes.size(); ++k) {
+    auto &s = c->scenes[k];
+    if (!s.field.empty())
+      HIPCHK(hipMemcpy(c->pool + offs[k], s.field.data(), s.field.size() * 4, hipMemcpyHostToDevice));
+    c->hscenes.push_back(to_device_scene(c, s, offs[k]));
+  }
+  HIPCHK(hipMemcpy(c->d_scenes, c->hscenes.data(), c->hscenes.size() * sizeof(Scene), hipMemcpyHostToDevice));
+  clear_graphs(c);
+  return PBF_OK;
+}
+
+int create_ctx(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc *descs, bool batch, pbf_ctx **out) {
+  pbf_ctx *c = nullptr;
+  if (!params || !out || nscenes == 0) return PBF_E_INVALID;
+  *out = nullptr;
+  c = new pbf_ctx();
+  c->params = *params;
+  pbf_params &p = c->params;
+  if (p.particle_radius <= 0.0) { int r = fail(c, PBF_E_INVALID, "particle_radius must be > 0"); delete c; return r; }
+  if (p.kernel_radius <= 0.0) p.kernel_radius = 4.0 * p.particle_radius;
+  if (p.rest_density <= 0.0) p.rest_density = 1000.0;
+  if (p.time_step <= 0.0) p.time_step = 1.0 / 300.0;
+  if (p.max_neighbors == 0) p.max_neighbors = 96;
+  if (p.grid_mode == PBF_GRID_HASH && (p.hash_log2 < 4 || p.hash_log2 > 28)) {
+    int r = fail(c, PBF_E_INVALID, "hash_log2 must be in [4, 28]"); delete c; return r;
+  }
+  c->cfg = {5u, 0.01f, 0.1f, 0.5f, 9.8f};  // src/common/config.ts:22-28
+  c->batch = batch;
+  uint32_t cell_base = 0, pbase = 0;
+  uint64_t total_cells = 0, total_particles = 0;
+  for (uint32_t k = 0; k < nscenes; ++k) {
+    SceneHost s;
+    for (int a = 0; a < 3; ++a) {
+      s.lo[a] = descs ? descs[k].domain_lo[a] : p.domain_lo[a];
+      s.hi[a] = descs ? descs[k].domain_hi[a] : p.domain_hi[a];
+      if (!(s.hi[a] > s.lo[a])) { int r = fail(c, PBF_E_INVALID, "scene domain must have hi > lo"); delete c; return r; }
+    }
+    s.count = descs ? descs[k].particle_count : 0;
+    compute_grid(p, s);
+    s.cell_base = cell_base;
+    s.pbase = pbase;
+    total_cells += s.ncells;
+    total_particles += s.count;
+    if (total_cells > 0x7fffffffull) { int r = fail(c, PBF_E_CAPACITY, "too many grid cells"); delete c; return r; }
+    cell_base += s.ncells;
+    pbase += s.count;
+    c->scenes.push_back(s);
+  }
+  if (batch && total_particles > p.capacity) {
+    int r = fail(c, PBF_E_CAPACITY, "To Many Particles!"); delete c; return r;
+  }
+  c->ncells = (uint32_t)total_cells;
+  c->ncap = ((p.capacity + 255u) / 256u) * 256u;
+  if (c->ncap == 0) c->ncap = 256;
+  c->ld = c->ncap;
+  c->maxn = p.max_neighbors;
+  hipError_t e = hipGetDevice(&c->device);
+  if (e != hipSuccess) { int r = fail(c, PBF_E_DEVICE, std::string("hipGetDevice: ") + hipGetErrorString(e)); delete c; return r; }
+  int rc = PBF_OK;
+  auto step = [&](hipError_t err, const char *what) {
+    if (rc == PBF_OK && err != hipSuccess) rc = fail(c, PBF_E_DEVICE, std::string(what) + ": " + hipGetErrorString(err));
+  };
+  step(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  const size_t N = c->ncap;
+  step(dalloc(&c->pos, N), "alloc"); step(dalloc(&c->vel, N), "alloc"); step(dalloc(&c->acc, N), "alloc");
+  step(dalloc(&c->om, N), "alloc"); step(dalloc(&c->nrm, N), "alloc"); step(dalloc(&c->XA, N), "alloc");
+  step(dalloc(&c->XB, N), "alloc"); step(dalloc(&c->Xold, N), "alloc"); step(dalloc(&c->B, N), "alloc");
+  step(dalloc(&c->stage4, N), "alloc");
+  step(dalloc(&c->ids, N), "alloc"); step(dalloc(&c->cellkey, N), "alloc"); step(dalloc(&c->rank, N), "alloc");
+  step(dalloc(&c->tmp_key, N), "alloc"); step(dalloc(&c->tmp_src, N), "alloc"); step(dalloc(&c->ncount, N), "alloc");
+  step(dalloc(&c->nbr, (size_t)c->maxn * c->ld), "alloc");
+  step(dalloc(&c->lam, N), "alloc"); step(dalloc(&c->lam_sorted, N), "alloc");
+  step(dalloc(&c->counts, (size_t)c->ncells), "alloc");
+  step(dalloc(&c->offsets, (size_t)c->ncells + 1), "alloc");
+  step(dalloc(&c->dbg_counts, (size_t)c->ncells), "alloc");
+  step(dalloc(&c->dbg_ids, N), "alloc"); step(dalloc(&c->ids_save, N), "alloc");
+  step(dalloc(&c->flags, 2), "alloc");
+  step(dalloc(&c->d_scenes, nscenes), "alloc");
+  step(dalloc(&c->sid, N), "alloc");
+  step(dalloc(&c->ranges, 2 * (size_t)nscenes), "alloc");
+  step(dalloc(&c->ci, N), "alloc");
+  step(dalloc(&c->per_scene, 2 * (size_t)nscenes), "alloc");
+  step(dalloc(&c->hist, 2 * (size_t)c->hist_cap), "alloc");
+  step(dalloc(&c->hist_ctr, 1), "alloc");
+  // scan workspace: one level per factor of 2048
+  {
+    uint32_t len = c->ncells;
+    while (rc == PBF_OK && len > 2048u) {
+      uint32_t nb = (len + 2047u) / 2048u;
+      uint32_t *s = nullptr, *q = nullptr;
+      step(dalloc(&s, nb), "alloc");
+      step(dalloc(&q, (size_t)nb + 1), "alloc");
+      c->scan_sums.push_back(s);
+      c->scan_prefix.push_back(q);
+      len = nb;
+    }
+    c->scan_sums.push_back(nullptr);
+    c->scan_prefix.push_back(nullptr);
+  }
+  if (rc == PBF_OK) {
+    step(hipMemset(c->counts, 0, (size_t)c->ncells * 4), "hipMemset");
+    step(hipMemset(c->offsets, 0, ((size_t)c->ncells + 1) * 4), "hipMemset");
+    step(hipMemset(c->flags, 0, 8), "hipMemset");
+    step(hipMemset(c->hist_ctr, 0, 4), "hipMemset");
+    step(hipMemset(c->ncount, 0, N * 4), "hipMemset");
+  }
+  if (rc == PBF_OK) rc = upload_scenes(c);
+  if (rc != PBF_OK) {
+    std::string msg = c->err;
+    pbf_destroy(c);
+    (void)msg;
+    return rc;
+  }
+  *out = c;
+  return PBF_OK;
+}
+
+int upload_positions(pbf_ctx *c, const float *xyzw, uint32_t count) {
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->batch) {
+    uint64_t want = 0;
+    for (auto &s : c->scenes) want += s.count;
+    if (want != count) return fail(c, PBF_E_INVALID, "batched context: particle count must equal the sum of scene counts");
+  } else {
+    c->scenes[0].count = count;
+    c->scenes[0].pbase = 0;
+  }
+  c->n = count;
+  const size_t N = count;
+  std::vector<float> zero(4 * (N ? N : 1), 0.0f);
+  std::vector<float> p4(4 * (N ? N : 1), 0.0f);
+  for (size_t i = 0; i < N; ++i) {
+    p4[4 * i] = xyzw[4 * i];
+    p4[4 * i + 1] = xyzw[4 * i + 1];
+    p4[4 * i + 2] = xyzw[4 * i + 2];
+    p4[4 * i + 3] = 0.0f;
+  }
+  if (N) {
+    HIPCHK(hipMemcpy(c->pos, p4.data(), N * 16, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->vel, zero.data(), N * 16, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->acc, zero.data(), N * 16, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->om, zero.data(), N * 16, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->nrm, zero.data(), N * 16, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->B, zero.data(), N * 16, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(c->lam, 0, N * 4));
+    HIPCHK(hipMemset(c->ncount, 0, N * 4));
+  }
+  std::vector<uint32_t> idv(N ? N : 1), sidv(N ? N : 1);
+  for (size_t i = 0; i < N; ++i) idv[i] = (uint32_t)i;
+  std::vector<uint32_t> rng;
+  for (size_t k = 0; k < c->scenes.size(); ++k) {
+    auto &s = c->scenes[k];
+    for (uint32_t q = 0; q < s.count; ++q) sidv[s.pbase + q] = (uint32_t)k;
+    rng.push_back(s.pbase);
+    rng.push_back(s.count);
+  }
+  if (N) {
+    HIPCHK(hipMemcpy(c->ids, idv.data(), N * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->sid, sidv.data(), N * 4, hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(c->ranges, rng.data(), rng.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(c->counts, 0, (size_t)c->ncells * 4));
+  c->have_lists = false;
+  c->last_was_search = false;
+  return upload_scenes(c);
+}
+
+int read_sorted4(pbf_ctx *c, const float4 *src, const uint32_t *ids, float *out, uint32_t count) {
+  if (count != c->n) return fail(c, PBF_E_LENGTH, "readback count must equal the particle count");
+  int rc = sync_ctx(c);
+  if (rc != PBF_OK) return rc;
+  launch_unsort4(c->stream, c->n, src, ids, c->stage4);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, c->stage4, (size_t)c->n * 16, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PBF_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+extern "C" {
+
+const char *pbf_version(void) { return "pbf-mi355x 1.0 (gfx950)"; }
+
+int pbf_create(const pbf_params *params, pbf_ctx **out) { return create_ctx(params, 1, nullptr, false, out); }
+
+int pbf_batch_create(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc *scenes, pbf_ctx **out) {
+  if (!scenes) return PBF_E_INVALID;
+  return create_ctx(params, nscenes, scenes, true, out);
+}
+
+int pbf_destroy(pbf_ctx *c) {
+  if (!c) return PBF_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  clear_graphs(c);
+  drain_events(c);
+  for (auto e : c->event_pool) hipEventDestroy(e);
+  void *bufs[] = {c->pos, c->vel, c->acc, c->om, c->nrm, c->XA, c->XB, c->Xold, c->B, c->xs, c->stage4, c->ids,
+                  c->cellkey, c->rank, c->tmp_key, c->tmp_src, c->ncount, c->nbr, c->lam, c->lam_sorted, c->counts,
+                  c->offsets, c->flags, c->d_scenes, c->sid, c->ranges, c->pool, c->ci, c->per_scene, c->hist,
+                  c->hist_ctr, c->dbg_counts, c->dbg_ids, c->ids_save};
+  for (void *b : bufs)
+    if (b) hipFree(b);
+  for (auto p : c->scan_sums)
+    if (p) hipFree(p);
+  for (auto p : c->scan_prefix)
+    if (p) hipFree(p);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return PBF_OK;
+}
+
+int pbf_reset(pbf_ctx *c) {
+  if (!c) return PBF_E_INVALID;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->staged.clear();
+  c->staged_serial = 0;
+  c->n = 0;
+  if (!c->batch) c->scenes[0].count = 0;
+  for (auto &s : c->scenes) {
+    s.field_kind = PBF_FIELD_NONE;
+    s.field.clear();
+  }
+  c->have_lists = false;
+  c->last_was_search = false;
+  HIPCHK(hipMemset(c->counts, 0, (size_t)c->ncells * 4));
+  HIPCHK(hipMemset(c->flags, 0, 8));
+  HIPCHK(hipMemset(c->hist_ctr, 0, 4));
+  return upload_scenes(c);
+}
+
+int pbf_set_config(pbf_ctx *c, const pbf_config *cfg) {
+  if (!c || !cfg) return PBF_E_INVALID;
+  c->cfg = *cfg;
+  clear_graphs(c);
+  return PBF_OK;
+}
+
+int pbf_voxelize_cube(pbf_ctx *c, const double mn[3], const double mx[3], double jitter, uint64_t seed,
+                      uint32_t *added) {
+  if (!c || !mn || !mx) return PBF_E_INVALID;
+  uint32_t cnt = 0;
+  int rc = pbf_voxelize_cube_host(c->params.particle_radius, mn, mx, jitter, seed, 0, nullptr, 0, &cnt);
+  if (rc != PBF_OK) return fail(c, rc, "voxelizeCube: invalid arguments");
+  size_t at = c->staged.size();
+  c->staged.resize(at + 4 * (size_t)cnt);
+  rc = pbf_voxelize_cube_host(c->params.particle_radius, mn, mx, jitter, seed, c->staged_serial, c->staged.data() + at,
+                              cnt, &cnt);
+  if (rc != PBF_OK) return fail(c, rc, "voxelizeCube failed");
+  c->staged_serial += cnt;
+  if (added) *added = cnt;
+  return PBF_OK;
+}
+
+int pbf_voxelize_sphere(pbf_ctx *c, const double center[3], double radius, double jitter, uint64_t seed,
+                        uint32_t *added) {
+  if (!c || !center) return PBF_E_INVALID;
+  uint32_t cnt = 0;
+  int rc = pbf_voxelize_sphere_host(c->params.particle_radius, center, radius, jitter, seed, 0, nullptr, 0, &cnt);
+  if (rc != PBF_OK) return fail(c, rc, "voxelizeSphere: invalid arguments");
+  size_t at = c->staged.size();
+  c->staged.resize(at + 4 * (size_t)cnt);
+  rc = pbf_voxelize_sphere_host(c->params.particle_radius, center, radius, jitter, seed, c->staged_serial,
+                                c->staged.data() + at, cnt, &cnt);
+  if (rc != PBF_OK) return fail(c, rc, "voxelizeSphere failed");
+  c->staged_serial += cnt;
+  if (added) *added = cnt;
+  return PBF_OK;
+}
+
+uint32_t pbf_staged_count(const pbf_ctx *c) { return c ? (uint32_t)(c->staged.size() / 4) : 0u; }
+
+int pbf_set_particle_position(pbf_ctx *c) {
+  if (!c) return PBF_E_INVALID;
+  uint32_t count = (uint32_t)(c->staged.size() / 4);
+  return pbf_set_positions(c, c->staged.data(), c->staged.size(), count);
+}
+
+int pbf_set_positions(pbf_ctx *c, const float *xyzw, size_t length, uint32_t count) {
+  if (!c) return PBF_E_INVALID;
+  if (length != (size_t)count * 4) return fail(c, PBF_E_LENGTH, "Illegal length of Particle Position Buffer!");
+  if (count > c->params.capacity) return fail(c, PBF_E_CAPACITY, "To Many Particles!");
+  if (count && !xyzw) return PBF_E_INVALID;
+  return upload_positions(c, xyzw, count);
+}
+
+int pbf_set_cubic_field(pbf_ctx *c, uint32_t scene, const int32_t res[3], const double lo[3], const double hi[3],
+                        const float *sdf, const float *vol, size_t length) {
+  if (!c || !res || !lo || !hi || !sdf || !vol) return PBF_E_INVALID;
+  if (scene >= c->scenes.size()) return fail(c, PBF_E_INVALID, "scene index out of range");
+  int r3[3] = {res[0], res[1], res[2]};
+  if (r3[0] < 1 || r3[1] < 1 || r3[2] < 1) return fail(c, PBF_E_FORMAT, "Invalid Boundary Model File!");
+  size_t fl = cubic_field_length(r3);
+  if (length != fl) return fail(c, PBF_E_FORMAT, "Invalid Boundary Model File!");
+  auto &s = c->scenes[scene];
+  s.field.assign(sdf, sdf + fl);
+  s.field.insert(s.field.end(), vol, vol + fl);
+  s.field_kind = PBF_FIELD_CUBIC;
+  for (int a = 0; a < 3; ++a) { s.fres[a] = res[a]; s.flo[a] = lo[a]; s.fhi[a] = hi[a]; }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return upload_scenes(c);
+}
+
+int pbf_set_boundary_data(pbf_ctx *c, const char *text, size_t length) {
+  if (!c || !text) return PBF_E_INVALID;
+  int res[3];
+  float *sdf = nullptr, *vol = nullptr;
+  size_t fl = 0;
+  if (parse_cdm(text, length, res, &sdf, &vol, &fl) != 0) return fail(c, PBF_E_FORMAT, "Invalid Boundary Model File!");
+  const double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {1.0, 1.0, 1.0};
+  const int32_t r3[3] = {res[0], res[1], res[2]};
+  int rc = pbf_set_cubic_field(c, 0, r3, lo, hi, sdf, vol, fl);
+  free(sdf);
+  free(vol);
+  return rc;
+}
+
+int pbf_set_volume_map(pbf_ctx *c, uint32_t scene, const int32_t res[3], const double lo[3], const double hi[3],
+                       const float *sdf_vol) {
+  if (!c || !res || !lo || !hi || !sdf_vol) return PBF_E_INVALID;
+  if (scene >= c->scenes.size()) return fail(c, PBF_E_INVALID, "scene index out of range");
+  if (res[0] < 1 || res[1] < 1 || res[2] < 1) return fail(c, PBF_E_INVALID, "volume map resolution must be >= 1");
+  size_t nn = (size_t)(res[0] + 1) * (res[1] + 1) * (res[2] + 1);
+  auto &s = c->scenes[scene];
+  s.field.assign(sdf_vol, sdf_vol + 2 * nn);
+  s.field_kind = PBF_FIELD_TRILINEAR;
+  for (int a = 0; a < 3; ++a) { s.fres[a] = res[a]; s.flo[a] = lo[a]; s.fhi[a] = hi[a]; }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return upload_scenes(c);
+}
+
+int pbf_clear_boundary(pbf_ctx *c) {
+  if (!c) return PBF_E_INVALID;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (auto &s : c->scenes) {
+    s.field_kind = PBF_FIELD_NONE;
+    s.field.clear();
+  }
+  return upload_scenes(c);
+}
+
+int pbf_step(pbf_ctx *c, float dt, uint32_t substeps, uint32_t iterations) {
+  if (!c) return PBF_E_INVALID;
+  if (!(dt > 0.0f) || substeps == 0) return fail(c, PBF_E_INVALID, "dt must be > 0 and substeps >= 1");
+  HIPCHK(hipSetDevice(c->device));
+  if (c->n == 0) return PBF_OK;
+  const uint32_t iters = iterations ? iterations : c->cfg.iteration;
+  const float sdt = (float)((double)dt / (double)substeps);
+  for (uint32_t s = 0; s < substeps; ++s) {
+    int rc = run_substep(c, sdt, iters);
+    if (rc != PBF_OK) return rc;
+  }
+  return PBF_OK;
+}
+
+int pbf_run(pbf_ctx *c) {
+  if (!c) return PBF_E_INVALID;
+  return pbf_step(c, (float)c->params.time_step, 1, c->cfg.iteration ? c->cfg.iteration : 1);
+}
+
+int pbf_sync(pbf_ctx *c) {
+  if (!c) return PBF_E_INVALID;
+  return sync_ctx(c);
+}
+
+int pbf_read_positions(pbf_ctx *c, float *xyzw, uint32_t count) {
+  if (!c || (!xyzw && count)) return PBF_E_INVALID;
+  return read_sorted4(c, c->pos, c->ids, xyzw, count);
+}
+
+int pbf_read_velocities(pbf_ctx *c, float *xyzw, uint32_t count) {
+  if (!c || (!xyzw && count)) return PBF_E_INVALID;
+  return read_sorted4(c, c->vel, c->ids, xyzw, count);
+}
+
+int pbf_read_field(pbf_ctx *c, uint32_t field, float *out4, uint32_t count) {
+  if (!c || (!out4 && count)) return PBF_E_INVALID;
+  const uint32_t *ids = c->last_was_search ? c->dbg_ids : c->ids;
+  switch (field) {
+    case PBF_FIELD_POSITION: return read_sorted4(c, c->pos, c->ids, out4, count);
+    case PBF_FIELD_VELOCITY: return read_sorted4(c, c->vel, c->ids, out4, count);
+    case PBF_FIELD_ACCEL: return read_sorted4(c, c->acc, c->ids, out4, count);
+    case PBF_FIELD_OMEGA: return read_sorted4(c, c->om, c->ids, out4, count);
+    case PBF_FIELD_NORMAL: return read_sorted4(c, c->nrm, c->ids, out4, count);
+    case PBF_FIELD_BOUNDARY: return read_sorted4(c, c->B, c->ids, out4, count);
+    case PBF_FIELD_PREDICTED: return read_sorted4(c, c->XA, ids, out4, count);
+    case PBF_FIELD_LAMBDA: {
+      if (count != c->n) return fail(c, PBF_E_LENGTH, "readback count must equal the particle count");
+      int rc = sync_ctx(c);
+      if (rc != PBF_OK) return rc;
+      launch_unsort1(c->stream, c->n, c->lam, c->ids, c->stage4);
+      HIPCHK(hipMemcpyAsync(out4, c->stage4, (size_t)c->n * 16, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      return PBF_OK;
+    }
+    default: return fail(c, PBF_E_INVALID, "unknown field");
+  }
+}
+
+uint32_t pbf_num_cells(const pbf_ctx *c) { return c ? c->ncells : 0u; }
+
+int pbf_search(pbf_ctx *c, float dt) {
+  if (!c) return PBF_E_INVALID;
+  if (!(dt > 0.0f)) return fail(c, PBF_E_INVALID, "dt must be > 0");
+  HIPCHK(hipSetDevice(c->device));
+  if (c->n == 0) return PBF_OK;
+  LaunchCtx L = make_launch(c, dt);
+  PassTimer T{c, false};
+  HIPCHK(hipMemcpyAsync(c->ids_save, c->ids, (size_t)c->n * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->lam_sorted, c->lam, (size_t)c->n * 4, hipMemcpyDeviceToDevice, c->stream));
+  enqueue_search(c, L, T, true);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->dbg_ids, c->ids, (size_t)c->n * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->ids, c->ids_save, (size_t)c->n * 4, hipMemcpyDeviceToDevice, c->stream));
+  c->last_was_search = true;
+  c->have_dbg_counts = true;
+  c->have_lists = true;
+  return sync_ctx(c);
+}
+
+int pbf_read_cells(pbf_ctx *c, uint32_t *counts, uint32_t *offsets) {
+  if (!c) return PBF_E_INVALID;
+  int rc = sync_ctx(c);
+  if (rc != PBF_OK) return rc;
+  std::vector<uint32_t> off((size_t)c->ncells + 1);
+  HIPCHK(hipMemcpy(off.data(), c->offsets, off.size() * 4, hipMemcpyDeviceToHost));
+  if (offsets) std::memcpy(offsets, off.data(), off.size() * 4);
+  if (counts) {
+    if (c->have_dbg_counts) {
+      HIPCHK(hipMemcpy(counts, c->dbg_counts, (size_t)c->ncells * 4, hipMemcpyDeviceToHost));
+    } else {
+      for (uint32_t k = 0; k < c->ncells; ++k) counts[k] = off[k + 1] - off[k];
+    }
+  }
+  return PBF_OK;
+}
+
+int pbf_read_neighbors(pbf_ctx *c, uint32_t *nbr_off, uint32_t *nbr_ids, uint64_t capacity, uint64_t *pairs) {
+  if (!c) return PBF_E_INVALID;
+  int rc = sync_ctx(c);
+  if (rc != PBF_OK) return rc;
+  if (!c->have_lists) return fail(c, PBF_E_STATE, "no neighbour lists yet (run pbf_step or pbf_search first)");
+  const uint32_t n = c->n;
+  std::vector<uint32_t> cnt(n ? n : 1), ids(n ? n : 1);
+  if (n) {
+    HIPCHK(hipMemcpy(cnt.data(), c->ncount, (size_t)n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ids.data(), c->last_was_search ? c->dbg_ids : c->ids, (size_t)n * 4, hipMemcpyDeviceToHost));
+  }
+  uint32_t mx = 0;
+  for (uint32_t i = 0; i < n; ++i) mx = std::max(mx, cnt[i]);
+  std::vector<uint32_t> table((size_t)mx * c->ld);
+  if (mx) HIPCHK(hipMemcpy(table.data(), c->nbr, table.size() * 4, hipMemcpyDeviceToHost));
+  std::vector<uint32_t> by_id(n ? n : 1), slot_of(n ? n : 1);
+  for (uint32_t i = 0; i < n; ++i) { by_id[ids[i]] = cnt[i]; slot_of[ids[i]] = i; }
+  uint64_t tot = 0;
+  for (uint32_t q = 0; q < n; ++q) {
+    if (nbr_off) nbr_off[q] = (uint32_t)tot;
+    tot += by_id[q];
+  }
+  if (nbr_off) nbr_off[n] = (uint32_t)tot;
+  if (pairs) *pairs = tot;
+  if (nbr_ids) {
+    if (capacity < tot) return fail(c, PBF_E_CAPACITY, "neighbour id buffer too small");
+    uint64_t w = 0;
+    for (uint32_t q = 0; q < n; ++q) {
+      uint32_t s = slot_of[q];
+      for (uint32_t k = 0; k < cnt[s]; ++k) nbr_ids[w++] = ids[table[(size_t)k * c->ld + s]];
+    }
+  }
+  return PBF_OK;
+}
+
+int pbf_density_error(pbf_ctx *c, uint32_t scene, double *mean, double *maxv) {
+  if (!c) return PBF_E_INVALID;
+  int rc = sync_ctx(c);
+  if (rc != PBF_OK) return rc;
+  if (!c->have_lists || c->last_was_search)
+    return fail(c, PBF_E_STATE, "no completed substep to evaluate (run pbf_step first)");
+  const uint32_t ns = (uint32_t)c->scenes.size();
+  LaunchCtx L = make_launch(c, (float)c->params.time_step);
+  launch_diag(L, c->pos, c->nbr, c->ncount, c->ci, c->ranges, ns, c->per_scene, nullptr, 0, nullptr);
+  HIPCHK(hipGetLastError());
+  std::vector<double> ps(2 * (size_t)ns);
+  HIPCHK(hipMemcpyAsync(ps.data(), c->per_scene, ps.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (scene < ns) {
+    uint32_t cnt = c->scenes[scene].count;
+    if (mean) *mean = cnt ? ps[2 * scene] / (double)cnt : 0.0;
+    if (maxv) *maxv = ps[2 * scene + 1];
+  } else {
+    double t = 0.0, m = 0.0;
+    for (uint32_t s = 0; s < ns; ++s) { t += ps[2 * s]; m = std::max(m, ps[2 * s + 1]); }
+    if (mean) *mean = c->n ? t / (double)c->n : 0.0;
+    if (maxv) *maxv = m;
+  }
+  return PBF_OK;
+}
+
+int pbf_record_density_error(pbf_ctx *c, uint32_t enable) {
+  if (!c) return PBF_E_INVALID;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->record_diag = enable != 0;
+  HIPCHK(hipMemset(c->hist_ctr, 0, 4));
+  return PBF_OK;
+}
+
+int pbf_read_density_history(pbf_ctx *c, double *mean_max, uint32_t capacity, uint32_t *count) {
+  if (!c) return PBF_E_INVALID;
+  int rc = sync_ctx(c);
+  if (rc != PBF_OK) return rc;
+  uint32_t ctr = 0;
+  HIPCHK(hipMemcpy(&ctr, c->hist_ctr, 4, hipMemcpyDeviceToHost));
+  uint32_t avail = std::min(ctr, c->hist_cap);
+  uint32_t take = std::min(avail, capacity);
+  std::vector<double> h(2 * (size_t)c->hist_cap);
+  HIPCHK(hipMemcpy(h.data(), c->hist, h.size() * 8, hipMemcpyDeviceToHost));
+  // oldest first
+  uint32_t start = ctr - avail;
+  for (uint32_t k = 0; k < take; ++k) {
+    uint32_t slot = (start + k) % c->hist_cap;
+    mean_max[2 * k] = h[2 * slot];
+    mean_max[2 * k + 1] = h[2 * slot + 1];
+  }
+  if (count) *count = take;
+  return PBF_OK;
+}
+
+uint32_t pbf_particle_count(const pbf_ctx *c) { return c ? c->n : 0u; }
+uint32_t pbf_scene_count(const pbf_ctx *c) { return c ? (uint32_t)c->scenes.size() : 0u; }
+const char *pbf_last_error(const pbf_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int pbf_pass_times(pbf_ctx *c, double *ms, uint32_t *launches, uint32_t capacity, uint32_t *count,
+                   const char **names) {
+  if (!c) return PBF_E_INVALID;
+  int rc = sync_ctx(c);
+  if (rc != PBF_OK) return rc;
+  uint32_t k = std::min<uint32_t>(capacity, P_NPASS);
+  for (uint32_t q = 0; q < k; ++q) {
+    if (ms) ms[q] = c->pass_ms[q];
+    if (launches) launches[q] = c->pass_count[q];
+    c->pass_ms[q] = 0.0;
+    c->pass_count[q] = 0;
+  }
+  if (count) *count = P_NPASS;
+  if (names) *names = kPassNames;
+  return PBF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,295 @@
+// pbf_host.cpp -- host-side helpers of libpbf: the seeded jitter PRNG, the
+// .cdm boundary-file parser and the analytic volume-map builder.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/pbf.h"
+
+namespace pbf {
+
+// SplitMix64 finaliser (Steele, Lea, Flood 2014).
+uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Values of one 32-node cubic field: corner nodes (R+1)^3, then the three
+// edge-node sets (src/simulator/boundary/volumeMap.ts:6-11).
+size_t cubic_field_length(const int r[3]) {
+  return (size_t)(r[0] + 1) * (r[1] + 1) * (r[2] + 1) + (size_t)(2 * r[0]) * (r[1] + 1) * (r[2] + 1) +
+         (size_t)(r[0] + 1) * (2 * r[1]) * (r[2] + 1) + (size_t)(r[0] + 1) * (r[1] + 1) * (2 * r[2]);
+}
+
+namespace {
+struct Line {
+  const char *b;
+  const char *e;
+};
+
+std::vector<Line> split_lines(const char *t, size_t n) {
+  std::vector<Line> out;
+  const char *s = t, *end = t + n;
+  while (s <= end) {
+    const char *q = s;
+    while (q < end && *q != '\n') ++q;
+    out.push_back({s, q});
+    if (q >= end) break;
+    s = q + 1;
+  }
+  return out;
+}
+
+// Parse every whitespace-separated number on a line (the reference splits on
+// ' ' and drops the trailing token, volumeMap.ts:28-29).
+bool parse_numbers(const Line &l, std::vector<float> &out) {
+  std::string s(l.b, l.e);
+  const char *p = s.c_str();
+  while (*p) {
+    while (*p == ' ' || *p == '\t' || *p == '\r') ++p;
+    if (!*p) break;
+    char *q = nullptr;
+    double v = std::strtod(p, &q);
+    if (q == p) return false;
+    out.push_back((float)v);
+    p = q;
+  }
+  return true;
+}
+}  // namespace
+
+// .cdm text: line 0 "Rx Ry Rz", line 1 count, line 2 SDF values, line 3
+// count, line 4 volume values (src/simulator/boundary/volumeMap.ts:23-39).
+int parse_cdm(const char *text, size_t len, int res[3], float **sdf, float **vol, size_t *flen) {
+  auto lines = split_lines(text, len);
+  if (lines.size() < 5) return -1;
+  std::vector<float> head, a, b;
+  if (!parse_numbers(lines[0], head) || head.size() != 3) return -1;
+  for (int k = 0; k < 3; ++k) {
+    if (!(head[k] >= 1.0f && head[k] <= 4096.0f) || head[k] != std::floor(head[k])) return -1;
+    res[k] = (int)head[k];
+  }
+  if (!parse_numbers(lines[2], a) || !parse_numbers(lines[4], b)) return -1;
+  size_t fl = cubic_field_length(res);
+  if (a.size() != fl || b.size() != fl) return -1;
+  *sdf = (float *)std::malloc(fl * 4);
+  *vol = (float *)std::malloc(fl * 4);
+  std::memcpy(*sdf, a.data(), fl * 4);
+  std::memcpy(*vol, b.data(), fl * 4);
+  *flen = fl;
+  return 0;
+}
+
+// Seeded jitter of particle `serial`, axis `axis`: uniform in [-amp, amp].
+double jitter_value(uint64_t seed, uint64_t serial, int axis, double amp) {
+  if (amp == 0.0) return 0.0;
+  uint64_t z = splitmix64(seed ^ (0x9E3779B97F4A7C15ull * (3ull * serial + (uint64_t)axis + 1ull)));
+  double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  return (2.0 * u - 1.0) * amp;
+}
+
+namespace {
+// Signed distance, positive in the fluid.
+struct Solid {
+  uint32_t kind;
+  double box_lo[3], box_hi[3];
+  double sc[3], sr;            // sphere
+  double tc[3], tR, tr;        // torus (axis y)
+  double sdf(const double *p) const {
+    // inside of the box: distance to the nearest wall; outside: -distance
+    double din = 1e300, q2 = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      double lo = p[a] - box_lo[a], hi = box_hi[a] - p[a];
+      din = std::fmin(din, std::fmin(lo, hi));
+      double o = std::fmax(std::fmax(-lo, -hi), 0.0);
+      q2 += o * o;
+    }
+    double d = din >= 0.0 ? din : -std::sqrt(q2 > 0.0 ? q2 : din * din);
+    if (kind == 1) {
+      double dx = p[0] - sc[0], dy = p[1] - sc[1], dz = p[2] - sc[2];
+      d = std::fmin(d, std::sqrt(dx * dx + dy * dy + dz * dz) - sr);
+      double ex = p[0] - tc[0], ey = p[1] - tc[1], ez = p[2] - tc[2];
+      double qx = std::sqrt(ex * ex + ez * ez) - tR;
+      d = std::fmin(d, std::sqrt(qx * qx + ey * ey) - tr);
+    }
+    return d;
+  }
+};
+}  // namespace
+
+}  // namespace pbf
+
+// Volume map of an analytic solid.  With particle_radius > 0 the stored
+// volume is the boundary volume V_b whose single mirrored sample reproduces
+// the density deficit of the solid part of the kernel ball:
+//   V_b(x) = 0.9 * int_{solid} W(x - y) dy / W(max(d + r/2, 2r)),
+// 0.9 being the fluid's packing (V = 0.9 (2r)^3, SPH.ts:49) and
+// max(d + r/2, 2r) the mirror distance of the sampler.  With
+// particle_radius == 0 it is the plain solid volume inside the ball.
+extern "C" int pbf_build_volume_map(uint32_t kind, const double *shape, const int32_t res[3], const double lo[3],
+                                    const double hi[3], double kernel_radius, double particle_radius,
+                                    float *sdf_vol) {
+  using namespace pbf;
+  if (!res || !lo || !hi || !sdf_vol || kind > 1 || !(kernel_radius > 0.0)) return PBF_E_INVALID;
+  if (res[0] < 1 || res[1] < 1 || res[2] < 1) return PBF_E_INVALID;
+  Solid S;
+  std::memset(&S, 0, sizeof(S));
+  S.kind = kind;
+  for (int a = 0; a < 3; ++a) { S.box_lo[a] = lo[a]; S.box_hi[a] = hi[a]; }
+  if (kind == 1) {
+    if (!shape) return PBF_E_INVALID;
+    for (int a = 0; a < 3; ++a) { S.sc[a] = shape[a]; S.tc[a] = shape[4 + a]; }
+    S.sr = shape[3];
+    S.tR = shape[7];
+    S.tr = shape[8];
+  } else if (shape) {
+    for (int a = 0; a < 3; ++a) { S.box_lo[a] = shape[a]; S.box_hi[a] = shape[3 + a]; }
+  }
+  const double h = kernel_radius;
+  // stratified quadrature of the solid volume inside the kernel ball
+  const int M = 16;
+  const double PI = 3.141592653589793;
+  const double kp = 315.0 / (64.0 * PI * std::pow(h, 9));
+  const double r = particle_radius;
+  std::vector<double> qp, qw;
+  for (int i = 0; i < M; ++i)
+    for (int j = 0; j < M; ++j)
+      for (int k = 0; k < M; ++k) {
+        double o[3] = {(-1.0 + (2.0 * i + 1.0) / M) * h, (-1.0 + (2.0 * j + 1.0) / M) * h,
+                       (-1.0 + (2.0 * k + 1.0) / M) * h};
+        double rr = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+        if (rr <= h * h) {
+          qp.push_back(o[0]); qp.push_back(o[1]); qp.push_back(o[2]);
+          double t = h * h - rr;
+          qw.push_back(kp * t * t * t);
+        }
+      }
+  const double cellv = std::pow(2.0 * h / M, 3);
+  const double ballv = 4.0 / 3.0 * 3.141592653589793 * h * h * h;
+  const size_t nx = (size_t)res[0] + 1, ny = (size_t)res[1] + 1, nz = (size_t)res[2] + 1;
+  const size_t total = nx * ny * nz;
+  unsigned nt = std::thread::hardware_concurrency();
+  if (nt == 0) nt = 1;
+  if (nt > 16) nt = 16;
+  auto work = [&](size_t b, size_t e) {
+    for (size_t q = b; q < e; ++q) {
+      size_t i = q % nx, j = (q / nx) % ny, k = q / (nx * ny);
+      double p[3] = {lo[0] + (hi[0] - lo[0]) * (double)i / res[0], lo[1] + (hi[1] - lo[1]) * (double)j / res[1],
+                     lo[2] + (hi[2] - lo[2]) * (double)k / res[2]};
+      double d = S.sdf(p);
+      double v;
+      if (d >= h) v = 0.0;
+      else {
+        double vin = 0.0, win = 0.0;
+        if (d <= -h) { vin = ballv; win = 1.0; }
+        else {
+          for (size_t t = 0, u = 0; t < qp.size(); t += 3, ++u) {
+            double y[3] = {p[0] + qp[t], p[1] + qp[t + 1], p[2] + qp[t + 2]};
+            if (S.sdf(y) < 0.0) { vin += cellv; win += qw[u] * cellv; }
+          }
+        }
+        if (r > 0.0) {
+          double s = std::fmax(d + 0.5 * r, 2.0 * r);
+          double t = h * h - s * s;
+          double ws = t > 0.0 ? kp * t * t * t : 0.0;
+          v = ws > 0.0 ? 0.9 * win / ws : 0.0;
+        } else {
+          v = vin;
+        }
+      }
+      sdf_vol[2 * q] = (float)d;
+      sdf_vol[2 * q + 1] = (float)v;
+    }
+  };
+  std::vector<std::thread> th;
+  size_t chunk = (total + nt - 1) / nt;
+  for (unsigned t = 0; t < nt; ++t) {
+    size_t b = t * chunk, e = std::min(total, b + chunk);
+    if (b < e) th.emplace_back(work, b, e);
+  }
+  for (auto &x : th) x.join();
+  return PBF_OK;
+}
+
+// SPH.voxelizeCube (SPH.ts:99-121): lattice of spacing 2r from `mn`, i (x)
+// outermost, k (z) innermost, f64 arithmetic rounded once to f32; plus the
+// seeded jitter of particle serial0 + q.  out == NULL: count only.
+extern "C" int pbf_voxelize_cube_host(double r, const double mn[3], const double mx[3], double jitter, uint64_t seed,
+                                      uint64_t serial0, float *out, uint64_t capacity, uint32_t *count) {
+  using namespace pbf;
+  if (!mn || !mx || !(r > 0.0)) return PBF_E_INVALID;
+  const double diam = 2.0 * r;
+  const long nx = (long)std::floor((mx[0] - mn[0]) / diam);
+  const long ny = (long)std::floor((mx[1] - mn[1]) / diam);
+  const long nz = (long)std::floor((mx[2] - mn[2]) / diam);
+  uint64_t total = (nx > 0 && ny > 0 && nz > 0) ? (uint64_t)nx * ny * nz : 0;
+  if (count) *count = (uint32_t)total;
+  if (!out) return PBF_OK;
+  if (total > capacity) return PBF_E_CAPACITY;
+  uint64_t q = 0;
+  for (long i = 0; i < nx; ++i)
+    for (long j = 0; j < ny; ++j)
+      for (long k = 0; k < nz; ++k, ++q) {
+        double p[3] = {(double)i * diam + mn[0], (double)j * diam + mn[1], (double)k * diam + mn[2]};
+        for (int a = 0; a < 3; ++a) out[4 * q + a] = (float)(p[a] + jitter_value(seed, serial0 + q, a, jitter));
+        out[4 * q + 3] = 0.0f;
+      }
+  return PBF_OK;
+}
+
+// SPH.voxelizeSphere (SPH.ts:123-149): lattice points of spacing 2r with
+// |p|^2 <= R^2 around `center`.
+extern "C" int pbf_voxelize_sphere_host(double r, const double center[3], double radius, double jitter, uint64_t seed,
+                                        uint64_t serial0, float *out, uint64_t capacity, uint32_t *count) {
+  using namespace pbf;
+  if (!center || !(r > 0.0)) return PBF_E_INVALID;
+  const double diam = 2.0 * r;
+  const long n = (long)std::floor(radius / r / 2.0);
+  const double r2 = radius * radius;
+  uint64_t q = 0;
+  for (long i = -n; i <= n; ++i)
+    for (long j = -n; j <= n; ++j)
+      for (long k = -n; k <= n; ++k) {
+        double p[3] = {(double)i * diam, (double)j * diam, (double)k * diam};
+        if ((p[0] * p[0] + p[1] * p[1]) + p[2] * p[2] <= r2) {
+          if (out) {
+            if (q >= capacity) return PBF_E_CAPACITY;
+            for (int a = 0; a < 3; ++a)
+              out[4 * q + a] = (float)(p[a] + center[a] + jitter_value(seed, serial0 + q, a, jitter));
+            out[4 * q + 3] = 0.0f;
+          }
+          ++q;
+        }
+      }
+  if (count) *count = (uint32_t)q;
+  return PBF_OK;
+}
+
+extern "C" int pbf_parse_cdm(const char *text, size_t length, int32_t res[3], float *sdf, float *vol,
+                             size_t capacity, size_t *field_length) {
+  using namespace pbf;
+  if (!text || !res) return PBF_E_INVALID;
+  int r3[3];
+  float *a = nullptr, *b = nullptr;
+  size_t fl = 0;
+  if (parse_cdm(text, length, r3, &a, &b, &fl) != 0) return PBF_E_FORMAT;
+  for (int k = 0; k < 3; ++k) res[k] = r3[k];
+  if (field_length) *field_length = fl;
+  int rc = PBF_OK;
+  if (sdf || vol) {
+    if (capacity < fl) rc = PBF_E_CAPACITY;
+    else {
+      if (sdf) std::memcpy(sdf, a, fl * 4);
+      if (vol) std::memcpy(vol, b, fl * 4);
+    }
+  }
+  std::free(a);
+  std::free(b);
+  return rc;
+}

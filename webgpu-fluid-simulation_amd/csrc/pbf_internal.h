@@ -1,0 +1,57 @@
+// pbf_internal.h -- launchers shared by the engine and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pbf_device.h"
+
+namespace pbf {
+
+// Everything a pass launch needs besides its buffers.
+struct LaunchCtx {
+  hipStream_t stream;
+  uint32_t n;
+  bool multi;   // more than one scene: per-slot scene lookup
+  bool hash;    // spatial-hash grid
+  bool field;   // at least one scene has a boundary field
+  Scene s0;     // the scene when !multi
+  const Scene *scenes;
+  const uint32_t *sid;
+  const float *pool;
+  Phys P;
+};
+
+void launch_predict_bin(const LaunchCtx &L, const float4 *pos, const float4 *vel, const float4 *acc, float4 *xs,
+                        uint32_t *cellkey, uint32_t *rank, uint32_t *counts);
+void launch_scan(hipStream_t st, uint32_t *in, uint32_t n, uint32_t *out, int zero_in, uint32_t *const *sums,
+                 uint32_t *const *prefix, int level);
+void launch_scatter(const LaunchCtx &L, const uint32_t *cellkey, const uint32_t *rank, const uint32_t *offsets,
+                    const uint32_t *ids, uint32_t *tmp_key, uint32_t *tmp_src);
+void launch_settle(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *tmp_src, const uint32_t *cellkey,
+                   const uint32_t *offsets, const float4 *xs, const float4 *pos, const float *lam_in, float4 *X0,
+                   float4 *Xold, uint32_t *ids, float *lam_out);
+void launch_build_list(const LaunchCtx &L, const float4 *X0, const uint32_t *offsets, uint32_t *nbr, uint32_t *ncount,
+                       uint32_t *flags);
+void launch_lambda(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr, const uint32_t *ncount);
+void launch_dp(const LaunchCtx &L, const float4 *X, const float4 *B, const uint32_t *nbr, const uint32_t *ncount,
+               float4 *Xn);
+void launch_density(const LaunchCtx &L, const float4 *X, const float4 *Xold, const uint32_t *nbr,
+                    const uint32_t *ncount, float4 *pos, float4 *vel, float *lam);
+void launch_vorticity(const LaunchCtx &L, const float4 *pos, const float4 *vel, const uint32_t *nbr,
+                      const uint32_t *ncount, float4 *om, float4 *nrm);
+void launch_xsph(const LaunchCtx &L, const float4 *pos, const float4 *vel, const float4 *om, const float4 *nrm,
+                 const uint32_t *nbr, const uint32_t *ncount, float4 *acc);
+void launch_diag(const LaunchCtx &L, const float4 *pos, const uint32_t *nbr, const uint32_t *ncount, double *ci,
+                 const uint32_t *ranges, uint32_t nscenes, double *per_scene, double *hist, uint32_t hist_cap,
+                 uint32_t *hist_ctr);
+void launch_unsort4(hipStream_t st, uint32_t n, const float4 *src, const uint32_t *ids, float4 *dst);
+void launch_unsort1(hipStream_t st, uint32_t n, const float *src, const uint32_t *ids, float4 *dst);
+void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v);
+void launch_iota(hipStream_t st, uint32_t *p, uint32_t n);
+
+// Host utilities (pbf_host.cpp).
+uint64_t splitmix64(uint64_t x);
+int parse_cdm(const char *text, size_t len, int res[3], float **sdf, float **vol, size_t *flen);
+size_t cubic_field_length(const int res[3]);
+
+}  // namespace pbf

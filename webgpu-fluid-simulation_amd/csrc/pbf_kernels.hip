@@ -1,0 +1,710 @@
+// pbf_kernels.hip -- the PBF substep as hand-written CDNA4 kernels.
+//
+// Data lives in structure-of-float4 arrays that are physically re-sorted into
+// cell order every substep, so every neighbour gather hits a short run of
+// nearby slots (L2-resident per XCD, see xcd_block).  Pass order of one
+// substep (reference order: src/simulator/PBF/PBF.ts:398-448):
+//
+//   predict_bin   x* = clamp(x + dt (v + dt (g + a))), cell key, wave-
+//                 aggregated atomic rank                     (K1 + K2)
+//   scan          counts -> offsets, LDS Blelloch, 3 kernels (K3)
+//   scatter       (id, source slot) to the cell-sorted slot  (K4)
+//   settle        stable order inside each cell (ascending id), move data
+//   build_list    27-cell walk on frozen x*, slot-major list (K5 + K6 + K7)
+//   lambda        boundary sample + density constraint + lambda (K8 + K9)
+//   dp            position correction + clamp (K10 + K11)    x I
+//   density       density + velocity (K12)
+//   vorticity     omega and surface normal (K13)
+//   xsph          XSPH + vorticity force + surface tension (K14)
+//   diag          FP64 density-constraint error (optional)
+#include <hip/hip_runtime.h>
+
+#include "pbf_device.h"
+#include "pbf_internal.h"
+
+namespace pbf {
+
+constexpr int BS = 256;
+
+template <bool MULTI>
+__device__ __forceinline__ const Scene &scene_at(const Scene &s0, const Scene *__restrict__ scenes,
+                                                 const uint32_t *__restrict__ sid, uint32_t i) {
+  if constexpr (MULTI) return scenes[sid[i]];
+  else return s0;
+}
+
+__device__ __forceinline__ uint32_t slot_index() {
+  return xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t w = (uint32_t)__shfl_xor((int)v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+// Rank of this lane's key among equal keys, one global atomic per distinct
+// key per wavefront (keys of cell-sorted particles repeat within a wave).
+__device__ __forceinline__ uint32_t wave_atomic_rank(uint32_t *counts, uint32_t key, bool active) {
+  const uint32_t lane = __lane_id();
+  unsigned long long pending = __ballot(active);
+  const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  uint32_t rank = 0;
+  while (pending) {
+    int leader = __ffsll((long long)pending) - 1;
+    uint32_t lkey = (uint32_t)__shfl((int)key, leader, 64);
+    bool mine = active && key == lkey;
+    unsigned long long same = __ballot(mine) & pending;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(&counts[lkey], (uint32_t)__popcll(same));
+    base = (uint32_t)__shfl((int)base, leader, 64);
+    if (mine) rank = base + (uint32_t)__popcll(same & lt);
+    pending &= ~same;
+  }
+  return rank;
+}
+
+// ---------------------------------------------------------------------------
+// K1 + K2: predict and bin.
+template <bool MULTI>
+__global__ void __launch_bounds__(BS) k_predict_bin(uint32_t n, const float4 *__restrict__ pos,
+                                                    const float4 *__restrict__ vel, const float4 *__restrict__ acc,
+                                                    float4 *__restrict__ xs, uint32_t *__restrict__ cellkey,
+                                                    uint32_t *__restrict__ rank, uint32_t *__restrict__ counts,
+                                                    Scene s0, const Scene *__restrict__ scenes,
+                                                    const uint32_t *__restrict__ sid, Phys P) {
+  const uint32_t i = slot_index();
+  const bool active = i < n;
+  uint32_t key = 0;
+  if (active) {
+    const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
+    float4 x = pos[i], v = vel[i], a = acc[i];
+    float vx = v.x + P.dt * (P.gx + a.x);
+    float vy = v.y + P.dt * (P.gy + a.y);
+    float vz = v.z + P.dt * (P.gz + a.z);
+    float px = x.x + P.dt * vx, py = x.y + P.dt * vy, pz = x.z + P.dt * vz;
+    clamp_rand(sc, px, py, pz);
+    xs[i] = make_float4(px, py, pz, 0.0f);
+    key = sc.cell_base + cell_local(sc, cell_axis(sc, px, 0), cell_axis(sc, py, 1), cell_axis(sc, pz, 2));
+    cellkey[i] = key;
+  }
+  uint32_t r = wave_atomic_rank(counts, key, active);
+  if (active) rank[i] = r;
+}
+
+// ---------------------------------------------------------------------------
+// K3: exclusive scan, reduce-then-scan with a Blelloch tree in LDS.
+constexpr int SCAN_V = 8;
+constexpr uint32_t SCAN_E = BS * SCAN_V;  // 2048 elements per block
+__device__ __forceinline__ int pad32(int i) { return i + (i >> 5); }
+
+__global__ void __launch_bounds__(BS) k_scan_reduce(const uint32_t *__restrict__ in, uint32_t n,
+                                                    uint32_t *__restrict__ sums) {
+  __shared__ uint32_t sh[BS / 64];
+  const uint32_t base = blockIdx.x * SCAN_E + threadIdx.x * SCAN_V;
+  uint32_t s = 0;
+  if (base + SCAN_V <= n) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(in + base);
+    uint4 a = p[0], b = p[1];
+    s = ((a.x + a.y) + (a.z + a.w)) + ((b.x + b.y) + (b.z + b.w));
+  } else {
+    for (int k = 0; k < SCAN_V; ++k)
+      if (base + k < n) s += in[base + k];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += (uint32_t)__shfl_xor((int)s, o, 64);
+  if (__lane_id() == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < BS / 64; ++w) t += sh[w];
+    sums[blockIdx.x] = t;
+  }
+}
+
+// out[0..n] = exclusive scan of in[0..n) (+ prefix[block]); out[n] = total.
+__global__ void __launch_bounds__(BS) k_scan_down(uint32_t *__restrict__ in, uint32_t n,
+                                                  const uint32_t *__restrict__ prefix, uint32_t *__restrict__ out,
+                                                  int zero_in) {
+  __shared__ uint32_t sh[BS + BS / 32];
+  const int t = threadIdx.x;
+  const uint32_t base = blockIdx.x * SCAN_E + (uint32_t)t * SCAN_V;
+  uint32_t v[SCAN_V];
+  if (base + SCAN_V <= n) {
+    uint4 *p = reinterpret_cast<uint4 *>(in + base);
+    uint4 a = p[0], b = p[1];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    if (zero_in) { p[0] = make_uint4(0, 0, 0, 0); p[1] = make_uint4(0, 0, 0, 0); }
+  } else {
+#pragma unroll
+    for (int k = 0; k < SCAN_V; ++k) {
+      v[k] = (base + k < n) ? in[base + k] : 0u;
+      if (zero_in && base + k < n) in[base + k] = 0u;
+    }
+  }
+  uint32_t e[SCAN_V];
+  e[0] = 0;
+#pragma unroll
+  for (int k = 1; k < SCAN_V; ++k) e[k] = e[k - 1] + v[k - 1];
+  const uint32_t tot = e[SCAN_V - 1] + v[SCAN_V - 1];
+  sh[pad32(t)] = tot;
+  // Blelloch up-sweep
+  int offset = 1;
+  for (int d = BS >> 1; d > 0; d >>= 1) {
+    __syncthreads();
+    if (t < d) {
+      int ai = offset * (2 * t + 1) - 1, bi = offset * (2 * t + 2) - 1;
+      sh[pad32(bi)] += sh[pad32(ai)];
+    }
+    offset <<= 1;
+  }
+  if (t == 0) sh[pad32(BS - 1)] = 0;
+  // down-sweep
+  for (int d = 1; d < BS; d <<= 1) {
+    offset >>= 1;
+    __syncthreads();
+    if (t < d) {
+      int ai = offset * (2 * t + 1) - 1, bi = offset * (2 * t + 2) - 1;
+      uint32_t tmp = sh[pad32(ai)];
+      sh[pad32(ai)] = sh[pad32(bi)];
+      sh[pad32(bi)] += tmp;
+    }
+  }
+  __syncthreads();
+  const uint32_t tp = sh[pad32(t)] + (prefix ? prefix[blockIdx.x] : 0u);
+  if (base + SCAN_V <= n) {
+    uint4 *o = reinterpret_cast<uint4 *>(out + base);
+    o[0] = make_uint4(tp + e[0], tp + e[1], tp + e[2], tp + e[3]);
+    o[1] = make_uint4(tp + e[4], tp + e[5], tp + e[6], tp + e[7]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < SCAN_V; ++k)
+      if (base + k < n) out[base + k] = tp + e[k];
+  }
+  if (n > 0 && base <= n - 1 && n - 1 < base + SCAN_V) {
+    uint32_t k = n - 1 - base;
+    uint32_t last = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_V; ++q)
+      if ((uint32_t)q == k) last = tp + e[q] + v[q];
+    out[n] = last;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K4: scatter ids into cell order (rank inside a cell is still arbitrary).
+__global__ void __launch_bounds__(BS) k_scatter(uint32_t n, const uint32_t *__restrict__ cellkey,
+                                                const uint32_t *__restrict__ rank, const uint32_t *__restrict__ offsets,
+                                                const uint32_t *__restrict__ ids, uint32_t *__restrict__ tmp_key,
+                                                uint32_t *__restrict__ tmp_src) {
+  const uint32_t i = slot_index();
+  if (i >= n) return;
+  uint32_t s = offsets[cellkey[i]] + rank[i];
+  tmp_key[s] = ids[i];
+  tmp_src[s] = i;
+}
+
+// Stable order inside each cell (ascending particle id), then move the
+// particle data into its final slot.
+__global__ void __launch_bounds__(BS) k_settle(uint32_t n, const uint32_t *__restrict__ tmp_key,
+                                               const uint32_t *__restrict__ tmp_src,
+                                               const uint32_t *__restrict__ cellkey,
+                                               const uint32_t *__restrict__ offsets, const float4 *__restrict__ xs,
+                                               const float4 *__restrict__ pos, const float *__restrict__ lam_in,
+                                               float4 *__restrict__ X0, float4 *__restrict__ Xold,
+                                               uint32_t *__restrict__ ids, float *__restrict__ lam_out) {
+  const uint32_t s = slot_index();
+  if (s >= n) return;
+  const uint32_t p = tmp_src[s], key = tmp_key[s];
+  const uint32_t c = cellkey[p];
+  const uint32_t o = offsets[c], e = offsets[c + 1];
+  uint32_t r = 0;
+  for (uint32_t t = o; t < e; ++t) r += tmp_key[t] < key ? 1u : 0u;
+  const uint32_t f = o + r;
+  X0[f] = xs[p];
+  Xold[f] = pos[p];
+  ids[f] = key;
+  if (lam_out) lam_out[f] = lam_in[p];
+}
+
+// ---------------------------------------------------------------------------
+// K5-K7: neighbour list over the 27 cells of the frozen predicted position.
+// Dense grid: the three cells (x, y, z-1..z+1) are one contiguous slot range.
+template <bool MULTI, bool HASH>
+__global__ void __launch_bounds__(BS) k_build_list(uint32_t n, const float4 *__restrict__ X0,
+                                                   const uint32_t *__restrict__ offsets, uint32_t *__restrict__ nbr,
+                                                   uint32_t *__restrict__ ncount, uint32_t *__restrict__ flags,
+                                                   Scene s0, const Scene *__restrict__ scenes,
+                                                   const uint32_t *__restrict__ sid, Phys P) {
+  const uint32_t i = slot_index();
+  uint32_t cnt = 0;
+  if (i < n) {
+    const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
+    const float4 xi = X0[i];
+    const int cx = cell_axis(sc, xi.x, 0), cy = cell_axis(sc, xi.y, 1), cz = cell_axis(sc, xi.z, 2);
+    const int D0 = sc.dims[0], D1 = sc.dims[1], D2 = sc.dims[2];
+    const uint32_t base = sc.cell_base;
+    uint32_t *row = nbr + i;
+    auto visit = [&](uint32_t t0, uint32_t t1) {
+      for (uint32_t j = t0; j < t1; ++j) {
+        const float4 xj = X0[j];
+        float ex = xj.x - xi.x, ey = xj.y - xi.y, ez = xj.z - xi.z;
+        float d2 = (ex * ex + ey * ey) + ez * ez;
+        if (d2 <= P.h2) {
+          if (cnt < P.maxn) row[(size_t)cnt * P.ld] = j;
+          ++cnt;
+        }
+      }
+    };
+    if constexpr (!HASH) {
+      const int zlo = cz > 0 ? cz - 1 : 0, zhi = cz + 1 < D2 ? cz + 1 : D2 - 1;
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int nx = cx + dx;
+        if (nx < 0 || nx >= D0) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+          const int ny = cy + dy;
+          if (ny < 0 || ny >= D1) continue;
+          uint32_t c0 = base + cell_local(sc, nx, ny, zlo);
+          uint32_t c1 = base + cell_local(sc, nx, ny, zhi);
+          visit(offsets[c0], offsets[c1 + 1]);
+        }
+      }
+    } else {
+      uint32_t seen[27];
+      int nseen = 0;
+      for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dz = -1; dz <= 1; ++dz) {
+            const int nx = cx + dx, ny = cy + dy, nz = cz + dz;
+            if (nx < 0 || ny < 0 || nz < 0 || nx >= D0 || ny >= D1 || nz >= D2) continue;
+            const uint32_t k = base + cell_local(sc, nx, ny, nz);
+            bool dup = false;
+            for (int q = 0; q < nseen; ++q) dup |= (seen[q] == k);
+            if (dup) continue;
+            seen[nseen++] = k;
+            visit(offsets[k], offsets[k + 1]);
+          }
+    }
+    ncount[i] = cnt < P.maxn ? cnt : P.maxn;
+  }
+  uint32_t wm = wave_max_u32(cnt);
+  if (__lane_id() == 0 && wm > 0) {
+    atomicMax(&flags[1], wm);
+    if (wm > P.maxn) atomicOr(&flags[0], 1u);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K8 + K9: boundary sample and lambda.  lambda goes to X[i].w.
+template <bool MULTI, bool FIELD>
+__global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__restrict__ B,
+                                               const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ ncount,
+                                               const float *__restrict__ pool, Scene s0,
+                                               const Scene *__restrict__ scenes, const uint32_t *__restrict__ sid,
+                                               Phys P) {
+  const uint32_t i = slot_index();
+  if (i >= n) return;
+  const float4 xi = X[i];
+  float4 b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if constexpr (FIELD) {
+    const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
+    b = boundary_sample(P, sc, pool, xi.x, xi.y, xi.z);
+    B[i] = b;
+  }
+  const uint32_t cnt = ncount[i];
+  const uint32_t *row = nbr + i;
+  float gx = 0.0f, gy = 0.0f, gz = 0.0f, s2 = 0.0f, rho = 0.0f;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    const uint32_t j = row[(size_t)k * P.ld];
+    const float4 xj = X[j];
+    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    float d2 = (dx * dx + dy * dy) + dz * dz;
+    float f = spiky_f(P, d2);
+    float ax = f * dx, ay = f * dy, az = f * dz;
+    gx += ax; gy += ay; gz += az;
+    s2 += (ax * ax + ay * ay) + az * az;
+    rho += poly6(P, d2);
+  }
+  rho *= P.V;
+  gx *= P.V; gy *= P.V; gz *= P.V;
+  s2 *= P.V2;
+  if (FIELD && b.w > 0.0f) {
+    float dx = xi.x - b.x, dy = xi.y - b.y, dz = xi.z - b.z;
+    float d2 = (dx * dx + dy * dy) + dz * dz;
+    float f = spiky_f(P, d2);
+    gx += b.w * (f * dx); gy += b.w * (f * dy); gz += b.w * (f * dz);
+    rho += b.w * poly6(P, d2);
+  }
+  float c = rho - 1.0f;
+  float C = c > 0.0f ? c : 0.0f;
+  float den = ((gx * gx + gy * gy) + gz * gz) + s2;
+  reinterpret_cast<float *>(X)[4 * (size_t)i + 3] = (-C) / (den + P.eps_lambda);
+}
+
+// K10 + K11: position correction and wall clamp.
+template <bool MULTI, bool FIELD>
+__global__ void __launch_bounds__(BS) k_dp(uint32_t n, const float4 *__restrict__ X, const float4 *__restrict__ B,
+                                           const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ ncount,
+                                           float4 *__restrict__ Xn, Scene s0, const Scene *__restrict__ scenes,
+                                           const uint32_t *__restrict__ sid, Phys P) {
+  const uint32_t i = slot_index();
+  if (i >= n) return;
+  const float4 xi = X[i];
+  const float li = xi.w;
+  const uint32_t cnt = ncount[i];
+  const uint32_t *row = nbr + i;
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    const uint32_t j = row[(size_t)k * P.ld];
+    const float4 xj = X[j];
+    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    float d2 = (dx * dx + dy * dy) + dz * dz;
+    float f = spiky_f(P, d2);
+    float cl = li + xj.w;
+    ax += cl * (f * dx); ay += cl * (f * dy); az += cl * (f * dz);
+  }
+  ax *= P.V; ay *= P.V; az *= P.V;
+  if constexpr (FIELD) {
+    const float4 b = B[i];
+    if (b.w > 0.0f) {
+      float dx = xi.x - b.x, dy = xi.y - b.y, dz = xi.z - b.z;
+      float d2 = (dx * dx + dy * dy) + dz * dz;
+      float f = spiky_f(P, d2);
+      float cb = b.w * li;
+      ax += cb * (f * dx); ay += cb * (f * dy); az += cb * (f * dz);
+    }
+  }
+  const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
+  float px = xi.x + 0.5f * ax, py = xi.y + 0.5f * ay, pz = xi.z + 0.5f * az;
+  clamp_rand(sc, px, py, pz);
+  Xn[i] = make_float4(px, py, pz, li);
+}
+
+// ---------------------------------------------------------------------------
+// K12: density and velocity; the state arrays now hold the new order.
+__global__ void __launch_bounds__(BS) k_density(uint32_t n, const float4 *__restrict__ X,
+                                                const float4 *__restrict__ Xold, const uint32_t *__restrict__ nbr,
+                                                const uint32_t *__restrict__ ncount, float4 *__restrict__ pos,
+                                                float4 *__restrict__ vel, float *__restrict__ lam, Phys P) {
+  const uint32_t i = slot_index();
+  if (i >= n) return;
+  const float4 xi = X[i];
+  const uint32_t cnt = ncount[i];
+  const uint32_t *row = nbr + i;
+  float rho = 0.0f;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    const float4 xj = X[row[(size_t)k * P.ld]];
+    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    float d2 = (dx * dx + dy * dy) + dz * dz;
+    rho += poly6(P, d2);
+  }
+  rho *= P.m;
+  const float4 xo = Xold[i];
+  vel[i] = make_float4((xi.x - xo.x) * P.inv_dt, (xi.y - xo.y) * P.inv_dt, (xi.z - xo.z) * P.inv_dt, 0.0f);
+  pos[i] = make_float4(xi.x, xi.y, xi.z, rho);
+  lam[i] = xi.w;
+}
+
+// K13: vorticity and the Akinci surface normal.
+__global__ void __launch_bounds__(BS) k_vorticity(uint32_t n, const float4 *__restrict__ pos,
+                                                  const float4 *__restrict__ vel, const uint32_t *__restrict__ nbr,
+                                                  const uint32_t *__restrict__ ncount, float4 *__restrict__ om,
+                                                  float4 *__restrict__ nrm, Phys P) {
+  const uint32_t i = slot_index();
+  if (i >= n) return;
+  const float4 xi = pos[i], vi = vel[i];
+  const uint32_t cnt = ncount[i];
+  const uint32_t *row = nbr + i;
+  float wx = 0.0f, wy = 0.0f, wz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    const uint32_t j = row[(size_t)k * P.ld];
+    const float4 xj = pos[j], vj = vel[j];
+    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    float d2 = (dx * dx + dy * dy) + dz * dz;
+    float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
+    float vol = P.m / xj.w;
+    float f = spiky_f(P, d2);
+    float gx = f * dx, gy = f * dy, gz = f * dz;
+    float cx = uy * gz - uz * gy, cy = uz * gx - ux * gz, cz = ux * gy - uy * gx;
+    wx += vol * cx; wy += vol * cy; wz += vol * cz;
+    nx += vol * gx; ny += vol * gy; nz += vol * gz;
+  }
+  om[i] = make_float4(wx, wy, wz, sqrtf((wx * wx + wy * wy) + wz * wz));
+  nrm[i] = make_float4(P.h * nx, P.h * ny, P.h * nz, 0.0f);
+}
+
+// K14: XSPH viscosity, vorticity confinement force and surface tension.
+__global__ void __launch_bounds__(BS) k_xsph(uint32_t n, const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+                                             const float4 *__restrict__ om, const float4 *__restrict__ nrm,
+                                             const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ ncount,
+                                             float4 *__restrict__ acc, Phys P) {
+  const uint32_t i = slot_index();
+  if (i >= n) return;
+  const float4 xi = pos[i], vi = vel[i], ni = nrm[i], wi = om[i];
+  const float rhoi = xi.w;
+  const uint32_t cnt = ncount[i];
+  const uint32_t *row = nbr + i;
+  float ux = 0.0f, uy = 0.0f, uz = 0.0f, ex = 0.0f, ey = 0.0f, ez = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    const uint32_t j = row[(size_t)k * P.ld];
+    const float4 xj = pos[j], vj = vel[j], nj = nrm[j];
+    const float wj = om[j].w;
+    const float rhoj = xj.w;
+    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    float d2 = (dx * dx + dy * dy) + dz * dz;
+    float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
+    float K = P.two_rho0 / (rhoi + rhoj);
+    float W = poly6(P, d2);
+    ux += (vx * W) / rhoj; uy += (vy * W) / rhoj; uz += (vz * W) / rhoj;
+    float f = spiky_f(P, d2);
+    ex += (wj * (f * dx)) / rhoj; ey += (wj * (f * dy)) / rhoj; ez += (wj * (f * dz)) / rhoj;
+    float fc = cohesion_f(P, d2);
+    tx -= K * (P.m * (fc * dx) + (ni.x - nj.x));
+    ty -= K * (P.m * (fc * dy) + (ni.y - nj.y));
+    tz -= K * (P.m * (fc * dz) + (ni.z - nj.z));
+  }
+  float el = sqrtf((ex * ex + ey * ey) + ez * ez);
+  float hx = 0.0f, hy = 0.0f, hz = 0.0f;
+  if (el > 0.0f) { hx = ex / el; hy = ey / el; hz = ez / el; }
+  float cx = hy * wi.z - hz * wi.y, cy = hz * wi.x - hx * wi.z, cz = hx * wi.y - hy * wi.x;
+  acc[i] = make_float4((P.xsph_k * ux + P.vort * cx) + P.tension * tx, (P.xsph_k * uy + P.vort * cy) + P.tension * ty,
+                       (P.xsph_k * uz + P.vort * cz) + P.tension * tz, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 density-constraint error per particle, then one block per scene.
+template <bool MULTI, bool FIELD>
+__global__ void __launch_bounds__(BS) k_diag(uint32_t n, const float4 *__restrict__ pos,
+                                             const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ ncount,
+                                             const float *__restrict__ pool, Scene s0,
+                                             const Scene *__restrict__ scenes, const uint32_t *__restrict__ sid,
+                                             Phys P, double *__restrict__ ci) {
+  const uint32_t i = slot_index();
+  if (i >= n) return;
+  const float4 xi = pos[i];
+  const uint32_t cnt = ncount[i];
+  const uint32_t *row = nbr + i;
+  double rho = 0.0;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    const float4 xj = pos[row[(size_t)k * P.ld]];
+    double dx = (double)xi.x - (double)xj.x, dy = (double)xi.y - (double)xj.y, dz = (double)xi.z - (double)xj.z;
+    double d2 = (dx * dx + dy * dy) + dz * dz;
+    if (d2 <= P.h2_64) {
+      double t = P.h2_64 - d2;
+      rho += P.Kp64 * ((t * t) * t);
+    }
+  }
+  rho *= P.V64;
+  if constexpr (FIELD) {
+    const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
+    float4 b = boundary_sample(P, sc, pool, xi.x, xi.y, xi.z);
+    if (b.w > 0.0f) {
+      double dx = (double)xi.x - (double)b.x, dy = (double)xi.y - (double)b.y, dz = (double)xi.z - (double)b.z;
+      double d2 = (dx * dx + dy * dy) + dz * dz;
+      if (d2 <= P.h2_64) {
+        double t = P.h2_64 - d2;
+        rho += (double)b.w * (P.Kp64 * ((t * t) * t));
+      }
+    }
+  }
+  double c = rho - 1.0;
+  ci[i] = c > 0.0 ? c : 0.0;
+}
+
+// block s reduces slots [ranges[2s], ranges[2s]+ranges[2s+1]); writes sum, max.
+// With hist != nullptr the totals over all blocks are also appended to a ring.
+__global__ void __launch_bounds__(1024) k_diag_reduce(const double *__restrict__ ci, const uint32_t *__restrict__ ranges,
+                                                      double *__restrict__ out) {
+  __shared__ double ssum[16], smax[16];
+  const uint32_t s = blockIdx.x;
+  const uint32_t b = ranges[2 * s], cnt = ranges[2 * s + 1];
+  double sum = 0.0, mx = 0.0;
+  for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
+    double v = ci[b + k];
+    sum += v;
+    mx = v > mx ? v : mx;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    sum += __shfl_xor(sum, o, 64);
+    double w = __shfl_xor(mx, o, 64);
+    mx = w > mx ? w : mx;
+  }
+  if (__lane_id() == 0) { ssum[threadIdx.x >> 6] = sum; smax[threadIdx.x >> 6] = mx; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0, m = 0.0;
+    for (uint32_t w = 0; w < blockDim.x / 64; ++w) { t += ssum[w]; m = smax[w] > m ? smax[w] : m; }
+    out[2 * s] = t;
+    out[2 * s + 1] = m;
+  }
+}
+
+// Sum the per-scene totals into history[ctr % cap] = (mean, max) over all.
+__global__ void k_diag_history(const double *__restrict__ per_scene, uint32_t nscenes, uint32_t n,
+                               double *__restrict__ hist, uint32_t cap, uint32_t *__restrict__ ctr) {
+  if (threadIdx.x != 0) return;
+  double t = 0.0, m = 0.0;
+  for (uint32_t s = 0; s < nscenes; ++s) {
+    t += per_scene[2 * s];
+    m = per_scene[2 * s + 1] > m ? per_scene[2 * s + 1] : m;
+  }
+  uint32_t c = *ctr;
+  hist[2 * (c % cap)] = n ? t / (double)n : 0.0;
+  hist[2 * (c % cap) + 1] = m;
+  *ctr = c + 1;
+}
+
+// ---------------------------------------------------------------------------
+// Utilities.
+__global__ void __launch_bounds__(BS) k_unsort4(uint32_t n, const float4 *__restrict__ src,
+                                                const uint32_t *__restrict__ ids, float4 *__restrict__ dst) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[ids[i]] = src[i];
+}
+__global__ void __launch_bounds__(BS) k_unsort1(uint32_t n, const float *__restrict__ src,
+                                                const uint32_t *__restrict__ ids, float4 *__restrict__ dst) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[ids[i]] = make_float4(src[i], 0.0f, 0.0f, 0.0f);
+}
+__global__ void __launch_bounds__(BS) k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void __launch_bounds__(BS) k_iota(uint32_t *p, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+static inline uint32_t nblocks(uint32_t n, uint32_t bs) { return (n + bs - 1) / bs; }
+
+#define LAUNCH(kernel, grid, stream, ...) hipLaunchKernelGGL(kernel, dim3(grid), dim3(BS), 0, stream, __VA_ARGS__)
+
+void launch_predict_bin(const LaunchCtx &L, const float4 *pos, const float4 *vel, const float4 *acc, float4 *xs,
+                        uint32_t *cellkey, uint32_t *rank, uint32_t *counts) {
+  if (!L.n) return;
+  if (L.multi)
+    LAUNCH(k_predict_bin<true>, nblocks(L.n, BS), L.stream, L.n, pos, vel, acc, xs, cellkey, rank, counts, L.s0,
+           L.scenes, L.sid, L.P);
+  else
+    LAUNCH(k_predict_bin<false>, nblocks(L.n, BS), L.stream, L.n, pos, vel, acc, xs, cellkey, rank, counts, L.s0,
+           L.scenes, L.sid, L.P);
+}
+
+void launch_scan(hipStream_t st, uint32_t *in, uint32_t n, uint32_t *out, int zero_in, uint32_t *const *sums,
+                 uint32_t *const *prefix, int level) {
+  if (n <= SCAN_E) {
+    hipLaunchKernelGGL(k_scan_down, dim3(1), dim3(BS), 0, st, in, n, (const uint32_t *)nullptr, out, zero_in);
+    return;
+  }
+  uint32_t nb = (n + SCAN_E - 1) / SCAN_E;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(BS), 0, st, in, n, sums[level]);
+  launch_scan(st, sums[level], nb, prefix[level], 0, sums, prefix, level + 1);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(BS), 0, st, in, n, (const uint32_t *)prefix[level], out, zero_in);
+}
+
+void launch_scatter(const LaunchCtx &L, const uint32_t *cellkey, const uint32_t *rank, const uint32_t *offsets,
+                    const uint32_t *ids, uint32_t *tmp_key, uint32_t *tmp_src) {
+  if (!L.n) return;
+  LAUNCH(k_scatter, nblocks(L.n, BS), L.stream, L.n, cellkey, rank, offsets, ids, tmp_key, tmp_src);
+}
+
+void launch_settle(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *tmp_src, const uint32_t *cellkey,
+                   const uint32_t *offsets, const float4 *xs, const float4 *pos, const float *lam_in, float4 *X0,
+                   float4 *Xold, uint32_t *ids, float *lam_out) {
+  if (!L.n) return;
+  LAUNCH(k_settle, nblocks(L.n, BS), L.stream, L.n, tmp_key, tmp_src, cellkey, offsets, xs, pos, lam_in, X0, Xold,
+         ids, lam_out);
+}
+
+void launch_build_list(const LaunchCtx &L, const float4 *X0, const uint32_t *offsets, uint32_t *nbr, uint32_t *ncount,
+                       uint32_t *flags) {
+  if (!L.n) return;
+  const uint32_t g = nblocks(L.n, BS);
+  if (L.multi) {
+    if (L.hash) LAUNCH((k_build_list<true, true>), g, L.stream, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
+    else LAUNCH((k_build_list<true, false>), g, L.stream, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
+  } else {
+    if (L.hash) LAUNCH((k_build_list<false, true>), g, L.stream, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
+    else LAUNCH((k_build_list<false, false>), g, L.stream, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
+  }
+}
+
+void launch_lambda(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr, const uint32_t *ncount) {
+  if (!L.n) return;
+  const uint32_t g = nblocks(L.n, BS);
+  if (L.multi) {
+    if (L.field) LAUNCH((k_lambda<true, true>), g, L.stream, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P);
+    else LAUNCH((k_lambda<true, false>), g, L.stream, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P);
+  } else {
+    if (L.field) LAUNCH((k_lambda<false, true>), g, L.stream, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P);
+    else LAUNCH((k_lambda<false, false>), g, L.stream, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P);
+  }
+}
+
+void launch_dp(const LaunchCtx &L, const float4 *X, const float4 *B, const uint32_t *nbr, const uint32_t *ncount,
+               float4 *Xn) {
+  if (!L.n) return;
+  const uint32_t g = nblocks(L.n, BS);
+  if (L.multi) {
+    if (L.field) LAUNCH((k_dp<true, true>), g, L.stream, L.n, X, B, nbr, ncount, Xn, L.s0, L.scenes, L.sid, L.P);
+    else LAUNCH((k_dp<true, false>), g, L.stream, L.n, X, B, nbr, ncount, Xn, L.s0, L.scenes, L.sid, L.P);
+  } else {
+    if (L.field) LAUNCH((k_dp<false, true>), g, L.stream, L.n, X, B, nbr, ncount, Xn, L.s0, L.scenes, L.sid, L.P);
+    else LAUNCH((k_dp<false, false>), g, L.stream, L.n, X, B, nbr, ncount, Xn, L.s0, L.scenes, L.sid, L.P);
+  }
+}
+
+void launch_density(const LaunchCtx &L, const float4 *X, const float4 *Xold, const uint32_t *nbr,
+                    const uint32_t *ncount, float4 *pos, float4 *vel, float *lam) {
+  if (!L.n) return;
+  LAUNCH(k_density, nblocks(L.n, BS), L.stream, L.n, X, Xold, nbr, ncount, pos, vel, lam, L.P);
+}
+
+void launch_vorticity(const LaunchCtx &L, const float4 *pos, const float4 *vel, const uint32_t *nbr,
+                      const uint32_t *ncount, float4 *om, float4 *nrm) {
+  if (!L.n) return;
+  LAUNCH(k_vorticity, nblocks(L.n, BS), L.stream, L.n, pos, vel, nbr, ncount, om, nrm, L.P);
+}
+
+void launch_xsph(const LaunchCtx &L, const float4 *pos, const float4 *vel, const float4 *om, const float4 *nrm,
+                 const uint32_t *nbr, const uint32_t *ncount, float4 *acc) {
+  if (!L.n) return;
+  LAUNCH(k_xsph, nblocks(L.n, BS), L.stream, L.n, pos, vel, om, nrm, nbr, ncount, acc, L.P);
+}
+
+void launch_diag(const LaunchCtx &L, const float4 *pos, const uint32_t *nbr, const uint32_t *ncount, double *ci,
+                 const uint32_t *ranges, uint32_t nscenes, double *per_scene, double *hist, uint32_t hist_cap,
+                 uint32_t *hist_ctr) {
+  if (!L.n) return;
+  const uint32_t g = nblocks(L.n, BS);
+  if (L.multi) {
+    if (L.field) LAUNCH((k_diag<true, true>), g, L.stream, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
+    else LAUNCH((k_diag<true, false>), g, L.stream, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
+  } else {
+    if (L.field) LAUNCH((k_diag<false, true>), g, L.stream, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
+    else LAUNCH((k_diag<false, false>), g, L.stream, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
+  }
+  hipLaunchKernelGGL(k_diag_reduce, dim3(nscenes), dim3(1024), 0, L.stream, ci, ranges, per_scene);
+  if (hist) hipLaunchKernelGGL(k_diag_history, dim3(1), dim3(64), 0, L.stream, per_scene, nscenes, L.n, hist, hist_cap, hist_ctr);
+}
+
+void launch_unsort4(hipStream_t st, uint32_t n, const float4 *src, const uint32_t *ids, float4 *dst) {
+  if (n) hipLaunchKernelGGL(k_unsort4, dim3(nblocks(n, BS)), dim3(BS), 0, st, n, src, ids, dst);
+}
+void launch_unsort1(hipStream_t st, uint32_t n, const float *src, const uint32_t *ids, float4 *dst) {
+  if (n) hipLaunchKernelGGL(k_unsort1, dim3(nblocks(n, BS)), dim3(BS), 0, st, n, src, ids, dst);
+}
+void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v) {
+  if (n) hipLaunchKernelGGL(k_fill_u32, dim3(nblocks(n, BS)), dim3(BS), 0, st, p, n, v);
+}
+void launch_iota(hipStream_t st, uint32_t *p, uint32_t n) {
+  if (n) hipLaunchKernelGGL(k_iota, dim3(nblocks(n, BS)), dim3(BS), 0, st, p, n);
+}
+
+}  // namespace pbf

@@ -1,0 +1,351 @@
+"""Python host mirror of the reference simulator API over libpbf's C ABI.
+
+The class ``PBF`` keeps the member names of the reference's ``SPH`` /
+``PBFConfig`` / ``PBF`` classes (LinzhouLi/WebGPU-Fluid-Simulation,
+src/simulator/SPH.ts, src/simulator/PBF/PBFConfig.ts, src/simulator/PBF/PBF.ts)
+and raises ``PBFError`` with the reference's messages where it has them
+('To Many Particles!', 'Illegal length of Particle Position Buffer!',
+'Invalid Boundary Model File!').  All physics runs in libpbf.so (HIP, gfx950);
+there is no CPU fallback: constructing a context without a GPU fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpbf.so")
+
+PBF_OK = 0
+PBF_GRID_DENSE, PBF_GRID_HASH = 0, 1
+PBF_FLAG_NO_GRAPH, PBF_FLAG_CUBIC_FIXED, PBF_FLAG_TIMING = 1, 2, 4
+FIELDS = {"position": 0, "velocity": 1, "accel": 2, "lambda": 3, "omega": 4, "normal": 5,
+          "boundary": 6, "predicted": 7}
+
+
+class PBFError(RuntimeError):
+    """Error raised for a non-zero status of the C ABI (message = pbf_last_error)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [("capacity", C.c_uint32), ("max_neighbors", C.c_uint32),
+                ("particle_radius", C.c_double), ("kernel_radius", C.c_double),
+                ("rest_density", C.c_double), ("time_step", C.c_double),
+                ("domain_lo", C.c_double * 3), ("domain_hi", C.c_double * 3),
+                ("grid_mode", C.c_uint32), ("hash_log2", C.c_uint32), ("flags", C.c_uint32),
+                ("reserved", C.c_uint32 * 5)]
+
+
+class Config(C.Structure):
+    _fields_ = [("iteration", C.c_uint32), ("xsph", C.c_float), ("vorticity", C.c_float),
+                ("surface_tension", C.c_float), ("gravity", C.c_float)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("domain_lo", C.c_double * 3), ("domain_hi", C.c_double * 3),
+                ("particle_count", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libpbf.so (raises OSError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"libpbf.so not built at {path}; run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    P, I, U32, U64, D, F, S = C.c_void_p, C.c_int, C.c_uint32, C.c_uint64, C.c_double, C.c_float, C.c_size_t
+    pd3 = C.POINTER(C.c_double)
+    pf = C.POINTER(C.c_float)
+    pu = C.POINTER(C.c_uint32)
+    sig = {
+        "pbf_create": (I, [C.POINTER(Params), C.POINTER(P)]),
+        "pbf_batch_create": (I, [C.POINTER(Params), U32, C.POINTER(SceneDesc), C.POINTER(P)]),
+        "pbf_destroy": (I, [P]), "pbf_reset": (I, [P]),
+        "pbf_set_config": (I, [P, C.POINTER(Config)]),
+        "pbf_voxelize_cube": (I, [P, pd3, pd3, D, U64, pu]),
+        "pbf_voxelize_sphere": (I, [P, pd3, D, D, U64, pu]),
+        "pbf_voxelize_cube_host": (I, [D, pd3, pd3, D, U64, U64, pf, U64, pu]),
+        "pbf_voxelize_sphere_host": (I, [D, pd3, D, D, U64, U64, pf, U64, pu]),
+        "pbf_staged_count": (U32, [P]),
+        "pbf_set_particle_position": (I, [P]),
+        "pbf_set_positions": (I, [P, pf, S, U32]),
+        "pbf_set_boundary_data": (I, [P, C.c_char_p, S]),
+        "pbf_parse_cdm": (I, [C.c_char_p, S, C.POINTER(C.c_int32), pf, pf, S, C.POINTER(S)]),
+        "pbf_set_cubic_field": (I, [P, U32, C.POINTER(C.c_int32), pd3, pd3, pf, pf, S]),
+        "pbf_set_volume_map": (I, [P, U32, C.POINTER(C.c_int32), pd3, pd3, pf]),
+        "pbf_clear_boundary": (I, [P]),
+        "pbf_build_volume_map": (I, [U32, pd3, C.POINTER(C.c_int32), pd3, pd3, D, D, pf]),
+        "pbf_step": (I, [P, F, U32, U32]), "pbf_run": (I, [P]), "pbf_sync": (I, [P]),
+        "pbf_read_positions": (I, [P, pf, U32]), "pbf_read_velocities": (I, [P, pf, U32]),
+        "pbf_read_field": (I, [P, U32, pf, U32]),
+        "pbf_num_cells": (U32, [P]),
+        "pbf_read_cells": (I, [P, pu, pu]),
+        "pbf_read_neighbors": (I, [P, pu, pu, U64, C.POINTER(U64)]),
+        "pbf_search": (I, [P, F]),
+        "pbf_density_error": (I, [P, U32, C.POINTER(D), C.POINTER(D)]),
+        "pbf_record_density_error": (I, [P, U32]),
+        "pbf_read_density_history": (I, [P, C.POINTER(D), U32, pu]),
+        "pbf_particle_count": (U32, [P]), "pbf_scene_count": (U32, [P]),
+        "pbf_last_error": (C.c_char_p, [P]), "pbf_version": (C.c_char_p, []),
+        "pbf_pass_times": (I, [P, C.POINTER(D), pu, U32, pu, C.POINTER(C.c_char_p)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _d3(v: Sequence[float]):
+    return (C.c_double * 3)(*[float(x) for x in v])
+
+
+def _i3(v: Sequence[int]):
+    return (C.c_int32 * 3)(*[int(x) for x in v])
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _uptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def build_volume_map(kind: int, res, lo, hi, kernel_radius: float, shape=None,
+                     particle_radius: float = 0.0) -> np.ndarray:
+    """Analytic volume map (res+1)^3 x (sdf, vol); kind 0 box, 1 box + sphere + torus."""
+    lib = load_library()
+    n = (res[0] + 1) * (res[1] + 1) * (res[2] + 1)
+    out = np.zeros((n, 2), np.float32)
+    sh = None
+    if shape is not None:
+        sh = (C.c_double * len(shape))(*[float(x) for x in shape])
+    rc = lib.pbf_build_volume_map(kind, sh, _i3(res), _d3(lo), _d3(hi), float(kernel_radius), float(particle_radius),
+                                  _fptr(out))
+    if rc != PBF_OK:
+        raise PBFError(rc, "pbf_build_volume_map failed")
+    return out
+
+
+def parse_cdm(text) -> tuple:
+    """Parse .cdm text without a GPU: (res, sdf, vol); raises PBFError on a bad file."""
+    lib = load_library()
+    b = text.encode() if isinstance(text, str) else bytes(text)
+    res = (C.c_int32 * 3)()
+    fl = C.c_size_t()
+    rc = lib.pbf_parse_cdm(b, len(b), res, None, None, 0, C.byref(fl))
+    if rc != PBF_OK:
+        raise PBFError(rc, "Invalid Boundary Model File!")
+    sdf = np.zeros(fl.value, np.float32)
+    vol = np.zeros(fl.value, np.float32)
+    rc = lib.pbf_parse_cdm(b, len(b), res, _fptr(sdf), _fptr(vol), fl.value, C.byref(fl))
+    if rc != PBF_OK:
+        raise PBFError(rc, "Invalid Boundary Model File!")
+    return tuple(res), sdf, vol
+
+
+class PBF:
+    """Mirror of the reference ``PBF`` simulator (PBF.ts:27-575) on libpbf."""
+
+    def __init__(self, particle_radius: float = 0.006, kernel_radius: float = 0.025, capacity: int = 1 << 17,
+                 rest_density: float = 1000.0, time_step: float = 1.0 / 300.0, domain_lo=(0.0, 0.0, 0.0),
+                 domain_hi=(1.0, 1.0, 1.0), max_neighbors: int = 96, grid_mode: int = PBF_GRID_DENSE,
+                 hash_log2: int = 22, flags: int = 0, scenes: Optional[Sequence[dict]] = None):
+        self.lib = load_library()
+        p = Params()
+        p.capacity = capacity
+        p.max_neighbors = max_neighbors
+        p.particle_radius = particle_radius
+        p.kernel_radius = kernel_radius
+        p.rest_density = rest_density
+        p.time_step = time_step
+        p.domain_lo = _d3(domain_lo)
+        p.domain_hi = _d3(domain_hi)
+        p.grid_mode = grid_mode
+        p.hash_log2 = hash_log2
+        p.flags = flags
+        self.params = p
+        self.ctx = C.c_void_p()
+        if scenes:
+            arr = (SceneDesc * len(scenes))()
+            for k, s in enumerate(scenes):
+                arr[k].domain_lo = _d3(s["domain_lo"])
+                arr[k].domain_hi = _d3(s["domain_hi"])
+                arr[k].particle_count = int(s["count"])
+            rc = self.lib.pbf_batch_create(C.byref(p), len(scenes), arr, C.byref(self.ctx))
+        else:
+            rc = self.lib.pbf_create(C.byref(p), C.byref(self.ctx))
+        if rc != PBF_OK:
+            raise PBFError(rc, f"pbf_create failed ({rc})")
+        self.pause = True
+        self.stepCount = 1
+        self.particlePositionArray: list = []
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if self.ctx:
+            self.lib.pbf_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != PBF_OK:
+            raise PBFError(rc, self.lib.pbf_last_error(self.ctx).decode())
+
+    # -- reference API ------------------------------------------------------
+    def reset(self):
+        """PBF.reset (PBF.ts:71-89)."""
+        self.particlePositionArray = []
+        self._check(self.lib.pbf_reset(self.ctx))
+
+    @property
+    def particleCount(self) -> int:
+        return int(self.lib.pbf_particle_count(self.ctx)) or int(self.lib.pbf_staged_count(self.ctx))
+
+    def voxelizeCube(self, mn, mx, jitter: float = 0.0, seed: int = 0) -> int:
+        """SPH.voxelizeCube (SPH.ts:99-121)."""
+        added = C.c_uint32()
+        self._check(self.lib.pbf_voxelize_cube(self.ctx, _d3(mn), _d3(mx), float(jitter), int(seed), C.byref(added)))
+        return int(added.value)
+
+    def voxelizeSphere(self, center, radius: float, jitter: float = 0.0, seed: int = 0) -> int:
+        """SPH.voxelizeSphere (SPH.ts:123-149)."""
+        added = C.c_uint32()
+        self._check(self.lib.pbf_voxelize_sphere(self.ctx, _d3(center), float(radius), float(jitter), int(seed),
+                                                 C.byref(added)))
+        return int(added.value)
+
+    def setParticlePosition(self, positions: Optional[np.ndarray] = None, count: Optional[int] = None):
+        """PBFConfig.setParticlePosition (PBFConfig.ts:48-62); uploads staged or given float4 rows."""
+        if positions is None:
+            self._check(self.lib.pbf_set_particle_position(self.ctx))
+            return
+        a = np.ascontiguousarray(np.asarray(positions, np.float32).reshape(-1))
+        n = a.size // 4 if count is None else int(count)
+        self._check(self.lib.pbf_set_positions(self.ctx, _fptr(a), a.size, n))
+
+    def setBoundaryData(self, data: str):
+        """PBF.setBoundaryData (PBF.ts:91-94): .cdm text."""
+        b = data.encode() if isinstance(data, str) else bytes(data)
+        self._check(self.lib.pbf_set_boundary_data(self.ctx, b, len(b)))
+
+    def setCubicField(self, res, lo, hi, sdf: np.ndarray, vol: np.ndarray, scene: int = 0):
+        s = np.ascontiguousarray(sdf, np.float32)
+        v = np.ascontiguousarray(vol, np.float32)
+        self._check(self.lib.pbf_set_cubic_field(self.ctx, scene, _i3(res), _d3(lo), _d3(hi), _fptr(s), _fptr(v),
+                                                 s.size))
+
+    def setVolumeMap(self, res, lo, hi, sdf_vol: np.ndarray, scene: int = 0):
+        a = np.ascontiguousarray(sdf_vol, np.float32)
+        self._check(self.lib.pbf_set_volume_map(self.ctx, scene, _i3(res), _d3(lo), _d3(hi), _fptr(a)))
+
+    def setConfig(self, conf: dict):
+        """PBFConfig.setConfig (PBFConfig.ts:31-46)."""
+        c = Config()
+        c.iteration = int(conf.get("iteration", 5))
+        c.xsph = float(conf.get("XSPH", 0.01))
+        c.vorticity = float(conf.get("vorticity", 0.1))
+        c.surface_tension = float(conf.get("surfaceTension", 0.5))
+        c.gravity = float(conf.get("gravity", 9.8))
+        self._check(self.lib.pbf_set_config(self.ctx, C.byref(c)))
+
+    def start(self):
+        self.pause = False
+
+    def stop(self):
+        self.pause = True
+
+    def switch(self):
+        self.pause = not self.pause
+
+    def run(self):
+        """PBF.run (PBF.ts:398-448): one substep of time_step; no-op when paused."""
+        if self.pause:
+            return
+        self._check(self.lib.pbf_run(self.ctx))
+
+    # -- extended API (north-star surface) ----------------------------------
+    def step(self, dt: float, substeps: int = 1, iterations: int = 0):
+        self._check(self.lib.pbf_step(self.ctx, float(dt), int(substeps), int(iterations)))
+
+    def sync(self):
+        self._check(self.lib.pbf_sync(self.ctx))
+
+    def search(self, dt: float):
+        self._check(self.lib.pbf_search(self.ctx, float(dt)))
+
+    def read_positions(self) -> np.ndarray:
+        n = int(self.lib.pbf_particle_count(self.ctx))
+        out = np.zeros((n, 4), np.float32)
+        self._check(self.lib.pbf_read_positions(self.ctx, _fptr(out), n))
+        return out
+
+    def read_velocities(self) -> np.ndarray:
+        n = int(self.lib.pbf_particle_count(self.ctx))
+        out = np.zeros((n, 4), np.float32)
+        self._check(self.lib.pbf_read_velocities(self.ctx, _fptr(out), n))
+        return out
+
+    def read_field(self, name: str) -> np.ndarray:
+        n = int(self.lib.pbf_particle_count(self.ctx))
+        out = np.zeros((n, 4), np.float32)
+        self._check(self.lib.pbf_read_field(self.ctx, FIELDS[name], _fptr(out), n))
+        return out
+
+    def read_cells(self):
+        nc = int(self.lib.pbf_num_cells(self.ctx))
+        counts = np.zeros(nc, np.uint32)
+        offsets = np.zeros(nc + 1, np.uint32)
+        self._check(self.lib.pbf_read_cells(self.ctx, _uptr(counts), _uptr(offsets)))
+        return counts, offsets
+
+    def read_neighbors(self):
+        n = int(self.lib.pbf_particle_count(self.ctx))
+        off = np.zeros(n + 1, np.uint32)
+        pairs = C.c_uint64()
+        self._check(self.lib.pbf_read_neighbors(self.ctx, _uptr(off), None, 0, C.byref(pairs)))
+        ids = np.zeros(max(int(pairs.value), 1), np.uint32)
+        self._check(self.lib.pbf_read_neighbors(self.ctx, _uptr(off), _uptr(ids), ids.size, C.byref(pairs)))
+        return off, ids[: int(pairs.value)]
+
+    def density_error(self, scene: int = 0xFFFFFFFF):
+        m, x = C.c_double(), C.c_double()
+        self._check(self.lib.pbf_density_error(self.ctx, scene, C.byref(m), C.byref(x)))
+        return m.value, x.value
+
+    def record_density_error(self, enable: bool = True):
+        self._check(self.lib.pbf_record_density_error(self.ctx, 1 if enable else 0))
+
+    def density_history(self, cap: int = 4096) -> np.ndarray:
+        out = np.zeros((cap, 2), np.float64)
+        cnt = C.c_uint32()
+        self._check(self.lib.pbf_read_density_history(self.ctx, out.ctypes.data_as(C.POINTER(C.c_double)), cap,
+                                                      C.byref(cnt)))
+        return out[: cnt.value]
+
+    def pass_times(self) -> dict:
+        ms = (C.c_double * 32)()
+        cnt = (C.c_uint32 * 32)()
+        k = C.c_uint32()
+        names = C.c_char_p()
+        self._check(self.lib.pbf_pass_times(self.ctx, ms, cnt, 32, C.byref(k), C.byref(names)))
+        nm = names.value.decode().split("\n")
+        return {nm[q]: (ms[q], cnt[q]) for q in range(min(k.value, len(nm)))}
