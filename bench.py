@@ -21,7 +21,7 @@ sys.path[:0] = [os.path.join(ROOT, "webgpu-fluid-simulation_amd"), os.path.join(
 import numpy as np  # noqa: E402
 
 import scenes  # noqa: E402
-from pbf_host import PBF_FLAG_TIMING  # noqa: E402
+from pbf_host import PBF_FLAG_TIMING, copy_bandwidth  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
@@ -128,6 +128,8 @@ def main():
         log(f"sum of pass times {gpu_ms:.1f} ms of {el * 1e3:.1f} ms wall")
 
     whole = substep_bytes(N, C, sc.iterations, B) * substeps / el / 1e9
+    copy_gbs = copy_bandwidth(1 << 30, 20)
+    log(f"streaming copy probe: {copy_gbs:.0f} GB/s (read + write, 1 GiB)")
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: oracle for ~{args.cpu_seconds:.0f} s ...")
@@ -155,6 +157,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "whole_substep_algorithmic_GBs": round(whole, 1),
+        "measured_copy_GBs": round(copy_gbs, 1),
         "density_error_after_run": {"mean": mean_err, "max": max_err},
         "passes": per_pass,
     }

@@ -203,6 +203,12 @@ const char *pbf_version(void);
 int pbf_pass_times(pbf_ctx *ctx, double *ms, uint32_t *launches, uint32_t capacity, uint32_t *count,
                    const char **names);
 
+/* Streaming-copy probe: copies `bytes` device-to-device `iterations` times
+ * with a plain 16-byte-per-lane kernel and returns read + write GB/s.  The
+ * measured HBM ceiling next to the 8 TB/s specification, and the reference
+ * kernel for scaling hardware byte counters. No reference counterpart. */
+int pbf_copy_bandwidth(size_t bytes, uint32_t iterations, double *gbps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,9 +7,10 @@
  *
  * Numerics (fixed here because the reference leaves them open):
  *   - physical constants are formed in double and rounded once to float;
- *   - no contraction (built with -ffp-contract=off); every sum is written in
+ *   - no implicit contraction (built with -ffp-contract=off); fused
+ *     multiply-adds only where fmaf() is written; every sum is written in
  *     the order it is evaluated;
- *   - |x_ij|^2 = (dx*dx + dy*dy) + dz*dz; poly6 uses it directly;
+ *   - |x_ij|^2 = fma(dz, dz, fma(dy, dy, dx*dx)); poly6 uses it directly;
  *   - spiky / cohesion gradients are f(r) * x_ij with one division by r;
  *   - sin() in the wall clamp is the Cephes single-precision sinf algorithm
  *     (S. L. Moshier), evaluated with plain float operations;
@@ -82,9 +83,9 @@ static inline float fractf_(float v) { return v - floorf(v); }
  * box [lo, hi]: bounds lo + 1e-4*rand, hi - 1e-4*rand. */
 static inline void clamp_rand(const orc_sim *s, float *p) {
   const float A = 12.9898f, B = 78.233f;
-  float b0 = p[1] * A + p[2] * B;
-  float b1 = p[0] * A + p[2] * B;
-  float b2 = p[0] * A + p[1] * B;
+  float b0 = fmaf(p[2], B, p[1] * A);
+  float b1 = fmaf(p[2], B, p[0] * A);
+  float b2 = fmaf(p[1], B, p[0] * A);
   float rnd[3];
   rnd[0] = fractf_(orc_sinf(b0) * 43758.5453f);
   rnd[1] = fractf_(orc_sinf(b1) * 43758.5453f);
@@ -106,6 +107,8 @@ void orc_clamp(const orc_sim *s, int n, const float *x3, float *out3) {
   }
 }
 
+static inline float dist2(float dx, float dy, float dz) { return fmaf(dz, dz, fmaf(dy, dy, dx * dx)); }
+
 /* ------------------------------------------------------------------ */
 /* SPH kernels (src/simulator/PBF/shader/common.ts:1-40).              */
 static inline float k_poly6(const orc_sim *s, float d2) {
@@ -126,7 +129,7 @@ static inline float k_cohesion_f(const orc_sim *s, float d2) {
   float r = sqrtf(d2);
   float a = s->h - r;
   float t = ((a * a) * a) * ((r * r) * r);
-  float res = (r > s->half_h) ? t : (2.0f * t + s->bias);
+  float res = (r > s->half_h) ? t : fmaf(2.0f, t, s->bias);
   return (s->Kc * res) / r;
 }
 
@@ -338,28 +341,28 @@ static void field_eval(const orc_sim *s, const float *x, float grad[3], float *s
     for (int k = 0; k < 8; ++k) { cs[k] = s->fsv[2 * o[k]]; cv[k] = s->fsv[2 * o[k] + 1]; }
     float fx = f[0], fy = f[1], fz = f[2];
     /* corners: 0 (000) 1 (100) 2 (010) 3 (110) 4 (001) 5 (101) 6 (011) 7 (111) */
-    float c00 = cs[0] + fx * (cs[1] - cs[0]);
-    float c10 = cs[2] + fx * (cs[3] - cs[2]);
-    float c01 = cs[4] + fx * (cs[5] - cs[4]);
-    float c11 = cs[6] + fx * (cs[7] - cs[6]);
-    float c0 = c00 + fy * (c10 - c00);
-    float c1 = c01 + fy * (c11 - c01);
-    *sdf = c0 + fz * (c1 - c0);
+    float c00 = fmaf(fx, cs[1] - cs[0], cs[0]);
+    float c10 = fmaf(fx, cs[3] - cs[2], cs[2]);
+    float c01 = fmaf(fx, cs[5] - cs[4], cs[4]);
+    float c11 = fmaf(fx, cs[7] - cs[6], cs[6]);
+    float c0 = fmaf(fy, c10 - c00, c00);
+    float c1 = fmaf(fy, c11 - c01, c01);
+    *sdf = fmaf(fz, c1 - c0, c0);
     float ex00 = cs[1] - cs[0], ex10 = cs[3] - cs[2], ex01 = cs[5] - cs[4], ex11 = cs[7] - cs[6];
-    float ex0 = ex00 + fy * (ex10 - ex00);
-    float ex1 = ex01 + fy * (ex11 - ex01);
-    grad[0] = (ex0 + fz * (ex1 - ex0)) * s->finv[0];
+    float ex0 = fmaf(fy, ex10 - ex00, ex00);
+    float ex1 = fmaf(fy, ex11 - ex01, ex01);
+    grad[0] = fmaf(fz, ex1 - ex0, ex0) * s->finv[0];
     float ey0 = c10 - c00, ey1 = c11 - c01;
-    grad[1] = (ey0 + fz * (ey1 - ey0)) * s->finv[1];
+    grad[1] = fmaf(fz, ey1 - ey0, ey0) * s->finv[1];
     grad[2] = (c1 - c0) * s->finv[2];
     if (want_vol) {
-      float v00 = cv[0] + fx * (cv[1] - cv[0]);
-      float v10 = cv[2] + fx * (cv[3] - cv[2]);
-      float v01 = cv[4] + fx * (cv[5] - cv[4]);
-      float v11 = cv[6] + fx * (cv[7] - cv[6]);
-      float v0 = v00 + fy * (v10 - v00);
-      float v1 = v01 + fy * (v11 - v01);
-      *vol = v0 + fz * (v1 - v0);
+      float v00 = fmaf(fx, cv[1] - cv[0], cv[0]);
+      float v10 = fmaf(fx, cv[3] - cv[2], cv[2]);
+      float v01 = fmaf(fx, cv[5] - cv[4], cv[4]);
+      float v11 = fmaf(fx, cv[7] - cv[6], cv[6]);
+      float v0 = fmaf(fy, v10 - v00, v00);
+      float v1 = fmaf(fy, v11 - v01, v01);
+      *vol = fmaf(fz, v1 - v0, v0);
     }
   }
 }
@@ -370,16 +373,16 @@ static void boundary_sample(const orc_sim *s, const float *x, float out[4]) {
   if (!s->field_kind) return;
   float g[3], d, vol = 0.0f;
   field_eval(s, x, g, &d, &vol, 0);
-  float len = sqrtf((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
+  float len = sqrtf(dist2(g[0], g[1], g[2]));
   if (d > 0.0f && d < s->h && len > 1e-9f) {
     field_eval(s, x, g, &d, &vol, 1);
     if (vol > 0.0f) {
       float nx = g[0] / len, ny = g[1] / len, nz = g[2] / len;
       float a = d + s->half_r;
       float dd = a > s->two_r ? a : s->two_r;
-      out[0] = x[0] - dd * nx;
-      out[1] = x[1] - dd * ny;
-      out[2] = x[2] - dd * nz;
+      out[0] = fmaf(-dd, nx, x[0]);
+      out[1] = fmaf(-dd, ny, x[1]);
+      out[2] = fmaf(-dd, nz, x[2]);
       out[3] = vol;
     }
   }
@@ -542,8 +545,8 @@ static void predict(orc_sim *s, float dt) {
   for (int i = 0; i < n; ++i) {
     float p[3];
     for (int c = 0; c < 3; ++c) {
-      float vel = s->v[4 * i + c] + dt * (g[c] + s->a[4 * i + c]);
-      p[c] = s->x[4 * i + c] + dt * vel;
+      float vel = fmaf(dt, g[c] + s->a[4 * i + c], s->v[4 * i + c]);
+      p[c] = fmaf(dt, vel, s->x[4 * i + c]);
     }
     clamp_rand(s, p);
     s->xs[4 * i] = p[0]; s->xs[4 * i + 1] = p[1]; s->xs[4 * i + 2] = p[2]; s->xs[4 * i + 3] = 0.0f;
@@ -597,8 +600,7 @@ static void search(orc_sim *s) {
             for (uint32_t t = s->off[k]; t < s->off[k + 1]; ++t) {
               uint32_t j = s->sorted[t];
               const float *xj = s->xs + 4 * j;
-              float ex = xj[0] - xi[0], ey = xj[1] - xi[1], ez = xj[2] - xi[2];
-              float d2 = (ex * ex + ey * ey) + ez * ez;
+              float d2 = dist2(xj[0] - xi[0], xj[1] - xi[1], xj[2] - xi[2]);
               if (d2 <= h2) {
                 if (pass) s->nbr[w++] = j;
                 else cntn++;
@@ -646,11 +648,11 @@ void orc_substep(orc_sim *s, float dt, int iterations) {
       for (uint32_t q = s->noff[i]; q < s->noff[i + 1]; ++q) {
         const float *xj = s->xi + 4 * s->nbr[q];
         float dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
-        float d2 = (dx * dx + dy * dy) + dz * dz;
+        float d2 = dist2(dx, dy, dz);
         float f = k_spiky_f(s, d2);
         float ax = f * dx, ay = f * dy, az = f * dz;
         gx += ax; gy += ay; gz += az;
-        s2 += (ax * ax + ay * ay) + az * az;
+        s2 = fmaf(az, az, fmaf(ay, ay, fmaf(ax, ax, s2)));
         rho += k_poly6(s, d2);
       }
       rho *= s->V;
@@ -659,14 +661,14 @@ void orc_substep(orc_sim *s, float dt, int iterations) {
       const float *b = s->bnd + 4 * i;
       if (b[3] > 0.0f) {
         float dx = xi[0] - b[0], dy = xi[1] - b[1], dz = xi[2] - b[2];
-        float d2 = (dx * dx + dy * dy) + dz * dz;
-        float f = k_spiky_f(s, d2);
-        gx += b[3] * (f * dx); gy += b[3] * (f * dy); gz += b[3] * (f * dz);
-        rho += b[3] * k_poly6(s, d2);
+        float d2 = dist2(dx, dy, dz);
+        float fb = b[3] * k_spiky_f(s, d2);
+        gx = fmaf(fb, dx, gx); gy = fmaf(fb, dy, gy); gz = fmaf(fb, dz, gz);
+        rho = fmaf(b[3], k_poly6(s, d2), rho);
       }
       float c = rho - 1.0f;
       float C = c > 0.0f ? c : 0.0f;
-      float den = ((gx * gx + gy * gy) + gz * gz) + s2;
+      float den = fmaf(gz, gz, fmaf(gy, gy, fmaf(gx, gx, s2)));
       s->lam[4 * i] = (-C) / (den + s->eps_lambda);
     }
     /* K10 + K11 delta p and apply (pressureConstrain.ts:109-179) */
@@ -679,21 +681,17 @@ void orc_substep(orc_sim *s, float dt, int iterations) {
         uint32_t j = s->nbr[q];
         const float *xj = s->xi + 4 * j;
         float dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
-        float d2 = (dx * dx + dy * dy) + dz * dz;
-        float f = k_spiky_f(s, d2);
-        float cl = li + s->lam[4 * j];
-        ax += cl * (f * dx); ay += cl * (f * dy); az += cl * (f * dz);
+        float fl = (li + s->lam[4 * j]) * k_spiky_f(s, dist2(dx, dy, dz));
+        ax = fmaf(fl, dx, ax); ay = fmaf(fl, dy, ay); az = fmaf(fl, dz, az);
       }
       ax *= s->V; ay *= s->V; az *= s->V;
       const float *b = s->bnd + 4 * i;
       if (b[3] > 0.0f) {
         float dx = xi[0] - b[0], dy = xi[1] - b[1], dz = xi[2] - b[2];
-        float d2 = (dx * dx + dy * dy) + dz * dz;
-        float f = k_spiky_f(s, d2);
-        float cb = b[3] * li;
-        ax += cb * (f * dx); ay += cb * (f * dy); az += cb * (f * dz);
+        float fb = (b[3] * li) * k_spiky_f(s, dist2(dx, dy, dz));
+        ax = fmaf(fb, dx, ax); ay = fmaf(fb, dy, ay); az = fmaf(fb, dz, az);
       }
-      float p[3] = {xi[0] + 0.5f * ax, xi[1] + 0.5f * ay, xi[2] + 0.5f * az};
+      float p[3] = {fmaf(0.5f, ax, xi[0]), fmaf(0.5f, ay, xi[1]), fmaf(0.5f, az, xi[2])};
       clamp_rand(s, p);
       float *o = s->xn + 4 * i;
       o[0] = p[0]; o[1] = p[1]; o[2] = p[2]; o[3] = 0.0f;
@@ -708,9 +706,7 @@ void orc_substep(orc_sim *s, float dt, int iterations) {
     float rho = 0.0f;
     for (uint32_t q = s->noff[i]; q < s->noff[i + 1]; ++q) {
       const float *xj = s->xi + 4 * s->nbr[q];
-      float dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
-      float d2 = (dx * dx + dy * dy) + dz * dz;
-      rho += k_poly6(s, d2);
+      rho += k_poly6(s, dist2(xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]));
     }
     rho *= s->m;
     for (int c = 0; c < 3; ++c) s->v[4 * i + c] = (xi[c] - s->x[4 * i + c]) * inv_dt;
@@ -731,17 +727,16 @@ void orc_substep(orc_sim *s, float dt, int iterations) {
       uint32_t j = s->nbr[q];
       const float *xj = s->x + 4 * j, *vj = s->v + 4 * j;
       float dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
-      float d2 = (dx * dx + dy * dy) + dz * dz;
       float ux = vj[0] - vi[0], uy = vj[1] - vi[1], uz = vj[2] - vi[2];
       float vol = s->m / xj[3];
-      float f = k_spiky_f(s, d2);
+      float f = k_spiky_f(s, dist2(dx, dy, dz));
       float gx = f * dx, gy = f * dy, gz = f * dz;
-      float cx = uy * gz - uz * gy, cy = uz * gx - ux * gz, cz = ux * gy - uy * gx;
-      wx += vol * cx; wy += vol * cy; wz += vol * cz;
-      nx += vol * gx; ny += vol * gy; nz += vol * gz;
+      float cx = fmaf(uy, gz, -(uz * gy)), cy = fmaf(uz, gx, -(ux * gz)), cz = fmaf(ux, gy, -(uy * gx));
+      wx = fmaf(vol, cx, wx); wy = fmaf(vol, cy, wy); wz = fmaf(vol, cz, wz);
+      nx = fmaf(vol, gx, nx); ny = fmaf(vol, gy, ny); nz = fmaf(vol, gz, nz);
     }
     s->om[4 * i] = wx; s->om[4 * i + 1] = wy; s->om[4 * i + 2] = wz;
-    s->om[4 * i + 3] = sqrtf((wx * wx + wy * wy) + wz * wz);
+    s->om[4 * i + 3] = sqrtf(dist2(wx, wy, wz));
     s->nrm[4 * i] = s->h * nx; s->nrm[4 * i + 1] = s->h * ny; s->nrm[4 * i + 2] = s->h * nz;
     s->nrm[4 * i + 3] = 0.0f;
   }
@@ -758,25 +753,29 @@ void orc_substep(orc_sim *s, float dt, int iterations) {
       float wj = s->om[4 * j + 3];
       float rhoj = xj[3];
       float dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
-      float d2 = (dx * dx + dy * dy) + dz * dz;
+      float d2 = dist2(dx, dy, dz);
       float vx = vj[0] - vi[0], vy = vj[1] - vi[1], vz = vj[2] - vi[2];
+      float irj = 1.0f / rhoj;
       float K = s->two_rho0 / (rhoi + rhoj);
-      float W = k_poly6(s, d2);
-      ux += (vx * W) / rhoj; uy += (vy * W) / rhoj; uz += (vz * W) / rhoj;
-      float f = k_spiky_f(s, d2);
-      ex += (wj * (f * dx)) / rhoj; ey += (wj * (f * dy)) / rhoj; ez += (wj * (f * dz)) / rhoj;
-      float fc = k_cohesion_f(s, d2);
-      tx -= K * (s->m * (fc * dx) + (ni[0] - nj[0]));
-      ty -= K * (s->m * (fc * dy) + (ni[1] - nj[1]));
-      tz -= K * (s->m * (fc * dz) + (ni[2] - nj[2]));
+      float sw = k_poly6(s, d2) * irj;
+      ux = fmaf(vx, sw, ux); uy = fmaf(vy, sw, uy); uz = fmaf(vz, sw, uz);
+      float se = (wj * k_spiky_f(s, d2)) * irj;
+      ex = fmaf(se, dx, ex); ey = fmaf(se, dy, ey); ez = fmaf(se, dz, ez);
+      float mc = s->m * k_cohesion_f(s, d2);
+      tx = fmaf(-K, fmaf(mc, dx, ni[0] - nj[0]), tx);
+      ty = fmaf(-K, fmaf(mc, dy, ni[1] - nj[1]), ty);
+      tz = fmaf(-K, fmaf(mc, dz, ni[2] - nj[2]), tz);
     }
-    float el = sqrtf((ex * ex + ey * ey) + ez * ez);
+    float el = sqrtf(dist2(ex, ey, ez));
     float hx = 0.0f, hy = 0.0f, hz = 0.0f;
-    if (el > 0.0f) { hx = ex / el; hy = ey / el; hz = ez / el; }
-    float cx = hy * wi[2] - hz * wi[1], cy = hz * wi[0] - hx * wi[2], cz = hx * wi[1] - hy * wi[0];
-    s->a[4 * i] = (kx * ux + s->vort * cx) + s->tension * tx;
-    s->a[4 * i + 1] = (kx * uy + s->vort * cy) + s->tension * ty;
-    s->a[4 * i + 2] = (kx * uz + s->vort * cz) + s->tension * tz;
+    if (el > 0.0f) {
+      float ie = 1.0f / el;
+      hx = ex * ie; hy = ey * ie; hz = ez * ie;
+    }
+    float cx = fmaf(hy, wi[2], -(hz * wi[1])), cy = fmaf(hz, wi[0], -(hx * wi[2])), cz = fmaf(hx, wi[1], -(hy * wi[0]));
+    s->a[4 * i] = fmaf(s->tension, tx, fmaf(s->vort, cx, kx * ux));
+    s->a[4 * i + 1] = fmaf(s->tension, ty, fmaf(s->vort, cy, kx * uy));
+    s->a[4 * i + 2] = fmaf(s->tension, tz, fmaf(s->vort, cz, kx * uz));
     s->a[4 * i + 3] = 0.0f;
   }
 }

@@ -1,11 +1,12 @@
 // pbf_device.h -- device-side math of the PBF substep (gfx950).
 //
 // Numerics follow one fixed specification so that every pass is reproducible
-// bit for bit: constants formed in double and rounded once, no contraction
-// (the library is built with -ffp-contract=off), sums written in evaluation
-// order, correctly rounded division and square root, and the wall-clamp
-// jitter's sin() evaluated with the Cephes single-precision algorithm
-// (S. L. Moshier) in plain float arithmetic.
+// bit for bit: constants formed in double and rounded once, no implicit
+// contraction (the library is built with -ffp-contract=off) and fused
+// multiply-adds only where written as fmaf(), sums in evaluation order,
+// correctly rounded division and square root, and the wall-clamp jitter's
+// sin() evaluated with the Cephes single-precision algorithm (S. L. Moshier)
+// in plain float arithmetic.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -75,9 +76,9 @@ __device__ __forceinline__ float fract_(float v) { return v - floorf(v); }
 // Wall clamp with per-axis pseudo-random inset of at most 1e-4.
 __device__ __forceinline__ void clamp_rand(const Scene &sc, float &px, float &py, float &pz) {
   const float A = 12.9898f, B = 78.233f;
-  float b0 = py * A + pz * B;
-  float b1 = px * A + pz * B;
-  float b2 = px * A + py * B;
+  float b0 = fmaf(pz, B, py * A);
+  float b1 = fmaf(pz, B, px * A);
+  float b2 = fmaf(py, B, px * A);
   float r0 = fract_(cephes_sinf(b0) * 43758.5453f);
   float r1 = fract_(cephes_sinf(b1) * 43758.5453f);
   float r2 = fract_(cephes_sinf(b2) * 43758.5453f);
@@ -112,9 +113,11 @@ __device__ __forceinline__ float cohesion_f(const Phys &P, float d2) {
   float r = sqrtf(d2);
   float a = P.h - r;
   float t = ((a * a) * a) * ((r * r) * r);
-  float res = (r > P.half_h) ? t : (2.0f * t + P.bias);
+  float res = (r > P.half_h) ? t : fmaf(2.0f, t, P.bias);
   return (P.Kc * res) / r;
 }
+
+__device__ __forceinline__ float dist2(float dx, float dy, float dz) { return fmaf(dz, dz, fmaf(dy, dy, dx * dx)); }
 
 // ---------------------------------------------------------------------------
 // Grid.
@@ -257,12 +260,14 @@ __device__ __forceinline__ float sum32(const float *N, const float *v) {
   return ((q0 + q1) + q2) + q3;
 }
 
-// sdf value and gradient (and volume when want_vol) of a scene's field.
+// sdf value, gradient and volume of a scene's field.  KIND 1: cubic 32-node
+// element (.cdm layout), 2: trilinear node grid, 0: decided at run time.
+template <int KIND>
 __device__ __forceinline__ void field_eval(const Scene &sc, const float *__restrict__ pool, float x0, float x1, float x2,
                                            float g[3], float &sdf, float &vol, bool want_vol) {
   const float *F = pool + sc.field_off;
   float xv[3] = {x0, x1, x2};
-  if (sc.field_kind == 1) {
+  if (KIND == 1 || (KIND == 0 && sc.field_kind == 1)) {
     uint32_t cell[3];
     float sv[3];
 #pragma unroll
@@ -331,50 +336,49 @@ __device__ __forceinline__ void field_eval(const Scene &sc, const float *__restr
       cv[k] = t.y;
     }
     float fx = f[0], fy = f[1], fz = f[2];
-    float c00 = cs[0] + fx * (cs[1] - cs[0]);
-    float c10 = cs[2] + fx * (cs[3] - cs[2]);
-    float c01 = cs[4] + fx * (cs[5] - cs[4]);
-    float c11 = cs[6] + fx * (cs[7] - cs[6]);
-    float c0 = c00 + fy * (c10 - c00);
-    float c1 = c01 + fy * (c11 - c01);
-    sdf = c0 + fz * (c1 - c0);
+    float c00 = fmaf(fx, cs[1] - cs[0], cs[0]);
+    float c10 = fmaf(fx, cs[3] - cs[2], cs[2]);
+    float c01 = fmaf(fx, cs[5] - cs[4], cs[4]);
+    float c11 = fmaf(fx, cs[7] - cs[6], cs[6]);
+    float c0 = fmaf(fy, c10 - c00, c00);
+    float c1 = fmaf(fy, c11 - c01, c01);
+    sdf = fmaf(fz, c1 - c0, c0);
     float ex00 = cs[1] - cs[0], ex10 = cs[3] - cs[2], ex01 = cs[5] - cs[4], ex11 = cs[7] - cs[6];
-    float ex0 = ex00 + fy * (ex10 - ex00);
-    float ex1 = ex01 + fy * (ex11 - ex01);
-    g[0] = (ex0 + fz * (ex1 - ex0)) * sc.finv[0];
+    float ex0 = fmaf(fy, ex10 - ex00, ex00);
+    float ex1 = fmaf(fy, ex11 - ex01, ex01);
+    g[0] = fmaf(fz, ex1 - ex0, ex0) * sc.finv[0];
     float ey0 = c10 - c00, ey1 = c11 - c01;
-    g[1] = (ey0 + fz * (ey1 - ey0)) * sc.finv[1];
+    g[1] = fmaf(fz, ey1 - ey0, ey0) * sc.finv[1];
     g[2] = (c1 - c0) * sc.finv[2];
-    if (want_vol) {
-      float v00 = cv[0] + fx * (cv[1] - cv[0]);
-      float v10 = cv[2] + fx * (cv[3] - cv[2]);
-      float v01 = cv[4] + fx * (cv[5] - cv[4]);
-      float v11 = cv[6] + fx * (cv[7] - cv[6]);
-      float v0 = v00 + fy * (v10 - v00);
-      float v1 = v01 + fy * (v11 - v01);
-      vol = v0 + fz * (v1 - v0);
-    }
+    float v00 = fmaf(fx, cv[1] - cv[0], cv[0]);
+    float v10 = fmaf(fx, cv[3] - cv[2], cv[2]);
+    float v01 = fmaf(fx, cv[5] - cv[4], cv[4]);
+    float v11 = fmaf(fx, cv[7] - cv[6], cv[6]);
+    float v0 = fmaf(fy, v10 - v00, v00);
+    float v1 = fmaf(fy, v11 - v01, v01);
+    vol = fmaf(fz, v1 - v0, v0);
   }
 }
 
 // Mirrored boundary particle (x_b, V_b) of a fluid particle at x: the field
 // point max(d + r/2, 2r) below x along the SDF normal, when 0 < d < h.
+template <int KIND>
 __device__ __forceinline__ float4 boundary_sample(const Phys &P, const Scene &sc, const float *__restrict__ pool,
                                                   float x0, float x1, float x2) {
   float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (!sc.field_kind) return out;
   float g[3], d, vol = 0.0f;
-  field_eval(sc, pool, x0, x1, x2, g, d, vol, false);
-  float len = sqrtf((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
+  field_eval<KIND>(sc, pool, x0, x1, x2, g, d, vol, false);
+  float len = sqrtf(dist2(g[0], g[1], g[2]));
   if (d > 0.0f && d < P.h && len > 1e-9f) {
-    field_eval(sc, pool, x0, x1, x2, g, d, vol, true);
+    if (KIND == 1 || (KIND == 0 && sc.field_kind == 1)) field_eval<KIND>(sc, pool, x0, x1, x2, g, d, vol, true);
     if (vol > 0.0f) {
       float nx = g[0] / len, ny = g[1] / len, nz = g[2] / len;
       float a = d + P.half_r;
       float dd = a > P.two_r ? a : P.two_r;
-      out.x = x0 - dd * nx;
-      out.y = x1 - dd * ny;
-      out.z = x2 - dd * nz;
+      out.x = fmaf(-dd, nx, x0);
+      out.y = fmaf(-dd, ny, x1);
+      out.z = fmaf(-dd, nz, x2);
       out.w = vol;
     }
   }

@@ -162,6 +162,11 @@ LaunchCtx make_launch(pbf_ctx *c, float dt) {
   L.multi = c->scenes.size() > 1;
   L.hash = c->params.grid_mode == PBF_GRID_HASH;
   L.field = c->any_field;
+  L.fmode = 0;
+  for (auto &s : c->scenes) {
+    if (s.field_kind == PBF_FIELD_CUBIC) L.fmode = 1;
+    else if (s.field_kind == PBF_FIELD_TRILINEAR && L.fmode == 0) L.fmode = 2;
+  }
   L.s0 = c->hscenes.empty() ? Scene{} : c->hscenes[0];
   L.scenes = c->d_scenes;
   L.sid = c->sid;
@@ -928,6 +933,35 @@ int pbf_pass_times(pbf_ctx *c, double *ms, uint32_t *launches, uint32_t capacity
   }
   if (count) *count = P_NPASS;
   if (names) *names = kPassNames;
+  return PBF_OK;
+}
+
+int pbf_copy_bandwidth(size_t bytes, uint32_t iterations, double *gbps) {
+  pbf_ctx *c = nullptr;
+  if (bytes < 16 || iterations == 0 || !gbps) return PBF_E_INVALID;
+  bytes &= ~(size_t)15;
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  HIPCHK(hipMalloc(&a, bytes));
+  HIPCHK(hipMalloc(&b, bytes));
+  HIPCHK(hipMemsetAsync(a, 1, bytes, st));
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  launch_copy_probe(st, a, b, bytes);  // warm-up
+  HIPCHK(hipEventRecord(e0, st));
+  for (uint32_t k = 0; k < iterations; ++k) launch_copy_probe(st, a, b, bytes);
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *gbps = 2.0 * (double)bytes * iterations / (ms * 1e-3) / 1e9;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  (void)hipStreamDestroy(st);
   return PBF_OK;
 }
 

@@ -14,6 +14,7 @@ struct LaunchCtx {
   bool multi;   // more than one scene: per-slot scene lookup
   bool hash;    // spatial-hash grid
   bool field;   // at least one scene has a boundary field
+  int fmode;    // 0 no field, 1 sample in a separate pass (cubic present), 2 all trilinear
   Scene s0;     // the scene when !multi
   const Scene *scenes;
   const uint32_t *sid;
@@ -48,6 +49,7 @@ void launch_unsort4(hipStream_t st, uint32_t n, const float4 *src, const uint32_
 void launch_unsort1(hipStream_t st, uint32_t n, const float *src, const uint32_t *ids, float4 *dst);
 void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v);
 void launch_iota(hipStream_t st, uint32_t *p, uint32_t n);
+void launch_copy_probe(hipStream_t st, const void *src, void *dst, size_t bytes);
 
 // Host utilities (pbf_host.cpp).
 uint64_t splitmix64(uint64_t x);

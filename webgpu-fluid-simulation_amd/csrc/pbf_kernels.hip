@@ -19,6 +19,8 @@
 //   diag          FP64 density-constraint error (optional)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "pbf_device.h"
 #include "pbf_internal.h"
 
@@ -82,10 +84,10 @@ __global__ void __launch_bounds__(BS) k_predict_bin(uint32_t n, const float4 *__
   if (active) {
     const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
     float4 x = pos[i], v = vel[i], a = acc[i];
-    float vx = v.x + P.dt * (P.gx + a.x);
-    float vy = v.y + P.dt * (P.gy + a.y);
-    float vz = v.z + P.dt * (P.gz + a.z);
-    float px = x.x + P.dt * vx, py = x.y + P.dt * vy, pz = x.z + P.dt * vz;
+    float vx = fmaf(P.dt, P.gx + a.x, v.x);
+    float vy = fmaf(P.dt, P.gy + a.y, v.y);
+    float vz = fmaf(P.dt, P.gz + a.z, v.z);
+    float px = fmaf(P.dt, vx, x.x), py = fmaf(P.dt, vy, x.y), pz = fmaf(P.dt, vz, x.z);
     clamp_rand(sc, px, py, pz);
     xs[i] = make_float4(px, py, pz, 0.0f);
     key = sc.cell_base + cell_local(sc, cell_axis(sc, px, 0), cell_axis(sc, py, 1), cell_axis(sc, pz, 2));
@@ -249,10 +251,10 @@ __global__ void __launch_bounds__(BS) k_build_list(uint32_t n, const float4 *__r
     const uint32_t base = sc.cell_base;
     uint32_t *row = nbr + i;
     auto visit = [&](uint32_t t0, uint32_t t1) {
+#pragma unroll 4
       for (uint32_t j = t0; j < t1; ++j) {
         const float4 xj = X0[j];
-        float ex = xj.x - xi.x, ey = xj.y - xi.y, ez = xj.z - xi.z;
-        float d2 = (ex * ex + ey * ey) + ez * ez;
+        float d2 = dist2(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
         if (d2 <= P.h2) {
           if (cnt < P.maxn) row[(size_t)cnt * P.ld] = j;
           ++cnt;
@@ -298,8 +300,23 @@ __global__ void __launch_bounds__(BS) k_build_list(uint32_t n, const float4 *__r
 }
 
 // ---------------------------------------------------------------------------
-// K8 + K9: boundary sample and lambda.  lambda goes to X[i].w.
-template <bool MULTI, bool FIELD>
+// K8: boundary sample of every particle at the current iterate (fields of
+// any kind, the cubic 32-node element included).
+template <bool MULTI>
+__global__ void __launch_bounds__(BS) k_boundary(uint32_t n, const float4 *__restrict__ X, float4 *__restrict__ B,
+                                                 const float *__restrict__ pool, Scene s0,
+                                                 const Scene *__restrict__ scenes, const uint32_t *__restrict__ sid,
+                                                 Phys P) {
+  const uint32_t i = slot_index();
+  if (i >= n) return;
+  const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
+  const float4 xi = X[i];
+  B[i] = boundary_sample<0>(P, sc, pool, xi.x, xi.y, xi.z);
+}
+
+// K9: density constraint and lambda, lambda to X[i].w.  FK: 0 no boundary,
+// 1 boundary samples already in B (k_boundary), 2 trilinear maps sampled here.
+template <bool MULTI, int FK>
 __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__restrict__ B,
                                                const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ ncount,
                                                const float *__restrict__ pool, Scene s0,
@@ -309,43 +326,46 @@ __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__
   if (i >= n) return;
   const float4 xi = X[i];
   float4 b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if constexpr (FIELD) {
+  if constexpr (FK == 2) {
     const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
-    b = boundary_sample(P, sc, pool, xi.x, xi.y, xi.z);
+    b = boundary_sample<2>(P, sc, pool, xi.x, xi.y, xi.z);
     B[i] = b;
+  } else if constexpr (FK == 1) {
+    b = B[i];
   }
   const uint32_t cnt = ncount[i];
   const uint32_t *row = nbr + i;
   float gx = 0.0f, gy = 0.0f, gz = 0.0f, s2 = 0.0f, rho = 0.0f;
+#pragma unroll 4
   for (uint32_t k = 0; k < cnt; ++k) {
     const uint32_t j = row[(size_t)k * P.ld];
     const float4 xj = X[j];
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-    float d2 = (dx * dx + dy * dy) + dz * dz;
+    float d2 = dist2(dx, dy, dz);
     float f = spiky_f(P, d2);
     float ax = f * dx, ay = f * dy, az = f * dz;
     gx += ax; gy += ay; gz += az;
-    s2 += (ax * ax + ay * ay) + az * az;
+    s2 = fmaf(az, az, fmaf(ay, ay, fmaf(ax, ax, s2)));
     rho += poly6(P, d2);
   }
   rho *= P.V;
   gx *= P.V; gy *= P.V; gz *= P.V;
   s2 *= P.V2;
-  if (FIELD && b.w > 0.0f) {
+  if (FK && b.w > 0.0f) {
     float dx = xi.x - b.x, dy = xi.y - b.y, dz = xi.z - b.z;
-    float d2 = (dx * dx + dy * dy) + dz * dz;
-    float f = spiky_f(P, d2);
-    gx += b.w * (f * dx); gy += b.w * (f * dy); gz += b.w * (f * dz);
-    rho += b.w * poly6(P, d2);
+    float d2 = dist2(dx, dy, dz);
+    float fb = b.w * spiky_f(P, d2);
+    gx = fmaf(fb, dx, gx); gy = fmaf(fb, dy, gy); gz = fmaf(fb, dz, gz);
+    rho = fmaf(b.w, poly6(P, d2), rho);
   }
   float c = rho - 1.0f;
   float C = c > 0.0f ? c : 0.0f;
-  float den = ((gx * gx + gy * gy) + gz * gz) + s2;
+  float den = fmaf(gz, gz, fmaf(gy, gy, fmaf(gx, gx, s2)));
   reinterpret_cast<float *>(X)[4 * (size_t)i + 3] = (-C) / (den + P.eps_lambda);
 }
 
 // K10 + K11: position correction and wall clamp.
-template <bool MULTI, bool FIELD>
+template <bool MULTI, int FK>
 __global__ void __launch_bounds__(BS) k_dp(uint32_t n, const float4 *__restrict__ X, const float4 *__restrict__ B,
                                            const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ ncount,
                                            float4 *__restrict__ Xn, Scene s0, const Scene *__restrict__ scenes,
@@ -357,28 +377,25 @@ __global__ void __launch_bounds__(BS) k_dp(uint32_t n, const float4 *__restrict_
   const uint32_t cnt = ncount[i];
   const uint32_t *row = nbr + i;
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
+#pragma unroll 4
   for (uint32_t k = 0; k < cnt; ++k) {
     const uint32_t j = row[(size_t)k * P.ld];
     const float4 xj = X[j];
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-    float d2 = (dx * dx + dy * dy) + dz * dz;
-    float f = spiky_f(P, d2);
-    float cl = li + xj.w;
-    ax += cl * (f * dx); ay += cl * (f * dy); az += cl * (f * dz);
+    float fl = (li + xj.w) * spiky_f(P, dist2(dx, dy, dz));
+    ax = fmaf(fl, dx, ax); ay = fmaf(fl, dy, ay); az = fmaf(fl, dz, az);
   }
   ax *= P.V; ay *= P.V; az *= P.V;
-  if constexpr (FIELD) {
+  if constexpr (FK != 0) {
     const float4 b = B[i];
     if (b.w > 0.0f) {
       float dx = xi.x - b.x, dy = xi.y - b.y, dz = xi.z - b.z;
-      float d2 = (dx * dx + dy * dy) + dz * dz;
-      float f = spiky_f(P, d2);
-      float cb = b.w * li;
-      ax += cb * (f * dx); ay += cb * (f * dy); az += cb * (f * dz);
+      float fb = (b.w * li) * spiky_f(P, dist2(dx, dy, dz));
+      ax = fmaf(fb, dx, ax); ay = fmaf(fb, dy, ay); az = fmaf(fb, dz, az);
     }
   }
   const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
-  float px = xi.x + 0.5f * ax, py = xi.y + 0.5f * ay, pz = xi.z + 0.5f * az;
+  float px = fmaf(0.5f, ax, xi.x), py = fmaf(0.5f, ay, xi.y), pz = fmaf(0.5f, az, xi.z);
   clamp_rand(sc, px, py, pz);
   Xn[i] = make_float4(px, py, pz, li);
 }
@@ -395,11 +412,10 @@ __global__ void __launch_bounds__(BS) k_density(uint32_t n, const float4 *__rest
   const uint32_t cnt = ncount[i];
   const uint32_t *row = nbr + i;
   float rho = 0.0f;
+#pragma unroll 4
   for (uint32_t k = 0; k < cnt; ++k) {
     const float4 xj = X[row[(size_t)k * P.ld]];
-    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-    float d2 = (dx * dx + dy * dy) + dz * dz;
-    rho += poly6(P, d2);
+    rho += poly6(P, dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z));
   }
   rho *= P.m;
   const float4 xo = Xold[i];
@@ -419,20 +435,20 @@ __global__ void __launch_bounds__(BS) k_vorticity(uint32_t n, const float4 *__re
   const uint32_t cnt = ncount[i];
   const uint32_t *row = nbr + i;
   float wx = 0.0f, wy = 0.0f, wz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
+#pragma unroll 2
   for (uint32_t k = 0; k < cnt; ++k) {
     const uint32_t j = row[(size_t)k * P.ld];
     const float4 xj = pos[j], vj = vel[j];
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-    float d2 = (dx * dx + dy * dy) + dz * dz;
     float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
     float vol = P.m / xj.w;
-    float f = spiky_f(P, d2);
+    float f = spiky_f(P, dist2(dx, dy, dz));
     float gx = f * dx, gy = f * dy, gz = f * dz;
-    float cx = uy * gz - uz * gy, cy = uz * gx - ux * gz, cz = ux * gy - uy * gx;
-    wx += vol * cx; wy += vol * cy; wz += vol * cz;
-    nx += vol * gx; ny += vol * gy; nz += vol * gz;
+    float cx = fmaf(uy, gz, -(uz * gy)), cy = fmaf(uz, gx, -(ux * gz)), cz = fmaf(ux, gy, -(uy * gx));
+    wx = fmaf(vol, cx, wx); wy = fmaf(vol, cy, wy); wz = fmaf(vol, cz, wz);
+    nx = fmaf(vol, gx, nx); ny = fmaf(vol, gy, ny); nz = fmaf(vol, gz, nz);
   }
-  om[i] = make_float4(wx, wy, wz, sqrtf((wx * wx + wy * wy) + wz * wz));
+  om[i] = make_float4(wx, wy, wz, sqrtf(dist2(wx, wy, wz)));
   nrm[i] = make_float4(P.h * nx, P.h * ny, P.h * nz, 0.0f);
 }
 
@@ -448,35 +464,41 @@ __global__ void __launch_bounds__(BS) k_xsph(uint32_t n, const float4 *__restric
   const uint32_t cnt = ncount[i];
   const uint32_t *row = nbr + i;
   float ux = 0.0f, uy = 0.0f, uz = 0.0f, ex = 0.0f, ey = 0.0f, ez = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
+#pragma unroll 2
   for (uint32_t k = 0; k < cnt; ++k) {
     const uint32_t j = row[(size_t)k * P.ld];
     const float4 xj = pos[j], vj = vel[j], nj = nrm[j];
     const float wj = om[j].w;
     const float rhoj = xj.w;
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-    float d2 = (dx * dx + dy * dy) + dz * dz;
+    float d2 = dist2(dx, dy, dz);
     float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
+    float irj = 1.0f / rhoj;
     float K = P.two_rho0 / (rhoi + rhoj);
-    float W = poly6(P, d2);
-    ux += (vx * W) / rhoj; uy += (vy * W) / rhoj; uz += (vz * W) / rhoj;
-    float f = spiky_f(P, d2);
-    ex += (wj * (f * dx)) / rhoj; ey += (wj * (f * dy)) / rhoj; ez += (wj * (f * dz)) / rhoj;
-    float fc = cohesion_f(P, d2);
-    tx -= K * (P.m * (fc * dx) + (ni.x - nj.x));
-    ty -= K * (P.m * (fc * dy) + (ni.y - nj.y));
-    tz -= K * (P.m * (fc * dz) + (ni.z - nj.z));
+    float sw = poly6(P, d2) * irj;
+    ux = fmaf(vx, sw, ux); uy = fmaf(vy, sw, uy); uz = fmaf(vz, sw, uz);
+    float se = (wj * spiky_f(P, d2)) * irj;
+    ex = fmaf(se, dx, ex); ey = fmaf(se, dy, ey); ez = fmaf(se, dz, ez);
+    float mc = P.m * cohesion_f(P, d2);
+    tx = fmaf(-K, fmaf(mc, dx, ni.x - nj.x), tx);
+    ty = fmaf(-K, fmaf(mc, dy, ni.y - nj.y), ty);
+    tz = fmaf(-K, fmaf(mc, dz, ni.z - nj.z), tz);
   }
-  float el = sqrtf((ex * ex + ey * ey) + ez * ez);
+  float el = sqrtf(dist2(ex, ey, ez));
   float hx = 0.0f, hy = 0.0f, hz = 0.0f;
-  if (el > 0.0f) { hx = ex / el; hy = ey / el; hz = ez / el; }
-  float cx = hy * wi.z - hz * wi.y, cy = hz * wi.x - hx * wi.z, cz = hx * wi.y - hy * wi.x;
-  acc[i] = make_float4((P.xsph_k * ux + P.vort * cx) + P.tension * tx, (P.xsph_k * uy + P.vort * cy) + P.tension * ty,
-                       (P.xsph_k * uz + P.vort * cz) + P.tension * tz, 0.0f);
+  if (el > 0.0f) {
+    float ie = 1.0f / el;
+    hx = ex * ie; hy = ey * ie; hz = ez * ie;
+  }
+  float cx = fmaf(hy, wi.z, -(hz * wi.y)), cy = fmaf(hz, wi.x, -(hx * wi.z)), cz = fmaf(hx, wi.y, -(hy * wi.x));
+  acc[i] = make_float4(fmaf(P.tension, tx, fmaf(P.vort, cx, P.xsph_k * ux)),
+                       fmaf(P.tension, ty, fmaf(P.vort, cy, P.xsph_k * uy)),
+                       fmaf(P.tension, tz, fmaf(P.vort, cz, P.xsph_k * uz)), 0.0f);
 }
 
 // ---------------------------------------------------------------------------
 // FP64 density-constraint error per particle, then one block per scene.
-template <bool MULTI, bool FIELD>
+template <bool MULTI, int FK>
 __global__ void __launch_bounds__(BS) k_diag(uint32_t n, const float4 *__restrict__ pos,
                                              const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ ncount,
                                              const float *__restrict__ pool, Scene s0,
@@ -498,9 +520,9 @@ __global__ void __launch_bounds__(BS) k_diag(uint32_t n, const float4 *__restric
     }
   }
   rho *= P.V64;
-  if constexpr (FIELD) {
+  if constexpr (FK != 0) {
     const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
-    float4 b = boundary_sample(P, sc, pool, xi.x, xi.y, xi.z);
+    float4 b = boundary_sample<FK == 2 ? 2 : 0>(P, sc, pool, xi.x, xi.y, xi.z);
     if (b.w > 0.0f) {
       double dx = (double)xi.x - (double)b.x, dy = (double)xi.y - (double)b.y, dz = (double)xi.z - (double)b.z;
       double d2 = (dx * dx + dy * dy) + dz * dz;
@@ -574,6 +596,13 @@ __global__ void __launch_bounds__(BS) k_fill_u32(uint32_t *p, uint32_t n, uint32
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+// Plain streaming copy (16 B per lane, grid-stride): the bandwidth probe and
+// the scale reference for hardware byte counters.
+__global__ void __launch_bounds__(BS) k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
 __global__ void __launch_bounds__(BS) k_iota(uint32_t *p, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = i;
@@ -635,29 +664,30 @@ void launch_build_list(const LaunchCtx &L, const float4 *X0, const uint32_t *off
   }
 }
 
+#define FK_DISPATCH(KERNEL, MULTI, G, ...)                                             \
+  do {                                                                                  \
+    if (L.fmode == 2) LAUNCH((KERNEL<MULTI, 2>), G, L.stream, __VA_ARGS__);             \
+    else if (L.fmode == 1) LAUNCH((KERNEL<MULTI, 1>), G, L.stream, __VA_ARGS__);        \
+    else LAUNCH((KERNEL<MULTI, 0>), G, L.stream, __VA_ARGS__);                          \
+  } while (0)
+
 void launch_lambda(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr, const uint32_t *ncount) {
   if (!L.n) return;
   const uint32_t g = nblocks(L.n, BS);
-  if (L.multi) {
-    if (L.field) LAUNCH((k_lambda<true, true>), g, L.stream, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P);
-    else LAUNCH((k_lambda<true, false>), g, L.stream, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P);
-  } else {
-    if (L.field) LAUNCH((k_lambda<false, true>), g, L.stream, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P);
-    else LAUNCH((k_lambda<false, false>), g, L.stream, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P);
+  if (L.fmode == 1) {
+    if (L.multi) LAUNCH(k_boundary<true>, g, L.stream, L.n, X, B, L.pool, L.s0, L.scenes, L.sid, L.P);
+    else LAUNCH(k_boundary<false>, g, L.stream, L.n, X, B, L.pool, L.s0, L.scenes, L.sid, L.P);
   }
+  if (L.multi) FK_DISPATCH(k_lambda, true, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P);
+  else FK_DISPATCH(k_lambda, false, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P);
 }
 
 void launch_dp(const LaunchCtx &L, const float4 *X, const float4 *B, const uint32_t *nbr, const uint32_t *ncount,
                float4 *Xn) {
   if (!L.n) return;
   const uint32_t g = nblocks(L.n, BS);
-  if (L.multi) {
-    if (L.field) LAUNCH((k_dp<true, true>), g, L.stream, L.n, X, B, nbr, ncount, Xn, L.s0, L.scenes, L.sid, L.P);
-    else LAUNCH((k_dp<true, false>), g, L.stream, L.n, X, B, nbr, ncount, Xn, L.s0, L.scenes, L.sid, L.P);
-  } else {
-    if (L.field) LAUNCH((k_dp<false, true>), g, L.stream, L.n, X, B, nbr, ncount, Xn, L.s0, L.scenes, L.sid, L.P);
-    else LAUNCH((k_dp<false, false>), g, L.stream, L.n, X, B, nbr, ncount, Xn, L.s0, L.scenes, L.sid, L.P);
-  }
+  if (L.multi) FK_DISPATCH(k_dp, true, g, L.n, X, B, nbr, ncount, Xn, L.s0, L.scenes, L.sid, L.P);
+  else FK_DISPATCH(k_dp, false, g, L.n, X, B, nbr, ncount, Xn, L.s0, L.scenes, L.sid, L.P);
 }
 
 void launch_density(const LaunchCtx &L, const float4 *X, const float4 *Xold, const uint32_t *nbr,
@@ -683,13 +713,8 @@ void launch_diag(const LaunchCtx &L, const float4 *pos, const uint32_t *nbr, con
                  uint32_t *hist_ctr) {
   if (!L.n) return;
   const uint32_t g = nblocks(L.n, BS);
-  if (L.multi) {
-    if (L.field) LAUNCH((k_diag<true, true>), g, L.stream, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
-    else LAUNCH((k_diag<true, false>), g, L.stream, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
-  } else {
-    if (L.field) LAUNCH((k_diag<false, true>), g, L.stream, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
-    else LAUNCH((k_diag<false, false>), g, L.stream, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
-  }
+  if (L.multi) FK_DISPATCH(k_diag, true, g, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
+  else FK_DISPATCH(k_diag, false, g, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
   hipLaunchKernelGGL(k_diag_reduce, dim3(nscenes), dim3(1024), 0, L.stream, ci, ranges, per_scene);
   if (hist) hipLaunchKernelGGL(k_diag_history, dim3(1), dim3(64), 0, L.stream, per_scene, nscenes, L.n, hist, hist_cap, hist_ctr);
 }
@@ -702,6 +727,11 @@ void launch_unsort1(hipStream_t st, uint32_t n, const float *src, const uint32_t
 }
 void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v) {
   if (n) hipLaunchKernelGGL(k_fill_u32, dim3(nblocks(n, BS)), dim3(BS), 0, st, p, n, v);
+}
+void launch_copy_probe(hipStream_t st, const void *src, void *dst, size_t bytes) {
+  size_t n = bytes / 16;
+  uint32_t grid = (uint32_t)std::min<size_t>((n + BS - 1) / BS, 256u * 64u);
+  if (n) hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(BS), 0, st, (const uint4 *)src, (uint4 *)dst, n);
 }
 void launch_iota(hipStream_t st, uint32_t *p, uint32_t n) {
   if (n) hipLaunchKernelGGL(k_iota, dim3(nblocks(n, BS)), dim3(BS), 0, st, p, n);

@@ -99,6 +99,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "pbf_particle_count": (U32, [P]), "pbf_scene_count": (U32, [P]),
         "pbf_last_error": (C.c_char_p, [P]), "pbf_version": (C.c_char_p, []),
         "pbf_pass_times": (I, [P, C.POINTER(D), pu, U32, pu, C.POINTER(C.c_char_p)]),
+        "pbf_copy_bandwidth": (I, [S, U32, C.POINTER(D)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -155,6 +156,16 @@ def parse_cdm(text) -> tuple:
     if rc != PBF_OK:
         raise PBFError(rc, "Invalid Boundary Model File!")
     return tuple(res), sdf, vol
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, iterations: int = 20) -> float:
+    """Device-to-device streaming copy rate (read + write GB/s)."""
+    lib = load_library()
+    g = C.c_double()
+    rc = lib.pbf_copy_bandwidth(int(nbytes), int(iterations), C.byref(g))
+    if rc != PBF_OK:
+        raise PBFError(rc, "pbf_copy_bandwidth failed")
+    return g.value
 
 
 class PBF:
