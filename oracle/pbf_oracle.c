@@ -728,7 +728,7 @@ void orc_substep(orc_sim *s, float dt, int iterations) {
       const float *xj = s->x + 4 * j, *vj = s->v + 4 * j;
       float dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
       float ux = vj[0] - vi[0], uy = vj[1] - vi[1], uz = vj[2] - vi[2];
-      float vol = s->m / xj[3];
+      float vol = s->m * (1.0f / xj[3]);
       float f = k_spiky_f(s, dist2(dx, dy, dz));
       float gx = f * dx, gy = f * dy, gz = f * dz;
       float cx = fmaf(uy, gz, -(uz * gy)), cy = fmaf(uz, gx, -(ux * gz)), cz = fmaf(ux, gy, -(uy * gx));

@@ -437,8 +437,9 @@ int create_ctx(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc 
   c->ncells = (uint32_t)total_cells;
   c->ncap = ((p.capacity + 255u) / 256u) * 256u;
   if (c->ncap == 0) c->ncap = 256;
-  c->ld = c->ncap;
-  c->maxn = p.max_neighbors;
+  // particle-major neighbour rows, 64-byte aligned
+  c->maxn = ((p.max_neighbors + 15u) / 16u) * 16u;
+  c->ld = c->maxn;
   hipError_t e = hipGetDevice(&c->device);
   if (e != hipSuccess) { int r = fail(c, PBF_E_DEVICE, std::string("hipGetDevice: ") + hipGetErrorString(e)); delete c; return r; }
   int rc = PBF_OK;
@@ -453,7 +454,7 @@ int create_ctx(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc 
   step(dalloc(&c->stage4, N), "alloc");
   step(dalloc(&c->ids, N), "alloc"); step(dalloc(&c->cellkey, N), "alloc"); step(dalloc(&c->rank, N), "alloc");
   step(dalloc(&c->tmp_key, N), "alloc"); step(dalloc(&c->tmp_src, N), "alloc"); step(dalloc(&c->ncount, N), "alloc");
-  step(dalloc(&c->nbr, (size_t)c->maxn * c->ld), "alloc");
+  step(dalloc(&c->nbr, (size_t)c->ncap * c->ld), "alloc");
   step(dalloc(&c->lam, N), "alloc"); step(dalloc(&c->lam_sorted, N), "alloc");
   step(dalloc(&c->counts, (size_t)c->ncells), "alloc");
   step(dalloc(&c->offsets, (size_t)c->ncells + 1), "alloc");
@@ -551,11 +552,11 @@ int upload_positions(pbf_ctx *c, const float *xyzw, uint32_t count) {
   return upload_scenes(c);
 }
 
-int read_sorted4(pbf_ctx *c, const float4 *src, const uint32_t *ids, float *out, uint32_t count) {
+int read_sorted4(pbf_ctx *c, const float4 *src, const uint32_t *ids, float *out, uint32_t count, int clear_w = 0) {
   if (count != c->n) return fail(c, PBF_E_LENGTH, "readback count must equal the particle count");
   int rc = sync_ctx(c);
   if (rc != PBF_OK) return rc;
-  launch_unsort4(c->stream, c->n, src, ids, c->stage4);
+  launch_unsort4(c->stream, c->n, src, ids, c->stage4, clear_w);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, c->stage4, (size_t)c->n * 16, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -760,7 +761,7 @@ int pbf_read_positions(pbf_ctx *c, float *xyzw, uint32_t count) {
 
 int pbf_read_velocities(pbf_ctx *c, float *xyzw, uint32_t count) {
   if (!c || (!xyzw && count)) return PBF_E_INVALID;
-  return read_sorted4(c, c->vel, c->ids, xyzw, count);
+  return read_sorted4(c, c->vel, c->ids, xyzw, count, 1);
 }
 
 int pbf_read_field(pbf_ctx *c, uint32_t field, float *out4, uint32_t count) {
@@ -768,10 +769,10 @@ int pbf_read_field(pbf_ctx *c, uint32_t field, float *out4, uint32_t count) {
   const uint32_t *ids = c->last_was_search ? c->dbg_ids : c->ids;
   switch (field) {
     case PBF_FIELD_POSITION: return read_sorted4(c, c->pos, c->ids, out4, count);
-    case PBF_FIELD_VELOCITY: return read_sorted4(c, c->vel, c->ids, out4, count);
+    case PBF_FIELD_VELOCITY: return read_sorted4(c, c->vel, c->ids, out4, count, 1);
     case PBF_FIELD_ACCEL: return read_sorted4(c, c->acc, c->ids, out4, count);
     case PBF_FIELD_OMEGA: return read_sorted4(c, c->om, c->ids, out4, count);
-    case PBF_FIELD_NORMAL: return read_sorted4(c, c->nrm, c->ids, out4, count);
+    case PBF_FIELD_NORMAL: return read_sorted4(c, c->nrm, c->ids, out4, count, 1);
     case PBF_FIELD_BOUNDARY: return read_sorted4(c, c->B, c->ids, out4, count);
     case PBF_FIELD_PREDICTED: return read_sorted4(c, c->XA, ids, out4, count);
     case PBF_FIELD_LAMBDA: {
@@ -838,7 +839,7 @@ int pbf_read_neighbors(pbf_ctx *c, uint32_t *nbr_off, uint32_t *nbr_ids, uint64_
   }
   uint32_t mx = 0;
   for (uint32_t i = 0; i < n; ++i) mx = std::max(mx, cnt[i]);
-  std::vector<uint32_t> table((size_t)mx * c->ld);
+  std::vector<uint32_t> table((size_t)n * c->ld);
   if (mx) HIPCHK(hipMemcpy(table.data(), c->nbr, table.size() * 4, hipMemcpyDeviceToHost));
   std::vector<uint32_t> by_id(n ? n : 1), slot_of(n ? n : 1);
   for (uint32_t i = 0; i < n; ++i) { by_id[ids[i]] = cnt[i]; slot_of[ids[i]] = i; }
@@ -854,7 +855,7 @@ int pbf_read_neighbors(pbf_ctx *c, uint32_t *nbr_off, uint32_t *nbr_ids, uint64_
     uint64_t w = 0;
     for (uint32_t q = 0; q < n; ++q) {
       uint32_t s = slot_of[q];
-      for (uint32_t k = 0; k < cnt[s]; ++k) nbr_ids[w++] = ids[table[(size_t)k * c->ld + s]];
+      for (uint32_t k = 0; k < cnt[s]; ++k) nbr_ids[w++] = ids[table[(size_t)s * c->ld + k]];
     }
   }
   return PBF_OK;

@@ -45,7 +45,7 @@ void launch_xsph(const LaunchCtx &L, const float4 *pos, const float4 *vel, const
 void launch_diag(const LaunchCtx &L, const float4 *pos, const uint32_t *nbr, const uint32_t *ncount, double *ci,
                  const uint32_t *ranges, uint32_t nscenes, double *per_scene, double *hist, uint32_t hist_cap,
                  uint32_t *hist_ctr);
-void launch_unsort4(hipStream_t st, uint32_t n, const float4 *src, const uint32_t *ids, float4 *dst);
+void launch_unsort4(hipStream_t st, uint32_t n, const float4 *src, const uint32_t *ids, float4 *dst, int clear_w);
 void launch_unsort1(hipStream_t st, uint32_t n, const float *src, const uint32_t *ids, float4 *dst);
 void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v);
 void launch_iota(hipStream_t st, uint32_t *p, uint32_t n);

@@ -39,6 +39,25 @@ __device__ __forceinline__ uint32_t slot_index() {
   return xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
 }
 
+// Neighbour rows are particle-major: row i holds the slots of particle i's
+// neighbours in walk order, 16-byte aligned, read four at a time.
+template <typename F>
+__device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ nbr, uint32_t i, uint32_t stride,
+                                                  uint32_t cnt, F &&f) {
+  const uint4 *row = reinterpret_cast<const uint4 *>(nbr + (size_t)i * stride);
+  uint32_t k = 0;
+  for (; k + 4 <= cnt; k += 4) {
+    const uint4 q = row[k >> 2];
+    f(q.x); f(q.y); f(q.z); f(q.w);
+  }
+  if (k < cnt) {
+    const uint4 q = row[k >> 2];
+    f(q.x);
+    if (k + 1 < cnt) f(q.y);
+    if (k + 2 < cnt) f(q.z);
+  }
+}
+
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -249,14 +268,22 @@ __global__ void __launch_bounds__(BS) k_build_list(uint32_t n, const float4 *__r
     const int cx = cell_axis(sc, xi.x, 0), cy = cell_axis(sc, xi.y, 1), cz = cell_axis(sc, xi.z, 2);
     const int D0 = sc.dims[0], D1 = sc.dims[1], D2 = sc.dims[2];
     const uint32_t base = sc.cell_base;
-    uint32_t *row = nbr + i;
+    uint4 *row = reinterpret_cast<uint4 *>(nbr + (size_t)i * P.ld);
+    uint4 acc = make_uint4(0u, 0u, 0u, 0u);
     auto visit = [&](uint32_t t0, uint32_t t1) {
 #pragma unroll 4
       for (uint32_t j = t0; j < t1; ++j) {
         const float4 xj = X0[j];
         float d2 = dist2(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
         if (d2 <= P.h2) {
-          if (cnt < P.maxn) row[(size_t)cnt * P.ld] = j;
+          if (cnt < P.maxn) {
+            const uint32_t q = cnt & 3u;
+            acc.x = q == 0 ? j : acc.x;
+            acc.y = q == 1 ? j : acc.y;
+            acc.z = q == 2 ? j : acc.z;
+            acc.w = q == 3 ? j : acc.w;
+            if (q == 3) row[cnt >> 2] = acc;
+          }
           ++cnt;
         }
       }
@@ -290,6 +317,7 @@ __global__ void __launch_bounds__(BS) k_build_list(uint32_t n, const float4 *__r
             visit(offsets[k], offsets[k + 1]);
           }
     }
+    if ((cnt & 3u) && cnt < P.maxn) row[cnt >> 2] = acc;
     ncount[i] = cnt < P.maxn ? cnt : P.maxn;
   }
   uint32_t wm = wave_max_u32(cnt);
@@ -333,12 +361,8 @@ __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__
   } else if constexpr (FK == 1) {
     b = B[i];
   }
-  const uint32_t cnt = ncount[i];
-  const uint32_t *row = nbr + i;
   float gx = 0.0f, gy = 0.0f, gz = 0.0f, s2 = 0.0f, rho = 0.0f;
-#pragma unroll 4
-  for (uint32_t k = 0; k < cnt; ++k) {
-    const uint32_t j = row[(size_t)k * P.ld];
+  for_each_neighbor(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
     const float4 xj = X[j];
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float d2 = dist2(dx, dy, dz);
@@ -347,7 +371,7 @@ __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__
     gx += ax; gy += ay; gz += az;
     s2 = fmaf(az, az, fmaf(ay, ay, fmaf(ax, ax, s2)));
     rho += poly6(P, d2);
-  }
+  });
   rho *= P.V;
   gx *= P.V; gy *= P.V; gz *= P.V;
   s2 *= P.V2;
@@ -374,17 +398,13 @@ __global__ void __launch_bounds__(BS) k_dp(uint32_t n, const float4 *__restrict_
   if (i >= n) return;
   const float4 xi = X[i];
   const float li = xi.w;
-  const uint32_t cnt = ncount[i];
-  const uint32_t *row = nbr + i;
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-#pragma unroll 4
-  for (uint32_t k = 0; k < cnt; ++k) {
-    const uint32_t j = row[(size_t)k * P.ld];
+  for_each_neighbor(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
     const float4 xj = X[j];
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float fl = (li + xj.w) * spiky_f(P, dist2(dx, dy, dz));
     ax = fmaf(fl, dx, ax); ay = fmaf(fl, dy, ay); az = fmaf(fl, dz, az);
-  }
+  });
   ax *= P.V; ay *= P.V; az *= P.V;
   if constexpr (FK != 0) {
     const float4 b = B[i];
@@ -409,17 +429,15 @@ __global__ void __launch_bounds__(BS) k_density(uint32_t n, const float4 *__rest
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = X[i];
-  const uint32_t cnt = ncount[i];
-  const uint32_t *row = nbr + i;
   float rho = 0.0f;
-#pragma unroll 4
-  for (uint32_t k = 0; k < cnt; ++k) {
-    const float4 xj = X[row[(size_t)k * P.ld]];
+  for_each_neighbor(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
+    const float4 xj = X[j];
     rho += poly6(P, dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z));
-  }
+  });
   rho *= P.m;
   const float4 xo = Xold[i];
-  vel[i] = make_float4((xi.x - xo.x) * P.inv_dt, (xi.y - xo.y) * P.inv_dt, (xi.z - xo.z) * P.inv_dt, 0.0f);
+  // w carries 1/rho for the vorticity and XSPH passes
+  vel[i] = make_float4((xi.x - xo.x) * P.inv_dt, (xi.y - xo.y) * P.inv_dt, (xi.z - xo.z) * P.inv_dt, 1.0f / rho);
   pos[i] = make_float4(xi.x, xi.y, xi.z, rho);
   lam[i] = xi.w;
 }
@@ -432,24 +450,22 @@ __global__ void __launch_bounds__(BS) k_vorticity(uint32_t n, const float4 *__re
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = pos[i], vi = vel[i];
-  const uint32_t cnt = ncount[i];
-  const uint32_t *row = nbr + i;
   float wx = 0.0f, wy = 0.0f, wz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
-#pragma unroll 2
-  for (uint32_t k = 0; k < cnt; ++k) {
-    const uint32_t j = row[(size_t)k * P.ld];
+  for_each_neighbor(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
     const float4 xj = pos[j], vj = vel[j];
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
-    float vol = P.m / xj.w;
+    float vol = P.m * vj.w;
     float f = spiky_f(P, dist2(dx, dy, dz));
     float gx = f * dx, gy = f * dy, gz = f * dz;
     float cx = fmaf(uy, gz, -(uz * gy)), cy = fmaf(uz, gx, -(ux * gz)), cz = fmaf(ux, gy, -(uy * gx));
     wx = fmaf(vol, cx, wx); wy = fmaf(vol, cy, wy); wz = fmaf(vol, cz, wz);
     nx = fmaf(vol, gx, nx); ny = fmaf(vol, gy, ny); nz = fmaf(vol, gz, nz);
-  }
-  om[i] = make_float4(wx, wy, wz, sqrtf(dist2(wx, wy, wz)));
-  nrm[i] = make_float4(P.h * nx, P.h * ny, P.h * nz, 0.0f);
+  });
+  const float wl = sqrtf(dist2(wx, wy, wz));
+  om[i] = make_float4(wx, wy, wz, wl);
+  // w carries |omega| so XSPH gathers one float4 less per neighbour
+  nrm[i] = make_float4(P.h * nx, P.h * ny, P.h * nz, wl);
 }
 
 // K14: XSPH viscosity, vorticity confinement force and surface tension.
@@ -461,19 +477,15 @@ __global__ void __launch_bounds__(BS) k_xsph(uint32_t n, const float4 *__restric
   if (i >= n) return;
   const float4 xi = pos[i], vi = vel[i], ni = nrm[i], wi = om[i];
   const float rhoi = xi.w;
-  const uint32_t cnt = ncount[i];
-  const uint32_t *row = nbr + i;
   float ux = 0.0f, uy = 0.0f, uz = 0.0f, ex = 0.0f, ey = 0.0f, ez = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
-#pragma unroll 2
-  for (uint32_t k = 0; k < cnt; ++k) {
-    const uint32_t j = row[(size_t)k * P.ld];
+  for_each_neighbor(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
     const float4 xj = pos[j], vj = vel[j], nj = nrm[j];
-    const float wj = om[j].w;
+    const float wj = nj.w;
     const float rhoj = xj.w;
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float d2 = dist2(dx, dy, dz);
     float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
-    float irj = 1.0f / rhoj;
+    float irj = vj.w;
     float K = P.two_rho0 / (rhoi + rhoj);
     float sw = poly6(P, d2) * irj;
     ux = fmaf(vx, sw, ux); uy = fmaf(vy, sw, uy); uz = fmaf(vz, sw, uz);
@@ -483,7 +495,7 @@ __global__ void __launch_bounds__(BS) k_xsph(uint32_t n, const float4 *__restric
     tx = fmaf(-K, fmaf(mc, dx, ni.x - nj.x), tx);
     ty = fmaf(-K, fmaf(mc, dy, ni.y - nj.y), ty);
     tz = fmaf(-K, fmaf(mc, dz, ni.z - nj.z), tz);
-  }
+  });
   float el = sqrtf(dist2(ex, ey, ez));
   float hx = 0.0f, hy = 0.0f, hz = 0.0f;
   if (el > 0.0f) {
@@ -507,18 +519,16 @@ __global__ void __launch_bounds__(BS) k_diag(uint32_t n, const float4 *__restric
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = pos[i];
-  const uint32_t cnt = ncount[i];
-  const uint32_t *row = nbr + i;
   double rho = 0.0;
-  for (uint32_t k = 0; k < cnt; ++k) {
-    const float4 xj = pos[row[(size_t)k * P.ld]];
+  for_each_neighbor(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
+    const float4 xj = pos[j];
     double dx = (double)xi.x - (double)xj.x, dy = (double)xi.y - (double)xj.y, dz = (double)xi.z - (double)xj.z;
     double d2 = (dx * dx + dy * dy) + dz * dz;
     if (d2 <= P.h2_64) {
       double t = P.h2_64 - d2;
       rho += P.Kp64 * ((t * t) * t);
     }
-  }
+  });
   rho *= P.V64;
   if constexpr (FK != 0) {
     const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
@@ -583,9 +593,14 @@ __global__ void k_diag_history(const double *__restrict__ per_scene, uint32_t ns
 // ---------------------------------------------------------------------------
 // Utilities.
 __global__ void __launch_bounds__(BS) k_unsort4(uint32_t n, const float4 *__restrict__ src,
-                                                const uint32_t *__restrict__ ids, float4 *__restrict__ dst) {
+                                                const uint32_t *__restrict__ ids, float4 *__restrict__ dst,
+                                                int clear_w) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[ids[i]] = src[i];
+  if (i < n) {
+    float4 v = src[i];
+    if (clear_w) v.w = 0.0f;
+    dst[ids[i]] = v;
+  }
 }
 __global__ void __launch_bounds__(BS) k_unsort1(uint32_t n, const float *__restrict__ src,
                                                 const uint32_t *__restrict__ ids, float4 *__restrict__ dst) {
@@ -719,8 +734,8 @@ void launch_diag(const LaunchCtx &L, const float4 *pos, const uint32_t *nbr, con
   if (hist) hipLaunchKernelGGL(k_diag_history, dim3(1), dim3(64), 0, L.stream, per_scene, nscenes, L.n, hist, hist_cap, hist_ctr);
 }
 
-void launch_unsort4(hipStream_t st, uint32_t n, const float4 *src, const uint32_t *ids, float4 *dst) {
-  if (n) hipLaunchKernelGGL(k_unsort4, dim3(nblocks(n, BS)), dim3(BS), 0, st, n, src, ids, dst);
+void launch_unsort4(hipStream_t st, uint32_t n, const float4 *src, const uint32_t *ids, float4 *dst, int clear_w) {
+  if (n) hipLaunchKernelGGL(k_unsort4, dim3(nblocks(n, BS)), dim3(BS), 0, st, n, src, ids, dst, clear_w);
 }
 void launch_unsort1(hipStream_t st, uint32_t n, const float *src, const uint32_t *ids, float4 *dst) {
   if (n) hipLaunchKernelGGL(k_unsort1, dim3(nblocks(n, BS)), dim3(BS), 0, st, n, src, ids, dst);
