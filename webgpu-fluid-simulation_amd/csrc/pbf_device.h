@@ -22,7 +22,7 @@ struct Phys {
   float gx, gy, gz;
   float xsph_k;  // (xsph * inv_dt) * m
   float vort, tension;
-  uint32_t ld;   // row stride of the particle-major neighbour table (multiple of 16)
+  uint32_t ld;   // leading dimension of the slot-major neighbour table
   uint32_t maxn; // neighbour capacity per particle
   double Kp64, h2_64, V64;
 };

@@ -437,9 +437,9 @@ int create_ctx(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc 
   c->ncells = (uint32_t)total_cells;
   c->ncap = ((p.capacity + 255u) / 256u) * 256u;
   if (c->ncap == 0) c->ncap = 256;
-  // particle-major neighbour rows, 64-byte aligned
-  c->maxn = ((p.max_neighbors + 15u) / 16u) * 16u;
-  c->ld = c->maxn;
+  // slot-major neighbour table: entry k of slot i at nbr[k * ld + i]
+  c->maxn = p.max_neighbors;
+  c->ld = c->ncap;
   hipError_t e = hipGetDevice(&c->device);
   if (e != hipSuccess) { int r = fail(c, PBF_E_DEVICE, std::string("hipGetDevice: ") + hipGetErrorString(e)); delete c; return r; }
   int rc = PBF_OK;
@@ -454,7 +454,7 @@ int create_ctx(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc 
   step(dalloc(&c->stage4, N), "alloc");
   step(dalloc(&c->ids, N), "alloc"); step(dalloc(&c->cellkey, N), "alloc"); step(dalloc(&c->rank, N), "alloc");
   step(dalloc(&c->tmp_key, N), "alloc"); step(dalloc(&c->tmp_src, N), "alloc"); step(dalloc(&c->ncount, N), "alloc");
-  step(dalloc(&c->nbr, (size_t)c->ncap * c->ld), "alloc");
+  step(dalloc(&c->nbr, (size_t)c->maxn * c->ld), "alloc");
   step(dalloc(&c->lam, N), "alloc"); step(dalloc(&c->lam_sorted, N), "alloc");
   step(dalloc(&c->counts, (size_t)c->ncells), "alloc");
   step(dalloc(&c->offsets, (size_t)c->ncells + 1), "alloc");
@@ -839,7 +839,7 @@ int pbf_read_neighbors(pbf_ctx *c, uint32_t *nbr_off, uint32_t *nbr_ids, uint64_
   }
   uint32_t mx = 0;
   for (uint32_t i = 0; i < n; ++i) mx = std::max(mx, cnt[i]);
-  std::vector<uint32_t> table((size_t)n * c->ld);
+  std::vector<uint32_t> table((size_t)mx * c->ld);
   if (mx) HIPCHK(hipMemcpy(table.data(), c->nbr, table.size() * 4, hipMemcpyDeviceToHost));
   std::vector<uint32_t> by_id(n ? n : 1), slot_of(n ? n : 1);
   for (uint32_t i = 0; i < n; ++i) { by_id[ids[i]] = cnt[i]; slot_of[ids[i]] = i; }
@@ -855,7 +855,7 @@ int pbf_read_neighbors(pbf_ctx *c, uint32_t *nbr_off, uint32_t *nbr_ids, uint64_
     uint64_t w = 0;
     for (uint32_t q = 0; q < n; ++q) {
       uint32_t s = slot_of[q];
-      for (uint32_t k = 0; k < cnt[s]; ++k) nbr_ids[w++] = ids[table[(size_t)s * c->ld + k]];
+      for (uint32_t k = 0; k < cnt[s]; ++k) nbr_ids[w++] = ids[table[(size_t)k * c->ld + s]];
     }
   }
   return PBF_OK;

@@ -39,23 +39,15 @@ __device__ __forceinline__ uint32_t slot_index() {
   return xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
 }
 
-// Neighbour rows are particle-major: row i holds the slots of particle i's
-// neighbours in walk order, 16-byte aligned, read four at a time.
-template <typename F>
-__device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ nbr, uint32_t i, uint32_t stride,
+// The neighbour table is slot-major: entry k of particle i sits at
+// nbr[k * ld + i], so one load instruction of a wavefront reads 256
+// contiguous bytes.  Visits the neighbours in walk order.
+template <int UNROLL = 4, typename F>
+__device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ nbr, uint32_t i, uint32_t ld,
                                                   uint32_t cnt, F &&f) {
-  const uint4 *row = reinterpret_cast<const uint4 *>(nbr + (size_t)i * stride);
-  uint32_t k = 0;
-  for (; k + 4 <= cnt; k += 4) {
-    const uint4 q = row[k >> 2];
-    f(q.x); f(q.y); f(q.z); f(q.w);
-  }
-  if (k < cnt) {
-    const uint4 q = row[k >> 2];
-    f(q.x);
-    if (k + 1 < cnt) f(q.y);
-    if (k + 2 < cnt) f(q.z);
-  }
+  const uint32_t *row = nbr + i;
+#pragma unroll UNROLL
+  for (uint32_t k = 0; k < cnt; ++k) f(row[(size_t)k * ld]);
 }
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
@@ -254,36 +246,41 @@ __global__ void __launch_bounds__(BS) k_settle(uint32_t n, const uint32_t *__res
 // ---------------------------------------------------------------------------
 // K5-K7: neighbour list over the 27 cells of the frozen predicted position.
 // Dense grid: the three cells (x, y, z-1..z+1) are one contiguous slot range.
+// The first LIST_R entries of each particle are staged in LDS and written as
+// whole table rows (coalesced) once the block is done; later entries are
+// stored directly.
+constexpr int LIST_BS = 128;
+constexpr int LIST_R = 32;
+
 template <bool MULTI, bool HASH>
-__global__ void __launch_bounds__(BS) k_build_list(uint32_t n, const float4 *__restrict__ X0,
-                                                   const uint32_t *__restrict__ offsets, uint32_t *__restrict__ nbr,
-                                                   uint32_t *__restrict__ ncount, uint32_t *__restrict__ flags,
-                                                   Scene s0, const Scene *__restrict__ scenes,
-                                                   const uint32_t *__restrict__ sid, Phys P) {
-  const uint32_t i = slot_index();
+__global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4 *__restrict__ X0,
+                                                        const uint32_t *__restrict__ offsets,
+                                                        uint32_t *__restrict__ nbr, uint32_t *__restrict__ ncount,
+                                                        uint32_t *__restrict__ flags, Scene s0,
+                                                        const Scene *__restrict__ scenes,
+                                                        const uint32_t *__restrict__ sid, Phys P) {
+  __shared__ uint32_t tile[LIST_R * LIST_BS];
+  __shared__ uint32_t block_max;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t base_slot = xcd_block(blockIdx.x, gridDim.x) * LIST_BS;
+  const uint32_t i = base_slot + tid;
+  if (tid == 0) block_max = 0;
+  __syncthreads();
   uint32_t cnt = 0;
   if (i < n) {
     const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
     const float4 xi = X0[i];
     const int cx = cell_axis(sc, xi.x, 0), cy = cell_axis(sc, xi.y, 1), cz = cell_axis(sc, xi.z, 2);
     const int D0 = sc.dims[0], D1 = sc.dims[1], D2 = sc.dims[2];
-    const uint32_t base = sc.cell_base;
-    uint4 *row = reinterpret_cast<uint4 *>(nbr + (size_t)i * P.ld);
-    uint4 acc = make_uint4(0u, 0u, 0u, 0u);
+    const uint32_t cbase = sc.cell_base;
     auto visit = [&](uint32_t t0, uint32_t t1) {
 #pragma unroll 4
       for (uint32_t j = t0; j < t1; ++j) {
         const float4 xj = X0[j];
         float d2 = dist2(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
         if (d2 <= P.h2) {
-          if (cnt < P.maxn) {
-            const uint32_t q = cnt & 3u;
-            acc.x = q == 0 ? j : acc.x;
-            acc.y = q == 1 ? j : acc.y;
-            acc.z = q == 2 ? j : acc.z;
-            acc.w = q == 3 ? j : acc.w;
-            if (q == 3) row[cnt >> 2] = acc;
-          }
+          if (cnt < (uint32_t)LIST_R) tile[cnt * LIST_BS + tid] = j;
+          else if (cnt < P.maxn) nbr[(size_t)cnt * P.ld + i] = j;
           ++cnt;
         }
       }
@@ -296,8 +293,8 @@ __global__ void __launch_bounds__(BS) k_build_list(uint32_t n, const float4 *__r
         for (int dy = -1; dy <= 1; ++dy) {
           const int ny = cy + dy;
           if (ny < 0 || ny >= D1) continue;
-          uint32_t c0 = base + cell_local(sc, nx, ny, zlo);
-          uint32_t c1 = base + cell_local(sc, nx, ny, zhi);
+          uint32_t c0 = cbase + cell_local(sc, nx, ny, zlo);
+          uint32_t c1 = cbase + cell_local(sc, nx, ny, zhi);
           visit(offsets[c0], offsets[c1 + 1]);
         }
       }
@@ -309,7 +306,7 @@ __global__ void __launch_bounds__(BS) k_build_list(uint32_t n, const float4 *__r
           for (int dz = -1; dz <= 1; ++dz) {
             const int nx = cx + dx, ny = cy + dy, nz = cz + dz;
             if (nx < 0 || ny < 0 || nz < 0 || nx >= D0 || ny >= D1 || nz >= D2) continue;
-            const uint32_t k = base + cell_local(sc, nx, ny, nz);
+            const uint32_t k = cbase + cell_local(sc, nx, ny, nz);
             bool dup = false;
             for (int q = 0; q < nseen; ++q) dup |= (seen[q] == k);
             if (dup) continue;
@@ -317,14 +314,21 @@ __global__ void __launch_bounds__(BS) k_build_list(uint32_t n, const float4 *__r
             visit(offsets[k], offsets[k + 1]);
           }
     }
-    if ((cnt & 3u) && cnt < P.maxn) row[cnt >> 2] = acc;
     ncount[i] = cnt < P.maxn ? cnt : P.maxn;
   }
-  uint32_t wm = wave_max_u32(cnt);
-  if (__lane_id() == 0 && wm > 0) {
-    atomicMax(&flags[1], wm);
-    if (wm > P.maxn) atomicOr(&flags[0], 1u);
+  const uint32_t wm = wave_max_u32(cnt);
+  if (__lane_id() == 0) {
+    atomicMax(&block_max, wm);
+    if (wm > 0) {
+      atomicMax(&flags[1], wm);
+      if (wm > P.maxn) atomicOr(&flags[0], 1u);
+    }
   }
+  __syncthreads();
+  uint32_t rows = block_max < (uint32_t)LIST_R ? block_max : (uint32_t)LIST_R;
+  rows = rows < P.maxn ? rows : P.maxn;
+  if (i < n)
+    for (uint32_t k = 0; k < rows; ++k) nbr[(size_t)k * P.ld + i] = tile[k * LIST_BS + tid];
 }
 
 // ---------------------------------------------------------------------------
@@ -451,7 +455,7 @@ __global__ void __launch_bounds__(BS) k_vorticity(uint32_t n, const float4 *__re
   if (i >= n) return;
   const float4 xi = pos[i], vi = vel[i];
   float wx = 0.0f, wy = 0.0f, wz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
-  for_each_neighbor(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
+  for_each_neighbor<2>(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
     const float4 xj = pos[j], vj = vel[j];
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
@@ -478,7 +482,7 @@ __global__ void __launch_bounds__(BS) k_xsph(uint32_t n, const float4 *__restric
   const float4 xi = pos[i], vi = vel[i], ni = nrm[i], wi = om[i];
   const float rhoi = xi.w;
   float ux = 0.0f, uy = 0.0f, uz = 0.0f, ex = 0.0f, ey = 0.0f, ez = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
-  for_each_neighbor(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
+  for_each_neighbor<2>(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
     const float4 xj = pos[j], vj = vel[j], nj = nrm[j];
     const float wj = nj.w;
     const float rhoj = xj.w;
@@ -669,13 +673,13 @@ void launch_settle(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *
 void launch_build_list(const LaunchCtx &L, const float4 *X0, const uint32_t *offsets, uint32_t *nbr, uint32_t *ncount,
                        uint32_t *flags) {
   if (!L.n) return;
-  const uint32_t g = nblocks(L.n, BS);
+  const dim3 g(nblocks(L.n, LIST_BS)), b(LIST_BS);
   if (L.multi) {
-    if (L.hash) LAUNCH((k_build_list<true, true>), g, L.stream, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
-    else LAUNCH((k_build_list<true, false>), g, L.stream, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
+    if (L.hash) hipLaunchKernelGGL((k_build_list<true, true>), g, b, 0, L.stream, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
+    else hipLaunchKernelGGL((k_build_list<true, false>), g, b, 0, L.stream, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
   } else {
-    if (L.hash) LAUNCH((k_build_list<false, true>), g, L.stream, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
-    else LAUNCH((k_build_list<false, false>), g, L.stream, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
+    if (L.hash) hipLaunchKernelGGL((k_build_list<false, true>), g, b, 0, L.stream, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
+    else hipLaunchKernelGGL((k_build_list<false, false>), g, b, 0, L.stream, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
   }
 }
 
