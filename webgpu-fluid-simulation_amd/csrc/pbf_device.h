@@ -96,25 +96,28 @@ __device__ __forceinline__ void clamp_rand(const Scene &sc, float &px, float &py
 }
 
 // ---------------------------------------------------------------------------
-// SPH kernels as functions of d2 = |x_ij|^2.
+// SPH kernels as functions of d2 = |x_ij|^2.  Written without branches
+// (select after computing) so a neighbour loop stays one basic block and its
+// gathers can be issued ahead of the arithmetic; the selected values are the
+// same as those of the guarded forms.
 __device__ __forceinline__ float poly6(const Phys &P, float d2) {
-  if (!(d2 <= P.h2)) return 0.0f;
   float t = P.h2 - d2;
-  return P.Kp * ((t * t) * t);
+  float w = P.Kp * ((t * t) * t);
+  return d2 <= P.h2 ? w : 0.0f;
 }
 __device__ __forceinline__ float spiky_f(const Phys &P, float d2) {
-  if (!(d2 > 0.0f && d2 <= P.h2)) return 0.0f;
   float r = sqrtf(d2);
   float q = P.h - r;
-  return (P.Ks * (q * q)) / r;
+  float f = (P.Ks * (q * q)) / r;
+  return (d2 > 0.0f && d2 <= P.h2) ? f : 0.0f;
 }
 __device__ __forceinline__ float cohesion_f(const Phys &P, float d2) {
-  if (!(d2 > 0.0f && d2 <= P.h2)) return 0.0f;
   float r = sqrtf(d2);
   float a = P.h - r;
   float t = ((a * a) * a) * ((r * r) * r);
   float res = (r > P.half_h) ? t : fmaf(2.0f, t, P.bias);
-  return (P.Kc * res) / r;
+  float f = (P.Kc * res) / r;
+  return (d2 > 0.0f && d2 <= P.h2) ? f : 0.0f;
 }
 
 __device__ __forceinline__ float dist2(float dx, float dy, float dz) { return fmaf(dz, dz, fmaf(dy, dy, dx * dx)); }

@@ -46,8 +46,20 @@ template <int UNROLL = 4, typename F>
 __device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ nbr, uint32_t i, uint32_t ld,
                                                   uint32_t cnt, F &&f) {
   const uint32_t *row = nbr + i;
-#pragma unroll UNROLL
-  for (uint32_t k = 0; k < cnt; ++k) f(row[(size_t)k * ld]);
+  uint32_t k = 0;
+  if constexpr (UNROLL >= 4) {
+    for (; k + 4 <= cnt; k += 4) {  // four index loads in flight, then four bodies
+      const uint32_t j0 = row[(size_t)k * ld], j1 = row[(size_t)(k + 1) * ld];
+      const uint32_t j2 = row[(size_t)(k + 2) * ld], j3 = row[(size_t)(k + 3) * ld];
+      f(j0); f(j1); f(j2); f(j3);
+    }
+  } else {
+    for (; k + 2 <= cnt; k += 2) {
+      const uint32_t j0 = row[(size_t)k * ld], j1 = row[(size_t)(k + 1) * ld];
+      f(j0); f(j1);
+    }
+  }
+  for (; k < cnt; ++k) f(row[(size_t)k * ld]);
 }
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
@@ -273,17 +285,21 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
     const int cx = cell_axis(sc, xi.x, 0), cy = cell_axis(sc, xi.y, 1), cz = cell_axis(sc, xi.z, 2);
     const int D0 = sc.dims[0], D1 = sc.dims[1], D2 = sc.dims[2];
     const uint32_t cbase = sc.cell_base;
-    auto visit = [&](uint32_t t0, uint32_t t1) {
-#pragma unroll 4
-      for (uint32_t j = t0; j < t1; ++j) {
-        const float4 xj = X0[j];
-        float d2 = dist2(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
-        if (d2 <= P.h2) {
-          if (cnt < (uint32_t)LIST_R) tile[cnt * LIST_BS + tid] = j;
-          else if (cnt < P.maxn) nbr[(size_t)cnt * P.ld + i] = j;
-          ++cnt;
-        }
+    auto test = [&](const float4 &xj, uint32_t j) {
+      float d2 = dist2(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
+      if (d2 <= P.h2) {
+        if (cnt < (uint32_t)LIST_R) tile[cnt * LIST_BS + tid] = j;
+        else if (cnt < P.maxn) nbr[(size_t)cnt * P.ld + i] = j;
+        ++cnt;
       }
+    };
+    auto visit = [&](uint32_t t0, uint32_t t1) {
+      uint32_t j = t0;
+      for (; j + 4 <= t1; j += 4) {  // four candidate loads in flight
+        const float4 a = X0[j], b = X0[j + 1], c = X0[j + 2], d = X0[j + 3];
+        test(a, j); test(b, j + 1); test(c, j + 2); test(d, j + 3);
+      }
+      for (; j < t1; ++j) test(X0[j], j);
     };
     if constexpr (!HASH) {
       const int zlo = cz > 0 ? cz - 1 : 0, zhi = cz + 1 < D2 ? cz + 1 : D2 - 1;
