@@ -85,6 +85,7 @@ def main():
     C = int(np.prod([int(np.ceil((sc.domain_hi[a] - sc.domain_lo[a]) / sc.kernel_radius - 1e-7))
                      for a in range(3)]))
     log(f"scene {sc.name}: N={N} cells={C} I={sc.iterations} substeps/step={sc.substeps}")
+    # per-pass HIP events ride on the kernels' own dispatch packets
     flags = 0 if args.no_timing else PBF_FLAG_TIMING
     sim = scenes.make_engine(sc, flags=flags)
     for _ in range(args.warmup):
@@ -97,6 +98,17 @@ def main():
     sim.sync()
     el = time.perf_counter() - t0
     passes = sim.pass_times()
+    # the same number of steps replayed from hipGraphs without pass events
+    sim_g = scenes.make_engine(sc)
+    for _ in range(args.warmup):
+        sim_g.step(sc.dt, sc.substeps, sc.iterations)
+    sim_g.sync()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        sim_g.step(sc.dt, sc.substeps, sc.iterations)
+    sim_g.sync()
+    el_graph = time.perf_counter() - t1
+    sim_g.close()
     mean_err, max_err = sim.density_error()
     pos = sim.read_positions()
     assert np.isfinite(pos).all(), "non-finite positions"
@@ -156,6 +168,7 @@ def main():
                    "grid_cells": C, "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "value_graph_replay_no_events": round(N * substeps / el_graph, 1),
         "whole_substep_algorithmic_GBs": round(whole, 1),
         "measured_copy_GBs": round(copy_gbs, 1),
         "density_error_after_run": {"mean": mean_err, "max": max_err},

@@ -151,7 +151,7 @@ Phys make_phys(const pbf_ctx *c, float dt) {
 }
 
 void clear_graphs(pbf_ctx *c) {
-  for (auto &g : c->graphs) hipGraphExecDestroy(g.exec);
+  for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
   c->graphs.clear();
 }
 
@@ -182,25 +182,33 @@ hipEvent_t take_event(pbf_ctx *c) {
     return e;
   }
   hipEvent_t e = nullptr;
-  hipEventCreate(&e);
+  if (hipEventCreate(&e) != hipSuccess) e = nullptr;
   return e;
 }
 
+// Per-pass timing: begin() returns the launch context with a start/stop
+// event pair attached to the pass's first/last dispatch packet.
 struct PassTimer {
   pbf_ctx *c;
   bool on;
   int pass = -1;
-  hipEvent_t a = nullptr;
-  void begin(int p) {
-    if (!on) return;
+  hipEvent_t a = nullptr, b = nullptr;
+  LaunchCtx begin(int p, const LaunchCtx &L) {
+    LaunchCtx t = L;
+    if (!on) return t;
     pass = p;
     a = take_event(c);
-    hipEventRecord(a, c->stream);
+    b = take_event(c);
+    if (!a || !b) {
+      pass = -1;
+      return t;
+    }
+    t.ev0 = a;
+    t.ev1 = b;
+    return t;
   }
   void end() {
     if (!on || pass < 0) return;
-    hipEvent_t b = take_event(c);
-    hipEventRecord(b, c->stream);
     c->pending.push_back({pass, a, b});
     pass = -1;
   }
@@ -221,22 +229,23 @@ void drain_events(pbf_ctx *c) {
 
 // Predict, bin, scan, reorder and neighbour lists.  dbg: keep cell counts.
 void enqueue_search(pbf_ctx *c, const LaunchCtx &L, PassTimer &T, bool dbg) {
-  T.begin(P_PREDICT);
-  launch_predict_bin(L, c->pos, c->vel, c->acc, c->XB, c->cellkey, c->rank, c->counts);
+  LaunchCtx t = T.begin(P_PREDICT, L);
+  launch_predict_bin(t, c->pos, c->vel, c->acc, c->XB, c->cellkey, c->rank, c->counts);
   T.end();
-  if (dbg) hipMemcpyAsync(c->dbg_counts, c->counts, (size_t)c->ncells * 4, hipMemcpyDeviceToDevice, c->stream);
-  T.begin(P_SCAN);
-  launch_scan(c->stream, c->counts, c->ncells, c->offsets, 1, c->scan_sums.data(), c->scan_prefix.data(), 0);
+  if (dbg) (void)hipMemcpyAsync(c->dbg_counts, c->counts, (size_t)c->ncells * 4, hipMemcpyDeviceToDevice, c->stream);
+  t = T.begin(P_SCAN, L);
+  launch_scan(c->stream, c->counts, c->ncells, c->offsets, 1, c->scan_sums.data(), c->scan_prefix.data(), 0, t.ev0,
+              t.ev1);
   T.end();
-  T.begin(P_SCATTER);
-  launch_scatter(L, c->cellkey, c->rank, c->offsets, c->ids, c->tmp_key, c->tmp_src);
+  t = T.begin(P_SCATTER, L);
+  launch_scatter(t, c->cellkey, c->rank, c->offsets, c->ids, c->tmp_key, c->tmp_src);
   T.end();
-  T.begin(P_SETTLE);
-  launch_settle(L, c->tmp_key, c->tmp_src, c->cellkey, c->offsets, c->XB, c->pos, c->lam, c->XA, c->Xold, c->ids,
+  t = T.begin(P_SETTLE, L);
+  launch_settle(t, c->tmp_key, c->tmp_src, c->cellkey, c->offsets, c->XB, c->pos, c->lam, c->XA, c->Xold, c->ids,
                 c->lam_sorted);
   T.end();
-  T.begin(P_LIST);
-  launch_build_list(L, c->XA, c->offsets, c->nbr, c->ncount, c->flags);
+  t = T.begin(P_LIST, L);
+  launch_build_list(t, c->XA, c->offsets, c->nbr, c->ncount, c->flags);
   T.end();
 }
 
@@ -245,26 +254,26 @@ void enqueue_substep(pbf_ctx *c, const LaunchCtx &L, uint32_t iters, bool diag, 
   enqueue_search(c, L, T, false);
   float4 *X = c->XA, *Xn = c->XB;
   for (uint32_t it = 0; it < iters; ++it) {
-    T.begin(P_LAMBDA);
-    launch_lambda(L, X, c->B, c->nbr, c->ncount);
+    LaunchCtx t = T.begin(P_LAMBDA, L);
+    launch_lambda(t, X, c->B, c->nbr, c->ncount);
     T.end();
-    T.begin(P_DP);
-    launch_dp(L, X, c->B, c->nbr, c->ncount, Xn);
+    t = T.begin(P_DP, L);
+    launch_dp(t, X, c->B, c->nbr, c->ncount, Xn);
     T.end();
     std::swap(X, Xn);
   }
-  T.begin(P_DENSITY);
-  launch_density(L, X, c->Xold, c->nbr, c->ncount, c->pos, c->vel, c->lam);
+  LaunchCtx t = T.begin(P_DENSITY, L);
+  launch_density(t, X, c->Xold, c->nbr, c->ncount, c->pos, c->vel, c->lam);
   T.end();
-  T.begin(P_VORT);
-  launch_vorticity(L, c->pos, c->vel, c->nbr, c->ncount, c->om, c->nrm);
+  t = T.begin(P_VORT, L);
+  launch_vorticity(t, c->pos, c->vel, c->nbr, c->ncount, c->om, c->nrm);
   T.end();
-  T.begin(P_XSPH);
-  launch_xsph(L, c->pos, c->vel, c->om, c->nrm, c->nbr, c->ncount, c->acc);
+  t = T.begin(P_XSPH, L);
+  launch_xsph(t, c->pos, c->vel, c->om, c->nrm, c->nbr, c->ncount, c->acc);
   T.end();
   if (diag) {
-    T.begin(P_DIAG);
-    launch_diag(L, c->pos, c->nbr, c->ncount, c->ci, c->ranges, (uint32_t)c->scenes.size(), c->per_scene, c->hist,
+    t = T.begin(P_DIAG, L);
+    launch_diag(t, c->pos, c->nbr, c->ncount, c->ci, c->ranges, (uint32_t)c->scenes.size(), c->per_scene, c->hist,
                 c->hist_cap, c->hist_ctr);
     T.end();
   }
@@ -297,7 +306,7 @@ int run_substep(pbf_ctx *c, float dt, uint32_t iters) {
     if (le != hipSuccess) return fail(c, PBF_E_DEVICE, std::string("launch during capture: ") + hipGetErrorString(le));
     if (ee != hipSuccess) return fail(c, PBF_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
     hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     if (ie != hipSuccess) return fail(c, PBF_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
     if (c->graphs.size() >= 16) clear_graphs(c);
     c->graphs.push_back({bits, iters, (uint32_t)c->record_diag, exec});
@@ -580,22 +589,22 @@ int pbf_batch_create(const pbf_params *params, uint32_t nscenes, const pbf_scene
 
 int pbf_destroy(pbf_ctx *c) {
   if (!c) return PBF_OK;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   clear_graphs(c);
   drain_events(c);
-  for (auto e : c->event_pool) hipEventDestroy(e);
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
   void *bufs[] = {c->pos, c->vel, c->acc, c->om, c->nrm, c->XA, c->XB, c->Xold, c->B, c->xs, c->stage4, c->ids,
                   c->cellkey, c->rank, c->tmp_key, c->tmp_src, c->ncount, c->nbr, c->lam, c->lam_sorted, c->counts,
                   c->offsets, c->flags, c->d_scenes, c->sid, c->ranges, c->pool, c->ci, c->per_scene, c->hist,
                   c->hist_ctr, c->dbg_counts, c->dbg_ids, c->ids_save};
   for (void *b : bufs)
-    if (b) hipFree(b);
+    if (b) (void)hipFree(b);
   for (auto p : c->scan_sums)
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   for (auto p : c->scan_prefix)
-    if (p) hipFree(p);
-  if (c->stream) hipStreamDestroy(c->stream);
+    if (p) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PBF_OK;
 }

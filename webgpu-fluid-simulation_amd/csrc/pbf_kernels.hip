@@ -17,6 +17,7 @@
 //   vorticity     omega and surface normal (K13)
 //   xsph          XSPH + vorticity force + surface tension (K14)
 //   diag          FP64 density-constraint error (optional)
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -644,46 +645,51 @@ __global__ void __launch_bounds__(BS) k_iota(uint32_t *p, uint32_t n) {
 }
 
 // ---------------------------------------------------------------------------
-// Launchers.
+// Launchers.  With timing events set, the first kernel of a pass carries the
+// start event and the last one the stop event on its own dispatch packet
+// (hipExtLaunchKernelGGL), so the interval is the kernels' execution time.
 static inline uint32_t nblocks(uint32_t n, uint32_t bs) { return (n + bs - 1) / bs; }
 
-#define LAUNCH(kernel, grid, stream, ...) hipLaunchKernelGGL(kernel, dim3(grid), dim3(BS), 0, stream, __VA_ARGS__)
+template <typename K, typename... Args>
+static void go(hipStream_t st, hipEvent_t e0, hipEvent_t e1, K kernel, dim3 g, dim3 b, Args... args) {
+  if (e0 || e1) hipExtLaunchKernelGGL(kernel, g, b, 0, st, e0, e1, 0, args...);
+  else hipLaunchKernelGGL(kernel, g, b, 0, st, args...);
+}
 
 void launch_predict_bin(const LaunchCtx &L, const float4 *pos, const float4 *vel, const float4 *acc, float4 *xs,
                         uint32_t *cellkey, uint32_t *rank, uint32_t *counts) {
   if (!L.n) return;
-  if (L.multi)
-    LAUNCH(k_predict_bin<true>, nblocks(L.n, BS), L.stream, L.n, pos, vel, acc, xs, cellkey, rank, counts, L.s0,
-           L.scenes, L.sid, L.P);
-  else
-    LAUNCH(k_predict_bin<false>, nblocks(L.n, BS), L.stream, L.n, pos, vel, acc, xs, cellkey, rank, counts, L.s0,
-           L.scenes, L.sid, L.P);
+  const dim3 g(nblocks(L.n, BS)), b(BS);
+  if (L.multi) go(L.stream, L.ev0, L.ev1, k_predict_bin<true>, g, b, L.n, pos, vel, acc, xs, cellkey, rank, counts, L.s0, L.scenes, L.sid, L.P);
+  else go(L.stream, L.ev0, L.ev1, k_predict_bin<false>, g, b, L.n, pos, vel, acc, xs, cellkey, rank, counts, L.s0, L.scenes, L.sid, L.P);
 }
 
 void launch_scan(hipStream_t st, uint32_t *in, uint32_t n, uint32_t *out, int zero_in, uint32_t *const *sums,
-                 uint32_t *const *prefix, int level) {
+                 uint32_t *const *prefix, int level, hipEvent_t e0, hipEvent_t e1) {
+  const dim3 b(BS);
   if (n <= SCAN_E) {
-    hipLaunchKernelGGL(k_scan_down, dim3(1), dim3(BS), 0, st, in, n, (const uint32_t *)nullptr, out, zero_in);
+    go(st, e0, e1, k_scan_down, dim3(1), b, in, n, (const uint32_t *)nullptr, out, zero_in);
     return;
   }
   uint32_t nb = (n + SCAN_E - 1) / SCAN_E;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(BS), 0, st, in, n, sums[level]);
-  launch_scan(st, sums[level], nb, prefix[level], 0, sums, prefix, level + 1);
-  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(BS), 0, st, in, n, (const uint32_t *)prefix[level], out, zero_in);
+  go(st, e0, nullptr, k_scan_reduce, dim3(nb), b, (const uint32_t *)in, n, sums[level]);
+  launch_scan(st, sums[level], nb, prefix[level], 0, sums, prefix, level + 1, nullptr, nullptr);
+  go(st, nullptr, e1, k_scan_down, dim3(nb), b, in, n, (const uint32_t *)prefix[level], out, zero_in);
 }
 
 void launch_scatter(const LaunchCtx &L, const uint32_t *cellkey, const uint32_t *rank, const uint32_t *offsets,
                     const uint32_t *ids, uint32_t *tmp_key, uint32_t *tmp_src) {
   if (!L.n) return;
-  LAUNCH(k_scatter, nblocks(L.n, BS), L.stream, L.n, cellkey, rank, offsets, ids, tmp_key, tmp_src);
+  go(L.stream, L.ev0, L.ev1, k_scatter, dim3(nblocks(L.n, BS)), dim3(BS), L.n, cellkey, rank, offsets, ids, tmp_key,
+     tmp_src);
 }
 
 void launch_settle(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *tmp_src, const uint32_t *cellkey,
                    const uint32_t *offsets, const float4 *xs, const float4 *pos, const float *lam_in, float4 *X0,
                    float4 *Xold, uint32_t *ids, float *lam_out) {
   if (!L.n) return;
-  LAUNCH(k_settle, nblocks(L.n, BS), L.stream, L.n, tmp_key, tmp_src, cellkey, offsets, xs, pos, lam_in, X0, Xold,
-         ids, lam_out);
+  go(L.stream, L.ev0, L.ev1, k_settle, dim3(nblocks(L.n, BS)), dim3(BS), L.n, tmp_key, tmp_src, cellkey, offsets, xs,
+     pos, lam_in, X0, Xold, ids, lam_out);
 }
 
 void launch_build_list(const LaunchCtx &L, const float4 *X0, const uint32_t *offsets, uint32_t *nbr, uint32_t *ncount,
@@ -691,67 +697,74 @@ void launch_build_list(const LaunchCtx &L, const float4 *X0, const uint32_t *off
   if (!L.n) return;
   const dim3 g(nblocks(L.n, LIST_BS)), b(LIST_BS);
   if (L.multi) {
-    if (L.hash) hipLaunchKernelGGL((k_build_list<true, true>), g, b, 0, L.stream, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
-    else hipLaunchKernelGGL((k_build_list<true, false>), g, b, 0, L.stream, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
+    if (L.hash) go(L.stream, L.ev0, L.ev1, k_build_list<true, true>, g, b, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
+    else go(L.stream, L.ev0, L.ev1, k_build_list<true, false>, g, b, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
   } else {
-    if (L.hash) hipLaunchKernelGGL((k_build_list<false, true>), g, b, 0, L.stream, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
-    else hipLaunchKernelGGL((k_build_list<false, false>), g, b, 0, L.stream, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
+    if (L.hash) go(L.stream, L.ev0, L.ev1, k_build_list<false, true>, g, b, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
+    else go(L.stream, L.ev0, L.ev1, k_build_list<false, false>, g, b, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
   }
 }
 
-#define FK_DISPATCH(KERNEL, MULTI, G, ...)                                             \
-  do {                                                                                  \
-    if (L.fmode == 2) LAUNCH((KERNEL<MULTI, 2>), G, L.stream, __VA_ARGS__);             \
-    else if (L.fmode == 1) LAUNCH((KERNEL<MULTI, 1>), G, L.stream, __VA_ARGS__);        \
-    else LAUNCH((KERNEL<MULTI, 0>), G, L.stream, __VA_ARGS__);                          \
-  } while (0)
+template <bool MULTI, template <bool, int> class Sel, typename... Args>
+static void fk_dispatch(const LaunchCtx &L, hipEvent_t e0, hipEvent_t e1, dim3 g, Args... args) {
+  if (L.fmode == 2) go(L.stream, e0, e1, Sel<MULTI, 2>::fn(), g, dim3(BS), args...);
+  else if (L.fmode == 1) go(L.stream, e0, e1, Sel<MULTI, 1>::fn(), g, dim3(BS), args...);
+  else go(L.stream, e0, e1, Sel<MULTI, 0>::fn(), g, dim3(BS), args...);
+}
+template <bool M, int F> struct SelLambda { static auto fn() { return k_lambda<M, F>; } };
+template <bool M, int F> struct SelDp { static auto fn() { return k_dp<M, F>; } };
+template <bool M, int F> struct SelDiag { static auto fn() { return k_diag<M, F>; } };
 
 void launch_lambda(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr, const uint32_t *ncount) {
   if (!L.n) return;
-  const uint32_t g = nblocks(L.n, BS);
+  const dim3 g(nblocks(L.n, BS));
+  hipEvent_t e0 = L.ev0;
   if (L.fmode == 1) {
-    if (L.multi) LAUNCH(k_boundary<true>, g, L.stream, L.n, X, B, L.pool, L.s0, L.scenes, L.sid, L.P);
-    else LAUNCH(k_boundary<false>, g, L.stream, L.n, X, B, L.pool, L.s0, L.scenes, L.sid, L.P);
+    if (L.multi) go(L.stream, e0, nullptr, k_boundary<true>, g, dim3(BS), L.n, (const float4 *)X, B, L.pool, L.s0, L.scenes, L.sid, L.P);
+    else go(L.stream, e0, nullptr, k_boundary<false>, g, dim3(BS), L.n, (const float4 *)X, B, L.pool, L.s0, L.scenes, L.sid, L.P);
+    e0 = nullptr;
   }
-  if (L.multi) FK_DISPATCH(k_lambda, true, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P);
-  else FK_DISPATCH(k_lambda, false, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P);
+  if (L.multi) fk_dispatch<true, SelLambda>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P);
+  else fk_dispatch<false, SelLambda>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P);
 }
 
 void launch_dp(const LaunchCtx &L, const float4 *X, const float4 *B, const uint32_t *nbr, const uint32_t *ncount,
                float4 *Xn) {
   if (!L.n) return;
-  const uint32_t g = nblocks(L.n, BS);
-  if (L.multi) FK_DISPATCH(k_dp, true, g, L.n, X, B, nbr, ncount, Xn, L.s0, L.scenes, L.sid, L.P);
-  else FK_DISPATCH(k_dp, false, g, L.n, X, B, nbr, ncount, Xn, L.s0, L.scenes, L.sid, L.P);
+  const dim3 g(nblocks(L.n, BS));
+  if (L.multi) fk_dispatch<true, SelDp>(L, L.ev0, L.ev1, g, L.n, X, B, nbr, ncount, Xn, L.s0, L.scenes, L.sid, L.P);
+  else fk_dispatch<false, SelDp>(L, L.ev0, L.ev1, g, L.n, X, B, nbr, ncount, Xn, L.s0, L.scenes, L.sid, L.P);
 }
 
 void launch_density(const LaunchCtx &L, const float4 *X, const float4 *Xold, const uint32_t *nbr,
                     const uint32_t *ncount, float4 *pos, float4 *vel, float *lam) {
   if (!L.n) return;
-  LAUNCH(k_density, nblocks(L.n, BS), L.stream, L.n, X, Xold, nbr, ncount, pos, vel, lam, L.P);
+  go(L.stream, L.ev0, L.ev1, k_density, dim3(nblocks(L.n, BS)), dim3(BS), L.n, X, Xold, nbr, ncount, pos, vel, lam, L.P);
 }
 
 void launch_vorticity(const LaunchCtx &L, const float4 *pos, const float4 *vel, const uint32_t *nbr,
                       const uint32_t *ncount, float4 *om, float4 *nrm) {
   if (!L.n) return;
-  LAUNCH(k_vorticity, nblocks(L.n, BS), L.stream, L.n, pos, vel, nbr, ncount, om, nrm, L.P);
+  go(L.stream, L.ev0, L.ev1, k_vorticity, dim3(nblocks(L.n, BS)), dim3(BS), L.n, pos, vel, nbr, ncount, om, nrm, L.P);
 }
 
 void launch_xsph(const LaunchCtx &L, const float4 *pos, const float4 *vel, const float4 *om, const float4 *nrm,
                  const uint32_t *nbr, const uint32_t *ncount, float4 *acc) {
   if (!L.n) return;
-  LAUNCH(k_xsph, nblocks(L.n, BS), L.stream, L.n, pos, vel, om, nrm, nbr, ncount, acc, L.P);
+  go(L.stream, L.ev0, L.ev1, k_xsph, dim3(nblocks(L.n, BS)), dim3(BS), L.n, pos, vel, om, nrm, nbr, ncount, acc, L.P);
 }
 
 void launch_diag(const LaunchCtx &L, const float4 *pos, const uint32_t *nbr, const uint32_t *ncount, double *ci,
                  const uint32_t *ranges, uint32_t nscenes, double *per_scene, double *hist, uint32_t hist_cap,
                  uint32_t *hist_ctr) {
   if (!L.n) return;
-  const uint32_t g = nblocks(L.n, BS);
-  if (L.multi) FK_DISPATCH(k_diag, true, g, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
-  else FK_DISPATCH(k_diag, false, g, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
-  hipLaunchKernelGGL(k_diag_reduce, dim3(nscenes), dim3(1024), 0, L.stream, ci, ranges, per_scene);
-  if (hist) hipLaunchKernelGGL(k_diag_history, dim3(1), dim3(64), 0, L.stream, per_scene, nscenes, L.n, hist, hist_cap, hist_ctr);
+  const dim3 g(nblocks(L.n, BS));
+  if (L.multi) fk_dispatch<true, SelDiag>(L, L.ev0, nullptr, g, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
+  else fk_dispatch<false, SelDiag>(L, L.ev0, nullptr, g, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
+  go(L.stream, nullptr, hist ? nullptr : L.ev1, k_diag_reduce, dim3(nscenes), dim3(1024), (const double *)ci, ranges, per_scene);
+  if (hist)
+    go(L.stream, nullptr, L.ev1, k_diag_history, dim3(1), dim3(64), (const double *)per_scene, nscenes, L.n, hist,
+       hist_cap, hist_ctr);
 }
 
 void launch_unsort4(hipStream_t st, uint32_t n, const float4 *src, const uint32_t *ids, float4 *dst, int clear_w) {
