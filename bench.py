@@ -73,6 +73,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-pass events (graph replay)")
+    ap.add_argument("--no-graph-compare", action="store_true", help="skip the extra graph-replay measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding).",
@@ -99,16 +100,18 @@ def main():
     el = time.perf_counter() - t0
     passes = sim.pass_times()
     # the same number of steps replayed from hipGraphs without pass events
-    sim_g = scenes.make_engine(sc)
-    for _ in range(args.warmup):
-        sim_g.step(sc.dt, sc.substeps, sc.iterations)
-    sim_g.sync()
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        sim_g.step(sc.dt, sc.substeps, sc.iterations)
-    sim_g.sync()
-    el_graph = time.perf_counter() - t1
-    sim_g.close()
+    el_graph = None
+    if not args.no_graph_compare and not args.no_timing:
+        sim_g = scenes.make_engine(sc)
+        for _ in range(args.warmup):
+            sim_g.step(sc.dt, sc.substeps, sc.iterations)
+        sim_g.sync()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            sim_g.step(sc.dt, sc.substeps, sc.iterations)
+        sim_g.sync()
+        el_graph = time.perf_counter() - t1
+        sim_g.close()
     mean_err, max_err = sim.density_error()
     pos = sim.read_positions()
     assert np.isfinite(pos).all(), "non-finite positions"
@@ -168,7 +171,7 @@ def main():
                    "grid_cells": C, "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "value_graph_replay_no_events": round(N * substeps / el_graph, 1),
+        "value_graph_replay_no_events": round(N * substeps / el_graph, 1) if el_graph else None,
         "whole_substep_algorithmic_GBs": round(whole, 1),
         "measured_copy_GBs": round(copy_gbs, 1),
         "density_error_after_run": {"mean": mean_err, "max": max_err},
