@@ -209,6 +209,62 @@ int pbf_pass_times(pbf_ctx *ctx, double *ms, uint32_t *launches, uint32_t capaci
  * kernel for scaling hardware byte counters. No reference counterpart. */
 int pbf_copy_bandwidth(size_t bytes, uint32_t iterations, double *gbps);
 
+/* ---- Learned warm start of the lambda solve ----------------------------
+ * No reference counterpart: the reference solves every substep from
+ * lambda = 0 with a fixed iteration count (PBF.ts:419-431).  A 16 -> 64 ->
+ * 64 -> 1 SiLU network (bf16 weights on MFMA, fp32 accumulation) predicts,
+ * from 16 per-particle features computed in iteration 0, the lambda summed
+ * over a long solve; that prediction is applied in iteration 0 and the
+ * remaining iterations are the plain Jacobi passes.
+ *
+ * Network blob: pbf_mlp_blob_size() floats,
+ *   W1[16][64] b1[64] W2[64][64] b2[64] w3[64] b3 fmu[16] fsig[16] tmu tsig
+ * (the first PBF_MLP_PARAMS are trained; features are normalised as
+ * (f - fmu) / fsig and the output is lambda = y * tsig + tmu).
+ * Features (iteration 0, per particle): 0 rho~-1, 1 C, 2 plain lambda,
+ * 3 previous substep's lambda sum, 4 previous substep's last lambda,
+ * 5 lambda denominator, 6 sum |grad_j C|^2, 7 |grad_i C|^2, 8 neighbour
+ * count, 9 boundary volume / V, 10 boundary distance / h, 11 velocity
+ * divergence * dt, 12 |v| dt / h, 13 v_y dt / h, 14 fluid part of rho~-1,
+ * 15 boundary part of rho~. */
+enum { PBF_MLP_PARAMS = 5313, PBF_MLP_FEATURES = 16 };
+enum { PBF_WARM_PLAIN = 0, PBF_WARM_LEARNED = 1 };
+
+uint32_t pbf_mlp_blob_size(void);
+/* Glorot-uniform weights from a SplitMix64 stream, zero biases, identity
+ * normalisation. */
+int pbf_mlp_init(uint64_t seed, float *blob);
+/* One forward + backward pass on the GPU over n samples: y (normalised
+ * output, n), grad (PBF_MLP_PARAMS, d loss / d parameter) and the loss
+ * mean((y - (target - tmu) / tsig)^2).  Any output pointer may be NULL. */
+int pbf_mlp_forward_backward(const float *blob, const float *features, const float *targets, uint32_t n, float *y,
+                             float *grad, float *loss);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) over shuffled mini-batches on the GPU;
+ * blob holds the initial network (and its normalisation) and receives the
+ * trained weights; loss_per_epoch (epochs floats) may be NULL. */
+int pbf_mlp_train(const float *features, const float *targets, uint64_t n, uint32_t epochs, uint32_t batch,
+                  float lr, uint64_t seed, float *blob, float *loss_per_epoch);
+/* Selects the solve: PBF_WARM_PLAIN, or PBF_WARM_LEARNED with the network
+ * in blob (NULL keeps the weights already set).  relax in (0, 1] scales the
+ * predicted lambda: the later iterations only remove compression, so an
+ * over-predicted lambda pushes particles apart for good (kinetic energy);
+ * relax < 1 keeps the warm start on the safe side.  fallback_threshold > 0
+ * checks each learned substep: when its mean density error is above the
+ * threshold (or not finite) the substep is undone and solved plainly with
+ * fallback_iterations (0: the same count). */
+int pbf_set_warm_start(pbf_ctx *ctx, const float *blob, size_t length, uint32_t mode, float relax,
+                       double fallback_threshold, uint32_t fallback_iterations);
+int pbf_warm_start_stats(const pbf_ctx *ctx, uint64_t *learned_substeps, uint64_t *fallbacks);
+/* Training pairs along the trajectory of the selected solver: from the
+ * current state, a solve of `long_iterations` plain Jacobi iterations gives,
+ * per particle (in the caller's order), the 16 features of iteration 0 and
+ * the lambda summed over all its iterations (the target); that solve is then
+ * undone and the state advances by one substep of the selected solver with
+ * `iterations` (0: the configured count), keeping the lambda sums that the
+ * next pair's features read. */
+int pbf_collect_training(pbf_ctx *ctx, float dt, uint32_t long_iterations, uint32_t iterations, float *features,
+                         float *targets, uint32_t capacity, uint32_t *count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -106,13 +106,31 @@ def test_volume_map_builder_box():
     kp = 315 / (64 * math.pi * h ** 9)
     for d in (0.0, 0.05):
         z = np.linspace(d, h, 20001)
-        slab = np.trapz(math.pi * kp * (h * h - z * z) ** 4 / 4, z)
+        slab = np.trapezoid(math.pi * kp * (h * h - z * z) ** 4 / 4, z)
         s = max(d + r / 2, 2 * r)
         want = 0.9 * slab / (kp * (h * h - s * s) ** 3)
         sel = (np.abs(box - d) < 1e-9) & (np.abs(nodes[:, 0] - 0.8) < 1e-9) & (np.abs(nodes[:, 2] - 0.8) < 1e-9)
         got = sv[sel & (nodes[:, 1] < 0.8), 1]
         assert got.size == 1
         assert got[0] == pytest.approx(want, rel=0.06)  # 16^3 midpoint quadrature
+
+
+def test_volume_map_builder_bounded_near_kernel_edge():
+    """V_b = 0.9 int W / W(d + r/2) has a pole at d = h - r/2; the builder caps
+    it at the deep-solid value 0.9 / W(2r).  Nodes 1.25 mm apart straddle the
+    pole on a column far from the side walls."""
+    h, r = 0.1, 0.025
+    res, lo, hi = (2, 1280, 2), (0.0, 0.0, 0.0), (1.6, 1.6, 1.6)
+    sv = pbf_host.build_volume_map(0, res, lo, hi, h, None, r)
+    kp = 315 / (64 * math.pi * h ** 9)
+    vcap = 0.9 / (kp * (h * h - 4 * r * r) ** 3)
+    assert np.all(np.isfinite(sv))
+    assert sv[:, 1].max() <= vcap * (1 + 1e-6)
+    col = sv.reshape(3, 1281, 3, 2)[1, :, 1]          # x = z = 0.8, y = 0 .. 1.6
+    d = col[:, 0]
+    near = (d > h - r / 2 - 0.004) & (d < h - r / 2)
+    assert near.sum() >= 2 and np.all(col[near, 1] <= vcap * (1 + 1e-6))
+    assert np.all(col[d >= h, 1] == 0.0) and np.all(col[d <= 0.05, 1] > 0.0)
 
 
 def test_volume_map_builder_obstacle_sdf():

@@ -23,9 +23,15 @@ using namespace pbf;
 namespace {
 
 enum Pass {
-  P_PREDICT, P_SCAN, P_SCATTER, P_SETTLE, P_LIST, P_LAMBDA, P_DP, P_DENSITY, P_VORT, P_XSPH, P_DIAG, P_NPASS
+  P_PREDICT, P_SCAN, P_SCATTER, P_SETTLE, P_LIST, P_LAMBDA, P_DP, P_DENSITY, P_VORT, P_XSPH, P_DIAG, P_MLP, P_NPASS
 };
-const char *kPassNames = "predict_bin\nscan\nscatter\nsettle\nbuild_list\nlambda\ndp\ndensity\nvorticity\nxsph\ndiag";
+const char *kPassNames =
+    "predict_bin\nscan\nscatter\nsettle\nbuild_list\nlambda\ndp\ndensity\nvorticity\nxsph\ndiag\nmlp_infer";
+
+// Substep kinds: the plain Jacobi solve, the learned warm start (network
+// lambda at iteration 0), and training-pair collection (features at
+// iteration 0, lambda summed over all iterations).
+enum SubstepMode : uint32_t { M_PLAIN = 0, M_LEARNED = 1, M_COLLECT = 2 };
 
 struct SceneHost {
   double lo[3], hi[3];
@@ -39,7 +45,7 @@ struct SceneHost {
 };
 
 struct GraphEntry {
-  uint32_t dt_bits, iters, diag;
+  uint32_t dt_bits, iters, diag, mode;
   hipGraphExec_t exec;
 };
 
@@ -91,6 +97,19 @@ struct pbf_ctx {
   std::vector<PendingEvent> pending;
   double pass_ms[P_NPASS] = {0};
   uint32_t pass_count[P_NPASS] = {0};
+  // learned warm start
+  void *mlp = nullptr;           // network weights on the device
+  uint32_t warm_mode = M_PLAIN;  // M_PLAIN or M_LEARNED
+  double fb_threshold = 0.0;     // mean density error that rejects a learned substep (0: never)
+  uint32_t fb_iters = 0;         // iterations of the plain re-solve
+  float relax_set = -1.0f;       // scale applied to the network's lambda
+  float *lam_acc = nullptr, *acc_sorted = nullptr, *feats = nullptr;
+  float4 *vel_sorted = nullptr;
+  // state saved before a learned substep that may be rejected
+  float4 *snap_pos = nullptr, *snap_vel = nullptr, *snap_acc = nullptr;
+  uint32_t *snap_ids = nullptr;
+  float *snap_lam = nullptr, *snap_lacc = nullptr;
+  uint64_t learned_substeps = 0, fallbacks = 0;
 };
 
 namespace {
@@ -228,7 +247,7 @@ void drain_events(pbf_ctx *c) {
 }
 
 // Predict, bin, scan, reorder and neighbour lists.  dbg: keep cell counts.
-void enqueue_search(pbf_ctx *c, const LaunchCtx &L, PassTimer &T, bool dbg) {
+void enqueue_search(pbf_ctx *c, const LaunchCtx &L, PassTimer &T, bool dbg, uint32_t mode = M_PLAIN) {
   LaunchCtx t = T.begin(P_PREDICT, L);
   launch_predict_bin(t, c->pos, c->vel, c->acc, c->XB, c->cellkey, c->rank, c->counts);
   T.end();
@@ -244,19 +263,29 @@ void enqueue_search(pbf_ctx *c, const LaunchCtx &L, PassTimer &T, bool dbg) {
   launch_settle(t, c->tmp_key, c->tmp_src, c->cellkey, c->offsets, c->XB, c->pos, c->lam, c->XA, c->Xold, c->ids,
                 c->lam_sorted);
   T.end();
+  if (mode != M_PLAIN)
+    launch_settle_extra(L, c->tmp_key, c->tmp_src, c->cellkey, c->offsets, c->lam_acc, c->acc_sorted, c->vel,
+                        c->vel_sorted);
   t = T.begin(P_LIST, L);
   launch_build_list(t, c->XA, c->offsets, c->nbr, c->ncount, c->flags);
   T.end();
 }
 
 // The launch sequence of one substep (PBF.ts:398-448).
-void enqueue_substep(pbf_ctx *c, const LaunchCtx &L, uint32_t iters, bool diag, PassTimer &T) {
-  enqueue_search(c, L, T, false);
+void enqueue_substep(pbf_ctx *c, const LaunchCtx &L, uint32_t iters, bool diag, PassTimer &T, uint32_t mode) {
+  enqueue_search(c, L, T, false, mode);
   float4 *X = c->XA, *Xn = c->XB;
+  LearnBufs lb{c->lam_acc, c->acc_sorted, c->lam_sorted, c->vel_sorted, c->feats};
   for (uint32_t it = 0; it < iters; ++it) {
     LaunchCtx t = T.begin(P_LAMBDA, L);
-    launch_lambda(t, X, c->B, c->nbr, c->ncount);
+    if (mode != M_PLAIN && it == 0) launch_lambda_features(t, X, c->B, c->nbr, c->ncount, lb);
+    else launch_lambda(t, X, c->B, c->nbr, c->ncount, mode != M_PLAIN ? c->lam_acc : nullptr);
     T.end();
+    if (mode == M_LEARNED && it == 0) {
+      t = T.begin(P_MLP, L);
+      launch_mlp_infer(c->stream, c->mlp, c->n, c->feats, X, c->lam_acc, t.ev0, t.ev1);
+      T.end();
+    }
     t = T.begin(P_DP, L);
     launch_dp(t, X, c->B, c->nbr, c->ncount, Xn);
     T.end();
@@ -279,7 +308,7 @@ void enqueue_substep(pbf_ctx *c, const LaunchCtx &L, uint32_t iters, bool diag, 
   }
 }
 
-int run_substep(pbf_ctx *c, float dt, uint32_t iters) {
+int run_substep(pbf_ctx *c, float dt, uint32_t iters, uint32_t mode, bool diag) {
   LaunchCtx L = make_launch(c, dt);
   const bool timing = (c->params.flags & PBF_FLAG_TIMING) != 0;
   const bool use_graph = !(c->params.flags & PBF_FLAG_NO_GRAPH) && !timing;
@@ -288,7 +317,7 @@ int run_substep(pbf_ctx *c, float dt, uint32_t iters) {
   c->have_dbg_counts = false;
   c->have_lists = true;
   if (!use_graph) {
-    enqueue_substep(c, L, iters, c->record_diag, T);
+    enqueue_substep(c, L, iters, diag, T, mode);
     HIPCHK(hipGetLastError());
     return PBF_OK;
   }
@@ -296,11 +325,11 @@ int run_substep(pbf_ctx *c, float dt, uint32_t iters) {
   std::memcpy(&bits, &dt, 4);
   hipGraphExec_t exec = nullptr;
   for (auto &g : c->graphs)
-    if (g.dt_bits == bits && g.iters == iters && g.diag == (uint32_t)c->record_diag) exec = g.exec;
+    if (g.dt_bits == bits && g.iters == iters && g.diag == (uint32_t)diag && g.mode == mode) exec = g.exec;
   if (!exec) {
     hipGraph_t graph = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_substep(c, L, iters, c->record_diag, T);
+    enqueue_substep(c, L, iters, diag, T, mode);
     hipError_t le = hipGetLastError();
     hipError_t ee = hipStreamEndCapture(c->stream, &graph);
     if (le != hipSuccess) return fail(c, PBF_E_DEVICE, std::string("launch during capture: ") + hipGetErrorString(le));
@@ -309,9 +338,76 @@ int run_substep(pbf_ctx *c, float dt, uint32_t iters) {
     (void)hipGraphDestroy(graph);
     if (ie != hipSuccess) return fail(c, PBF_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
     if (c->graphs.size() >= 16) clear_graphs(c);
-    c->graphs.push_back({bits, iters, (uint32_t)c->record_diag, exec});
+    c->graphs.push_back({bits, iters, (uint32_t)diag, mode, exec});
   }
   HIPCHK(hipGraphLaunch(exec, c->stream));
+  return PBF_OK;
+}
+
+// Buffers of the learned and collection modes (allocated on first use).
+int ensure_learn_buffers(pbf_ctx *c) {
+  if (c->lam_acc) return PBF_OK;
+  const size_t N = c->ncap;
+  HIPCHK(dalloc(&c->lam_acc, N));
+  HIPCHK(dalloc(&c->acc_sorted, N));
+  HIPCHK(dalloc(&c->feats, 16 * N));
+  HIPCHK(dalloc(&c->vel_sorted, N));
+  HIPCHK(hipMemsetAsync(c->lam_acc, 0, N * 4, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PBF_OK;
+}
+
+int ensure_snapshot(pbf_ctx *c) {
+  if (c->snap_pos) return PBF_OK;
+  const size_t N = c->ncap;
+  HIPCHK(dalloc(&c->snap_pos, N));
+  HIPCHK(dalloc(&c->snap_vel, N));
+  HIPCHK(dalloc(&c->snap_acc, N));
+  HIPCHK(dalloc(&c->snap_ids, N));
+  HIPCHK(dalloc(&c->snap_lam, N));
+  HIPCHK(dalloc(&c->snap_lacc, N));
+  return PBF_OK;
+}
+
+// Copies the state carried from one substep to the next (slot order).
+void copy_state(pbf_ctx *c, bool save) {
+  const size_t n = c->n;
+  struct { void *live, *snap; size_t bytes; } t[] = {
+      {c->pos, c->snap_pos, 16 * n}, {c->vel, c->snap_vel, 16 * n},   {c->acc, c->snap_acc, 16 * n},
+      {c->ids, c->snap_ids, 4 * n},  {c->lam, c->snap_lam, 4 * n},    {c->lam_acc, c->snap_lacc, 4 * n}};
+  for (auto &e : t) launch_copy(c->stream, save ? e.snap : e.live, save ? e.live : e.snap, e.bytes);
+}
+
+int mean_density_error(pbf_ctx *c, double *mean);
+
+// One learned substep; when a rejection threshold is set, the substep is
+// checked against it and replaced by a plain solve if the mean density
+// error is above it (or not finite).
+int run_learned_substep(pbf_ctx *c, float dt, uint32_t iters) {
+  c->learned_substeps += 1;
+  if (!(c->fb_threshold > 0.0)) return run_substep(c, dt, iters, M_LEARNED, c->record_diag);
+  int rc = ensure_snapshot(c);
+  if (rc != PBF_OK) return rc;
+  copy_state(c, true);
+  rc = run_substep(c, dt, iters, M_LEARNED, false);
+  if (rc != PBF_OK) return rc;
+  double mean = 0.0;
+  rc = mean_density_error(c, &mean);
+  if (rc != PBF_OK) return rc;
+  if (!(mean <= c->fb_threshold)) {
+    c->fallbacks += 1;
+    copy_state(c, false);
+    // the collection mode is the plain solve that also keeps the lambda sum,
+    // which the next learned substep reads
+    rc = run_substep(c, dt, c->fb_iters ? c->fb_iters : iters, M_COLLECT, false);
+    if (rc != PBF_OK) return rc;
+  }
+  if (c->record_diag) {
+    LaunchCtx L = make_launch(c, dt);
+    launch_diag(L, c->pos, c->nbr, c->ncount, c->ci, c->ranges, (uint32_t)c->scenes.size(), c->per_scene, c->hist,
+                c->hist_cap, c->hist_ctr);
+    HIPCHK(hipGetLastError());
+  }
   return PBF_OK;
 }
 
@@ -333,6 +429,21 @@ int sync_ctx(pbf_ctx *c) {
   HIPCHK(hipStreamSynchronize(c->stream));
   drain_events(c);
   return check_flags(c);
+}
+
+// Mean density error over all scenes of the last completed substep.
+int mean_density_error(pbf_ctx *c, double *mean) {
+  const uint32_t ns = (uint32_t)c->scenes.size();
+  LaunchCtx L = make_launch(c, (float)c->params.time_step);
+  launch_diag(L, c->pos, c->nbr, c->ncount, c->ci, c->ranges, ns, c->per_scene, nullptr, 0, nullptr);
+  HIPCHK(hipGetLastError());
+  std::vector<double> ps(2 * (size_t)ns);
+  HIPCHK(hipMemcpyAsync(ps.data(), c->per_scene, ps.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double t = 0.0;
+  for (uint32_t k = 0; k < ns; ++k) t += ps[2 * k];
+  *mean = c->n ? t / (double)c->n : 0.0;
+  return PBF_OK;
 }
 
 void compute_grid(const pbf_params &p, SceneHost &s) {
@@ -540,6 +651,7 @@ int upload_positions(pbf_ctx *c, const float *xyzw, uint32_t count) {
     HIPCHK(hipMemcpy(c->B, zero.data(), N * 16, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(c->lam, 0, N * 4));
     HIPCHK(hipMemset(c->ncount, 0, N * 4));
+    if (c->lam_acc) HIPCHK(hipMemset(c->lam_acc, 0, N * 4));
   }
   std::vector<uint32_t> idv(N ? N : 1), sidv(N ? N : 1);
   for (size_t i = 0; i < N; ++i) idv[i] = (uint32_t)i;
@@ -597,7 +709,9 @@ int pbf_destroy(pbf_ctx *c) {
   void *bufs[] = {c->pos, c->vel, c->acc, c->om, c->nrm, c->XA, c->XB, c->Xold, c->B, c->xs, c->stage4, c->ids,
                   c->cellkey, c->rank, c->tmp_key, c->tmp_src, c->ncount, c->nbr, c->lam, c->lam_sorted, c->counts,
                   c->offsets, c->flags, c->d_scenes, c->sid, c->ranges, c->pool, c->ci, c->per_scene, c->hist,
-                  c->hist_ctr, c->dbg_counts, c->dbg_ids, c->ids_save};
+                  c->hist_ctr, c->dbg_counts, c->dbg_ids, c->ids_save, c->lam_acc, c->acc_sorted, c->feats,
+                  c->vel_sorted, c->snap_pos, c->snap_vel, c->snap_acc, c->snap_ids, c->snap_lam, c->snap_lacc};
+  mlp_engine_free(&c->mlp);
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto p : c->scan_sums)
@@ -747,10 +861,92 @@ int pbf_step(pbf_ctx *c, float dt, uint32_t substeps, uint32_t iterations) {
   const uint32_t iters = iterations ? iterations : c->cfg.iteration;
   const float sdt = (float)((double)dt / (double)substeps);
   for (uint32_t s = 0; s < substeps; ++s) {
-    int rc = run_substep(c, sdt, iters);
+    int rc = c->warm_mode == M_LEARNED ? run_learned_substep(c, sdt, iters)
+                                       : run_substep(c, sdt, iters, M_PLAIN, c->record_diag);
     if (rc != PBF_OK) return rc;
   }
   return PBF_OK;
+}
+
+int pbf_set_warm_start(pbf_ctx *c, const float *blob, size_t length, uint32_t mode, float relax,
+                       double fallback_threshold, uint32_t fallback_iterations) {
+  if (!c) return PBF_E_INVALID;
+  if (mode > PBF_WARM_LEARNED) return fail(c, PBF_E_INVALID, "unknown warm-start mode");
+  if (!(relax > 0.0f && relax <= 1.0f)) return fail(c, PBF_E_INVALID, "relax must be in (0, 1]");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (blob) {
+    if (length != pbf_mlp_blob_size())
+      return fail(c, PBF_E_LENGTH, "network blob must hold pbf_mlp_blob_size() floats");
+    for (size_t q = 0; q < length; ++q)
+      if (!std::isfinite(blob[q])) return fail(c, PBF_E_FORMAT, "network blob holds a non-finite value");
+    int rc = mlp_engine_set(&c->mlp, blob, c->stream);
+    c->relax_set = -1.0f;
+    if (rc != PBF_OK) return fail(c, rc, "could not upload the network weights");
+  }
+  if (mode == PBF_WARM_LEARNED && !c->mlp) return fail(c, PBF_E_STATE, "learned warm start needs network weights");
+  if (c->mlp && relax != c->relax_set) {
+    int rc = mlp_engine_relax(c->mlp, relax, c->stream);
+    if (rc != PBF_OK) return fail(c, rc, "could not set the warm-start relaxation");
+    c->relax_set = relax;
+  }
+  if (fallback_threshold < 0.0 || !std::isfinite(fallback_threshold))
+    return fail(c, PBF_E_INVALID, "fallback_threshold must be finite and >= 0");
+  if (mode == PBF_WARM_LEARNED) {
+    int rc = ensure_learn_buffers(c);
+    if (rc != PBF_OK) return rc;
+  }
+  if (mode != c->warm_mode && c->lam_acc) HIPCHK(hipMemsetAsync(c->lam_acc, 0, (size_t)c->ncap * 4, c->stream));
+  c->warm_mode = mode;
+  c->fb_threshold = fallback_threshold;
+  c->fb_iters = fallback_iterations;
+  clear_graphs(c);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PBF_OK;
+}
+
+int pbf_warm_start_stats(const pbf_ctx *c, uint64_t *learned_substeps, uint64_t *fallbacks) {
+  if (!c) return PBF_E_INVALID;
+  if (learned_substeps) *learned_substeps = c->learned_substeps;
+  if (fallbacks) *fallbacks = c->fallbacks;
+  return PBF_OK;
+}
+
+int pbf_collect_training(pbf_ctx *c, float dt, uint32_t long_iterations, uint32_t iterations, float *features,
+                         float *targets, uint32_t capacity, uint32_t *count) {
+  if (!c) return PBF_E_INVALID;
+  if (!(dt > 0.0f) || long_iterations == 0) return fail(c, PBF_E_INVALID, "dt must be > 0 and long_iterations >= 1");
+  HIPCHK(hipSetDevice(c->device));
+  if (count) *count = c->n;
+  if (c->n == 0) return PBF_OK;
+  if (capacity < c->n) return fail(c, PBF_E_LENGTH, "capacity is below the particle count");
+  int rc = ensure_learn_buffers(c);
+  if (rc == PBF_OK) rc = ensure_snapshot(c);
+  if (rc != PBF_OK) return rc;
+  // the long solve from the current state gives the pair ...
+  copy_state(c, true);
+  rc = run_substep(c, dt, long_iterations, M_COLLECT, false);
+  if (rc == PBF_OK) rc = sync_ctx(c);
+  if (rc != PBF_OK) return rc;
+  // device rows are in slot order; return them in the caller's particle order
+  const size_t n = c->n;
+  std::vector<uint32_t> ids(n);
+  HIPCHK(hipMemcpy(ids.data(), c->ids, n * 4, hipMemcpyDeviceToHost));
+  if (features) {
+    std::vector<float> f(16 * n);
+    HIPCHK(hipMemcpy(f.data(), c->feats, n * 64, hipMemcpyDeviceToHost));
+    for (size_t s = 0; s < n; ++s) std::memcpy(features + 16 * (size_t)ids[s], &f[16 * s], 64);
+  }
+  if (targets) {
+    std::vector<float> t(n);
+    HIPCHK(hipMemcpy(t.data(), c->lam_acc, n * 4, hipMemcpyDeviceToHost));
+    for (size_t s = 0; s < n; ++s) targets[ids[s]] = t[s];
+  }
+  // ... and the state then advances by one substep of the selected solver
+  copy_state(c, false);
+  const uint32_t iters = iterations ? iterations : c->cfg.iteration;
+  if (c->warm_mode == M_LEARNED) return run_learned_substep(c, dt, iters);
+  return run_substep(c, dt, iters, M_COLLECT, c->record_diag);
 }
 
 int pbf_run(pbf_ctx *c) {

@@ -132,6 +132,10 @@ struct Solid {
 // 0.9 being the fluid's packing (V = 0.9 (2r)^3, SPH.ts:49) and
 // max(d + r/2, 2r) the mirror distance of the sampler.  With
 // particle_radius == 0 it is the plain solid volume inside the ball.
+// The ratio has a pole where d + r/2 reaches h while a thin cap of solid is
+// still inside the ball; V_b is therefore capped at its deep-solid value
+// 0.9 / W(2r) (a sample never stands for more than a ball full of solid),
+// which keeps interpolated values bounded near that shell.
 extern "C" int pbf_build_volume_map(uint32_t kind, const double *shape, const int32_t res[3], const double lo[3],
                                     const double hi[3], double kernel_radius, double particle_radius,
                                     float *sdf_vol) {
@@ -171,6 +175,8 @@ extern "C" int pbf_build_volume_map(uint32_t kind, const double *shape, const in
         }
       }
   const double cellv = std::pow(2.0 * h / M, 3);
+  const double t2r = h * h - 4.0 * particle_radius * particle_radius;
+  const double vcap = t2r > 0.0 ? 0.9 / (kp * t2r * t2r * t2r) : 0.0;
   const double ballv = 4.0 / 3.0 * 3.141592653589793 * h * h * h;
   const size_t nx = (size_t)res[0] + 1, ny = (size_t)res[1] + 1, nz = (size_t)res[2] + 1;
   const size_t total = nx * ny * nz;
@@ -198,7 +204,7 @@ extern "C" int pbf_build_volume_map(uint32_t kind, const double *shape, const in
           double s = std::fmax(d + 0.5 * r, 2.0 * r);
           double t = h * h - s * s;
           double ws = t > 0.0 ? kp * t * t * t : 0.0;
-          v = ws > 0.0 ? 0.9 * win / ws : 0.0;
+          v = ws > 0.0 ? std::fmin(0.9 * win / ws, vcap) : 0.0;
         } else {
           v = vin;
         }

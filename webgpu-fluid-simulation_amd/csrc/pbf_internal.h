@@ -7,6 +7,21 @@
 
 namespace pbf {
 
+// Device view of the warm-start MLP (bf16 MFMA operands + f32 vectors).
+struct MlpDev {
+  const short *W1T, *W2T, *W2N;
+  const float *b1, *b2, *w3, *misc;  // misc: b3, fmu[16], fsig[16], tmu, tsig
+};
+
+// Extra per-slot arrays of the learned / data-generation modes.
+struct LearnBufs {
+  float *lam_acc;        // sum of the lambdas applied this substep (current order)
+  const float *acc_prev; // previous substep's sum, reordered
+  const float *lam_prev; // previous substep's last lambda, reordered
+  const float4 *vel_prev;// previous substep's velocity, reordered
+  float *feats;          // 16 features per slot
+};
+
 // Everything a pass launch needs besides its buffers.
 struct LaunchCtx {
   hipStream_t stream;
@@ -34,7 +49,19 @@ void launch_settle(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *
                    float4 *Xold, uint32_t *ids, float *lam_out);
 void launch_build_list(const LaunchCtx &L, const float4 *X0, const uint32_t *offsets, uint32_t *nbr, uint32_t *ncount,
                        uint32_t *flags);
-void launch_lambda(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr, const uint32_t *ncount);
+void launch_lambda(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr, const uint32_t *ncount,
+                   float *lam_acc = nullptr);
+void launch_lambda_features(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr, const uint32_t *ncount,
+                            const LearnBufs &lb);
+void launch_settle_extra(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *tmp_src, const uint32_t *cellkey,
+                         const uint32_t *offsets, const float *acc_in, float *acc_out, const float4 *vel_in,
+                         float4 *vel_out);
+int mlp_engine_set(void **handle, const float *blob, hipStream_t st);
+int mlp_engine_relax(void *handle, float relax, hipStream_t st);
+void mlp_engine_free(void **handle);
+void launch_mlp_infer(hipStream_t st, void *handle, uint32_t n, const float *feats, float4 *X, float *lam_acc,
+                      hipEvent_t e0, hipEvent_t e1);
+void launch_copy(hipStream_t st, void *dst, const void *src, size_t bytes);
 void launch_dp(const LaunchCtx &L, const float4 *X, const float4 *B, const uint32_t *nbr, const uint32_t *ncount,
                float4 *Xn);
 void launch_density(const LaunchCtx &L, const float4 *X, const float4 *Xold, const uint32_t *nbr,

@@ -256,6 +256,33 @@ __global__ void __launch_bounds__(BS) k_settle(uint32_t n, const uint32_t *__res
   if (lam_out) lam_out[f] = lam_in[p];
 }
 
+// Learned / data-generation modes: carry the previous substep's lambda sum and
+// velocity along with the reorder (same stable rank as k_settle).
+__global__ void __launch_bounds__(BS) k_settle_extra(uint32_t n, const uint32_t *__restrict__ tmp_key,
+                                                     const uint32_t *__restrict__ tmp_src,
+                                                     const uint32_t *__restrict__ cellkey,
+                                                     const uint32_t *__restrict__ offsets,
+                                                     const float *__restrict__ acc_in, float *__restrict__ acc_out,
+                                                     const float4 *__restrict__ vel_in, float4 *__restrict__ vel_out) {
+  const uint32_t s = slot_index();
+  if (s >= n) return;
+  const uint32_t p = tmp_src[s], key = tmp_key[s];
+  const uint32_t c = cellkey[p];
+  const uint32_t o = offsets[c], e = offsets[c + 1];
+  uint32_t r = 0;
+  for (uint32_t t = o; t < e; ++t) r += tmp_key[t] < key ? 1u : 0u;
+  const uint32_t f = o + r;
+  acc_out[f] = acc_in[p];
+  vel_out[f] = vel_in[p];
+}
+
+__global__ void __launch_bounds__(BS) k_copy_words(uint32_t *__restrict__ dst, const uint32_t *__restrict__ src,
+                                                   size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // K5-K7: neighbour list over the 27 cells of the frozen predicted position.
 // Dense grid: the three cells (x, y, z-1..z+1) are one contiguous slot range.
@@ -365,12 +392,13 @@ __global__ void __launch_bounds__(BS) k_boundary(uint32_t n, const float4 *__res
 
 // K9: density constraint and lambda, lambda to X[i].w.  FK: 0 no boundary,
 // 1 boundary samples already in B (k_boundary), 2 trilinear maps sampled here.
-template <bool MULTI, int FK>
+// ACC: also add lambda to the per-substep sum (learned / data-generation modes).
+template <bool MULTI, int FK, bool ACC = false>
 __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__restrict__ B,
                                                const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ ncount,
                                                const float *__restrict__ pool, Scene s0,
                                                const Scene *__restrict__ scenes, const uint32_t *__restrict__ sid,
-                                               Phys P) {
+                                               Phys P, float *__restrict__ lam_acc) {
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = X[i];
@@ -406,7 +434,74 @@ __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__
   float c = rho - 1.0f;
   float C = c > 0.0f ? c : 0.0f;
   float den = fmaf(gz, gz, fmaf(gy, gy, fmaf(gx, gx, s2)));
-  reinterpret_cast<float *>(X)[4 * (size_t)i + 3] = (-C) / (den + P.eps_lambda);
+  const float lam = (-C) / (den + P.eps_lambda);
+  reinterpret_cast<float *>(X)[4 * (size_t)i + 3] = lam;
+  if constexpr (ACC) lam_acc[i] += lam;
+}
+
+// Iteration 0 of the learned and data-generation modes: the lambda pass plus
+// the 16 features of the warm-start network (previous-substep lambdas,
+// density, constraint gradient terms, boundary, velocity divergence).
+template <bool MULTI, int FK>
+__global__ void __launch_bounds__(BS) k_lambda_feat(uint32_t n, float4 *X, float4 *__restrict__ B,
+                                                    const uint32_t *__restrict__ nbr,
+                                                    const uint32_t *__restrict__ ncount,
+                                                    const float *__restrict__ pool, Scene s0,
+                                                    const Scene *__restrict__ scenes,
+                                                    const uint32_t *__restrict__ sid, Phys P, LearnBufs lb) {
+  const uint32_t i = slot_index();
+  if (i >= n) return;
+  const float4 xi = X[i];
+  float4 b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if constexpr (FK == 2) {
+    const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
+    b = boundary_sample<2>(P, sc, pool, xi.x, xi.y, xi.z);
+    B[i] = b;
+  } else if constexpr (FK == 1) {
+    b = B[i];
+  }
+  const float4 vi = lb.vel_prev[i];
+  const uint32_t cnt = ncount[i];
+  float gx = 0.0f, gy = 0.0f, gz = 0.0f, s2 = 0.0f, rho = 0.0f, dv = 0.0f;
+  for_each_neighbor<2>(nbr, i, P.ld, cnt, [&](uint32_t j) {
+    const float4 xj = X[j];
+    const float4 vj = lb.vel_prev[j];
+    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    float d2 = dist2(dx, dy, dz);
+    float f = spiky_f(P, d2);
+    float ax = f * dx, ay = f * dy, az = f * dz;
+    gx += ax; gy += ay; gz += az;
+    s2 = fmaf(az, az, fmaf(ay, ay, fmaf(ax, ax, s2)));
+    rho += poly6(P, d2);
+    dv = fmaf(vj.x - vi.x, ax, fmaf(vj.y - vi.y, ay, fmaf(vj.z - vi.z, az, dv)));
+  });
+  rho *= P.V;
+  gx *= P.V; gy *= P.V; gz *= P.V;
+  s2 *= P.V2;
+  const float rho_f = rho;
+  float rho_b = 0.0f, bdist = 1.0f;
+  if (FK && b.w > 0.0f) {
+    float dx = xi.x - b.x, dy = xi.y - b.y, dz = xi.z - b.z;
+    float d2 = dist2(dx, dy, dz);
+    float fb = b.w * spiky_f(P, d2);
+    gx = fmaf(fb, dx, gx); gy = fmaf(fb, dy, gy); gz = fmaf(fb, dz, gz);
+    rho_b = b.w * poly6(P, d2);
+    rho = fmaf(b.w, poly6(P, d2), rho);
+    bdist = sqrtf(d2) / P.h;
+  }
+  float c = rho - 1.0f;
+  float C = c > 0.0f ? c : 0.0f;
+  float g2 = fmaf(gz, gz, fmaf(gy, gy, gx * gx));
+  float den = fmaf(gz, gz, fmaf(gy, gy, fmaf(gx, gx, s2)));
+  const float lam = (-C) / (den + P.eps_lambda);
+  reinterpret_cast<float *>(X)[4 * (size_t)i + 3] = lam;
+  lb.lam_acc[i] = lam;
+  const float vs = P.dt / P.h;
+  float4 *F = reinterpret_cast<float4 *>(lb.feats + 16 * (size_t)i);
+  F[0] = make_float4(c, C, lam, lb.acc_prev[i]);
+  F[1] = make_float4(lb.lam_prev[i], den, s2, g2);
+  F[2] = make_float4((float)cnt, b.w / P.V, bdist, dv * P.V * P.dt);
+  F[3] = make_float4(sqrtf(dist2(vi.x, vi.y, vi.z)) * vs, vi.y * vs, rho_f - 1.0f, rho_b);
 }
 
 // K10 + K11: position correction and wall clamp.
@@ -711,21 +806,54 @@ static void fk_dispatch(const LaunchCtx &L, hipEvent_t e0, hipEvent_t e1, dim3 g
   else if (L.fmode == 1) go(L.stream, e0, e1, Sel<MULTI, 1>::fn(), g, dim3(BS), args...);
   else go(L.stream, e0, e1, Sel<MULTI, 0>::fn(), g, dim3(BS), args...);
 }
-template <bool M, int F> struct SelLambda { static auto fn() { return k_lambda<M, F>; } };
+template <bool M, int F> struct SelLambda { static auto fn() { return k_lambda<M, F, false>; } };
+template <bool M, int F> struct SelLambdaAcc { static auto fn() { return k_lambda<M, F, true>; } };
+template <bool M, int F> struct SelLambdaFeat { static auto fn() { return k_lambda_feat<M, F>; } };
 template <bool M, int F> struct SelDp { static auto fn() { return k_dp<M, F>; } };
 template <bool M, int F> struct SelDiag { static auto fn() { return k_diag<M, F>; } };
 
-void launch_lambda(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr, const uint32_t *ncount) {
+static hipEvent_t boundary_pass(const LaunchCtx &L, float4 *X, float4 *B, dim3 g) {
+  if (L.fmode != 1) return L.ev0;
+  if (L.multi) go(L.stream, L.ev0, nullptr, k_boundary<true>, g, dim3(BS), L.n, (const float4 *)X, B, L.pool, L.s0, L.scenes, L.sid, L.P);
+  else go(L.stream, L.ev0, nullptr, k_boundary<false>, g, dim3(BS), L.n, (const float4 *)X, B, L.pool, L.s0, L.scenes, L.sid, L.P);
+  return nullptr;
+}
+
+void launch_lambda(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr, const uint32_t *ncount,
+                   float *lam_acc) {
   if (!L.n) return;
   const dim3 g(nblocks(L.n, BS));
-  hipEvent_t e0 = L.ev0;
-  if (L.fmode == 1) {
-    if (L.multi) go(L.stream, e0, nullptr, k_boundary<true>, g, dim3(BS), L.n, (const float4 *)X, B, L.pool, L.s0, L.scenes, L.sid, L.P);
-    else go(L.stream, e0, nullptr, k_boundary<false>, g, dim3(BS), L.n, (const float4 *)X, B, L.pool, L.s0, L.scenes, L.sid, L.P);
-    e0 = nullptr;
+  hipEvent_t e0 = boundary_pass(L, X, B, g);
+  if (lam_acc) {
+    if (L.multi) fk_dispatch<true, SelLambdaAcc>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lam_acc);
+    else fk_dispatch<false, SelLambdaAcc>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lam_acc);
+  } else {
+    if (L.multi) fk_dispatch<true, SelLambda>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lam_acc);
+    else fk_dispatch<false, SelLambda>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lam_acc);
   }
-  if (L.multi) fk_dispatch<true, SelLambda>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P);
-  else fk_dispatch<false, SelLambda>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P);
+}
+
+void launch_lambda_features(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr, const uint32_t *ncount,
+                            const LearnBufs &lb) {
+  if (!L.n) return;
+  const dim3 g(nblocks(L.n, BS));
+  hipEvent_t e0 = boundary_pass(L, X, B, g);
+  if (L.multi) fk_dispatch<true, SelLambdaFeat>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lb);
+  else fk_dispatch<false, SelLambdaFeat>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lb);
+}
+
+void launch_settle_extra(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *tmp_src, const uint32_t *cellkey,
+                         const uint32_t *offsets, const float *acc_in, float *acc_out, const float4 *vel_in,
+                         float4 *vel_out) {
+  if (!L.n) return;
+  go(L.stream, nullptr, nullptr, k_settle_extra, dim3(nblocks(L.n, BS)), dim3(BS), L.n, tmp_key, tmp_src, cellkey,
+     offsets, acc_in, acc_out, vel_in, vel_out);
+}
+
+void launch_copy(hipStream_t st, void *dst, const void *src, size_t bytes) {
+  size_t n = bytes / 4;
+  uint32_t grid = (uint32_t)std::min<size_t>((n + BS - 1) / BS, 4096u);
+  if (n) hipLaunchKernelGGL(k_copy_words, dim3(grid), dim3(BS), 0, st, (uint32_t *)dst, (const uint32_t *)src, n);
 }
 
 void launch_dp(const LaunchCtx &L, const float4 *X, const float4 *B, const uint32_t *nbr, const uint32_t *ncount,

@@ -100,6 +100,13 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "pbf_last_error": (C.c_char_p, [P]), "pbf_version": (C.c_char_p, []),
         "pbf_pass_times": (I, [P, C.POINTER(D), pu, U32, pu, C.POINTER(C.c_char_p)]),
         "pbf_copy_bandwidth": (I, [S, U32, C.POINTER(D)]),
+        "pbf_mlp_blob_size": (U32, []),
+        "pbf_mlp_init": (I, [U64, pf]),
+        "pbf_mlp_forward_backward": (I, [pf, pf, pf, U32, pf, pf, pf]),
+        "pbf_mlp_train": (I, [pf, pf, U64, U32, U32, F, U64, pf, pf]),
+        "pbf_set_warm_start": (I, [P, pf, S, U32, F, D, U32]),
+        "pbf_warm_start_stats": (I, [P, C.POINTER(U64), C.POINTER(U64)]),
+        "pbf_collect_training": (I, [P, F, U32, U32, pf, pf, U32, pu]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -166,6 +173,58 @@ def copy_bandwidth(nbytes: int = 1 << 30, iterations: int = 20) -> float:
     if rc != PBF_OK:
         raise PBFError(rc, "pbf_copy_bandwidth failed")
     return g.value
+
+
+MLP_PARAMS = 5313
+MLP_FEATURES = 16
+WARM_PLAIN, WARM_LEARNED = 0, 1
+
+
+def mlp_blob_size() -> int:
+    return int(load_library().pbf_mlp_blob_size())
+
+
+def mlp_init(seed: int = 0) -> np.ndarray:
+    """Fresh network blob (Glorot-uniform weights, identity normalisation)."""
+    lib = load_library()
+    blob = np.zeros(lib.pbf_mlp_blob_size(), np.float32)
+    rc = lib.pbf_mlp_init(int(seed), _fptr(blob))
+    if rc != PBF_OK:
+        raise PBFError(rc, "pbf_mlp_init failed")
+    return blob
+
+
+def mlp_forward_backward(blob: np.ndarray, features: np.ndarray, targets: np.ndarray):
+    """GPU forward + backward: (y normalised, gradient[MLP_PARAMS], loss)."""
+    lib = load_library()
+    blob = np.ascontiguousarray(blob, np.float32)
+    f = np.ascontiguousarray(features, np.float32).reshape(-1, MLP_FEATURES)
+    t = np.ascontiguousarray(targets, np.float32).reshape(-1)
+    if f.shape[0] != t.shape[0]:
+        raise ValueError("features and targets disagree in length")
+    n = f.shape[0]
+    y = np.zeros(n, np.float32)
+    g = np.zeros(MLP_PARAMS, np.float32)
+    loss = np.zeros(1, np.float32)
+    rc = lib.pbf_mlp_forward_backward(_fptr(blob), _fptr(f), _fptr(t), n, _fptr(y), _fptr(g), _fptr(loss))
+    if rc != PBF_OK:
+        raise PBFError(rc, "pbf_mlp_forward_backward failed")
+    return y, g, float(loss[0])
+
+
+def mlp_train(features: np.ndarray, targets: np.ndarray, blob: np.ndarray, epochs: int = 10, batch: int = 4096,
+              lr: float = 1e-3, seed: int = 0):
+    """Adam on the GPU; returns (trained blob, mean loss per epoch)."""
+    lib = load_library()
+    blob = np.array(blob, np.float32, copy=True)
+    f = np.ascontiguousarray(features, np.float32).reshape(-1, MLP_FEATURES)
+    t = np.ascontiguousarray(targets, np.float32).reshape(-1)
+    losses = np.zeros(max(int(epochs), 1), np.float32)
+    rc = lib.pbf_mlp_train(_fptr(f), _fptr(t), f.shape[0], int(epochs), int(batch), float(lr), int(seed),
+                           _fptr(blob), _fptr(losses))
+    if rc != PBF_OK:
+        raise PBFError(rc, "pbf_mlp_train failed")
+    return blob, losses[: int(epochs)]
 
 
 class PBF:
@@ -360,3 +419,29 @@ class PBF:
         self._check(self.lib.pbf_pass_times(self.ctx, ms, cnt, 32, C.byref(k), C.byref(names)))
         nm = names.value.decode().split("\n")
         return {nm[q]: (ms[q], cnt[q]) for q in range(min(k.value, len(nm)))}
+
+    # -- learned warm start (no reference counterpart) ----------------------
+    def set_warm_start(self, blob: Optional[np.ndarray] = None, mode: int = WARM_LEARNED, relax: float = 1.0,
+                       fallback_threshold: float = 0.0, fallback_iterations: int = 0):
+        ptr, length = None, 0
+        if blob is not None:
+            blob = np.ascontiguousarray(blob, np.float32)
+            ptr, length = _fptr(blob), blob.size
+        self._check(self.lib.pbf_set_warm_start(self.ctx, ptr, length, int(mode), float(relax),
+                                                float(fallback_threshold), int(fallback_iterations)))
+
+    def warm_start_stats(self):
+        a, b = C.c_uint64(), C.c_uint64()
+        self._check(self.lib.pbf_warm_start_stats(self.ctx, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def collect_training(self, dt: float, long_iterations: int, iterations: int = 0):
+        """Training pair of the current state (features (n, 16), lambda sums of a
+        `long_iterations` solve (n,)), then one substep of the selected solver."""
+        n = int(self.lib.pbf_particle_count(self.ctx))
+        f = np.zeros((max(n, 1), MLP_FEATURES), np.float32)
+        t = np.zeros(max(n, 1), np.float32)
+        cnt = C.c_uint32()
+        self._check(self.lib.pbf_collect_training(self.ctx, float(dt), int(long_iterations), int(iterations),
+                                                  _fptr(f), _fptr(t), max(n, 1), C.byref(cnt)))
+        return f[: cnt.value], t[: cnt.value]
