@@ -65,6 +65,48 @@ def cpu_baseline(sc, budget_s: float):
                       f"{el:.1f} s, OpenMP {threads} threads"}
 
 
+WARM_BLOB = os.path.join(ROOT, "webgpu-fluid-simulation_amd", "warm_start", "dam_break.f32")
+TARGET_ERR = 0.005
+
+
+def warm_start_comparison(sc, substeps, plain_its, learned_its, relax, blob):
+    """Mean (over `substeps` substeps) density error and ms/substep of the plain
+    solve at each iteration count, and of the learned warm start with the
+    plain solve as fallback (a learned substep whose mean density error is
+    above the plain solve's at the same count is redone plainly)."""
+    import learned
+    out = {"scene": sc.name, "particles": sc.n, "substeps": substeps, "target_density_error": TARGET_ERR,
+           "plain": [], "learned": []}
+    plain_err = {}
+    for it in plain_its:
+        t = learned.run(sc, substeps, it, every=0)
+        e = float(np.mean(t.density_error))
+        plain_err[it] = e
+        out["plain"].append({"iterations": it, "density_error": e, "ms_per_substep": 1e3 * t.seconds / substeps})
+        log(f"  plain   I={it:2d}: density error {e:.4%}, {1e3 * t.seconds / substeps:.3f} ms/substep")
+    for it in learned_its:
+        thr = plain_err.get(it, TARGET_ERR)
+        try:
+            t = learned.run(sc, substeps, it, blob, relax, fallback_threshold=thr, every=0)
+        except Exception as e:  # noqa: BLE001 - an unstable learned run is a result, not a crash
+            out["learned"].append({"iterations": it, "relax": relax, "error": str(e)})
+            log(f"  learned I={it:2d}: failed: {e}")
+            continue
+        e = float(np.mean(t.density_error))
+        out["learned"].append({"iterations": it, "relax": relax, "density_error": e,
+                               "ms_per_substep": 1e3 * t.seconds / substeps, "fallbacks": t.fallbacks,
+                               "fallback_threshold": thr})
+        log(f"  learned I={it:2d}: density error {e:.4%}, {1e3 * t.seconds / substeps:.3f} ms/substep, "
+            f"{t.fallbacks} of {substeps} substeps fell back")
+    reach = [r for r in out["plain"] if r["density_error"] <= TARGET_ERR]
+    lreach = [r for r in out["learned"] if "density_error" in r and r["density_error"] <= TARGET_ERR]
+    out["plain_iterations_to_target"] = reach[0]["iterations"] if reach else None
+    out["learned_iterations_to_target"] = lreach[0]["iterations"] if lreach else None
+    if reach and lreach:
+        out["speedup_at_target"] = reach[0]["ms_per_substep"] / lreach[0]["ms_per_substep"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,13 +116,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-pass events (graph replay)")
     ap.add_argument("--no-graph-compare", action="store_true", help="skip the extra graph-replay measurement")
+    ap.add_argument("--no-learned", action="store_true", help="skip the plain vs learned warm-start comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding).",
               file=sys.stderr)
         sys.exit(2)
 
-    sc = scenes.cfg1()
+    sc = scenes.dam_break_1m()
     N = sc.n
     B = 1 if sc.field is not None else 0
     C = int(np.prod([int(np.ceil((sc.domain_hi[a] - sc.domain_lo[a]) / sc.kernel_radius - 1e-7))
@@ -151,8 +194,19 @@ def main():
         cpu = cpu_baseline(sc, args.cpu_seconds)
         log(f"cpu baseline {cpu['value'] / 1e6:.3f} M particle-substeps/s on {cpu['cores']} threads")
 
+    warm = None
+    if not args.no_learned and os.path.exists(WARM_BLOB):
+        blob = np.fromfile(WARM_BLOB, "<f4")
+        import train_warm_start as tw
+        log("warm start, held-out dam break the network was not trained on:")
+        small = warm_start_comparison(tw.family(tw.EVAL_SHAPE, tw.EVAL_SEED), 150, (2, 3, 4, 5, 6, 8), (2, 3, 4),
+                                      0.5, blob)
+        log("warm start, the 1M dam break:")
+        big = warm_start_comparison(sc, 100, (4, 8, 16, 32, 64), (4, 8), 0.5, blob)
+        warm = {"held_out_dam_break": small, "dam_break_1m": big}
+
     out = {
-        "metric": "particle-substeps/s at 1M (PBF dam break)",
+        "metric": "particle-substeps/s at 1M; iters to \u22640.5% density error, plain vs learned",
         "value": round(value, 1),
         "unit": "particle-substeps/s",
         "n_gpus": 1,
@@ -164,7 +218,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded lattice dam break)",
-        "config": {"workload": "cfg1 dam break: 1,048,576 particles (128x128x64), r=0.025, h=0.1, "
+        "config": {"workload": "dam break: 1,048,576 particles (128x128x64), r=0.025, h=0.1, "
                                "12.8^3 box, 128^3 trilinear box volume map, dt=1/60 x 2 substeps, 4 Jacobi "
                                "iterations, XSPH 0.01, vorticity 0.01, surface tension 0.1",
                    "particles": N, "substeps_per_step": sc.substeps, "iterations": sc.iterations,
@@ -175,6 +229,7 @@ def main():
         "whole_substep_algorithmic_GBs": round(whole, 1),
         "measured_copy_GBs": round(copy_gbs, 1),
         "density_error_after_run": {"mean": mean_err, "max": max_err},
+        "learned_warm_start": warm,
         "passes": per_pass,
     }
     sim.close()

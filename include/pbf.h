@@ -219,8 +219,10 @@ int pbf_copy_bandwidth(size_t bytes, uint32_t iterations, double *gbps);
  *
  * Network blob: pbf_mlp_blob_size() floats,
  *   W1[16][64] b1[64] W2[64][64] b2[64] w3[64] b3 fmu[16] fsig[16] tmu tsig
+ *   lmin
  * (the first PBF_MLP_PARAMS are trained; features are normalised as
- * (f - fmu) / fsig and the output is lambda = y * tsig + tmu).
+ * (f - fmu) / fsig and the output is lambda = max(y * tsig + tmu, lmin);
+ * a positive or non-finite output keeps the plain lambda).
  * Features (iteration 0, per particle): 0 rho~-1, 1 C, 2 plain lambda,
  * 3 previous substep's lambda sum, 4 previous substep's last lambda,
  * 5 lambda denominator, 6 sum |grad_j C|^2, 7 |grad_i C|^2, 8 neighbour
@@ -232,7 +234,7 @@ enum { PBF_WARM_PLAIN = 0, PBF_WARM_LEARNED = 1 };
 
 uint32_t pbf_mlp_blob_size(void);
 /* Glorot-uniform weights from a SplitMix64 stream, zero biases, identity
- * normalisation. */
+ * normalisation, lmin = -3e38 (no bound). */
 int pbf_mlp_init(uint64_t seed, float *blob);
 /* One forward + backward pass on the GPU over n samples: y (normalised
  * output, n), grad (PBF_MLP_PARAMS, d loss / d parameter) and the loss
@@ -250,8 +252,9 @@ int pbf_mlp_train(const float *features, const float *targets, uint64_t n, uint3
  * over-predicted lambda pushes particles apart for good (kinetic energy);
  * relax < 1 keeps the warm start on the safe side.  fallback_threshold > 0
  * checks each learned substep: when its mean density error is above the
- * threshold (or not finite) the substep is undone and solved plainly with
- * fallback_iterations (0: the same count). */
+ * threshold, a particle moved more than h in it, or either is not finite,
+ * the substep is undone and solved plainly with fallback_iterations (0: the
+ * same count). */
 int pbf_set_warm_start(pbf_ctx *ctx, const float *blob, size_t length, uint32_t mode, float relax,
                        double fallback_threshold, uint32_t fallback_iterations);
 int pbf_warm_start_stats(const pbf_ctx *ctx, uint64_t *learned_substeps, uint64_t *fallbacks);

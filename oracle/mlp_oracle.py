@@ -7,7 +7,7 @@ solves lambda from zero every substep, PBF.ts:419-431).
 
 Blob layout (float32):
   W1[16][64] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3 | fmu[16] |
-  fsig[16] | tmu | tsig
+  fsig[16] | tmu | tsig | lmin
 Forward:  x = (f - fmu) / fsig;  z1 = x W1 + b1;  a1 = silu(z1);
           z2 = a1 W2 + b2;  a2 = silu(z2);  y = a2 . w3 + b3
 Loss:     mean over samples of (y - (t - tmu) / tsig)^2.
@@ -25,7 +25,7 @@ from __future__ import annotations
 import numpy as np
 
 OFF_W1, OFF_B1, OFF_W2, OFF_B2, OFF_W3, OFF_B3 = 0, 1024, 1088, 5184, 5248, 5312
-OFF_FMU, OFF_FSIG, OFF_TMU, OFF_TSIG, BLOB, NPARAM = 5313, 5329, 5345, 5346, 5347, 5313
+OFF_FMU, OFF_FSIG, OFF_TMU, OFF_TSIG, OFF_LMIN, BLOB, NPARAM = 5313, 5329, 5345, 5346, 5347, 5348, 5313
 
 
 def bf16(x: np.ndarray) -> np.ndarray:
@@ -39,7 +39,7 @@ def unpack(blob: np.ndarray, dtype=np.float64):
     b = np.asarray(blob, np.float32).astype(dtype)
     return dict(W1=b[OFF_W1:OFF_B1].reshape(16, 64), b1=b[OFF_B1:OFF_W2], W2=b[OFF_W2:OFF_B2].reshape(64, 64),
                 b2=b[OFF_B2:OFF_W3], w3=b[OFF_W3:OFF_B3], b3=b[OFF_B3], fmu=b[OFF_FMU:OFF_FSIG],
-                fsig=b[OFF_FSIG:OFF_TMU], tmu=b[OFF_TMU], tsig=b[OFF_TSIG])
+                fsig=b[OFF_FSIG:OFF_TMU], tmu=b[OFF_TMU], tsig=b[OFF_TSIG], lmin=b[OFF_LMIN])
 
 
 def _sig(z):
@@ -102,9 +102,11 @@ def predict_lambda(blob: np.ndarray, feats: np.ndarray, lam_plain: np.ndarray, q
                    relax: float = 1.0) -> np.ndarray:
     """lambda applied in iteration 0 by the learned solve (k_mlp_infer):
     relax (y tsig + tmu), replaced by the plain lambda where it is positive
-    or not finite (lambda = -C / (den + eps) is never positive)."""
+    or not finite (lambda = -C / (den + eps) is never positive), then
+    bounded below by relax lmin."""
     P = unpack(blob, np.float64)
     y, _, _ = forward_backward(blob, feats, np.zeros(len(feats), np.float32), quant)
     lam = (relax * (y * P["tsig"] + P["tmu"])).astype(np.float32)
     bad = ~(lam <= 0.0) | ~np.isfinite(lam)
-    return np.where(bad, np.asarray(lam_plain, np.float32), lam)
+    lam = np.where(bad, np.asarray(lam_plain, np.float32), lam)
+    return np.maximum(lam, np.float32(relax * P["lmin"]))

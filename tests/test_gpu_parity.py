@@ -19,8 +19,8 @@ pytestmark = pytest.mark.gpu
 
 def small_scenes():
     return {
-        "cfg0_small": scenes.dam_break(16, 16, 8, map_res=32, name="cfg0_small"),
-        "cfg1_physics": scenes.cfg1_small(16, 16, 8, map_res=32),
+        "dam_break_small": scenes.dam_break(16, 16, 8, map_res=32, name="dam_break_small"),
+        "dam_break_physics": scenes.dam_break_physics(16, 16, 8, map_res=32),
         "ref_cube_drop": scenes.ref_cube_drop(),
         "ref_boundary_cubic": scenes.ref_boundary(scenes.unit_box_cdm_field()),
         "hash_obstacle": scenes.obstacle_drop(12, 16, 12, map_res=48, hash_log2=12, name="hash_obstacle"),
@@ -69,7 +69,7 @@ def test_one_step_parity(name):
     sim.close()
 
 
-@pytest.mark.parametrize("name", ["cfg0_small", "ref_cube_drop", "ref_boundary_cubic", "hash_obstacle"])
+@pytest.mark.parametrize("name", ["dam_break_small", "ref_cube_drop", "ref_boundary_cubic", "hash_obstacle"])
 def test_one_step_bit_exact(name):
     sc = SCENES[name]
     sim = scenes.make_engine(sc)
@@ -83,7 +83,7 @@ def test_one_step_bit_exact(name):
 
 
 def test_many_steps_stay_exact():
-    sc = scenes.cfg1_small(16, 16, 8, map_res=32)
+    sc = scenes.dam_break_physics(16, 16, 8, map_res=32)
     sim = scenes.make_engine(sc)
     o = make_oracle(sc)
     for s in range(30):
@@ -109,8 +109,8 @@ def test_density_error_fp64_matches_oracle():
     assert gx == pytest.approx(ox, rel=1e-12, abs=1e-18)
 
 
-def test_200_step_density_statistics_cfg0():
-    sc = scenes.cfg0()
+def test_200_step_density_statistics_dam_break_16k():
+    sc = scenes.dam_break_16k()
     print(f"[200 steps] n={sc.n}", flush=True)
     sim = scenes.make_engine(sc)
     sim.record_density_error(True)
@@ -153,7 +153,7 @@ def test_batched_scenes_match_independent_runs():
 
 
 def test_graph_replay_equals_direct_launches():
-    sc = scenes.cfg1_small(16, 16, 8, map_res=32)
+    sc = scenes.dam_break_physics(16, 16, 8, map_res=32)
     a = scenes.make_engine(sc)
     b = scenes.make_engine(sc, flags=PBF_FLAG_NO_GRAPH)
     for s in range(6):
@@ -168,7 +168,7 @@ def test_graph_replay_equals_direct_launches():
 
 
 def test_reference_error_messages():
-    sc = SCENES["cfg0_small"]
+    sc = SCENES["dam_break_small"]
     sim = scenes.make_engine(sc)
     with pytest.raises(PBFError, match="Illegal length of Particle Position Buffer!"):
         sim.setParticlePosition(np.zeros(10, np.float32), count=3)

@@ -8,6 +8,8 @@ known function, the collection mode against the plain solve (bit-exact),
 the features against their definitions, the learned lambda of iteration 0
 against the oracle's prediction, and the fallback to the plain solve.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -30,6 +32,7 @@ def _rng_blob(seed=3, scale=1.0):
     b[mo.OFF_FSIG:mo.OFF_TMU] = rng.uniform(0.5, 2.0, 16)
     b[mo.OFF_TMU] = -0.3
     b[mo.OFF_TSIG] = 1.7
+    b[mo.OFF_LMIN] = -1e30
     return b
 
 
@@ -94,6 +97,7 @@ def test_blob_init_on_host():
     assert np.abs(P["W1"]).std() > 0.05
     assert not P["b1"].any() and not P["b2"].any() and P["b3"] == 0.0
     assert (P["fsig"] == 1.0).all() and not P["fmu"].any() and P["tsig"] == 1.0 and P["tmu"] == 0.0
+    assert P["lmin"] <= -1e38
 
 
 def test_normalisation_from_data():
@@ -105,6 +109,7 @@ def test_normalisation_from_data():
     assert np.allclose(P["fmu"], f.mean(0), atol=1e-6) and P["fsig"][9] == 1.0
     assert np.allclose(P["fsig"][:9], f.std(0)[:9], rtol=1e-5)
     assert abs(P["tmu"] - t.mean()) < 1e-6 and abs(P["tsig"] - t.std()) < 1e-5
+    assert P["lmin"] == pytest.approx(1.5 * t.min(), rel=1e-6)
 
 
 # ---------------------------------------------------------------- GPU -----
@@ -214,6 +219,17 @@ def test_learned_iteration0_lambda_matches_oracle():
     got = r.read_field("lambda")[:, 0]
     want = mo.predict_lambda(blob, f, f[:, 2], relax=0.5)
     assert np.abs(got - want).max() <= 2e-3 * tsig, np.abs(got - want).max()
+    assert np.all(got >= 0.5 * blob[mo.OFF_LMIN])
+    # a tighter lower bound clamps part of the predictions
+    tight = blob.copy()
+    tight[mo.OFF_LMIN] = np.float32(np.median(mo.predict_lambda(blob, f, f[:, 2])))
+    q = scenes.make_engine(sc)
+    q.set_warm_start(tight, WARM_LEARNED)
+    q.step(dt, 1, 1)
+    got = q.read_field("lambda")[:, 0]
+    want = mo.predict_lambda(tight, f, f[:, 2])
+    assert np.any(want == tight[mo.OFF_LMIN]) and np.all(got >= tight[mo.OFF_LMIN])
+    assert np.abs(got - want).max() <= 4e-3 * tsig
 
 
 @pytest.mark.gpu
@@ -232,6 +248,11 @@ def test_fallback_replaces_a_rejected_substep_with_the_plain_solve():
     assert b.warm_start_stats() == (3, 3)
     for k in ("position", "velocity", "accel"):
         assert np.array_equal(a.read_field(k), b.read_field(k)), k
+    # a loose density threshold: the particles' speed rejects the substep
+    c = scenes.make_engine(sc)
+    c.set_warm_start(blob, WARM_LEARNED, fallback_threshold=10.0, fallback_iterations=3)
+    c.step(dt, 1, 2)
+    assert c.warm_start_stats() == (1, 1)
     # back to the plain solve
     b.set_warm_start(None, WARM_PLAIN)
     a.step(dt, 1, 3)
@@ -257,3 +278,33 @@ def test_warm_start_argument_errors():
         s.set_warm_start(_rng_blob(), WARM_LEARNED, relax=0.0)
     with pytest.raises(PBFError):
         s.set_warm_start(_rng_blob(), WARM_LEARNED, relax=1.5)
+
+
+WARM_BLOB = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "webgpu-fluid-simulation_amd",
+                         "warm_start", "dam_break.f32")
+
+
+@pytest.mark.gpu
+def test_learned_warm_start_gain_on_held_out_dam_break():
+    """The committed network (trained on other block shapes and seeds) at 3
+    iterations reaches a mean density error the plain solve needs 5 for,
+    with a trajectory (mean height, kinetic energy) at least as close to a
+    40-iteration reference run as the plain solve at 3 iterations."""
+    import learned
+    import train_warm_start as tw
+    blob = np.fromfile(WARM_BLOB, "<f4")
+    assert blob.size == mlp_blob_size()
+    ev = tw.family(tw.EVAL_SHAPE, tw.EVAL_SEED)
+    print(f"[warm-start gain] n={ev.n}", flush=True)
+    sub = 150
+    ref = learned.run(ev, sub, 40)
+    plain3, plain5 = learned.run(ev, sub, 3), learned.run(ev, sub, 5)
+    warm3 = learned.run(ev, sub, 3, blob, 0.5)
+    e_p3, e_p5, e_w3 = (float(t.density_error.mean()) for t in (plain3, plain5, warm3))
+    print(f"density error plain I=3 {e_p3:.5f}  plain I=5 {e_p5:.5f}  learned I=3 {e_w3:.5f}", flush=True)
+    assert e_w3 <= 0.005 < e_p3
+    assert e_w3 <= e_p5
+    dh_p, dk_p = learned.fidelity(plain3, ref)
+    dh_w, dk_w = learned.fidelity(warm3, ref)
+    print(f"fidelity plain I=3 ({dh_p:.4f}, {dk_p:.3f})  learned I=3 ({dh_w:.4f}, {dk_w:.3f})", flush=True)
+    assert dh_w <= dh_p and dk_w <= dk_p

@@ -379,10 +379,13 @@ void copy_state(pbf_ctx *c, bool save) {
 }
 
 int mean_density_error(pbf_ctx *c, double *mean);
+int max_speed(pbf_ctx *c, float dt, double *vmax);
 
 // One learned substep; when a rejection threshold is set, the substep is
-// checked against it and replaced by a plain solve if the mean density
-// error is above it (or not finite).
+// checked and replaced by a plain solve if its mean density error is above
+// the threshold or a particle moved more than h in it (|v| dt > h: an
+// over-predicted lambda pushes particles apart, which lowers the density
+// error), or either figure is not finite.
 int run_learned_substep(pbf_ctx *c, float dt, uint32_t iters) {
   c->learned_substeps += 1;
   if (!(c->fb_threshold > 0.0)) return run_substep(c, dt, iters, M_LEARNED, c->record_diag);
@@ -391,10 +394,11 @@ int run_learned_substep(pbf_ctx *c, float dt, uint32_t iters) {
   copy_state(c, true);
   rc = run_substep(c, dt, iters, M_LEARNED, false);
   if (rc != PBF_OK) return rc;
-  double mean = 0.0;
+  double mean = 0.0, vmax = 0.0;
   rc = mean_density_error(c, &mean);
+  if (rc == PBF_OK) rc = max_speed(c, dt, &vmax);
   if (rc != PBF_OK) return rc;
-  if (!(mean <= c->fb_threshold)) {
+  if (!(mean <= c->fb_threshold) || !(vmax <= 1.0)) {
     c->fallbacks += 1;
     copy_state(c, false);
     // the collection mode is the plain solve that also keeps the lambda sum,
@@ -443,6 +447,21 @@ int mean_density_error(pbf_ctx *c, double *mean) {
   double t = 0.0;
   for (uint32_t k = 0; k < ns; ++k) t += ps[2 * k];
   *mean = c->n ? t / (double)c->n : 0.0;
+  return PBF_OK;
+}
+
+// Largest |v| dt / h over all particles.
+int max_speed(pbf_ctx *c, float dt, double *vmax) {
+  const uint32_t ns = (uint32_t)c->scenes.size();
+  LaunchCtx L = make_launch(c, dt);
+  launch_speed(L, c->vel, c->ci, c->ranges, ns, c->per_scene);
+  HIPCHK(hipGetLastError());
+  std::vector<double> ps(2 * (size_t)ns);
+  HIPCHK(hipMemcpyAsync(ps.data(), c->per_scene, ps.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double m = 0.0;
+  for (uint32_t k = 0; k < ns; ++k) m = ps[2 * k + 1] > m || ps[2 * k + 1] != ps[2 * k + 1] ? ps[2 * k + 1] : m;
+  *vmax = m;
   return PBF_OK;
 }
 

@@ -691,6 +691,15 @@ __global__ void __launch_bounds__(1024) k_diag_reduce(const double *__restrict__
   }
 }
 
+// |v| dt / h per particle (the learned solve's rejection test).
+__global__ void __launch_bounds__(BS) k_speed(uint32_t n, const float4 *__restrict__ vel, float scale,
+                                              double *__restrict__ ci) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 v = vel[i];
+  ci[i] = (double)(sqrtf(dist2(v.x, v.y, v.z)) * scale);
+}
+
 // Sum the per-scene totals into history[ctr % cap] = (mean, max) over all.
 __global__ void k_diag_history(const double *__restrict__ per_scene, uint32_t nscenes, uint32_t n,
                                double *__restrict__ hist, uint32_t cap, uint32_t *__restrict__ ctr) {
@@ -893,6 +902,13 @@ void launch_diag(const LaunchCtx &L, const float4 *pos, const uint32_t *nbr, con
   if (hist)
     go(L.stream, nullptr, L.ev1, k_diag_history, dim3(1), dim3(64), (const double *)per_scene, nscenes, L.n, hist,
        hist_cap, hist_ctr);
+}
+
+void launch_speed(const LaunchCtx &L, const float4 *vel, double *ci, const uint32_t *ranges, uint32_t nscenes,
+                  double *per_scene) {
+  if (!L.n) return;
+  hipLaunchKernelGGL(k_speed, dim3(nblocks(L.n, BS)), dim3(BS), 0, L.stream, L.n, vel, L.P.dt / L.P.h, ci);
+  hipLaunchKernelGGL(k_diag_reduce, dim3(nscenes), dim3(1024), 0, L.stream, (const double *)ci, ranges, per_scene);
 }
 
 void launch_unsort4(hipStream_t st, uint32_t n, const float4 *src, const uint32_t *ids, float4 *dst, int clear_w) {

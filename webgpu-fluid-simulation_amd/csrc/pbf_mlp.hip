@@ -10,11 +10,12 @@
 // to layer in registers.  Weight gradients reduce over particles and go
 // through LDS.
 //
-// Packed parameter blob (float32, PBF_MLP_BLOB floats):
+// Packed parameter blob (float32, 5348 floats):
 //   W1[16][64] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3 | fmu[16] |
-//   fsig[16] | tmu | tsig
+//   fsig[16] | tmu | tsig | lmin
 // The first PBF_MLP_PARAMS (5313) entries are trained; the rest normalise
-// features (x = (f - fmu) / fsig) and the target (lambda = y tsig + tmu).
+// features (x = (f - fmu) / fsig) and the target (lambda = y tsig + tmu),
+// and bound the prediction from below (lambda >= lmin).
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -34,7 +35,7 @@ typedef float f16v __attribute__((ext_vector_type(16)));
 
 enum : uint32_t {
   OFF_W1 = 0, OFF_B1 = 1024, OFF_W2 = 1088, OFF_B2 = 5184, OFF_W3 = 5248, OFF_B3 = 5312,
-  OFF_FMU = 5313, OFF_FSIG = 5329, OFF_TMU = 5345, OFF_TSIG = 5346, BLOB = 5347, NPARAM = 5313
+  OFF_FMU = 5313, OFF_FSIG = 5329, OFF_TMU = 5345, OFF_TSIG = 5346, OFF_LMIN = 5347, BLOB = 5348, NPARAM = 5313
 };
 
 // float -> bfloat16, round to nearest even
@@ -121,6 +122,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(uint32_t n, const float *__re
       float *wp = reinterpret_cast<float *>(X) + 4 * (size_t)p + 3;
       float lam = fmaf(T.y, M.misc[OFF_TSIG - OFF_B3], M.misc[OFF_TMU - OFF_B3]);
       if (!(lam <= 0.0f) || !isfinite(lam)) lam = *wp;
+      lam = fmaxf(lam, M.misc[OFF_LMIN - OFF_B3]);
       *wp = lam;
       if (lam_acc) lam_acc[p] = lam;
     }
@@ -299,7 +301,7 @@ __global__ void k_mlp_pack(const float *__restrict__ w, short *__restrict__ W1T,
     W2T[o * 64 + i] = b;
     W2N[q] = b;
   }
-  if (q < BLOB - OFF_B1) {  // b1 .. tsig as f32 (W2 region copied too, unused)
+  if (q < BLOB - OFF_B1) {  // b1 .. lmin as f32 (W2 region copied too, unused)
     f32[q] = w[OFF_B1 + q];
   }
 }
@@ -381,6 +383,7 @@ __global__ void k_mlp_relax(const float *__restrict__ w, float *__restrict__ f32
   if (threadIdx.x == 0) {
     f32[OFF_TMU - OFF_B1] = relax * w[OFF_TMU];
     f32[OFF_TSIG - OFF_B1] = relax * w[OFF_TSIG];
+    f32[OFF_LMIN - OFF_B1] = relax * w[OFF_LMIN];
   }
 }
 
@@ -417,7 +420,8 @@ void launch_mlp_infer(hipStream_t st, void *handle, uint32_t n, const float *fea
 namespace {
 void init_blob(float *w, uint64_t seed) {
   // Glorot-uniform weights from the seeded SplitMix64 stream, zero biases,
-  // identity normalisation (callers overwrite fmu/fsig/tmu/tsig).
+  // identity normalisation, no lower bound (callers overwrite fmu, fsig,
+  // tmu, tsig and lmin).
   uint64_t s = seed;
   auto uni = [&](double a) {
     s = splitmix64(s);
@@ -430,6 +434,7 @@ void init_blob(float *w, uint64_t seed) {
   for (int q = 0; q < 64; ++q) w[OFF_W3 + q] = uni(a3);
   for (int f = 0; f < 16; ++f) w[OFF_FSIG + f] = 1.0f;
   w[OFF_TSIG] = 1.0f;
+  w[OFF_LMIN] = -3.0e38f;
 }
 
 struct TrainState {

@@ -17,19 +17,22 @@ iteration count every substep (PBF.ts:419-431).
 """
 from __future__ import annotations
 
+import time
 from dataclasses import dataclass
 from typing import Callable, List, Optional, Sequence
 
 import numpy as np
 
-from pbf_host import (MLP_FEATURES, WARM_LEARNED, WARM_PLAIN, mlp_init, mlp_train)
+from pbf_host import (MLP_FEATURES, WARM_LEARNED, WARM_PLAIN, PBFError, mlp_init, mlp_train)
 from scenes import Scene, make_engine
 
-OFF_FMU, OFF_FSIG, OFF_TMU, OFF_TSIG = 5313, 5329, 5345, 5346
+OFF_FMU, OFF_FSIG, OFF_TMU, OFF_TSIG, OFF_LMIN = 5313, 5329, 5345, 5346, 5347
+LMIN_MARGIN = 1.5
 
 
 def set_normalisation(blob: np.ndarray, features: np.ndarray, targets: np.ndarray) -> np.ndarray:
-    """Feature mean / std and target mean / std into the blob (std 0 -> 1)."""
+    """Feature mean / std and target mean / std into the blob (std 0 -> 1);
+    the prediction is bounded below by 1.5 x the smallest target seen."""
     b = np.array(blob, np.float32, copy=True)
     f = np.asarray(features, np.float64).reshape(-1, MLP_FEATURES)
     t = np.asarray(targets, np.float64).reshape(-1)
@@ -39,29 +42,40 @@ def set_normalisation(blob: np.ndarray, features: np.ndarray, targets: np.ndarra
     b[OFF_FSIG:OFF_TMU] = sd
     b[OFF_TMU] = t.mean()
     b[OFF_TSIG] = t.std() if t.std() > 1e-12 else 1.0
+    b[OFF_LMIN] = LMIN_MARGIN * min(float(t.min()), 0.0) if t.size else 0.0
     return b
 
 
 def collect(scene: Scene, substeps: int, long_iterations: int = 40, iterations: Optional[int] = None,
-            blob: Optional[np.ndarray] = None, skip: int = 0, stride: int = 1, relax: float = 1.0):
+            blob: Optional[np.ndarray] = None, skip: int = 0, stride: int = 1, relax: float = 1.0,
+            keep: float = 1.0, seed: int = 0, max_neighbors: int = 128, fallback_threshold: float = 0.0):
     """Training pairs along `substeps` substeps of `scene` with the deployed
-    solver (plain, or learned when `blob` is given)."""
-    sim = make_engine(scene)
+    solver (plain, or learned when `blob` is given); `keep` < 1 keeps a
+    seeded random fraction of the particles of every pair."""
+    rng = np.random.default_rng(seed)
+    sim = make_engine(scene, max_neighbors=max_neighbors)
     try:
         if blob is not None:
-            sim.set_warm_start(blob, WARM_LEARNED, relax)
+            sim.set_warm_start(blob, WARM_LEARNED, relax, fallback_threshold)
         it = iterations if iterations is not None else scene.iterations
         dt = scene.substep_dt()
         F: List[np.ndarray] = []
         T: List[np.ndarray] = []
         dropped = 0
+        collect.last_error = None
         for k in range(substeps):
             if k >= skip and (k - skip) % stride == 0:
-                f, t = sim.collect_training(dt, long_iterations, it)
+                try:
+                    f, t = sim.collect_training(dt, long_iterations, it)
+                except PBFError as e:  # an unstable learned run: keep what was gathered
+                    collect.last_error = f"substep {k}: {e}"
+                    break
                 # a long Jacobi solve can blow up where the short one does not;
                 # such pairs carry no target
                 ok = np.isfinite(f).all(1) & np.isfinite(t)
                 dropped += int((~ok).sum())
+                if keep < 1.0:
+                    ok &= rng.random(len(t)) < keep
                 F.append(f[ok])
                 T.append(t[ok])
             else:
@@ -92,26 +106,34 @@ class Trajectory:
     density_error: np.ndarray   # mean density error after every substep
     height: np.ndarray          # mean particle height at the checkpoints
     kinetic: np.ndarray         # mean kinetic energy per unit mass at the checkpoints
+    seconds: float = 0.0        # wall time of the substeps (no checkpoints: every = 0)
+    fallbacks: int = 0          # learned substeps replaced by the plain solve
 
 
 def run(scene: Scene, substeps: int, iterations: int, blob: Optional[np.ndarray] = None, relax: float = 1.0,
-        fallback_threshold: float = 0.0, every: int = 20) -> Trajectory:
+        fallback_threshold: float = 0.0, every: int = 20, max_neighbors: int = 128) -> Trajectory:
     """Density error per substep (FP64 diagnostics on the GPU) and, every
-    `every` substeps, the mean height and kinetic energy of the fluid."""
-    sim = make_engine(scene)
+    `every` substeps, the mean height and kinetic energy of the fluid
+    (every = 0: no checkpoints, and the wall time of the substeps)."""
+    sim = make_engine(scene, max_neighbors=max_neighbors)
     try:
         if blob is not None:
             sim.set_warm_start(blob, WARM_LEARNED, relax, fallback_threshold)
         sim.record_density_error(True)
         dt = scene.substep_dt()
         hs, ks = [], []
+        sim.sync()
+        t0 = time.perf_counter()
         for k in range(substeps):
             sim.step(dt, 1, iterations)
-            if (k + 1) % every == 0:
+            if every and (k + 1) % every == 0:
                 p, v = sim.read_positions(), sim.read_velocities()
                 hs.append(float(p[:, 1].mean()))
                 ks.append(float(0.5 * (v[:, :3].astype(np.float64) ** 2).sum(1).mean()))
-        return Trajectory(sim.density_history()[:, 0].copy(), np.array(hs), np.array(ks))
+        sim.sync()
+        secs = time.perf_counter() - t0
+        fb = sim.warm_start_stats()[1]
+        return Trajectory(sim.density_history()[:, 0].copy(), np.array(hs), np.array(ks), secs, fb)
     finally:
         sim.close()
 

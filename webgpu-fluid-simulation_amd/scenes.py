@@ -120,21 +120,21 @@ def dam_break(nx: int, ny: int, nz: int, r: float = 0.025, map_res: int = 64, xs
     return Scene(name, r, h, lo, hi, pos, fld, cfg, 1.0 / 60.0, 2, iterations, steps)
 
 
-def cfg0(**kw) -> Scene:
+def dam_break_16k(**kw) -> Scene:
     """16,384 particles (32x32x16), 64^3 box map, XSPH only, 200 steps."""
-    return dam_break(32, 32, 16, map_res=64, name="cfg0_dam_break_16k", seed=SEED + 0, **kw)
+    return dam_break(32, 32, 16, map_res=64, name="dam_break_16k", seed=SEED + 0, **kw)
 
 
-def cfg1(**kw) -> Scene:
+def dam_break_1m(**kw) -> Scene:
     """1,048,576 particles (128x128x64), 128^3 box map, vorticity 0.01, tension 0.1, 500 steps."""
     return dam_break(128, 128, 64, map_res=128, vorticity=0.01, tension=0.1, steps=500,
-                     name="cfg1_dam_break_1m", seed=SEED + 1, **kw)
+                     name="dam_break_1m", seed=SEED + 1, **kw)
 
 
-def cfg1_small(nx=32, ny=32, nz=16, map_res=32) -> Scene:
-    """cfg1's physics (vorticity + tension) at a size the oracle runs in seconds."""
+def dam_break_physics(nx=32, ny=32, nz=16, map_res=32) -> Scene:
+    """The 1M dam break's physics (vorticity + tension) at a size the oracle runs in seconds."""
     return dam_break(nx, ny, nz, map_res=map_res, vorticity=0.01, tension=0.1, steps=500,
-                     name=f"cfg1_physics_{nx*ny*nz}", seed=SEED + 1)
+                     name=f"dam_break_physics_{nx*ny*nz}", seed=SEED + 1)
 
 
 def obstacle_drop(nx: int, ny: int, nz: int, r: float = 0.025, map_res: int = 256, hash_log2: int = 22,
@@ -158,9 +158,9 @@ def obstacle_drop(nx: int, ny: int, nz: int, r: float = 0.025, map_res: int = 25
     return Scene(name, r, h, lo, hi, pos, fld, cfg, 1.0 / 60.0, 2, 4, 300, PBF_GRID_HASH, hash_log2)
 
 
-def cfg2(**kw) -> Scene:
+def obstacle_drop_4m(**kw) -> Scene:
     """4,194,304 particles (128x256x128) dropped on sphere + torus, 256^3 map, 2^22 hash table."""
-    return obstacle_drop(128, 256, 128, map_res=256, name="cfg2_obstacle_4m", **kw)
+    return obstacle_drop(128, 256, 128, map_res=256, name="obstacle_drop_4m", **kw)
 
 
 def batch_specs(nscenes: int = 256, nmin: int = 2048, nmax: int = 65536, seed: int = SEED + 3):
@@ -175,12 +175,12 @@ def batch_specs(nscenes: int = 256, nmin: int = 2048, nmax: int = 65536, seed: i
     return out
 
 
-def cfg3(nscenes: int = 256, map_res: int = 64, nmin: int = 2048, nmax: int = 65536) -> List[Scene]:
+def batch_scenes(nscenes: int = 256, map_res: int = 64, nmin: int = 2048, nmax: int = 65536) -> List[Scene]:
     """Independent dam breaks of uneven size, each with its own 64^3 box map."""
     scenes = []
     for k, (nx, ny, nz) in enumerate(batch_specs(nscenes, nmin, nmax)):
         scenes.append(dam_break(nx, ny, nz, map_res=map_res, seed=SEED + 3 + 1000 * (k + 1),
-                                name=f"cfg3_scene{k}"))
+                                name=f"batch_scene{k}"))
     return scenes
 
 

@@ -29,6 +29,7 @@ import numpy as np  # noqa: E402
 
 import learned  # noqa: E402
 import scenes  # noqa: E402
+from pbf_host import PBFError  # noqa: E402
 
 TRAIN_SHAPES = [(16, 16, 8), (20, 20, 10), (24, 16, 12), (28, 24, 14)]
 EVAL_SHAPE = (24, 18, 12)
@@ -40,6 +41,32 @@ def family(shape, seed):
     return scenes.dam_break(nx, ny, nz, map_res=32, vorticity=0.01, tension=0.1, seed=seed, name="warm_start")
 
 
+def evaluate(ev, sub, blob, long_iterations, plain_its, learned_its, relaxes, log=print):
+    """Rows of (solver, iterations, relax, density error, height / KE deviation
+    from a long-solve reference run)."""
+    ref = learned.run(ev, sub, long_iterations)
+    rows = []
+    for it in plain_its:
+        tr = learned.run(ev, sub, it)
+        dh, dk = learned.fidelity(tr, ref)
+        rows.append({"solver": "plain", "iterations": it, "relax": None,
+                     "density_error": float(tr.density_error.mean()), "height_dev": dh, "kinetic_dev": dk})
+        log(f"plain   I={it:2d}            err {rows[-1]['density_error']:.5f}  dh {dh:.4f}  dKE {dk:.3f}")
+    for it in learned_its:
+        for rx in relaxes:
+            try:
+                tr = learned.run(ev, sub, it, blob, rx)
+            except PBFError as e:  # e.g. a neighbour overflow of an unstable run
+                rows.append({"solver": "learned", "iterations": it, "relax": rx, "error": str(e)})
+                log(f"learned I={it:2d} relax {rx:.2f} failed: {e}")
+                continue
+            dh, dk = learned.fidelity(tr, ref)
+            rows.append({"solver": "learned", "iterations": it, "relax": rx,
+                         "density_error": float(tr.density_error.mean()), "height_dev": dh, "kinetic_dev": dk})
+            log(f"learned I={it:2d} relax {rx:.2f} err {rows[-1]['density_error']:.5f}  dh {dh:.4f}  dKE {dk:.3f}")
+    return float(ref.density_error.mean()), rows
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "warm_start"))
@@ -49,60 +76,66 @@ def main(argv=None):
     ap.add_argument("--relax", type=float, default=0.5)
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="fewer shapes and substeps")
+    ap.add_argument("--medium", default="64,64,32", help="a larger block, subsampled ('' to skip)")
+    ap.add_argument("--medium-keep", type=float, default=0.25)
+    ap.add_argument("--large", default="", help="the large block in training too, subsampled ('' to skip)")
+    ap.add_argument("--large-keep", type=float, default=0.03)
     ap.add_argument("--eval-relax", default="0.25,0.5,0.75")
+    ap.add_argument("--eval-iterations", default="1,2,3,4,6")
+    ap.add_argument("--eval-large", default="128,128,64", help="block of the large evaluation ('' to skip)")
+    ap.add_argument("--large-substeps", type=int, default=200)
     a = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)  # noqa: E731
     shapes = TRAIN_SHAPES[:2] if a.quick else TRAIN_SHAPES
     sub = 60 if a.quick else a.substeps
     t0 = time.time()
-    jobs = [(s, 100 + 7 * k + j) for k, s in enumerate(shapes) for j in range(2)]
+    jobs = [(s, 100 + 7 * k + j, 1.0) for k, s in enumerate(shapes) for j in range(2)]
+    if a.medium and not a.quick:
+        jobs.append((tuple(int(x) for x in a.medium.split(",")), 200, a.medium_keep))
+    if a.large and not a.quick:
+        jobs.append((tuple(int(x) for x in a.large.split(",")), 300, a.large_keep))
     F, T = [], []
-    for shape, seed in jobs:
-        f, t = learned.collect(family(shape, seed), sub, a.long, a.iterations)
-        print(f"collect {shape} seed {seed}: {len(t)} pairs, dropped {learned.collect.last_dropped}, "
-              f"target range [{t.min():.3g}, {t.max():.3g}]", flush=True)
+    for shape, seed, keep in jobs:
+        f, t = learned.collect(family(shape, seed), sub, a.long, a.iterations, keep=keep, seed=seed)
+        log(f"collect {shape} seed {seed}: {len(t)} pairs, dropped {learned.collect.last_dropped}, "
+            f"target range [{t.min():.3g}, {t.max():.3g}]")
         F.append(f)
         T.append(t)
     res = learned.train(np.concatenate(F), np.concatenate(T), a.epochs)
-    for shape, seed in jobs:  # DAgger: states the learned solver visits
-        f, t = learned.collect(family(shape, seed), sub, a.long, a.iterations, res.blob, relax=a.relax)
-        print(f"collect (learned) {shape} seed {seed}: {len(t)} pairs, dropped {learned.collect.last_dropped}",
-              flush=True)
+    for shape, seed, keep in jobs:  # DAgger: states the learned solver visits
+        f, t = learned.collect(family(shape, seed), sub, a.long, a.iterations, res.blob, relax=a.relax, keep=keep,
+                               seed=seed + 1, fallback_threshold=0.1)
+        log(f"collect (learned) {shape} seed {seed}: {len(t)} pairs, dropped {learned.collect.last_dropped}"
+            + (f", stopped at {learned.collect.last_error}" if learned.collect.last_error else ""))
         F.append(f)
         T.append(t)
     F, T = np.concatenate(F), np.concatenate(T)
     res = learned.train(F, T, a.epochs)
     t_train = time.time() - t0
-    print(f"samples {len(T)}  loss {res.losses[0]:.4f} -> {res.losses[-1]:.4f}  ({t_train:.1f} s)", flush=True)
-
-    ev = family(EVAL_SHAPE, EVAL_SEED)
-    ref = learned.run(ev, sub, a.long)
-    rows = []
-    for it in (1, 2, 3, 4, 5, 6, 8, 10, 12):
-        tr = learned.run(ev, sub, it)
-        dh, dk = learned.fidelity(tr, ref)
-        rows.append({"solver": "plain", "iterations": it, "relax": None,
-                     "density_error": float(tr.density_error.mean()), "height_dev": dh, "kinetic_dev": dk})
-        print(f"plain   I={it:2d}            err {rows[-1]['density_error']:.5f}  dh {dh:.4f}  dKE {dk:.3f}",
-              flush=True)
-    for it in (1, 2, 3, 4, 6):
-        for rx in [float(x) for x in a.eval_relax.split(",")]:
-            tr = learned.run(ev, sub, it, res.blob, rx)
-            dh, dk = learned.fidelity(tr, ref)
-            rows.append({"solver": "learned", "iterations": it, "relax": rx,
-                         "density_error": float(tr.density_error.mean()), "height_dev": dh, "kinetic_dev": dk})
-            print(f"learned I={it:2d} relax {rx:.2f} err {rows[-1]['density_error']:.5f}  dh {dh:.4f}  dKE {dk:.3f}",
-                  flush=True)
+    log(f"samples {len(T)}  loss {res.losses[0]:.4f} -> {res.losses[-1]:.4f}  ({t_train:.1f} s)")
+    relaxes = [float(x) for x in a.eval_relax.split(",")]
+    ref_err, rows = evaluate(family(EVAL_SHAPE, EVAL_SEED), sub, res.blob, a.long, (1, 2, 3, 4, 5, 6, 8, 10, 12),
+                             [int(x) for x in a.eval_iterations.split(",")], relaxes, log)
+    large = None
+    if a.eval_large and not a.quick:
+        shape = tuple(int(x) for x in a.eval_large.split(","))
+        log(f"-- large evaluation {shape}, {a.large_substeps} substeps")
+        lref, lrows = evaluate(family(shape, EVAL_SEED), a.large_substeps, res.blob, a.long, (2, 3, 4, 5, 6, 8, 12),
+                               (2, 3, 4), relaxes, log)
+        large = {"shape": shape, "seed": EVAL_SEED, "substeps": a.large_substeps, "reference_density_error": lref,
+                 "rows": lrows}
     os.makedirs(a.out, exist_ok=True)
     res.blob.astype("<f4").tofile(os.path.join(a.out, "dam_break.f32"))
     meta = {
         "network": "16-64-64-1 SiLU, bf16 weights, fp32 accumulate",
-        "train_shapes": shapes, "train_seeds": sorted({s for _, s in jobs}), "substeps": sub,
+        "train": [{"shape": s, "seed": sd, "keep": k} for s, sd, k in jobs], "substeps": sub,
         "long_iterations": a.long, "deployed_iterations": a.iterations, "collect_relax": a.relax,
         "epochs": a.epochs, "samples": int(len(T)), "loss_first_last": [float(res.losses[0]), float(res.losses[-1])],
         "physics": {"particle_radius": 0.025, "kernel_radius": 0.1, "substep_dt": "1/120 s", "xsph": 0.01,
                     "vorticity": 0.01, "surface_tension": 0.1},
         "eval": {"shape": EVAL_SHAPE, "seed": EVAL_SEED, "substeps": sub, "reference_iterations": a.long,
-                 "reference_density_error": float(ref.density_error.mean()), "rows": rows},
+                 "reference_density_error": ref_err, "rows": rows},
+        "eval_large": large,
         "train_seconds": t_train,
     }
     with open(os.path.join(a.out, "dam_break.json"), "w") as fh:
