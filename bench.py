@@ -69,7 +69,7 @@ WARM_BLOB = os.path.join(ROOT, "webgpu-fluid-simulation_amd", "warm_start", "dam
 TARGET_ERR = 0.005
 
 
-def warm_start_comparison(sc, substeps, plain_its, learned_its, relax, blob):
+def warm_start_comparison(sc, substeps, plain_its, learned_its, relax, blob, previous_its=(), previous_relax=0.5):
     """Mean (over `substeps` substeps) density error and ms/substep of the plain
     solve at each iteration count, and of the learned warm start with the
     plain solve as fallback (a learned substep whose mean density error is
@@ -98,6 +98,19 @@ def warm_start_comparison(sc, substeps, plain_its, learned_its, relax, blob):
                                "fallback_threshold": thr})
         log(f"  learned I={it:2d}: density error {e:.4%}, {1e3 * t.seconds / substeps:.3f} ms/substep, "
             f"{t.fallbacks} of {substeps} substeps fell back")
+    # the temporal warm start (previous substep's lambda sum, no network)
+    from pbf_host import WARM_PREVIOUS
+    out["previous"] = []
+    for it in previous_its:
+        try:
+            t = learned.run(sc, substeps, it, None, previous_relax, every=0, mode=WARM_PREVIOUS)
+        except Exception as e:  # noqa: BLE001
+            out["previous"].append({"iterations": it, "relax": previous_relax, "error": str(e)})
+            continue
+        e = float(np.mean(t.density_error))
+        out["previous"].append({"iterations": it, "relax": previous_relax, "density_error": e,
+                                "ms_per_substep": 1e3 * t.seconds / substeps})
+        log(f"  previous I={it:2d}: density error {e:.4%}, {1e3 * t.seconds / substeps:.3f} ms/substep")
     reach = [r for r in out["plain"] if r["density_error"] <= TARGET_ERR]
     lreach = [r for r in out["learned"] if "density_error" in r and r["density_error"] <= TARGET_ERR]
     out["plain_iterations_to_target"] = reach[0]["iterations"] if reach else None
@@ -200,9 +213,9 @@ def main():
         import train_warm_start as tw
         log("warm start, held-out dam break the network was not trained on:")
         small = warm_start_comparison(tw.family(tw.EVAL_SHAPE, tw.EVAL_SEED), 150, (2, 3, 4, 5, 6, 8), (2, 3, 4),
-                                      0.5, blob)
+                                      0.5, blob, (2, 3, 4), 0.8)
         log("warm start, the 1M dam break:")
-        big = warm_start_comparison(sc, 100, (4, 8, 16, 32, 64), (4, 8), 0.5, blob)
+        big = warm_start_comparison(sc, 100, (4, 8, 16, 32, 64), (4, 8), 0.5, blob, (4, 8, 16, 32), 0.5)
         warm = {"held_out_dam_break": small, "dam_break_1m": big}
 
     out = {

@@ -230,7 +230,9 @@ int pbf_copy_bandwidth(size_t bytes, uint32_t iterations, double *gbps);
  * divergence * dt, 12 |v| dt / h, 13 v_y dt / h, 14 fluid part of rho~-1,
  * 15 boundary part of rho~. */
 enum { PBF_MLP_PARAMS = 5313, PBF_MLP_FEATURES = 16 };
-enum { PBF_WARM_PLAIN = 0, PBF_WARM_LEARNED = 1 };
+/* PBF_WARM_PREVIOUS: iteration 0 applies relax x the previous substep's
+ * lambda sum, without a network (the classic temporal warm start). */
+enum { PBF_WARM_PLAIN = 0, PBF_WARM_LEARNED = 1, PBF_WARM_PREVIOUS = 2 };
 
 uint32_t pbf_mlp_blob_size(void);
 /* Glorot-uniform weights from a SplitMix64 stream, zero biases, identity

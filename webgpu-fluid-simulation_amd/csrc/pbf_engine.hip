@@ -29,9 +29,10 @@ const char *kPassNames =
     "predict_bin\nscan\nscatter\nsettle\nbuild_list\nlambda\ndp\ndensity\nvorticity\nxsph\ndiag\nmlp_infer";
 
 // Substep kinds: the plain Jacobi solve, the learned warm start (network
-// lambda at iteration 0), and training-pair collection (features at
-// iteration 0, lambda summed over all iterations).
-enum SubstepMode : uint32_t { M_PLAIN = 0, M_LEARNED = 1, M_COLLECT = 2 };
+// lambda at iteration 0), training-pair collection (features at iteration
+// 0, lambda summed over all iterations), and the temporal warm start
+// (iteration 0 applies the previous substep's lambda sum).
+enum SubstepMode : uint32_t { M_PLAIN = 0, M_LEARNED = 1, M_COLLECT = 2, M_PREVIOUS = 3 };
 
 struct SceneHost {
   double lo[3], hi[3];
@@ -103,6 +104,7 @@ struct pbf_ctx {
   double fb_threshold = 0.0;     // mean density error that rejects a learned substep (0: never)
   uint32_t fb_iters = 0;         // iterations of the plain re-solve
   float relax_set = -1.0f;       // scale applied to the network's lambda
+  float relax_prev = 1.0f;       // scale applied to the previous lambda sum
   float *lam_acc = nullptr, *acc_sorted = nullptr, *feats = nullptr;
   float4 *vel_sorted = nullptr;
   // state saved before a learned substep that may be rejected
@@ -285,6 +287,8 @@ void enqueue_substep(pbf_ctx *c, const LaunchCtx &L, uint32_t iters, bool diag, 
       t = T.begin(P_MLP, L);
       launch_mlp_infer(c->stream, c->mlp, c->n, c->feats, X, c->lam_acc, t.ev0, t.ev1);
       T.end();
+    } else if (mode == M_PREVIOUS && it == 0) {
+      launch_warm_previous(L, c->acc_sorted, X, c->lam_acc, c->relax_prev);
     }
     t = T.begin(P_DP, L);
     launch_dp(t, X, c->B, c->nbr, c->ncount, Xn);
@@ -388,11 +392,12 @@ int max_speed(pbf_ctx *c, float dt, double *vmax);
 // error), or either figure is not finite.
 int run_learned_substep(pbf_ctx *c, float dt, uint32_t iters) {
   c->learned_substeps += 1;
-  if (!(c->fb_threshold > 0.0)) return run_substep(c, dt, iters, M_LEARNED, c->record_diag);
+  const uint32_t mode = c->warm_mode == PBF_WARM_PREVIOUS ? M_PREVIOUS : M_LEARNED;
+  if (!(c->fb_threshold > 0.0)) return run_substep(c, dt, iters, mode, c->record_diag);
   int rc = ensure_snapshot(c);
   if (rc != PBF_OK) return rc;
   copy_state(c, true);
-  rc = run_substep(c, dt, iters, M_LEARNED, false);
+  rc = run_substep(c, dt, iters, mode, false);
   if (rc != PBF_OK) return rc;
   double mean = 0.0, vmax = 0.0;
   rc = mean_density_error(c, &mean);
@@ -880,8 +885,8 @@ int pbf_step(pbf_ctx *c, float dt, uint32_t substeps, uint32_t iterations) {
   const uint32_t iters = iterations ? iterations : c->cfg.iteration;
   const float sdt = (float)((double)dt / (double)substeps);
   for (uint32_t s = 0; s < substeps; ++s) {
-    int rc = c->warm_mode == M_LEARNED ? run_learned_substep(c, sdt, iters)
-                                       : run_substep(c, sdt, iters, M_PLAIN, c->record_diag);
+    int rc = c->warm_mode != PBF_WARM_PLAIN ? run_learned_substep(c, sdt, iters)
+                                            : run_substep(c, sdt, iters, M_PLAIN, c->record_diag);
     if (rc != PBF_OK) return rc;
   }
   return PBF_OK;
@@ -890,7 +895,7 @@ int pbf_step(pbf_ctx *c, float dt, uint32_t substeps, uint32_t iterations) {
 int pbf_set_warm_start(pbf_ctx *c, const float *blob, size_t length, uint32_t mode, float relax,
                        double fallback_threshold, uint32_t fallback_iterations) {
   if (!c) return PBF_E_INVALID;
-  if (mode > PBF_WARM_LEARNED) return fail(c, PBF_E_INVALID, "unknown warm-start mode");
+  if (mode > PBF_WARM_PREVIOUS) return fail(c, PBF_E_INVALID, "unknown warm-start mode");
   if (!(relax > 0.0f && relax <= 1.0f)) return fail(c, PBF_E_INVALID, "relax must be in (0, 1]");
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -911,10 +916,11 @@ int pbf_set_warm_start(pbf_ctx *c, const float *blob, size_t length, uint32_t mo
   }
   if (fallback_threshold < 0.0 || !std::isfinite(fallback_threshold))
     return fail(c, PBF_E_INVALID, "fallback_threshold must be finite and >= 0");
-  if (mode == PBF_WARM_LEARNED) {
+  if (mode != PBF_WARM_PLAIN) {
     int rc = ensure_learn_buffers(c);
     if (rc != PBF_OK) return rc;
   }
+  c->relax_prev = relax;
   if (mode != c->warm_mode && c->lam_acc) HIPCHK(hipMemsetAsync(c->lam_acc, 0, (size_t)c->ncap * 4, c->stream));
   c->warm_mode = mode;
   c->fb_threshold = fallback_threshold;
@@ -964,7 +970,7 @@ int pbf_collect_training(pbf_ctx *c, float dt, uint32_t long_iterations, uint32_
   // ... and the state then advances by one substep of the selected solver
   copy_state(c, false);
   const uint32_t iters = iterations ? iterations : c->cfg.iteration;
-  if (c->warm_mode == M_LEARNED) return run_learned_substep(c, dt, iters);
+  if (c->warm_mode != PBF_WARM_PLAIN) return run_learned_substep(c, dt, iters);
   return run_substep(c, dt, iters, M_COLLECT, c->record_diag);
 }
 

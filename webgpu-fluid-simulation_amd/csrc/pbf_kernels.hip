@@ -276,6 +276,19 @@ __global__ void __launch_bounds__(BS) k_settle_extra(uint32_t n, const uint32_t 
   vel_out[f] = vel_in[p];
 }
 
+// Temporal warm start: iteration 0 applies relax x the previous substep's
+// lambda sum (kept when it is not a finite non-positive number).
+__global__ void __launch_bounds__(BS) k_warm_previous(uint32_t n, const float *__restrict__ acc_prev, float4 *X,
+                                                      float *__restrict__ lam_acc, float relax) {
+  const uint32_t i = slot_index();
+  if (i >= n) return;
+  float *wp = reinterpret_cast<float *>(X) + 4 * (size_t)i + 3;
+  float lam = relax * acc_prev[i];
+  if (!(lam <= 0.0f) || !isfinite(lam)) lam = *wp;
+  *wp = lam;
+  lam_acc[i] = lam;
+}
+
 __global__ void __launch_bounds__(BS) k_copy_words(uint32_t *__restrict__ dst, const uint32_t *__restrict__ src,
                                                    size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -857,6 +870,11 @@ void launch_settle_extra(const LaunchCtx &L, const uint32_t *tmp_key, const uint
   if (!L.n) return;
   go(L.stream, nullptr, nullptr, k_settle_extra, dim3(nblocks(L.n, BS)), dim3(BS), L.n, tmp_key, tmp_src, cellkey,
      offsets, acc_in, acc_out, vel_in, vel_out);
+}
+
+void launch_warm_previous(const LaunchCtx &L, const float *acc_prev, float4 *X, float *lam_acc, float relax) {
+  if (!L.n) return;
+  go(L.stream, nullptr, nullptr, k_warm_previous, dim3(nblocks(L.n, BS)), dim3(BS), L.n, acc_prev, X, lam_acc, relax);
 }
 
 void launch_copy(hipStream_t st, void *dst, const void *src, size_t bytes) {

@@ -177,7 +177,7 @@ def copy_bandwidth(nbytes: int = 1 << 30, iterations: int = 20) -> float:
 
 MLP_PARAMS = 5313
 MLP_FEATURES = 16
-WARM_PLAIN, WARM_LEARNED = 0, 1
+WARM_PLAIN, WARM_LEARNED, WARM_PREVIOUS = 0, 1, 2
 
 
 def mlp_blob_size() -> int:
