@@ -120,6 +120,53 @@ def warm_start_comparison(sc, substeps, plain_its, learned_its, relax, blob, pre
     return out
 
 
+def batched_measure(substeps: int, warmup: int, tune: bool):
+    """The 256-scene batch (one context, every pass one launch over all
+    scenes, hipGraph replay) against one context per scene."""
+    t0 = time.perf_counter()
+    batch = scenes.batch_scenes()
+    log(f"batch: {len(batch)} scenes, {sum(s.n for s in batch)} particles, built in {time.perf_counter() - t0:.1f} s")
+    n = sum(s.n for s in batch)
+    dt = batch[0].substep_dt()
+    it = batch[0].iterations
+    sim = scenes.make_batch_engine(batch)
+    tuning = sim.autotune(dt, it, 3) if tune else None
+    for _ in range(warmup):
+        sim.step(dt, 1, it)
+    sim.sync()
+    t1 = time.perf_counter()
+    for _ in range(substeps):
+        sim.step(dt, 1, it)
+    sim.sync()
+    el = time.perf_counter() - t1
+    mean_err, max_err = sim.density_error()
+    sim.close()
+    # one context per scene, each on its own stream
+    sims = [scenes.make_engine(s) for s in batch]
+    for s in sims:
+        for _ in range(warmup):
+            s.step(dt, 1, it)
+    for s in sims:
+        s.sync()
+    t2 = time.perf_counter()
+    for _ in range(substeps):
+        for s in sims:
+            s.step(dt, 1, it)
+    for s in sims:
+        s.sync()
+    el_sep = time.perf_counter() - t2
+    for s in sims:
+        s.close()
+    out = {"scenes": len(batch), "particles": n, "substeps": substeps, "iterations": it,
+           "particle_substeps_per_s": n * substeps / el, "ms_per_substep": 1e3 * el / substeps,
+           "one_context_per_scene_particle_substeps_per_s": n * substeps / el_sep,
+           "speedup_vs_one_context_per_scene": el_sep / el, "density_error_after": {"mean": mean_err, "max": max_err},
+           "tuning": tuning}
+    log(f"batched: {out['particle_substeps_per_s'] / 1e6:.1f} M particle-substeps/s; one context per scene "
+        f"{out['one_context_per_scene_particle_substeps_per_s'] / 1e6:.1f} M ({out['speedup_vs_one_context_per_scene']:.2f}x)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +177,8 @@ def main():
     ap.add_argument("--no-timing", action="store_true", help="no per-pass events (graph replay)")
     ap.add_argument("--no-graph-compare", action="store_true", help="skip the extra graph-replay measurement")
     ap.add_argument("--no-learned", action="store_true", help="skip the plain vs learned warm-start comparison")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-scene batched measurement")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch setting (no autotuning)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding).",
@@ -145,6 +194,10 @@ def main():
     # per-pass HIP events ride on the kernels' own dispatch packets
     flags = 0 if args.no_timing else PBF_FLAG_TIMING
     sim = scenes.make_engine(sc, flags=flags)
+    tuning = None
+    if not args.no_tune:
+        tuning = sim.autotune(sc.substep_dt(), sc.iterations, 3)
+        log(f"autotuned launch setting: {tuning}")
     for _ in range(args.warmup):
         sim.step(sc.dt, sc.substeps, sc.iterations)
     sim.sync()
@@ -159,6 +212,8 @@ def main():
     el_graph = None
     if not args.no_graph_compare and not args.no_timing:
         sim_g = scenes.make_engine(sc)
+        if tuning:
+            sim_g.set_tuning(tuning["walk_block"], tuning["list_variant"])
         for _ in range(args.warmup):
             sim_g.step(sc.dt, sc.substeps, sc.iterations)
         sim_g.sync()
@@ -218,6 +273,10 @@ def main():
         big = warm_start_comparison(sc, 100, (4, 8, 16, 32, 64), (4, 8), 0.5, blob, (4, 8, 16, 32), 0.5)
         warm = {"held_out_dam_break": small, "dam_break_1m": big}
 
+    batched = None
+    if not args.no_batch:
+        batched = batched_measure(200, 3, not args.no_tune)
+
     out = {
         "metric": "particle-substeps/s at 1M; iters to \u22640.5% density error, plain vs learned",
         "value": round(value, 1),
@@ -243,6 +302,8 @@ def main():
         "measured_copy_GBs": round(copy_gbs, 1),
         "density_error_after_run": {"mean": mean_err, "max": max_err},
         "learned_warm_start": warm,
+        "batched": batched,
+        "tuning": tuning,
         "passes": per_pass,
     }
     sim.close()

@@ -150,6 +150,11 @@ int pbf_clear_boundary(pbf_ctx *ctx);
  * No reference counterpart: the reference ships precomputed .cdm maps. */
 int pbf_build_volume_map(uint32_t kind, const double *shape, const int32_t res[3], const double lo[3],
                          const double hi[3], double kernel_radius, double particle_radius, float *sdf_vol);
+/* The same map built on the GPU (same definition, quadrature and float64
+ * arithmetic order, one thread per node); host arrays in and out.
+ * PBF_E_DEVICE without a GPU. */
+int pbf_build_volume_map_gpu(uint32_t kind, const double *shape, const int32_t res[3], const double lo[3],
+                             const double hi[3], double kernel_radius, double particle_radius, float *sdf_vol);
 
 /* Reference: PBF.run (PBF.ts:398-448) encodes one substep with the
  * compile-time dt and the configured iteration count.  pbf_step runs
@@ -269,6 +274,23 @@ int pbf_warm_start_stats(const pbf_ctx *ctx, uint64_t *learned_substeps, uint64_
  * next pair's features read. */
 int pbf_collect_training(pbf_ctx *ctx, float dt, uint32_t long_iterations, uint32_t iterations, float *features,
                          float *targets, uint32_t capacity, uint32_t *count);
+
+/* ---- Launch tuning -------------------------------------------------------
+ * No reference counterpart (WebGPU fixes workgroup sizes in the shaders).
+ * The work-group size of each neighbour walk and the neighbour-list variant
+ * change only the time, never a result: every particle's arithmetic is the
+ * same under any setting. */
+typedef struct {
+  uint32_t walk_block[8];  /* lambda, dp, density, vorticity, xsph, diag, -, -: 64, 128, 192 or 256 */
+  uint32_t list_variant;   /* 0..3: (LDS rows, work-group) = (32,128), (16,128), (32,64), (48,64) */
+  uint32_t reserved[7];
+} pbf_tuning;
+int pbf_get_tuning(const pbf_ctx *ctx, pbf_tuning *out);
+int pbf_set_tuning(pbf_ctx *ctx, const pbf_tuning *tuning);
+/* Times `repeats` substeps of the current scene (dt, iterations) under each
+ * candidate setting, keeps per pass the fastest, and leaves the simulation
+ * state as it found it.  out (may be NULL) receives the chosen setting. */
+int pbf_autotune(pbf_ctx *ctx, float dt, uint32_t iterations, uint32_t repeats, pbf_tuning *out);
 
 #ifdef __cplusplus
 }

@@ -108,10 +108,12 @@ struct pbf_ctx {
   float *lam_acc = nullptr, *acc_sorted = nullptr, *feats = nullptr;
   float4 *vel_sorted = nullptr;
   // state saved before a learned substep that may be rejected
-  float4 *snap_pos = nullptr, *snap_vel = nullptr, *snap_acc = nullptr;
+  float4 *snap_pos = nullptr, *snap_vel = nullptr, *snap_acc = nullptr, *snap_om = nullptr, *snap_nrm = nullptr;
   uint32_t *snap_ids = nullptr;
   float *snap_lam = nullptr, *snap_lacc = nullptr;
   uint64_t learned_substeps = 0, fallbacks = 0;
+  // launch tuning
+  pbf_tuning tuning = {{256, 256, 256, 256, 256, 256, 256, 256}, 0, {0, 0, 0, 0, 0, 0, 0}};
 };
 
 namespace {
@@ -216,6 +218,10 @@ struct PassTimer {
   hipEvent_t a = nullptr, b = nullptr;
   LaunchCtx begin(int p, const LaunchCtx &L) {
     LaunchCtx t = L;
+    const int walk = p == P_LAMBDA ? 0 : p == P_DP ? 1 : p == P_DENSITY ? 2 : p == P_VORT ? 3 : p == P_XSPH ? 4
+                   : p == P_DIAG ? 5 : -1;
+    if (walk >= 0) t.bs = c->tuning.walk_block[walk];
+    t.lvar = c->tuning.list_variant;
     if (!on) return t;
     pass = p;
     a = take_event(c);
@@ -367,19 +373,24 @@ int ensure_snapshot(pbf_ctx *c) {
   HIPCHK(dalloc(&c->snap_pos, N));
   HIPCHK(dalloc(&c->snap_vel, N));
   HIPCHK(dalloc(&c->snap_acc, N));
+  HIPCHK(dalloc(&c->snap_om, N));
+  HIPCHK(dalloc(&c->snap_nrm, N));
   HIPCHK(dalloc(&c->snap_ids, N));
   HIPCHK(dalloc(&c->snap_lam, N));
   HIPCHK(dalloc(&c->snap_lacc, N));
   return PBF_OK;
 }
 
-// Copies the state carried from one substep to the next (slot order).
+// Copies the state carried from one substep to the next (slot order), and
+// the readable omega / normal fields.
 void copy_state(pbf_ctx *c, bool save) {
   const size_t n = c->n;
   struct { void *live, *snap; size_t bytes; } t[] = {
       {c->pos, c->snap_pos, 16 * n}, {c->vel, c->snap_vel, 16 * n},   {c->acc, c->snap_acc, 16 * n},
-      {c->ids, c->snap_ids, 4 * n},  {c->lam, c->snap_lam, 4 * n},    {c->lam_acc, c->snap_lacc, 4 * n}};
-  for (auto &e : t) launch_copy(c->stream, save ? e.snap : e.live, save ? e.live : e.snap, e.bytes);
+      {c->ids, c->snap_ids, 4 * n},  {c->lam, c->snap_lam, 4 * n},    {c->lam_acc, c->snap_lacc, 4 * n},
+      {c->om, c->snap_om, 16 * n},   {c->nrm, c->snap_nrm, 16 * n}};
+  for (auto &e : t)
+    if (e.live && e.snap) launch_copy(c->stream, save ? e.snap : e.live, save ? e.live : e.snap, e.bytes);
 }
 
 int mean_density_error(pbf_ctx *c, double *mean);
@@ -734,7 +745,8 @@ int pbf_destroy(pbf_ctx *c) {
                   c->cellkey, c->rank, c->tmp_key, c->tmp_src, c->ncount, c->nbr, c->lam, c->lam_sorted, c->counts,
                   c->offsets, c->flags, c->d_scenes, c->sid, c->ranges, c->pool, c->ci, c->per_scene, c->hist,
                   c->hist_ctr, c->dbg_counts, c->dbg_ids, c->ids_save, c->lam_acc, c->acc_sorted, c->feats,
-                  c->vel_sorted, c->snap_pos, c->snap_vel, c->snap_acc, c->snap_ids, c->snap_lam, c->snap_lacc};
+                  c->vel_sorted, c->snap_pos, c->snap_vel, c->snap_acc, c->snap_om, c->snap_nrm, c->snap_ids,
+                  c->snap_lam, c->snap_lacc};
   mlp_engine_free(&c->mlp);
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -927,6 +939,85 @@ int pbf_set_warm_start(pbf_ctx *c, const float *blob, size_t length, uint32_t mo
   c->fb_iters = fallback_iterations;
   clear_graphs(c);
   HIPCHK(hipStreamSynchronize(c->stream));
+  return PBF_OK;
+}
+
+int pbf_get_tuning(const pbf_ctx *c, pbf_tuning *out) {
+  if (!c || !out) return PBF_E_INVALID;
+  *out = c->tuning;
+  return PBF_OK;
+}
+
+int pbf_set_tuning(pbf_ctx *c, const pbf_tuning *t) {
+  if (!c || !t) return PBF_E_INVALID;
+  for (int k = 0; k < 8; ++k)
+    if (t->walk_block[k] < 64 || t->walk_block[k] > 256 || t->walk_block[k] % 64)
+      return fail(c, PBF_E_INVALID, "walk_block must be 64, 128, 192 or 256");
+  if (t->list_variant >= kListVariants) return fail(c, PBF_E_INVALID, "list_variant must be 0..3");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->tuning = *t;
+  clear_graphs(c);
+  return PBF_OK;
+}
+
+int pbf_autotune(pbf_ctx *c, float dt, uint32_t iterations, uint32_t repeats, pbf_tuning *out) {
+  if (!c) return PBF_E_INVALID;
+  if (!(dt > 0.0f) || repeats == 0) return fail(c, PBF_E_INVALID, "dt must be > 0 and repeats >= 1");
+  if (c->n == 0) return fail(c, PBF_E_STATE, "no particles to tune on");
+  HIPCHK(hipSetDevice(c->device));
+  int rc = sync_ctx(c);
+  if (rc == PBF_OK) rc = ensure_snapshot(c);
+  if (rc != PBF_OK) return rc;
+  const uint32_t iters = iterations ? iterations : c->cfg.iteration;
+  copy_state(c, true);
+  const pbf_tuning keep = c->tuning;
+  const uint32_t keep_flags = c->params.flags;
+  const bool keep_lists = c->have_lists, keep_search = c->last_was_search;
+  c->params.flags |= PBF_FLAG_TIMING;
+  for (int p = 0; p < P_NPASS; ++p) { c->pass_ms[p] = 0.0; c->pass_count[p] = 0; }
+  // candidate k: every walk at block[k], the list at variant k
+  const uint32_t blocks[kListVariants] = {256, 128, 64, 192};
+  double ms[kListVariants][P_NPASS] = {};
+  for (uint32_t k = 0; k < kListVariants && rc == PBF_OK; ++k) {
+    for (int w = 0; w < 8; ++w) c->tuning.walk_block[w] = blocks[k];
+    c->tuning.list_variant = k;
+    copy_state(c, false);
+    for (uint32_t r = 0; r <= repeats && rc == PBF_OK; ++r) {
+      rc = run_substep(c, dt, iters, M_PLAIN, false);
+      if (rc == PBF_OK) rc = sync_ctx(c);
+      if (r == 0)  // the first substep warms the code and caches up
+        for (int p = 0; p < P_NPASS; ++p) { c->pass_ms[p] = 0.0; c->pass_count[p] = 0; }
+    }
+    for (int p = 0; p < P_NPASS; ++p) {
+      ms[k][p] = c->pass_ms[p];
+      c->pass_ms[p] = 0.0;
+      c->pass_count[p] = 0;
+    }
+  }
+  c->params.flags = keep_flags;
+  copy_state(c, false);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->have_lists = keep_lists;
+  c->last_was_search = keep_search;
+  if (rc != PBF_OK) {
+    c->tuning = keep;
+    return rc;
+  }
+  pbf_tuning best = keep;
+  const int walk_pass[6] = {P_LAMBDA, P_DP, P_DENSITY, P_VORT, P_XSPH, P_DIAG};
+  for (int w = 0; w < 6; ++w) {
+    uint32_t bk = 0;
+    for (uint32_t k = 1; k < kListVariants; ++k)
+      if (ms[k][walk_pass[w]] < ms[bk][walk_pass[w]]) bk = k;
+    best.walk_block[w] = ms[bk][walk_pass[w]] > 0.0 ? blocks[bk] : keep.walk_block[w];
+  }
+  uint32_t bl = 0;
+  for (uint32_t k = 1; k < kListVariants; ++k)
+    if (ms[k][P_LIST] < ms[bl][P_LIST]) bl = k;
+  best.list_variant = bl;
+  c->tuning = best;
+  clear_graphs(c);
+  if (out) *out = best;
   return PBF_OK;
 }
 

@@ -36,7 +36,11 @@ struct LaunchCtx {
   const float *pool;
   Phys P;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // pass timing: start / stop on the dispatch packets
+  uint32_t bs = 256;   // work-group size of the neighbour walks (64..256, a multiple of 64)
+  uint32_t lvar = 0;   // neighbour-list kernel variant (LDS rows x work-group size)
 };
+
+constexpr uint32_t kListVariants = 4;
 
 void launch_predict_bin(const LaunchCtx &L, const float4 *pos, const float4 *vel, const float4 *acc, float4 *xs,
                         uint32_t *cellkey, uint32_t *rank, uint32_t *counts);

@@ -299,13 +299,12 @@ __global__ void __launch_bounds__(BS) k_copy_words(uint32_t *__restrict__ dst, c
 // ---------------------------------------------------------------------------
 // K5-K7: neighbour list over the 27 cells of the frozen predicted position.
 // Dense grid: the three cells (x, y, z-1..z+1) are one contiguous slot range.
-// The first LIST_R entries of each particle are staged in LDS and written as
-// whole table rows (coalesced) once the block is done; later entries are
-// stored directly.
-constexpr int LIST_BS = 128;
-constexpr int LIST_R = 32;
+// The first R entries of each particle are staged in LDS and written as
+// whole table rows (coalesced) once the block of B slots is done; later
+// entries are stored directly.  (R, B) is chosen at run time among
+// kListVariants instantiations (the autotuner's list knob).
 
-template <bool MULTI, bool HASH>
+template <bool MULTI, bool HASH, int LIST_R, int LIST_BS>
 __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4 *__restrict__ X0,
                                                         const uint32_t *__restrict__ offsets,
                                                         uint32_t *__restrict__ nbr, uint32_t *__restrict__ ncount,
@@ -325,6 +324,17 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
     const float4 xi = X0[i];
     const int cx = cell_axis(sc, xi.x, 0), cy = cell_axis(sc, xi.y, 1), cz = cell_axis(sc, xi.z, 2);
     const int D0 = sc.dims[0], D1 = sc.dims[1], D2 = sc.dims[2];
+    // Cells out of reach are skipped: the squared gap (in units of the cell
+    // size h) from the particle to a neighbour cell's box is a^2 + b^2 + c^2
+    // with a = fx (cell below), 0 (same), 1 - fx (cell above) per axis.  The
+    // 1e-3 slack covers the float rounding of the position, of the cell
+    // assignment and of the d2 <= h^2 test, so no candidate that could pass
+    // the test is skipped and the list (and its order) is unchanged.
+    const float fx = (xi.x - sc.lo[0]) * sc.inv_cell - (float)cx;
+    const float fy = (xi.y - sc.lo[1]) * sc.inv_cell - (float)cy;
+    const float fz = (xi.z - sc.lo[2]) * sc.inv_cell - (float)cz;
+    constexpr float kReach = 1.001f;
+    auto gap = [](int d, float f) { return d < 0 ? f * f : d > 0 ? (1.0f - f) * (1.0f - f) : 0.0f; };
     const uint32_t cbase = sc.cell_base;
     auto test = [&](const float4 &xj, uint32_t j) {
       float d2 = dist2(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
@@ -343,19 +353,26 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
       for (; j < t1; ++j) test(X0[j], j);
     };
     if constexpr (!HASH) {
-      const int zlo = cz > 0 ? cz - 1 : 0, zhi = cz + 1 < D2 ? cz + 1 : D2 - 1;
+      const float gz_lo = gap(-1, fz), gz_hi = gap(1, fz);
       for (int dx = -1; dx <= 1; ++dx) {
         const int nx = cx + dx;
         if (nx < 0 || nx >= D0) continue;
+        const float gx = gap(dx, fx);
         for (int dy = -1; dy <= 1; ++dy) {
           const int ny = cy + dy;
           if (ny < 0 || ny >= D1) continue;
+          const float gxy = gx + gap(dy, fy);
+          if (gxy > kReach) continue;
+          const int zlo = (cz > 0 && gxy + gz_lo <= kReach) ? cz - 1 : cz;
+          const int zhi = (cz + 1 < D2 && gxy + gz_hi <= kReach) ? cz + 1 : cz;
           uint32_t c0 = cbase + cell_local(sc, nx, ny, zlo);
           uint32_t c1 = cbase + cell_local(sc, nx, ny, zhi);
           visit(offsets[c0], offsets[c1 + 1]);
         }
       }
     } else {
+      // hashed buckets are visited at their first occurrence among the 27
+      // cells; skipping a cell could move that occurrence, so no pruning here
       uint32_t seen[27];
       int nseen = 0;
       for (int dx = -1; dx <= 1; ++dx)
@@ -387,6 +404,7 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
   if (i < n)
     for (uint32_t k = 0; k < rows; ++k) nbr[(size_t)k * P.ld + i] = tile[k * LIST_BS + tid];
 }
+
 
 // ---------------------------------------------------------------------------
 // K8: boundary sample of every particle at the current iterate (fields of
@@ -809,24 +827,39 @@ void launch_settle(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *
      pos, lam_in, X0, Xold, ids, lam_out);
 }
 
+template <int R, int B>
+static void build_list_rb(const LaunchCtx &L, const float4 *X0, const uint32_t *offsets, uint32_t *nbr,
+                          uint32_t *ncount, uint32_t *flags) {
+  const dim3 g(nblocks(L.n, B)), b(B);
+  if (L.multi) {
+    if (L.hash) go(L.stream, L.ev0, L.ev1, k_build_list<true, true, R, B>, g, b, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
+    else go(L.stream, L.ev0, L.ev1, k_build_list<true, false, R, B>, g, b, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
+  } else {
+    if (L.hash) go(L.stream, L.ev0, L.ev1, k_build_list<false, true, R, B>, g, b, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
+    else go(L.stream, L.ev0, L.ev1, k_build_list<false, false, R, B>, g, b, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
+  }
+}
+
+// List variants (LDS rows R, work-group size B): 0 (32, 128), 1 (16, 128),
+// 2 (32, 64), 3 (48, 64): 16, 8, 8 and 12 KB of LDS.  (Staging each
+// wavefront's nine candidate column ranges in LDS was measured too: 30-60 %
+// slower, the candidate loads are L1 hits and not what bounds this pass.)
 void launch_build_list(const LaunchCtx &L, const float4 *X0, const uint32_t *offsets, uint32_t *nbr, uint32_t *ncount,
                        uint32_t *flags) {
   if (!L.n) return;
-  const dim3 g(nblocks(L.n, LIST_BS)), b(LIST_BS);
-  if (L.multi) {
-    if (L.hash) go(L.stream, L.ev0, L.ev1, k_build_list<true, true>, g, b, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
-    else go(L.stream, L.ev0, L.ev1, k_build_list<true, false>, g, b, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
-  } else {
-    if (L.hash) go(L.stream, L.ev0, L.ev1, k_build_list<false, true>, g, b, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
-    else go(L.stream, L.ev0, L.ev1, k_build_list<false, false>, g, b, L.n, X0, offsets, nbr, ncount, flags, L.s0, L.scenes, L.sid, L.P);
+  switch (L.lvar) {
+    case 1: build_list_rb<16, 128>(L, X0, offsets, nbr, ncount, flags); break;
+    case 2: build_list_rb<32, 64>(L, X0, offsets, nbr, ncount, flags); break;
+    case 3: build_list_rb<48, 64>(L, X0, offsets, nbr, ncount, flags); break;
+    default: build_list_rb<32, 128>(L, X0, offsets, nbr, ncount, flags); break;
   }
 }
 
 template <bool MULTI, template <bool, int> class Sel, typename... Args>
 static void fk_dispatch(const LaunchCtx &L, hipEvent_t e0, hipEvent_t e1, dim3 g, Args... args) {
-  if (L.fmode == 2) go(L.stream, e0, e1, Sel<MULTI, 2>::fn(), g, dim3(BS), args...);
-  else if (L.fmode == 1) go(L.stream, e0, e1, Sel<MULTI, 1>::fn(), g, dim3(BS), args...);
-  else go(L.stream, e0, e1, Sel<MULTI, 0>::fn(), g, dim3(BS), args...);
+  if (L.fmode == 2) go(L.stream, e0, e1, Sel<MULTI, 2>::fn(), g, dim3(L.bs), args...);
+  else if (L.fmode == 1) go(L.stream, e0, e1, Sel<MULTI, 1>::fn(), g, dim3(L.bs), args...);
+  else go(L.stream, e0, e1, Sel<MULTI, 0>::fn(), g, dim3(L.bs), args...);
 }
 template <bool M, int F> struct SelLambda { static auto fn() { return k_lambda<M, F, false>; } };
 template <bool M, int F> struct SelLambdaAcc { static auto fn() { return k_lambda<M, F, true>; } };
@@ -844,8 +877,8 @@ static hipEvent_t boundary_pass(const LaunchCtx &L, float4 *X, float4 *B, dim3 g
 void launch_lambda(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr, const uint32_t *ncount,
                    float *lam_acc) {
   if (!L.n) return;
-  const dim3 g(nblocks(L.n, BS));
-  hipEvent_t e0 = boundary_pass(L, X, B, g);
+  hipEvent_t e0 = boundary_pass(L, X, B, dim3(nblocks(L.n, BS)));
+  const dim3 g(nblocks(L.n, L.bs));
   if (lam_acc) {
     if (L.multi) fk_dispatch<true, SelLambdaAcc>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lam_acc);
     else fk_dispatch<false, SelLambdaAcc>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lam_acc);
@@ -858,8 +891,8 @@ void launch_lambda(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr
 void launch_lambda_features(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr, const uint32_t *ncount,
                             const LearnBufs &lb) {
   if (!L.n) return;
-  const dim3 g(nblocks(L.n, BS));
-  hipEvent_t e0 = boundary_pass(L, X, B, g);
+  hipEvent_t e0 = boundary_pass(L, X, B, dim3(nblocks(L.n, BS)));
+  const dim3 g(nblocks(L.n, L.bs));
   if (L.multi) fk_dispatch<true, SelLambdaFeat>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lb);
   else fk_dispatch<false, SelLambdaFeat>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lb);
 }
@@ -886,7 +919,7 @@ void launch_copy(hipStream_t st, void *dst, const void *src, size_t bytes) {
 void launch_dp(const LaunchCtx &L, const float4 *X, const float4 *B, const uint32_t *nbr, const uint32_t *ncount,
                float4 *Xn) {
   if (!L.n) return;
-  const dim3 g(nblocks(L.n, BS));
+  const dim3 g(nblocks(L.n, L.bs));
   if (L.multi) fk_dispatch<true, SelDp>(L, L.ev0, L.ev1, g, L.n, X, B, nbr, ncount, Xn, L.s0, L.scenes, L.sid, L.P);
   else fk_dispatch<false, SelDp>(L, L.ev0, L.ev1, g, L.n, X, B, nbr, ncount, Xn, L.s0, L.scenes, L.sid, L.P);
 }
@@ -894,26 +927,26 @@ void launch_dp(const LaunchCtx &L, const float4 *X, const float4 *B, const uint3
 void launch_density(const LaunchCtx &L, const float4 *X, const float4 *Xold, const uint32_t *nbr,
                     const uint32_t *ncount, float4 *pos, float4 *vel, float *lam) {
   if (!L.n) return;
-  go(L.stream, L.ev0, L.ev1, k_density, dim3(nblocks(L.n, BS)), dim3(BS), L.n, X, Xold, nbr, ncount, pos, vel, lam, L.P);
+  go(L.stream, L.ev0, L.ev1, k_density, dim3(nblocks(L.n, L.bs)), dim3(L.bs), L.n, X, Xold, nbr, ncount, pos, vel, lam, L.P);
 }
 
 void launch_vorticity(const LaunchCtx &L, const float4 *pos, const float4 *vel, const uint32_t *nbr,
                       const uint32_t *ncount, float4 *om, float4 *nrm) {
   if (!L.n) return;
-  go(L.stream, L.ev0, L.ev1, k_vorticity, dim3(nblocks(L.n, BS)), dim3(BS), L.n, pos, vel, nbr, ncount, om, nrm, L.P);
+  go(L.stream, L.ev0, L.ev1, k_vorticity, dim3(nblocks(L.n, L.bs)), dim3(L.bs), L.n, pos, vel, nbr, ncount, om, nrm, L.P);
 }
 
 void launch_xsph(const LaunchCtx &L, const float4 *pos, const float4 *vel, const float4 *om, const float4 *nrm,
                  const uint32_t *nbr, const uint32_t *ncount, float4 *acc) {
   if (!L.n) return;
-  go(L.stream, L.ev0, L.ev1, k_xsph, dim3(nblocks(L.n, BS)), dim3(BS), L.n, pos, vel, om, nrm, nbr, ncount, acc, L.P);
+  go(L.stream, L.ev0, L.ev1, k_xsph, dim3(nblocks(L.n, L.bs)), dim3(L.bs), L.n, pos, vel, om, nrm, nbr, ncount, acc, L.P);
 }
 
 void launch_diag(const LaunchCtx &L, const float4 *pos, const uint32_t *nbr, const uint32_t *ncount, double *ci,
                  const uint32_t *ranges, uint32_t nscenes, double *per_scene, double *hist, uint32_t hist_cap,
                  uint32_t *hist_ctr) {
   if (!L.n) return;
-  const dim3 g(nblocks(L.n, BS));
+  const dim3 g(nblocks(L.n, L.bs));
   if (L.multi) fk_dispatch<true, SelDiag>(L, L.ev0, nullptr, g, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
   else fk_dispatch<false, SelDiag>(L, L.ev0, nullptr, g, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
   go(L.stream, nullptr, hist ? nullptr : L.ev1, k_diag_reduce, dim3(nscenes), dim3(1024), (const double *)ci, ranges, per_scene);

@@ -20,6 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpbf.so")
 
 PBF_OK = 0
+PBF_E_INVALID, PBF_E_CAPACITY, PBF_E_LENGTH, PBF_E_FORMAT, PBF_E_DEVICE, PBF_E_OVERFLOW, PBF_E_STATE = range(-1, -8, -1)
 PBF_GRID_DENSE, PBF_GRID_HASH = 0, 1
 PBF_FLAG_NO_GRAPH, PBF_FLAG_CUBIC_FIXED, PBF_FLAG_TIMING = 1, 2, 4
 FIELDS = {"position": 0, "velocity": 1, "accel": 2, "lambda": 3, "omega": 4, "normal": 5,
@@ -46,6 +47,10 @@ class Params(C.Structure):
 class Config(C.Structure):
     _fields_ = [("iteration", C.c_uint32), ("xsph", C.c_float), ("vorticity", C.c_float),
                 ("surface_tension", C.c_float), ("gravity", C.c_float)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("walk_block", C.c_uint32 * 8), ("list_variant", C.c_uint32), ("reserved", C.c_uint32 * 7)]
 
 
 class SceneDesc(C.Structure):
@@ -86,6 +91,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "pbf_set_volume_map": (I, [P, U32, C.POINTER(C.c_int32), pd3, pd3, pf]),
         "pbf_clear_boundary": (I, [P]),
         "pbf_build_volume_map": (I, [U32, pd3, C.POINTER(C.c_int32), pd3, pd3, D, D, pf]),
+        "pbf_build_volume_map_gpu": (I, [U32, pd3, C.POINTER(C.c_int32), pd3, pd3, D, D, pf]),
         "pbf_step": (I, [P, F, U32, U32]), "pbf_run": (I, [P]), "pbf_sync": (I, [P]),
         "pbf_read_positions": (I, [P, pf, U32]), "pbf_read_velocities": (I, [P, pf, U32]),
         "pbf_read_field": (I, [P, U32, pf, U32]),
@@ -107,6 +113,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "pbf_set_warm_start": (I, [P, pf, S, U32, F, D, U32]),
         "pbf_warm_start_stats": (I, [P, C.POINTER(U64), C.POINTER(U64)]),
         "pbf_collect_training": (I, [P, F, U32, U32, pf, pf, U32, pu]),
+        "pbf_get_tuning": (I, [P, C.POINTER(Tuning)]),
+        "pbf_set_tuning": (I, [P, C.POINTER(Tuning)]),
+        "pbf_autotune": (I, [P, F, U32, U32, C.POINTER(Tuning)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -132,17 +141,31 @@ def _uptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_uint32))
 
 
+_map_on_gpu = None
+
+
 def build_volume_map(kind: int, res, lo, hi, kernel_radius: float, shape=None,
-                     particle_radius: float = 0.0) -> np.ndarray:
-    """Analytic volume map (res+1)^3 x (sdf, vol); kind 0 box, 1 box + sphere + torus."""
+                     particle_radius: float = 0.0, where: str = "auto") -> np.ndarray:
+    """Analytic volume map (res+1)^3 x (sdf, vol); kind 0 box, 1 box + sphere + torus.
+    where: "gpu", "host", or "auto" (the GPU when one answers, else the host;
+    both give the same map)."""
+    global _map_on_gpu
     lib = load_library()
     n = (res[0] + 1) * (res[1] + 1) * (res[2] + 1)
     out = np.zeros((n, 2), np.float32)
     sh = None
     if shape is not None:
         sh = (C.c_double * len(shape))(*[float(x) for x in shape])
-    rc = lib.pbf_build_volume_map(kind, sh, _i3(res), _d3(lo), _d3(hi), float(kernel_radius), float(particle_radius),
-                                  _fptr(out))
+    args = (kind, sh, _i3(res), _d3(lo), _d3(hi), float(kernel_radius), float(particle_radius), _fptr(out))
+    if where in ("gpu", "auto") and _map_on_gpu is not False:
+        rc = lib.pbf_build_volume_map_gpu(*args)
+        if rc == PBF_OK:
+            _map_on_gpu = True
+            return out
+        if where == "gpu" or rc != PBF_E_DEVICE:
+            raise PBFError(rc, "pbf_build_volume_map_gpu failed")
+        _map_on_gpu = False
+    rc = lib.pbf_build_volume_map(*args)
     if rc != PBF_OK:
         raise PBFError(rc, "pbf_build_volume_map failed")
     return out
@@ -445,3 +468,25 @@ class PBF:
         self._check(self.lib.pbf_collect_training(self.ctx, float(dt), int(long_iterations), int(iterations),
                                                   _fptr(f), _fptr(t), max(n, 1), C.byref(cnt)))
         return f[: cnt.value], t[: cnt.value]
+
+    # -- launch tuning (no reference counterpart) ---------------------------
+    WALKS = ("lambda", "dp", "density", "vorticity", "xsph", "diag")
+
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        self._check(self.lib.pbf_get_tuning(self.ctx, C.byref(t)))
+        return {"walk_block": dict(zip(self.WALKS, list(t.walk_block)[:6])), "list_variant": int(t.list_variant)}
+
+    def set_tuning(self, walk_block=None, list_variant: int = 0):
+        t = Tuning()
+        wb = walk_block if walk_block is not None else {}
+        for k, name in enumerate(self.WALKS):
+            t.walk_block[k] = int(wb.get(name, 256)) if isinstance(wb, dict) else int(wb)
+        t.walk_block[6] = t.walk_block[7] = 256
+        t.list_variant = int(list_variant)
+        self._check(self.lib.pbf_set_tuning(self.ctx, C.byref(t)))
+
+    def autotune(self, dt: float, iterations: int = 0, repeats: int = 3) -> dict:
+        t = Tuning()
+        self._check(self.lib.pbf_autotune(self.ctx, float(dt), int(iterations), int(repeats), C.byref(t)))
+        return {"walk_block": dict(zip(self.WALKS, list(t.walk_block)[:6])), "list_variant": int(t.list_variant)}
