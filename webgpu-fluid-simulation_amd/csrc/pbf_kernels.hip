@@ -48,6 +48,15 @@ __device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ n
                                                   uint32_t cnt, F &&f) {
   const uint32_t *row = nbr + i;
   uint32_t k = 0;
+  if constexpr (UNROLL >= 8) {
+    for (; k + 8 <= cnt; k += 8) {  // eight index loads in flight, then eight bodies
+      uint32_t j[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) j[q] = row[(size_t)(k + q) * ld];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) f(j[q]);
+    }
+  }
   if constexpr (UNROLL >= 4) {
     for (; k + 4 <= cnt; k += 4) {  // four index loads in flight, then four bodies
       const uint32_t j0 = row[(size_t)k * ld], j1 = row[(size_t)(k + 1) * ld];
@@ -346,30 +355,37 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
     };
     auto visit = [&](uint32_t t0, uint32_t t1) {
       uint32_t j = t0;
-      for (; j + 4 <= t1; j += 4) {  // four candidate loads in flight
+      for (; j + 8 <= t1; j += 8) {  // eight candidate loads in flight
+        float4 c[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) c[q] = X0[j + q];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) test(c[q], j + q);
+      }
+      for (; j + 4 <= t1; j += 4) {
         const float4 a = X0[j], b = X0[j + 1], c = X0[j + 2], d = X0[j + 3];
         test(a, j); test(b, j + 1); test(c, j + 2); test(d, j + 3);
       }
       for (; j < t1; ++j) test(X0[j], j);
     };
     if constexpr (!HASH) {
+      // the 18 range bounds are loaded up front so their latencies overlap
       const float gz_lo = gap(-1, fz), gz_hi = gap(1, fz);
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int nx = cx + dx;
-        if (nx < 0 || nx >= D0) continue;
-        const float gx = gap(dx, fx);
-        for (int dy = -1; dy <= 1; ++dy) {
-          const int ny = cy + dy;
-          if (ny < 0 || ny >= D1) continue;
-          const float gxy = gx + gap(dy, fy);
-          if (gxy > kReach) continue;
-          const int zlo = (cz > 0 && gxy + gz_lo <= kReach) ? cz - 1 : cz;
-          const int zhi = (cz + 1 < D2 && gxy + gz_hi <= kReach) ? cz + 1 : cz;
-          uint32_t c0 = cbase + cell_local(sc, nx, ny, zlo);
-          uint32_t c1 = cbase + cell_local(sc, nx, ny, zhi);
-          visit(offsets[c0], offsets[c1 + 1]);
-        }
+      uint32_t t0[9], t1[9];
+#pragma unroll
+      for (int r = 0; r < 9; ++r) {
+        const int dx = r / 3 - 1, dy = r % 3 - 1;
+        const int nx = cx + dx, ny = cy + dy;
+        const float gxy = gap(dx, fx) + gap(dy, fy);
+        t0[r] = t1[r] = 0;
+        if (nx < 0 || nx >= D0 || ny < 0 || ny >= D1 || gxy > kReach) continue;
+        const int zlo = (cz > 0 && gxy + gz_lo <= kReach) ? cz - 1 : cz;
+        const int zhi = (cz + 1 < D2 && gxy + gz_hi <= kReach) ? cz + 1 : cz;
+        t0[r] = offsets[cbase + cell_local(sc, nx, ny, zlo)];
+        t1[r] = offsets[cbase + cell_local(sc, nx, ny, zhi) + 1];
       }
+#pragma unroll
+      for (int r = 0; r < 9; ++r) visit(t0[r], t1[r]);
     } else {
       // hashed buckets are visited at their first occurrence among the 27
       // cells; skipping a cell could move that occurrence, so no pruning here
@@ -546,7 +562,7 @@ __global__ void __launch_bounds__(BS) k_dp(uint32_t n, const float4 *__restrict_
   const float4 xi = X[i];
   const float li = xi.w;
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for_each_neighbor(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
+  for_each_neighbor<8>(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
     const float4 xj = X[j];
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float fl = (li + xj.w) * spiky_f(P, dist2(dx, dy, dz));
@@ -577,7 +593,7 @@ __global__ void __launch_bounds__(BS) k_density(uint32_t n, const float4 *__rest
   if (i >= n) return;
   const float4 xi = X[i];
   float rho = 0.0f;
-  for_each_neighbor(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
+  for_each_neighbor<8>(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
     const float4 xj = X[j];
     rho += poly6(P, dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z));
   });
@@ -598,7 +614,7 @@ __global__ void __launch_bounds__(BS) k_vorticity(uint32_t n, const float4 *__re
   if (i >= n) return;
   const float4 xi = pos[i], vi = vel[i];
   float wx = 0.0f, wy = 0.0f, wz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
-  for_each_neighbor<2>(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
+  for_each_neighbor<8>(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
     const float4 xj = pos[j], vj = vel[j];
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
@@ -625,7 +641,7 @@ __global__ void __launch_bounds__(BS) k_xsph(uint32_t n, const float4 *__restric
   const float4 xi = pos[i], vi = vel[i], ni = nrm[i], wi = om[i];
   const float rhoi = xi.w;
   float ux = 0.0f, uy = 0.0f, uz = 0.0f, ex = 0.0f, ey = 0.0f, ez = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
-  for_each_neighbor<2>(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
+  for_each_neighbor<4>(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
     const float4 xj = pos[j], vj = vel[j], nj = nrm[j];
     const float wj = nj.w;
     const float rhoj = xj.w;
