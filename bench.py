@@ -66,6 +66,20 @@ def cpu_baseline(sc, budget_s: float):
 
 
 WARM_BLOB = os.path.join(ROOT, "webgpu-fluid-simulation_amd", "warm_start", "dam_break.f32")
+PROFILE = os.path.join(ROOT, "profiles", "dam_break_1m_kernels_and_traffic.md")
+
+
+def profiled_traffic(pass_name):
+    """HBM bytes per launch of a pass from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (scaled by the copy-kernel factors), or None."""
+    try:
+        for line in open(PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                kv = dict(t.split("=") for t in line.split(":", 1)[1].split())
+                return int(kv[pass_name]) if pass_name in kv else None
+    except OSError:
+        return None
+    return None
 TARGET_ERR = 0.005
 
 
@@ -242,9 +256,13 @@ def main():
         avg_s = per_pass[dom]["ms_total"] / per_pass[dom]["launches"] / 1e3
         bytes_per_launch = pb[dom] * N
         ach = bytes_per_launch / avg_s / 1e9
+        traffic = profiled_traffic(dom)
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": round(avg_s * 1e6, 2)}
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+                "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": round(avg_s * 1e6, 2),
+                "traffic_GBs": round(traffic / avg_s / 1e9, 1) if traffic else None,
+                "traffic_source": "profiles/dam_break_1m_kernels_and_traffic.md (rocprofv3 FETCH_SIZE x2, "
+                                  "WRITE_SIZE x1 per dispatch)" if traffic else None}
         for name in per_pass:
             if name in pb:
                 a = pb[name] * N / (per_pass[name]["us_avg"] * 1e-6) / 1e9
