@@ -112,6 +112,16 @@ int pbf_voxelize_cube_host(double particle_radius, const double min[3], const do
                            uint64_t seed, uint64_t serial0, float *out, uint64_t capacity, uint32_t *count);
 int pbf_voxelize_sphere_host(double particle_radius, const double center[3], double radius, double jitter,
                              uint64_t seed, uint64_t serial0, float *out, uint64_t capacity, uint32_t *count);
+/* Reference: SPH.voxelizeMesh (SPH.ts:151-187): the 2r lattice over the mesh's
+ * world-space bounding box, keeping points whose ray along the mesh's local
+ * +z axis first hits a triangle facing along the ray.  vertices: nverts xyz
+ * (mesh-local), indices: 3 per triangle, world: 4x4 column-major
+ * (three.js Matrix4.elements) or NULL for the identity. */
+int pbf_voxelize_mesh(pbf_ctx *ctx, const float *vertices, uint32_t nverts, const uint32_t *indices, uint32_t ntris,
+                      const double world[16], double jitter, uint64_t seed, uint32_t *added);
+int pbf_voxelize_mesh_host(double particle_radius, const float *vertices, uint32_t nverts, const uint32_t *indices,
+                           uint32_t ntris, const double world[16], double jitter, uint64_t seed, uint64_t serial0,
+                           float *out, uint64_t capacity, uint32_t *count);
 /* Number of staged particles (SPH.particleCount before upload). */
 uint32_t pbf_staged_count(const pbf_ctx *ctx);
 /* Reference: PBFConfig.setParticlePosition (PBFConfig.ts:48-62): uploads the

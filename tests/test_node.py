@@ -24,7 +24,8 @@ def test_addon_exports_reference_api():
           "version:PBF.version()}))") % (ADDON, os.path.join(NODE_DIR, "PBF.js"))
     out = json.loads(subprocess.run([NODE, "-e", js], capture_output=True, text=True, check=True).stdout)
     for m in ("reset", "setConfig", "voxelizeCube", "voxelizeSphere", "setParticlePosition", "setBoundaryData",
-              "run", "start", "stop", "switch", "step", "readPositions", "readVelocities", "densityError"):
+              "run", "start", "stop", "switch", "step", "readPositions", "readVelocities", "densityError",
+              "voxelizeMesh", "setWarmStart", "autotune", "setTuning"):
         assert m in out["methods"], m
     assert out["version"].startswith("pbf-mi355x")
 

@@ -818,6 +818,23 @@ int pbf_voxelize_sphere(pbf_ctx *c, const double center[3], double radius, doubl
   return PBF_OK;
 }
 
+int pbf_voxelize_mesh(pbf_ctx *c, const float *vertices, uint32_t nverts, const uint32_t *indices, uint32_t ntris,
+                      const double world[16], double jitter, uint64_t seed, uint32_t *added) {
+  if (!c) return PBF_E_INVALID;
+  uint32_t cnt = 0;
+  int rc = pbf_voxelize_mesh_host(c->params.particle_radius, vertices, nverts, indices, ntris, world, jitter, seed, 0,
+                                  nullptr, 0, &cnt);
+  if (rc != PBF_OK) return fail(c, rc, "voxelizeMesh: invalid mesh or matrix");
+  size_t at = c->staged.size();
+  c->staged.resize(at + 4 * (size_t)cnt);
+  rc = pbf_voxelize_mesh_host(c->params.particle_radius, vertices, nverts, indices, ntris, world, jitter, seed,
+                              c->staged_serial, c->staged.data() + at, cnt, &cnt);
+  if (rc != PBF_OK) return fail(c, rc, "voxelizeMesh failed");
+  c->staged_serial += cnt;
+  if (added) *added = cnt;
+  return PBF_OK;
+}
+
 uint32_t pbf_staged_count(const pbf_ctx *c) { return c ? (uint32_t)(c->staged.size() / 4) : 0u; }
 
 int pbf_set_particle_position(pbf_ctx *c) {

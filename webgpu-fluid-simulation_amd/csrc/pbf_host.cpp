@@ -277,6 +277,164 @@ extern "C" int pbf_voxelize_sphere_host(double r, const double center[3], double
   return PBF_OK;
 }
 
+// SPH.voxelizeMesh (SPH.ts:151-187): the lattice of spacing 2r over the
+// mesh's world-space bounding box (i outermost, k innermost), keeping a point
+// when the first triangle hit by the ray from it along the mesh's local +z
+// axis faces along the ray (the ray leaves the solid first).  The reference
+// casts the ray with three-mesh-bvh (absent here); this is a plain
+// double-precision ray / triangle test (Moller-Trumbore) over a 2-D grid of
+// the triangles' local-x/y bounds.  Hits at t <= 1e-12 (the origin on a
+// face) and rays in a face's plane do not count -- the reference leaves
+// those degenerate cases to its ray caster.  `world` is the mesh's 4x4
+// world matrix in column-major order (three.js Matrix4.elements), NULL for
+// the identity.
+namespace {
+struct MeshRays {
+  std::vector<double> v;          // local-space vertices, xyz
+  std::vector<uint32_t> tri;      // 3 per triangle
+  double lo[2], cell[2];
+  int nx = 1, ny = 1;
+  std::vector<std::vector<uint32_t>> bins;
+
+  void build() {
+    const size_t nt = tri.size() / 3;
+    double hi[2];
+    lo[0] = lo[1] = 1e300;
+    hi[0] = hi[1] = -1e300;
+    for (size_t q = 0; q < v.size() / 3; ++q)
+      for (int a = 0; a < 2; ++a) {
+        lo[a] = std::fmin(lo[a], v[3 * q + a]);
+        hi[a] = std::fmax(hi[a], v[3 * q + a]);
+      }
+    const int side = (int)std::max(1.0, std::min(512.0, std::sqrt((double)nt)));
+    nx = ny = side;
+    for (int a = 0; a < 2; ++a) cell[a] = std::fmax(hi[a] - lo[a], 1e-300) / side;
+    bins.assign((size_t)nx * ny, {});
+    for (size_t t = 0; t < nt; ++t) {
+      double bl[2] = {1e300, 1e300}, bh[2] = {-1e300, -1e300};
+      for (int e = 0; e < 3; ++e)
+        for (int a = 0; a < 2; ++a) {
+          bl[a] = std::fmin(bl[a], v[3 * tri[3 * t + e] + a]);
+          bh[a] = std::fmax(bh[a], v[3 * tri[3 * t + e] + a]);
+        }
+      int i0 = bin(bl[0], 0), i1 = bin(bh[0], 0), j0 = bin(bl[1], 1), j1 = bin(bh[1], 1);
+      for (int i = i0; i <= i1; ++i)
+        for (int j = j0; j <= j1; ++j) bins[(size_t)j * nx + i].push_back((uint32_t)t);
+    }
+  }
+  int bin(double x, int a) const {
+    int k = (int)std::floor((x - lo[a]) / cell[a]);
+    int n = a == 0 ? nx : ny;
+    return k < 0 ? 0 : k >= n ? n - 1 : k;
+  }
+  // +1 the first hit faces along +z, -1 against it, 0 no hit
+  int first_hit(const double o[3]) const {
+    if (o[0] < lo[0] - cell[0] || o[1] < lo[1] - cell[1]) return 0;
+    const std::vector<uint32_t> &b = bins[(size_t)bin(o[1], 1) * nx + bin(o[0], 0)];
+    double best = 1e300;
+    int face = 0;
+    for (uint32_t t : b) {
+      const double *A = &v[3 * tri[3 * t]], *B = &v[3 * tri[3 * t + 1]], *C = &v[3 * tri[3 * t + 2]];
+      const double e1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+      const double e2[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+      // direction d = (0, 0, 1): p = d x e2, det = e1 . p
+      const double p[3] = {-e2[1], e2[0], 0.0};
+      const double det = e1[0] * p[0] + e1[1] * p[1];
+      if (std::fabs(det) < 1e-300) continue;
+      const double inv = 1.0 / det;
+      const double s[3] = {o[0] - A[0], o[1] - A[1], o[2] - A[2]};
+      const double u = (s[0] * p[0] + s[1] * p[1]) * inv;
+      if (u < 0.0 || u > 1.0) continue;
+      const double qv[3] = {s[1] * e1[2] - s[2] * e1[1], s[2] * e1[0] - s[0] * e1[2], s[0] * e1[1] - s[1] * e1[0]};
+      const double w = qv[2] * inv;
+      if (w < 0.0 || u + w > 1.0) continue;
+      const double tt = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) * inv;
+      if (tt <= 1e-12 || tt >= best) continue;
+      best = tt;
+      // face normal (C - B) x (A - B) as three.js Triangle.getNormal; its z
+      const double cb[2] = {C[0] - B[0], C[1] - B[1]}, ab[2] = {A[0] - B[0], A[1] - B[1]};
+      const double nz = cb[0] * ab[1] - cb[1] * ab[0];
+      face = nz > 0.0 ? 1 : -1;
+    }
+    return face;
+  }
+};
+
+bool invert4(const double m[16], double r[16]) {  // column-major 4x4 inverse (cofactors)
+  double inv[16];
+  inv[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] + m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+  inv[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] - m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+  inv[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] + m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+  inv[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] - m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+  inv[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] - m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+  inv[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] + m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+  inv[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] - m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+  inv[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] + m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+  inv[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] + m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+  inv[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] - m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+  inv[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] + m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+  inv[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] - m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+  inv[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] - m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+  inv[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] + m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+  inv[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] - m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+  inv[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] + m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+  double det = m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12];
+  if (!(std::fabs(det) > 1e-300)) return false;
+  for (int k = 0; k < 16; ++k) r[k] = inv[k] / det;
+  return true;
+}
+
+void apply4(const double m[16], const double p[3], double out[3]) {  // column-major, w = 1
+  for (int a = 0; a < 3; ++a) out[a] = m[a] * p[0] + m[4 + a] * p[1] + m[8 + a] * p[2] + m[12 + a];
+}
+}  // namespace
+
+extern "C" int pbf_voxelize_mesh_host(double r, const float *vertices, uint32_t nverts, const uint32_t *indices,
+                                      uint32_t ntris, const double world[16], double jitter, uint64_t seed,
+                                      uint64_t serial0, float *out, uint64_t capacity, uint32_t *count) {
+  using namespace pbf;
+  if (!vertices || !indices || nverts == 0 || ntris == 0 || !(r > 0.0)) return PBF_E_INVALID;
+  for (uint64_t q = 0; q < 3ull * ntris; ++q)
+    if (indices[q] >= nverts) return PBF_E_INVALID;
+  double M[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1}, Minv[16];
+  if (world) std::memcpy(M, world, sizeof(M));
+  if (!invert4(M, Minv)) return PBF_E_INVALID;
+  MeshRays R;
+  R.v.resize(3 * (size_t)nverts);
+  R.tri.assign(indices, indices + 3 * (size_t)ntris);
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (uint32_t q = 0; q < nverts; ++q) {
+    double p[3] = {vertices[3 * q], vertices[3 * q + 1], vertices[3 * q + 2]}, w[3];
+    for (int a = 0; a < 3; ++a) R.v[3 * q + a] = p[a];
+    apply4(M, p, w);
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::fmin(lo[a], w[a]);
+      hi[a] = std::fmax(hi[a], w[a]);
+    }
+  }
+  R.build();
+  const double diam = 2.0 * r;
+  const long nx = (long)std::floor((hi[0] - lo[0]) / diam);
+  const long ny = (long)std::floor((hi[1] - lo[1]) / diam);
+  const long nz = (long)std::floor((hi[2] - lo[2]) / diam);
+  uint64_t q = 0;
+  for (long i = 0; i < nx; ++i)
+    for (long j = 0; j < ny; ++j)
+      for (long k = 0; k < nz; ++k) {
+        double p[3] = {(double)i * diam + lo[0], (double)j * diam + lo[1], (double)k * diam + lo[2]}, o[3];
+        apply4(Minv, p, o);
+        if (R.first_hit(o) <= 0) continue;
+        if (out) {
+          if (q >= capacity) return PBF_E_CAPACITY;
+          for (int a = 0; a < 3; ++a) out[4 * q + a] = (float)(p[a] + jitter_value(seed, serial0 + q, a, jitter));
+          out[4 * q + 3] = 0.0f;
+        }
+        ++q;
+      }
+  if (count) *count = (uint32_t)q;
+  return PBF_OK;
+}
+
 extern "C" int pbf_parse_cdm(const char *text, size_t length, int32_t res[3], float *sdf, float *vol,
                              size_t capacity, size_t *field_length) {
   using namespace pbf;

@@ -20,6 +20,15 @@ class PBF {
   voxelizeSphere(center, radius, jitter, seed) {
     return addon.voxelizeSphere(this.handle, center, radius, jitter || 0, seed || 0);
   }
+  voxelizeMesh(positions, index, matrixWorld, jitter, seed) {
+    return addon.voxelizeMesh(this.handle, positions, index, matrixWorld || null, jitter || 0, seed || 0);
+  }
+  setWarmStart(blob, mode, relax, fallbackThreshold, fallbackIterations) {
+    addon.setWarmStart(this.handle, blob || null, mode === undefined ? 1 : mode, relax === undefined ? 0.5 : relax,
+                       fallbackThreshold || 0, fallbackIterations || 0);
+  }
+  autotune(dt, iterations, repeats) { return addon.autotune(this.handle, dt, iterations || 0, repeats || 3); }
+  setTuning(walkBlock, listVariant) { addon.setTuning(this.handle, walkBlock, listVariant); }
   setParticlePosition(positions, count) {
     if (positions) addon.setParticlePosition(this.handle, positions, count === undefined ? positions.length / 4 : count);
     else addon.setParticlePosition(this.handle);

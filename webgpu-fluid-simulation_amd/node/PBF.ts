@@ -42,6 +42,21 @@ export class PBF {
   public voxelizeSphere(center: Vec3, radius: number, jitter = 0, seed = 0): number {
     return addon.voxelizeSphere(this.handle, center, radius, jitter, seed);
   }
+  /** SPH.voxelizeMesh (SPH.ts:151-187): pass mesh.geometry.attributes.position.array,
+   *  mesh.geometry.index.array and mesh.matrixWorld.elements. */
+  public voxelizeMesh(positions: Float32Array, index: Uint32Array | Uint16Array, matrixWorld?: ArrayLike<number>,
+                      jitter = 0, seed = 0): number {
+    return addon.voxelizeMesh(this.handle, positions, index, matrixWorld ?? null, jitter, seed);
+  }
+  /** Learned warm start (no reference counterpart): mode 0 plain, 1 network, 2 previous lambda sum. */
+  public setWarmStart(blob: Float32Array | null, mode = 1, relax = 0.5, fallbackThreshold = 0, fallbackIterations = 0): void {
+    addon.setWarmStart(this.handle, blob, mode, relax, fallbackThreshold, fallbackIterations);
+  }
+  /** Times the launch settings on the current scene and keeps the fastest. */
+  public autotune(dt: number, iterations = 0, repeats = 3): { walkBlock: number[]; listVariant: number } {
+    return addon.autotune(this.handle, dt, iterations, repeats);
+  }
+  public setTuning(walkBlock: number[], listVariant: number): void { addon.setTuning(this.handle, walkBlock, listVariant); }
   /** PBFConfig.setParticlePosition (PBFConfig.ts:48-62): staged particles, or an explicit float4 array. */
   public setParticlePosition(positions?: Float32Array, count?: number): void {
     if (positions) addon.setParticlePosition(this.handle, positions, count ?? positions.length / 4);

@@ -242,6 +242,115 @@ napi_value VoxelizeSphere(napi_env env, napi_callback_info info) {
   return r;
 }
 
+// Index buffer of a THREE.BufferGeometry: Uint32Array or Uint16Array.
+bool index_array(napi_env env, napi_value v, std::vector<uint32_t> &out) {
+  napi_typedarray_type t;
+  size_t n;
+  void *p;
+  napi_value ab;
+  size_t off;
+  if (napi_get_typedarray_info(env, v, &t, &n, &p, &ab, &off) != napi_ok ||
+      (t != napi_uint32_array && t != napi_uint16_array)) {
+    napi_throw_type_error(env, nullptr, "expected a Uint32Array or Uint16Array index");
+    return false;
+  }
+  out.resize(n);
+  for (size_t k = 0; k < n; ++k)
+    out[k] = t == napi_uint32_array ? static_cast<uint32_t *>(p)[k] : static_cast<uint16_t *>(p)[k];
+  return true;
+}
+
+// 16 numbers (Matrix4.elements, column-major) from an array or typed array.
+bool matrix16(napi_env env, napi_value v, double m[16]) {
+  for (uint32_t k = 0; k < 16; ++k) {
+    napi_value e;
+    if (napi_get_element(env, v, k, &e) != napi_ok) {
+      napi_throw_type_error(env, nullptr, "expected 16 matrix elements");
+      return false;
+    }
+    m[k] = num(env, e);
+  }
+  return true;
+}
+
+napi_value VoxelizeMesh(napi_env env, napi_callback_info info) {
+  CTX_OR_RETURN(0);
+  float *vp;
+  size_t vlen;
+  if (!float_array(env, a.v[1], &vp, &vlen)) return nullptr;
+  std::vector<uint32_t> idx;
+  if (!index_array(env, a.v[2], idx)) return nullptr;
+  double m[16];
+  const double *mp = nullptr;
+  napi_valuetype t = napi_undefined;
+  if (a.n > 3) napi_typeof(env, a.v[3], &t);
+  if (t == napi_object) {
+    if (!matrix16(env, a.v[3], m)) return nullptr;
+    mp = m;
+  }
+  double jitter = a.n > 4 ? num(env, a.v[4]) : 0.0;
+  uint64_t seed = a.n > 5 ? (uint64_t)num(env, a.v[5]) : 0;
+  uint32_t added = 0;
+  CHECK(pbf_voxelize_mesh(c, vp, (uint32_t)(vlen / 3), idx.data(), (uint32_t)(idx.size() / 3), mp, jitter, seed,
+                          &added));
+  napi_value r;
+  napi_create_uint32(env, added, &r);
+  return r;
+}
+
+// setWarmStart(ctx, blob | null, mode, relax, fallbackThreshold, fallbackIterations)
+napi_value SetWarmStart(napi_env env, napi_callback_info info) {
+  CTX_OR_RETURN(0);
+  float *bp = nullptr;
+  size_t blen = 0;
+  napi_valuetype t = napi_undefined;
+  napi_typeof(env, a.v[1], &t);
+  if (t == napi_object && !float_array(env, a.v[1], &bp, &blen)) return nullptr;
+  uint32_t mode = (uint32_t)num(env, a.v[2], 1);
+  float relax = (float)num(env, a.v[3], 1.0);
+  double thr = num(env, a.v[4], 0.0);
+  uint32_t fb = (uint32_t)num(env, a.v[5], 0);
+  CHECK(pbf_set_warm_start(c, bp, blen, mode, relax, thr, fb));
+  return undefined(env);
+}
+
+napi_value tuning_object(napi_env env, const pbf_tuning &t) {
+  napi_value o, arr, v;
+  napi_create_object(env, &o);
+  napi_create_array_with_length(env, 6, &arr);
+  for (uint32_t k = 0; k < 6; ++k) {
+    napi_create_uint32(env, t.walk_block[k], &v);
+    napi_set_element(env, arr, k, v);
+  }
+  napi_set_named_property(env, o, "walkBlock", arr);
+  napi_create_uint32(env, t.list_variant, &v);
+  napi_set_named_property(env, o, "listVariant", v);
+  return o;
+}
+
+// autotune(ctx, dt, iterations, repeats) -> {walkBlock, listVariant}
+napi_value Autotune(napi_env env, napi_callback_info info) {
+  CTX_OR_RETURN(0);
+  pbf_tuning t;
+  CHECK(pbf_autotune(c, (float)num(env, a.v[1], 1.0 / 300.0), (uint32_t)num(env, a.v[2], 0),
+                     (uint32_t)num(env, a.v[3], 3), &t));
+  return tuning_object(env, t);
+}
+
+// setTuning(ctx, walkBlock[6], listVariant)
+napi_value SetTuning(napi_env env, napi_callback_info info) {
+  CTX_OR_RETURN(0);
+  pbf_tuning t;
+  CHECK(pbf_get_tuning(c, &t));
+  for (uint32_t k = 0; k < 6; ++k) {
+    napi_value e;
+    if (napi_get_element(env, a.v[1], k, &e) == napi_ok) t.walk_block[k] = (uint32_t)num(env, e, 256);
+  }
+  t.list_variant = (uint32_t)num(env, a.v[2], 0);
+  CHECK(pbf_set_tuning(c, &t));
+  return undefined(env);
+}
+
 napi_value SetParticlePosition(napi_env env, napi_callback_info info) {
   CTX_OR_RETURN(0);
   if (a.n >= 3) {
@@ -360,6 +469,10 @@ napi_value Init(napi_env env, napi_value exports) {
              {"setConfig", SetConfig},
              {"voxelizeCube", VoxelizeCube},
              {"voxelizeSphere", VoxelizeSphere},
+             {"voxelizeMesh", VoxelizeMesh},
+             {"setWarmStart", SetWarmStart},
+             {"autotune", Autotune},
+             {"setTuning", SetTuning},
              {"setParticlePosition", SetParticlePosition},
              {"setBoundaryData", SetBoundaryData},
              {"setVolumeMap", SetVolumeMap},

@@ -82,6 +82,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "pbf_voxelize_sphere": (I, [P, pd3, D, D, U64, pu]),
         "pbf_voxelize_cube_host": (I, [D, pd3, pd3, D, U64, U64, pf, U64, pu]),
         "pbf_voxelize_sphere_host": (I, [D, pd3, D, D, U64, U64, pf, U64, pu]),
+        "pbf_voxelize_mesh": (I, [P, pf, U32, pu, U32, C.POINTER(C.c_double), D, U64, pu]),
+        "pbf_voxelize_mesh_host": (I, [D, pf, U32, pu, U32, C.POINTER(C.c_double), D, U64, U64, pf, U64, pu]),
         "pbf_staged_count": (U32, [P]),
         "pbf_set_particle_position": (I, [P]),
         "pbf_set_positions": (I, [P, pf, S, U32]),
@@ -186,6 +188,26 @@ def parse_cdm(text) -> tuple:
     if rc != PBF_OK:
         raise PBFError(rc, "Invalid Boundary Model File!")
     return tuple(res), sdf, vol
+
+
+def voxelize_mesh(particle_radius: float, vertices, indices, world=None, jitter: float = 0.0,
+                  seed: int = 0) -> np.ndarray:
+    """GPU-free SPH.voxelizeMesh: float4 rows (w = 0)."""
+    lib = load_library()
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1)
+    t = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+    w = None if world is None else (C.c_double * 16)(*[float(x) for x in np.asarray(world).reshape(-1)])
+    cnt = C.c_uint32()
+    rc = lib.pbf_voxelize_mesh_host(float(particle_radius), _fptr(v), v.size // 3, _uptr(t), t.size // 3, w,
+                                    float(jitter), int(seed), 0, None, 0, C.byref(cnt))
+    if rc != PBF_OK:
+        raise PBFError(rc, "pbf_voxelize_mesh_host failed")
+    out = np.zeros((max(cnt.value, 1), 4), np.float32)
+    rc = lib.pbf_voxelize_mesh_host(float(particle_radius), _fptr(v), v.size // 3, _uptr(t), t.size // 3, w,
+                                    float(jitter), int(seed), 0, _fptr(out), cnt.value, C.byref(cnt))
+    if rc != PBF_OK:
+        raise PBFError(rc, "pbf_voxelize_mesh_host failed")
+    return out[: cnt.value]
 
 
 def copy_bandwidth(nbytes: int = 1 << 30, iterations: int = 20) -> float:
@@ -326,6 +348,17 @@ class PBF:
                                                  C.byref(added)))
         return int(added.value)
 
+
+    def voxelizeMesh(self, vertices, indices, world=None, jitter: float = 0.0, seed: int = 0) -> int:
+        """SPH.voxelizeMesh: vertices (n, 3) mesh-local, indices (m, 3), world 4x4
+        column-major (three.js Matrix4.elements) or None."""
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1)
+        t = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+        w = None if world is None else (C.c_double * 16)(*[float(x) for x in np.asarray(world).reshape(-1)])
+        added = C.c_uint32()
+        self._check(self.lib.pbf_voxelize_mesh(self.ctx, _fptr(v), v.size // 3, _uptr(t), t.size // 3, w,
+                                               float(jitter), int(seed), C.byref(added)))
+        return added.value
     def setParticlePosition(self, positions: Optional[np.ndarray] = None, count: Optional[int] = None):
         """PBFConfig.setParticlePosition (PBFConfig.ts:48-62); uploads staged or given float4 rows."""
         if positions is None:
