@@ -160,6 +160,17 @@ int pbf_clear_boundary(pbf_ctx *ctx);
  * No reference counterpart: the reference ships precomputed .cdm maps. */
 int pbf_build_volume_map(uint32_t kind, const double *shape, const int32_t res[3], const double lo[3],
                          const double hi[3], double kernel_radius, double particle_radius, float *sdf_vol);
+/* The same definition sampled at the nodes of the 32-node cubic element in
+ * the .cdm layout (corner, x-, y-, z-edge node sets) over [lo, hi]: a
+ * reference-format field for any analytic solid and resolution.  sdf / vol
+ * NULL: field_length only. */
+int pbf_build_cubic_field(uint32_t kind, const double *shape, const int32_t res[3], const double lo[3],
+                          const double hi[3], double kernel_radius, double particle_radius, float *sdf, float *vol,
+                          size_t capacity, size_t *field_length);
+/* .cdm text of a cubic field, the format volumeMap.ts:23-39 parses (9
+ * significant digits: exact float round trip).  out NULL: size only. */
+int pbf_write_cdm(const int32_t res[3], const float *sdf, const float *vol, size_t length, char *out,
+                  size_t capacity, size_t *written);
 /* The same map built on the GPU (same definition, quadrature and float64
  * arithmetic order, one thread per node); host arrays in and out.
  * PBF_E_DEVICE without a GPU. */

@@ -143,3 +143,26 @@ def test_volume_map_builder_obstacle_sdf():
     q = np.hypot(nodes[:, 0] - 0.5, nodes[:, 2] - 0.5) - 0.3
     tor = np.hypot(q, nodes[:, 1] - 0.5) - 0.08
     np.testing.assert_allclose(sv[:, 0], np.minimum.reduce([box, sph, tor]), atol=1e-6)
+
+
+def test_cubic_field_generator_and_cdm_writer():
+    """The 32-node generator samples the volume-map definition at the .cdm
+    node sets: its corner nodes are the trilinear map's nodes (same
+    evaluator, bit-identical), every SDF node is the closed-form box
+    distance, and .cdm text written and parsed again is bit-identical."""
+    res, lo, hi, h, r = (10, 8, 6), (0.0, 0.0, 0.0), (1.0, 0.8, 0.6), 0.1, 0.025
+    sdf, vol = pbf_host.build_cubic_field(0, res, lo, hi, h, None, r)
+    n0 = 11 * 9 * 7
+    assert sdf.size == n0 + 2 * 10 * 9 * 7 + 2 * 8 * 11 * 7 + 2 * 6 * 11 * 9
+    tri = pbf_host.build_volume_map(0, res, lo, hi, h, None, r, where="host")
+    assert np.array_equal(sdf[:n0], tri[:, 0]) and np.array_equal(vol[:n0], tri[:, 1])
+    pts = scenes.cubic_nodes(res)
+    box = np.concatenate([np.minimum.reduce([x * 1.0, 1.0 - x * 1.0, y * 0.8, 0.8 - y * 0.8, z * 0.6,
+                                             0.6 - z * 0.6]).ravel() for (x, y, z) in pts])
+    np.testing.assert_allclose(sdf, box, atol=2e-7)
+    text = pbf_host.write_cdm(res, sdf, vol)
+    assert text.startswith("10 8 6\r\n%d\r\n" % sdf.size)
+    res2, s2, v2 = pbf_host.parse_cdm(text)
+    assert res2 == res and np.array_equal(s2, sdf) and np.array_equal(v2, vol)
+    with pytest.raises(pbf_host.PBFError):
+        pbf_host.write_cdm(res, sdf[:-1], vol[:-1])

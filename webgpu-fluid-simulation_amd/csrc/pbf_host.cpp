@@ -2,6 +2,7 @@
 // .cdm boundary-file parser and the analytic volume-map builder.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -136,90 +137,184 @@ struct Solid {
 // still inside the ball; V_b is therefore capped at its deep-solid value
 // 0.9 / W(2r) (a sample never stands for more than a ball full of solid),
 // which keeps interpolated values bounded near that shell.
-extern "C" int pbf_build_volume_map(uint32_t kind, const double *shape, const int32_t res[3], const double lo[3],
-                                    const double hi[3], double kernel_radius, double particle_radius,
-                                    float *sdf_vol) {
-  using namespace pbf;
-  if (!res || !lo || !hi || !sdf_vol || kind > 1 || !(kernel_radius > 0.0)) return PBF_E_INVALID;
-  if (res[0] < 1 || res[1] < 1 || res[2] < 1) return PBF_E_INVALID;
-  Solid S;
-  std::memset(&S, 0, sizeof(S));
-  S.kind = kind;
-  for (int a = 0; a < 3; ++a) { S.box_lo[a] = lo[a]; S.box_hi[a] = hi[a]; }
-  if (kind == 1) {
-    if (!shape) return PBF_E_INVALID;
-    for (int a = 0; a < 3; ++a) { S.sc[a] = shape[a]; S.tc[a] = shape[4 + a]; }
-    S.sr = shape[3];
-    S.tR = shape[7];
-    S.tr = shape[8];
-  } else if (shape) {
-    for (int a = 0; a < 3; ++a) { S.box_lo[a] = shape[a]; S.box_hi[a] = shape[3 + a]; }
-  }
-  const double h = kernel_radius;
-  // stratified quadrature of the solid volume inside the kernel ball
-  const int M = 16;
-  const double PI = 3.141592653589793;
-  const double kp = 315.0 / (64.0 * PI * std::pow(h, 9));
-  const double r = particle_radius;
+namespace {
+// The map definition above at one point: solid SDF, then the quadrature of
+// the solid part of the kernel ball (stratified 16^3 midpoint cells).
+struct VolumeEval {
+  pbf::Solid S;
+  double h, r, kp, cellv, ballv, vcap;
   std::vector<double> qp, qw;
-  for (int i = 0; i < M; ++i)
-    for (int j = 0; j < M; ++j)
-      for (int k = 0; k < M; ++k) {
-        double o[3] = {(-1.0 + (2.0 * i + 1.0) / M) * h, (-1.0 + (2.0 * j + 1.0) / M) * h,
-                       (-1.0 + (2.0 * k + 1.0) / M) * h};
-        double rr = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
-        if (rr <= h * h) {
-          qp.push_back(o[0]); qp.push_back(o[1]); qp.push_back(o[2]);
-          double t = h * h - rr;
-          qw.push_back(kp * t * t * t);
+
+  int init(uint32_t kind, const double *shape, const double lo[3], const double hi[3], double kernel_radius,
+           double particle_radius) {
+    if (!lo || !hi || kind > 1 || !(kernel_radius > 0.0)) return PBF_E_INVALID;
+    std::memset(&S, 0, sizeof(S));
+    S.kind = kind;
+    for (int a = 0; a < 3; ++a) { S.box_lo[a] = lo[a]; S.box_hi[a] = hi[a]; }
+    if (kind == 1) {
+      if (!shape) return PBF_E_INVALID;
+      for (int a = 0; a < 3; ++a) { S.sc[a] = shape[a]; S.tc[a] = shape[4 + a]; }
+      S.sr = shape[3];
+      S.tR = shape[7];
+      S.tr = shape[8];
+    } else if (shape) {
+      for (int a = 0; a < 3; ++a) { S.box_lo[a] = shape[a]; S.box_hi[a] = shape[3 + a]; }
+    }
+    h = kernel_radius;
+    r = particle_radius;
+    const int M = 16;
+    const double PI = 3.141592653589793;
+    kp = 315.0 / (64.0 * PI * std::pow(h, 9));
+    for (int i = 0; i < M; ++i)
+      for (int j = 0; j < M; ++j)
+        for (int k = 0; k < M; ++k) {
+          double o[3] = {(-1.0 + (2.0 * i + 1.0) / M) * h, (-1.0 + (2.0 * j + 1.0) / M) * h,
+                         (-1.0 + (2.0 * k + 1.0) / M) * h};
+          double rr = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+          if (rr <= h * h) {
+            qp.push_back(o[0]); qp.push_back(o[1]); qp.push_back(o[2]);
+            double t = h * h - rr;
+            qw.push_back(kp * t * t * t);
+          }
+        }
+    cellv = std::pow(2.0 * h / M, 3);
+    const double t2r = h * h - 4.0 * r * r;
+    vcap = t2r > 0.0 ? 0.9 / (kp * t2r * t2r * t2r) : 0.0;
+    ballv = 4.0 / 3.0 * 3.141592653589793 * h * h * h;
+    return PBF_OK;
+  }
+
+  void eval(const double p[3], float &sdf, float &vol) const {
+    double d = S.sdf(p);
+    double v;
+    if (d >= h) v = 0.0;
+    else {
+      double vin = 0.0, win = 0.0;
+      if (d <= -h) { vin = ballv; win = 1.0; }
+      else {
+        for (size_t t = 0, u = 0; t < qp.size(); t += 3, ++u) {
+          double y[3] = {p[0] + qp[t], p[1] + qp[t + 1], p[2] + qp[t + 2]};
+          if (S.sdf(y) < 0.0) { vin += cellv; win += qw[u] * cellv; }
         }
       }
-  const double cellv = std::pow(2.0 * h / M, 3);
-  const double t2r = h * h - 4.0 * particle_radius * particle_radius;
-  const double vcap = t2r > 0.0 ? 0.9 / (kp * t2r * t2r * t2r) : 0.0;
-  const double ballv = 4.0 / 3.0 * 3.141592653589793 * h * h * h;
-  const size_t nx = (size_t)res[0] + 1, ny = (size_t)res[1] + 1, nz = (size_t)res[2] + 1;
-  const size_t total = nx * ny * nz;
+      if (r > 0.0) {
+        double s = std::fmax(d + 0.5 * r, 2.0 * r);
+        double t = h * h - s * s;
+        double ws = t > 0.0 ? kp * t * t * t : 0.0;
+        v = ws > 0.0 ? std::fmin(0.9 * win / ws, vcap) : 0.0;
+      } else {
+        v = vin;
+      }
+    }
+    sdf = (float)d;
+    vol = (float)v;
+  }
+};
+
+// Runs f(q) for q in [0, total) on up to 16 threads.
+template <typename F>
+void parallel_for(size_t total, F f) {
   unsigned nt = std::thread::hardware_concurrency();
   if (nt == 0) nt = 1;
   if (nt > 16) nt = 16;
-  auto work = [&](size_t b, size_t e) {
-    for (size_t q = b; q < e; ++q) {
-      size_t i = q % nx, j = (q / nx) % ny, k = q / (nx * ny);
-      double p[3] = {lo[0] + (hi[0] - lo[0]) * (double)i / res[0], lo[1] + (hi[1] - lo[1]) * (double)j / res[1],
-                     lo[2] + (hi[2] - lo[2]) * (double)k / res[2]};
-      double d = S.sdf(p);
-      double v;
-      if (d >= h) v = 0.0;
-      else {
-        double vin = 0.0, win = 0.0;
-        if (d <= -h) { vin = ballv; win = 1.0; }
-        else {
-          for (size_t t = 0, u = 0; t < qp.size(); t += 3, ++u) {
-            double y[3] = {p[0] + qp[t], p[1] + qp[t + 1], p[2] + qp[t + 2]};
-            if (S.sdf(y) < 0.0) { vin += cellv; win += qw[u] * cellv; }
-          }
-        }
-        if (r > 0.0) {
-          double s = std::fmax(d + 0.5 * r, 2.0 * r);
-          double t = h * h - s * s;
-          double ws = t > 0.0 ? kp * t * t * t : 0.0;
-          v = ws > 0.0 ? std::fmin(0.9 * win / ws, vcap) : 0.0;
-        } else {
-          v = vin;
-        }
-      }
-      sdf_vol[2 * q] = (float)d;
-      sdf_vol[2 * q + 1] = (float)v;
-    }
-  };
   std::vector<std::thread> th;
   size_t chunk = (total + nt - 1) / nt;
   for (unsigned t = 0; t < nt; ++t) {
     size_t b = t * chunk, e = std::min(total, b + chunk);
-    if (b < e) th.emplace_back(work, b, e);
+    if (b < e) th.emplace_back([=] { for (size_t q = b; q < e; ++q) f(q); });
   }
   for (auto &x : th) x.join();
+}
+}  // namespace
+
+extern "C" int pbf_build_volume_map(uint32_t kind, const double *shape, const int32_t res[3], const double lo[3],
+                                    const double hi[3], double kernel_radius, double particle_radius,
+                                    float *sdf_vol) {
+  if (!res || !sdf_vol) return PBF_E_INVALID;
+  if (res[0] < 1 || res[1] < 1 || res[2] < 1) return PBF_E_INVALID;
+  VolumeEval E;
+  int rc = E.init(kind, shape, lo, hi, kernel_radius, particle_radius);
+  if (rc != PBF_OK) return rc;
+  const size_t nx = (size_t)res[0] + 1, ny = (size_t)res[1] + 1, nz = (size_t)res[2] + 1;
+  parallel_for(nx * ny * nz, [&](size_t q) {
+    size_t i = q % nx, j = (q / nx) % ny, k = q / (nx * ny);
+    double p[3] = {lo[0] + (hi[0] - lo[0]) * (double)i / res[0], lo[1] + (hi[1] - lo[1]) * (double)j / res[1],
+                   lo[2] + (hi[2] - lo[2]) * (double)k / res[2]};
+    E.eval(p, sdf_vol[2 * q], sdf_vol[2 * q + 1]);
+  });
+  return PBF_OK;
+}
+
+// The same definition sampled at the nodes of the 32-node cubic element in
+// the .cdm layout (volumeMap.ts:23-39, discreteFieldShader.ts:1-12): corner
+// nodes [k][j][i] ((R+1)^3), then x-edge nodes [k][j][i2], y-edge nodes
+// [i][k][j2], z-edge nodes [j][i][k2] (2R(R+1)^2 each), an edge node i2 at
+// lattice coordinate floor(i2 / 2) + (1 + i2 % 2) / 3.  field_length = the
+// node count; sdf / vol receive it when capacity allows.
+extern "C" int pbf_build_cubic_field(uint32_t kind, const double *shape, const int32_t res[3], const double lo[3],
+                                     const double hi[3], double kernel_radius, double particle_radius, float *sdf,
+                                     float *vol, size_t capacity, size_t *field_length) {
+  using namespace pbf;
+  if (!res || res[0] < 1 || res[1] < 1 || res[2] < 1) return PBF_E_INVALID;
+  const int R[3] = {res[0], res[1], res[2]};
+  const size_t fl = cubic_field_length(R);
+  if (field_length) *field_length = fl;
+  if (!sdf && !vol) return PBF_OK;
+  if (!sdf || !vol) return PBF_E_INVALID;
+  if (capacity < fl) return PBF_E_CAPACITY;
+  VolumeEval E;
+  int rc = E.init(kind, shape, lo, hi, kernel_radius, particle_radius);
+  if (rc != PBF_OK) return rc;
+  const size_t n0 = (size_t)(R[0] + 1) * (R[1] + 1) * (R[2] + 1);
+  const size_t ne[3] = {(size_t)2 * R[0] * (R[1] + 1) * (R[2] + 1), (size_t)2 * R[1] * (R[0] + 1) * (R[2] + 1),
+                        (size_t)2 * R[2] * (R[0] + 1) * (R[1] + 1)};
+  auto lat = [&](int a, double u) { return lo[a] + (hi[a] - lo[a]) * u / R[a]; };
+  auto edge = [](size_t i2) { return (double)(i2 / 2) + (1.0 + (double)(i2 % 2)) / 3.0; };
+  parallel_for(fl, [&](size_t q) {
+    double p[3];
+    if (q < n0) {  // corner[k][j][i]
+      size_t i = q % (R[0] + 1), j = (q / (R[0] + 1)) % (R[1] + 1), k = q / ((size_t)(R[0] + 1) * (R[1] + 1));
+      p[0] = lat(0, (double)i); p[1] = lat(1, (double)j); p[2] = lat(2, (double)k);
+    } else if (q < n0 + ne[0]) {  // edgex[k][j][i2]
+      size_t t = q - n0, i2 = t % (2 * R[0]), j = (t / (2 * R[0])) % (R[1] + 1), k = t / (2 * (size_t)R[0] * (R[1] + 1));
+      p[0] = lat(0, edge(i2)); p[1] = lat(1, (double)j); p[2] = lat(2, (double)k);
+    } else if (q < n0 + ne[0] + ne[1]) {  // edgey[i][k][j2]
+      size_t t = q - n0 - ne[0], j2 = t % (2 * R[1]), k = (t / (2 * R[1])) % (R[2] + 1), i = t / (2 * (size_t)R[1] * (R[2] + 1));
+      p[0] = lat(0, (double)i); p[1] = lat(1, edge(j2)); p[2] = lat(2, (double)k);
+    } else {  // edgez[j][i][k2]
+      size_t t = q - n0 - ne[0] - ne[1], k2 = t % (2 * R[2]), i = (t / (2 * R[2])) % (R[0] + 1), j = t / (2 * (size_t)R[2] * (R[0] + 1));
+      p[0] = lat(0, (double)i); p[1] = lat(1, (double)j); p[2] = lat(2, edge(k2));
+    }
+    E.eval(p, sdf[q], vol[q]);
+  });
+  return PBF_OK;
+}
+
+// .cdm text of a cubic field (the layout volumeMap.ts:23-39 reads): "Rx Ry Rz",
+// the SDF count, the SDF values, the volume count, the volume values, each
+// value followed by a space, CRLF line ends; values with 9 significant
+// digits (exact float round trip).  out == NULL: size only (written gets the
+// byte count without a terminator).
+extern "C" int pbf_write_cdm(const int32_t res[3], const float *sdf, const float *vol, size_t length, char *out,
+                             size_t capacity, size_t *written) {
+  using namespace pbf;
+  if (!res || !sdf || !vol || res[0] < 1 || res[1] < 1 || res[2] < 1) return PBF_E_INVALID;
+  const int R[3] = {res[0], res[1], res[2]};
+  if (length != cubic_field_length(R)) return PBF_E_LENGTH;
+  std::string t;
+  t.reserve(length * 32 + 64);
+  char buf[64];
+  std::snprintf(buf, sizeof(buf), "%d %d %d\r\n%zu\r\n", R[0], R[1], R[2], length);
+  t += buf;
+  for (size_t q = 0; q < length; ++q) { std::snprintf(buf, sizeof(buf), "%.9g ", (double)sdf[q]); t += buf; }
+  std::snprintf(buf, sizeof(buf), "\r\n%zu\r\n", length);
+  t += buf;
+  for (size_t q = 0; q < length; ++q) { std::snprintf(buf, sizeof(buf), "%.9g ", (double)vol[q]); t += buf; }
+  t += "\r\n";
+  if (written) *written = t.size();
+  if (!out) return PBF_OK;
+  if (capacity < t.size()) return PBF_E_CAPACITY;
+  std::memcpy(out, t.data(), t.size());
   return PBF_OK;
 }
 

@@ -94,6 +94,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "pbf_clear_boundary": (I, [P]),
         "pbf_build_volume_map": (I, [U32, pd3, C.POINTER(C.c_int32), pd3, pd3, D, D, pf]),
         "pbf_build_volume_map_gpu": (I, [U32, pd3, C.POINTER(C.c_int32), pd3, pd3, D, D, pf]),
+        "pbf_build_cubic_field": (I, [U32, pd3, C.POINTER(C.c_int32), pd3, pd3, D, D, pf, pf, S, C.POINTER(S)]),
+        "pbf_write_cdm": (I, [C.POINTER(C.c_int32), pf, pf, S, C.c_char_p, S, C.POINTER(S)]),
         "pbf_step": (I, [P, F, U32, U32]), "pbf_run": (I, [P]), "pbf_sync": (I, [P]),
         "pbf_read_positions": (I, [P, pf, U32]), "pbf_read_velocities": (I, [P, pf, U32]),
         "pbf_read_field": (I, [P, U32, pf, U32]),
@@ -171,6 +173,40 @@ def build_volume_map(kind: int, res, lo, hi, kernel_radius: float, shape=None,
     if rc != PBF_OK:
         raise PBFError(rc, "pbf_build_volume_map failed")
     return out
+
+
+def build_cubic_field(kind: int, res, lo, hi, kernel_radius: float, shape=None, particle_radius: float = 0.0):
+    """(sdf, vol) at the 32-node element's nodes in the .cdm layout."""
+    lib = load_library()
+    sh = None if shape is None else (C.c_double * len(shape))(*[float(x) for x in shape])
+    fl = C.c_size_t()
+    rc = lib.pbf_build_cubic_field(kind, sh, _i3(res), _d3(lo), _d3(hi), float(kernel_radius), float(particle_radius),
+                                   None, None, 0, C.byref(fl))
+    if rc != PBF_OK:
+        raise PBFError(rc, "pbf_build_cubic_field failed")
+    sdf = np.zeros(fl.value, np.float32)
+    vol = np.zeros(fl.value, np.float32)
+    rc = lib.pbf_build_cubic_field(kind, sh, _i3(res), _d3(lo), _d3(hi), float(kernel_radius), float(particle_radius),
+                                   _fptr(sdf), _fptr(vol), fl.value, C.byref(fl))
+    if rc != PBF_OK:
+        raise PBFError(rc, "pbf_build_cubic_field failed")
+    return sdf, vol
+
+
+def write_cdm(res, sdf, vol) -> str:
+    """.cdm text of a cubic field."""
+    lib = load_library()
+    s = np.ascontiguousarray(sdf, np.float32)
+    v = np.ascontiguousarray(vol, np.float32)
+    n = C.c_size_t()
+    rc = lib.pbf_write_cdm(_i3(res), _fptr(s), _fptr(v), s.size, None, 0, C.byref(n))
+    if rc != PBF_OK:
+        raise PBFError(rc, "pbf_write_cdm failed")
+    buf = C.create_string_buffer(n.value + 1)
+    rc = lib.pbf_write_cdm(_i3(res), _fptr(s), _fptr(v), s.size, buf, n.value + 1, C.byref(n))
+    if rc != PBF_OK:
+        raise PBFError(rc, "pbf_write_cdm failed")
+    return buf.raw[: n.value].decode()
 
 
 def parse_cdm(text) -> tuple:
