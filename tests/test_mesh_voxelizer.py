@@ -120,6 +120,6 @@ def test_context_voxelize_mesh_stages_the_same_particles():
     sim = PBF(particle_radius=0.01, kernel_radius=0.04, capacity=1 << 16)
     n = sim.voxelizeMesh(v, t, jitter=0.001, seed=3)
     host = pbf_host.voxelize_mesh(0.01, v, t, jitter=0.001, seed=3)
-    assert n == host.shape[0] == sim.particleCount()
+    assert n == host.shape[0] == sim.particleCount
     sim.setParticlePosition()
     np.testing.assert_array_equal(sim.read_positions()[:, :3], host[:, :3])
