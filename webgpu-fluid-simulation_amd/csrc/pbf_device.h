@@ -96,6 +96,38 @@ __device__ __forceinline__ void clamp_rand(const Scene &sc, float &px, float &py
 }
 
 // ---------------------------------------------------------------------------
+// Correctly rounded square root and division for the operand ranges of the
+// pair terms.  The hardware v_sqrt_f32 / v_rcp_f32 are refined to the
+// correctly rounded result the usual way (square root: test the neighbours
+// s -/+ 1 ulp with an exact fma residual; division: one Newton step on the
+// reciprocal, then two fma remainder corrections of the quotient).  The
+// general forms add pre- and post-scaling for denormal, huge or special
+// operands; the pair terms never produce those (d2 is 0 or at least the
+// square of one position ulp, r lies in [1e-7, h], densities are ~rho0), so
+// the scaling is left out.  Results equal sqrtf() and '/' with
+// -fhip-fp32-correctly-rounded-divide-sqrt, which the parity tests check bit
+// for bit.
+__device__ __forceinline__ float sqrt_n(float x) {
+  float s = __builtin_amdgcn_sqrtf(x);
+  const int si = __float_as_int(s);
+  const float dn = __int_as_float(si - 1), up = __int_as_float(si + 1);
+  const float rd = fmaf(-dn, s, x), ru = fmaf(-up, s, x);
+  s = rd <= 0.0f ? dn : s;
+  s = ru > 0.0f ? up : s;
+  return s;
+}
+__device__ __forceinline__ float div_n(float a, float b) {
+  float y = __builtin_amdgcn_rcpf(b);
+  const float e = fmaf(-b, y, 1.0f);
+  y = fmaf(e, y, y);
+  float q = a * y;
+  float r = fmaf(-b, q, a);
+  q = fmaf(r, y, q);
+  r = fmaf(-b, q, a);
+  return fmaf(r, y, q);
+}
+
+// ---------------------------------------------------------------------------
 // SPH kernels as functions of d2 = |x_ij|^2.  Written without branches
 // (select after computing) so a neighbour loop stays one basic block and its
 // gathers can be issued ahead of the arithmetic; the selected values are the
@@ -106,17 +138,17 @@ __device__ __forceinline__ float poly6(const Phys &P, float d2) {
   return d2 <= P.h2 ? w : 0.0f;
 }
 __device__ __forceinline__ float spiky_f(const Phys &P, float d2) {
-  float r = sqrtf(d2);
+  float r = sqrt_n(d2);
   float q = P.h - r;
-  float f = (P.Ks * (q * q)) / r;
+  float f = div_n(P.Ks * (q * q), r);
   return (d2 > 0.0f && d2 <= P.h2) ? f : 0.0f;
 }
 __device__ __forceinline__ float cohesion_f(const Phys &P, float d2) {
-  float r = sqrtf(d2);
+  float r = sqrt_n(d2);
   float a = P.h - r;
   float t = ((a * a) * a) * ((r * r) * r);
   float res = (r > P.half_h) ? t : fmaf(2.0f, t, P.bias);
-  float f = (P.Kc * res) / r;
+  float f = div_n(P.Kc * res, r);
   return (d2 > 0.0f && d2 <= P.h2) ? f : 0.0f;
 }
 

@@ -649,7 +649,7 @@ __global__ void __launch_bounds__(BS) k_xsph(uint32_t n, const float4 *__restric
     float d2 = dist2(dx, dy, dz);
     float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
     float irj = vj.w;
-    float K = P.two_rho0 / (rhoi + rhoj);
+    float K = div_n(P.two_rho0, rhoi + rhoj);
     float sw = poly6(P, d2) * irj;
     ux = fmaf(vx, sw, ux); uy = fmaf(vy, sw, uy); uz = fmaf(vz, sw, uz);
     float se = (wj * spiky_f(P, d2)) * irj;
