@@ -320,7 +320,7 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
                                                         uint32_t *__restrict__ flags, Scene s0,
                                                         const Scene *__restrict__ scenes,
                                                         const uint32_t *__restrict__ sid, Phys P) {
-  __shared__ uint32_t tile[LIST_R * LIST_BS];
+  __shared__ uint32_t tile[(LIST_R + 1) * LIST_BS];  // row LIST_R: spare
   __shared__ uint32_t block_max;
   const uint32_t tid = threadIdx.x;
   const uint32_t base_slot = xcd_block(blockIdx.x, gridDim.x) * LIST_BS;
@@ -345,13 +345,16 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
     constexpr float kReach = 1.001f;
     auto gap = [](int d, float f) { return d < 0 ? f * f : d > 0 ? (1.0f - f) * (1.0f - f) : 0.0f; };
     const uint32_t cbase = sc.cell_base;
+    // Branch-free in the common case: every candidate is stored, accepted
+    // ones in the particle's next LDS row, rejected ones in the spare row R
+    // (overwritten freely); only entries past R leave the LDS.
     auto test = [&](const float4 &xj, uint32_t j) {
-      float d2 = dist2(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
-      if (d2 <= P.h2) {
-        if (cnt < (uint32_t)LIST_R) tile[cnt * LIST_BS + tid] = j;
-        else if (cnt < P.maxn) nbr[(size_t)cnt * P.ld + i] = j;
-        ++cnt;
-      }
+      const float d2 = dist2(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
+      const bool in = d2 <= P.h2;
+      const uint32_t row = in && cnt < (uint32_t)LIST_R ? cnt : (uint32_t)LIST_R;
+      tile[row * LIST_BS + tid] = j;
+      if (in && cnt >= (uint32_t)LIST_R && cnt < P.maxn) nbr[(size_t)cnt * P.ld + i] = j;
+      cnt += in ? 1u : 0u;
     };
     auto visit = [&](uint32_t t0, uint32_t t1) {
       uint32_t j = t0;
