@@ -1134,7 +1134,7 @@ int pbf_search(pbf_ctx *c, float dt) {
   HIPCHK(hipSetDevice(c->device));
   if (c->n == 0) return PBF_OK;
   LaunchCtx L = make_launch(c, dt);
-  PassTimer T{c, false};
+  PassTimer T{c, (c->params.flags & PBF_FLAG_TIMING) != 0};
   HIPCHK(hipMemcpyAsync(c->ids_save, c->ids, (size_t)c->n * 4, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->lam_sorted, c->lam, (size_t)c->n * 4, hipMemcpyDeviceToDevice, c->stream));
   enqueue_search(c, L, T, true);
