@@ -1,0 +1,123 @@
+"""Edge cases of the GPU path against the oracle: empty and single-particle
+systems, ragged counts around wavefront / work-group / scan-block sizes,
+particles on cell faces and domain walls, crowded cells, a three-level scan
+(16.7 M cells), and the capacity limit.  All comparisons are bit-exact."""
+import numpy as np
+import pytest
+
+import scenes
+from pairs import gpu_fields, make_oracle, oracle_fields, ulp_diff
+from pbf_host import PBF, PBFError
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(pos, L=1.6, name="edge", iters=3, field=True, r=0.025):
+    h = 4 * r
+    lo, hi = (0.0, 0.0, 0.0), (L, L, L)
+    fld = None
+    if field:
+        res = (16, 16, 16)
+        fld = scenes.Field("trilinear", res, lo, hi,
+                           sdf_vol=scenes.build_volume_map(0, res, lo, hi, h, None, r))
+    cfg = {"iteration": iters, "XSPH": 0.01, "vorticity": 0.01, "surfaceTension": 0.1, "gravity": 9.8}
+    return scenes.Scene(name, r, h, lo, hi, np.asarray(pos, np.float32), fld, cfg, 1.0 / 60.0, 2, iters, 10)
+
+
+def _bit_exact(sc, steps=2):
+    sim = scenes.make_engine(sc)
+    o = make_oracle(sc)
+    for _ in range(steps):
+        sim.step(sc.dt, sc.substeps, sc.iterations)
+        o.step(sc.dt, sc.substeps, sc.iterations)
+    g, c = gpu_fields(sim), oracle_fields(o)
+    for k in g:
+        assert ulp_diff(g[k], c[k]) == 0, (sc.name, k, ulp_diff(g[k], c[k]))
+    sim.close()
+
+
+def test_empty_context():
+    sim = PBF(particle_radius=0.025, kernel_radius=0.1, capacity=1024)
+    sim.setParticlePosition(np.zeros((0, 4), np.float32), 0)
+    sim.step(1.0 / 60.0, 2, 4)
+    sim.sync()
+    assert sim.particleCount == 0
+    with pytest.raises(PBFError):
+        sim.density_error()
+    sim.close()
+
+
+def test_single_particle_free_fall():
+    sc = _scene([[0.8, 1.2, 0.8, 0.0]], field=False, name="single")
+    _bit_exact(sc, steps=3)
+    # 6 substeps of free fall: y = y0 - g dt^2 (1 + 2 + ... + 6)
+    sim = scenes.make_engine(sc)
+    for _ in range(3):
+        sim.step(sc.dt, sc.substeps, sc.iterations)
+    dt = sc.dt / sc.substeps
+    assert sim.read_positions()[0, 1] == pytest.approx(1.2 - 9.8 * dt * dt * 21, abs=2e-6)
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 255, 257, 2047, 2049])
+def test_ragged_counts(n):
+    rng = np.random.default_rng(n)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rng.uniform(0.3, 0.3 + 0.05 * max(2.0, n ** (1 / 3)), (n, 3))
+    print(f"[ragged] n={n}", flush=True)
+    _bit_exact(_scene(pos, name=f"ragged{n}"))
+
+
+def test_cell_faces_and_walls():
+    h = 0.1
+    g = np.arange(0.0, 1.6 + 1e-9, h / 2)            # every other point on a cell face
+    X, Y, Z = np.meshgrid(g[:10], g[:12], g[:10], indexing="ij")
+    pos = np.zeros((X.size, 4), np.float32)
+    pos[:, 0], pos[:, 1], pos[:, 2] = X.ravel(), Y.ravel() + 0.2, Z.ravel()
+    pos[0, :3] = (0.0, 0.0, 0.0)                      # on the domain corner
+    pos[1, :3] = (1.6, 1.6, 1.6)
+    _bit_exact(_scene(pos, name="faces"))
+
+
+def test_crowded_cells():
+    rng = np.random.default_rng(3)
+    pos = np.zeros((600, 4), np.float32)
+    pos[:, :3] = 0.8 + rng.uniform(-0.15, 0.15, (600, 3))  # ~90 neighbours in the middle
+    sc = _scene(pos, name="crowded", iters=2)
+    sim = scenes.make_engine(sc, max_neighbors=160)
+    o = make_oracle(sc)
+    sim.step(sc.dt, sc.substeps, sc.iterations)
+    o.step(sc.dt, sc.substeps, sc.iterations)
+    g, c = gpu_fields(sim), oracle_fields(o)
+    for k in g:
+        assert ulp_diff(g[k], c[k]) == 0, k
+
+
+def test_three_level_scan():
+    # 256^3 = 16.7 M cells: the scan recurses 16.7 M -> 8192 -> 4 block sums
+    L = 25.6
+    rng = np.random.default_rng(8)
+    pos = np.zeros((5000, 4), np.float32)
+    pos[:, :3] = rng.uniform(0.0, L, (5000, 3))
+    pos[:100, :3] = L - rng.uniform(0.0, 0.3, (100, 3))  # the last cells too
+    sc = _scene(pos, L=L, field=False, name="scan3")
+    sim = scenes.make_engine(sc)
+    o = make_oracle(sc)
+    sim.search(sc.substep_dt())
+    o.search(sc.substep_dt())
+    gc, go = sim.read_cells()
+    oc, oo = o.cells()
+    assert gc.size == 256 ** 3 and int(gc.sum()) == 5000
+    assert np.array_equal(gc, oc) and np.array_equal(go, oo)
+    assert np.array_equal(sim.read_neighbors()[1], o.neighbors()[1])
+
+
+def test_capacity_limit():
+    pos = np.zeros((1000, 4), np.float32)
+    pos[:, :3] = np.random.default_rng(1).uniform(0.2, 0.6, (1000, 3))
+    sim = PBF(particle_radius=0.025, kernel_radius=0.1, capacity=1000, domain_hi=(1.6, 1.6, 1.6))
+    sim.setParticlePosition(pos)                      # exactly the capacity
+    sim.step(1.0 / 60.0, 1, 2)
+    sim.sync()
+    with pytest.raises(PBFError, match="To Many Particles!"):
+        sim.setParticlePosition(np.zeros((1001, 4), np.float32))
+    sim.close()
