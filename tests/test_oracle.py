@@ -236,6 +236,67 @@ def test_cubic_volume_path_index_quirk(quirk):
     assert (dev > 1e-3) if quirk else (dev < 1e-5)
 
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+REF_CDM = "/root/reference/public/model/domain_boundary.cdm"
+
+
+def _wall_profile():
+    import json
+    with open(os.path.join(GOLDEN, "domain_boundary_wall_profile.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("quirk,key", [(1, "reference_sampler"), (0, "corrected_sampler")])
+def test_cubic_sampler_on_reference_wall_nodes(quirk, key):
+    # The reference's own unit-box field (public/model/domain_boundary.cdm),
+    # restricted to the two elements the line x = y = 0.5 crosses near z = 0:
+    # the oracle reads only those 52 nodes, so it must reproduce the full-field
+    # values bit for bit, and an independent float64 evaluation of the same
+    # element to rounding.
+    doc = _wall_profile()
+    res, n = tuple(doc["res"]), doc["length"]
+    sdf, vol = np.zeros(n, np.float32), np.zeros(n, np.float32)
+    idx = np.array(doc["nodes"]["index"])
+    sdf[idx], vol[idx] = doc["nodes"]["sdf"], doc["nodes"]["vol"]
+    pts = np.array(doc["points"], np.float32)
+    o = make(np.zeros((1, 3)), r=0.006, h=0.025)
+    o.set_cubic_field(res, (0, 0, 0), (1, 1, 1), sdf, vol, quirk=quirk)
+    out = o.field_eval(pts)
+    assert np.array_equal(out[:, 3], np.array(doc[key]["sdf"], np.float32))
+    assert np.array_equal(out[:, 4], np.array(doc[key]["vol"], np.float32))
+    np.testing.assert_allclose(out[:, 3], serendipity_numpy(res, sdf.astype(np.float64), pts, 0), atol=1e-7)
+    np.testing.assert_allclose(out[:, 4], serendipity_numpy(res, vol.astype(np.float64), pts, quirk),
+                               rtol=1e-5, atol=1e-12)
+    # The SDF along the normal is the depth; the volume is the tabulated
+    # wall value at the face and first edge node, and zero from d = 0.05 on
+    # (two cells = twice the kernel radius of that scene).
+    np.testing.assert_allclose(out[:, 3], pts[:, 2], atol=1e-6)
+    assert out[0, 4] == vol[idx][np.argmax(vol[idx])] and np.all(out[3:, 4] == 0.0)
+    # The reference's index convention reads the d = 1/60 edge node again at
+    # d = 1/30, where the corrected element returns the tabulated 9.84e-7.
+    if quirk:
+        assert out[2, 4] == pytest.approx(out[1, 4], rel=1e-6)
+    else:
+        assert out[2, 4] == pytest.approx(9.8409e-7, rel=1e-4)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_CDM), reason="reference checkout not mounted")
+def test_wall_profile_fixture_matches_reference_file():
+    import pbf_host
+    doc = _wall_profile()
+    with open(REF_CDM, "rb") as f:
+        res, sdf, vol = pbf_host.parse_cdm(f.read())
+    assert list(res) == doc["res"] and sdf.size == doc["length"]
+    idx = np.array(doc["nodes"]["index"])
+    assert np.array_equal(sdf[idx], np.array(doc["nodes"]["sdf"], np.float32))
+    assert np.array_equal(vol[idx], np.array(doc["nodes"]["vol"], np.float32))
+    o = make(np.zeros((1, 3)), r=0.006, h=0.025)
+    for quirk, key in ((1, "reference_sampler"), (0, "corrected_sampler")):
+        o.set_cubic_field(res, (0, 0, 0), (1, 1, 1), sdf, vol, quirk=quirk)
+        out = o.field_eval(np.array(doc["points"], np.float32))
+        assert np.array_equal(out[:, 4], np.array(doc[key]["vol"], np.float32))
+
+
 def test_trilinear_field_linear_exact():
     res, lo, hi = (8, 6, 5), (0.0, -1.0, 0.5), (2.0, 1.0, 1.5)
     nodes = np.stack(np.meshgrid(*[np.linspace(lo[a], hi[a], res[a] + 1) for a in range(3)], indexing="ij"), -1)
