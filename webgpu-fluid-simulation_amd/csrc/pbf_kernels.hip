@@ -72,6 +72,48 @@ __device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ n
   for (; k < cnt; ++k) f(row[(size_t)k * ld]);
 }
 
+// Wavefront-uniform walk: every lane runs as many bodies as the largest count
+// among the active lanes of its wavefront -- batches of UNROLL, then one of 4,
+// then singles, each loop test a ballot -- instead of each lane's own
+// 8 / 4 / 1 split, which costs the wavefront the maximum of every phase
+// (39 bodies for counts 29..33).  The list build fills rows [count, largest
+// count of the wavefront) with the lane's own slot, so entries past a lane's
+// count are visited as (i, false) and the body makes them contribute exactly
+// nothing (d = 0: every gradient and difference term is 0; a poly6 sum takes
+// them with weight 0).  Valid entries are visited in walk order, so every sum
+// is the same as the per-lane loop's.
+template <int UNROLL, typename F>
+__device__ __forceinline__ void for_each_neighbor_w(const uint32_t *__restrict__ nbr, uint32_t i, uint32_t ld,
+                                                    uint32_t cnt, F &&f) {
+  // largest count among the active lanes, bit by bit from ballots
+  uint32_t m = 0;
+#pragma unroll
+  for (int b = 15; b >= 0; --b) {
+    const uint32_t t = m | (1u << b);
+    if (__ballot(cnt >= t)) m = t;
+  }
+  const uint32_t kmax = __builtin_amdgcn_readfirstlane(m);
+  const uint32_t *row = nbr + i;
+  uint32_t k = 0;
+  if constexpr (UNROLL >= 8) {
+    for (; k + 8 <= kmax; k += 8) {
+      uint32_t j[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) j[q] = row[(size_t)(k + q) * ld];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) f(j[q], k + q < cnt);
+    }
+  }
+  for (; k + 4 <= kmax; k += 4) {
+    uint32_t j[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) j[q] = row[(size_t)(k + q) * ld];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) f(j[q], k + q < cnt);
+  }
+  for (; k < kmax; ++k) f(row[(size_t)k * ld], k < cnt);
+}
+
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -321,12 +363,9 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
                                                         const Scene *__restrict__ scenes,
                                                         const uint32_t *__restrict__ sid, Phys P) {
   __shared__ uint32_t tile[(LIST_R + 1) * LIST_BS];  // row LIST_R: spare
-  __shared__ uint32_t block_max;
   const uint32_t tid = threadIdx.x;
   const uint32_t base_slot = xcd_block(blockIdx.x, gridDim.x) * LIST_BS;
   const uint32_t i = base_slot + tid;
-  if (tid == 0) block_max = 0;
-  __syncthreads();
   uint32_t cnt = 0;
   if (i < n) {
     const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
@@ -409,19 +448,20 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
     }
     ncount[i] = cnt < P.maxn ? cnt : P.maxn;
   }
-  const uint32_t wm = wave_max_u32(cnt);
-  if (__lane_id() == 0) {
-    atomicMax(&block_max, wm);
-    if (wm > 0) {
-      atomicMax(&flags[1], wm);
-      if (wm > P.maxn) atomicOr(&flags[0], 1u);
-    }
+  const uint32_t wm = wave_max_u32(cnt);  // lanes past n hold cnt = 0
+  if (__lane_id() == 0 && wm > 0) {
+    atomicMax(&flags[1], wm);
+    if (wm > P.maxn) atomicOr(&flags[0], 1u);
   }
-  __syncthreads();
-  uint32_t rows = block_max < (uint32_t)LIST_R ? block_max : (uint32_t)LIST_R;
-  rows = rows < P.maxn ? rows : P.maxn;
-  if (i < n)
-    for (uint32_t k = 0; k < rows; ++k) nbr[(size_t)k * P.ld + i] = tile[k * LIST_BS + tid];
+  // Rows [count, largest count of the wavefront) hold the lane's own slot:
+  // the wavefront-uniform walks read them as padding.
+  const uint32_t wcap = wm < P.maxn ? wm : P.maxn;
+  const uint32_t lrows = wcap < (uint32_t)LIST_R ? wcap : (uint32_t)LIST_R;
+  if (i < n) {
+    const uint32_t c = cnt < P.maxn ? cnt : P.maxn;
+    for (uint32_t k = 0; k < lrows; ++k) nbr[(size_t)k * P.ld + i] = k < c ? tile[k * LIST_BS + tid] : i;
+    for (uint32_t k = c > (uint32_t)LIST_R ? c : (uint32_t)LIST_R; k < wcap; ++k) nbr[(size_t)k * P.ld + i] = i;
+  }
 }
 
 
@@ -461,7 +501,9 @@ __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__
     b = B[i];
   }
   float gx = 0.0f, gy = 0.0f, gz = 0.0f, s2 = 0.0f, rho = 0.0f;
-  for_each_neighbor(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
+  // padding entries (j = i): zero gradient, poly6 taken with weight 0
+  // (fma(w, 1, rho) is rho + w exactly)
+  for_each_neighbor_w<4>(nbr, i, P.ld, ncount[i], [&](uint32_t j, bool valid) {
     const float4 xj = X[j];
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float d2 = dist2(dx, dy, dz);
@@ -469,7 +511,7 @@ __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__
     float ax = f * dx, ay = f * dy, az = f * dz;
     gx += ax; gy += ay; gz += az;
     s2 = fmaf(az, az, fmaf(ay, ay, fmaf(ax, ax, s2)));
-    rho += poly6(P, d2);
+    rho = fmaf(poly6(P, d2), valid ? 1.0f : 0.0f, rho);
   });
   rho *= P.V;
   gx *= P.V; gy *= P.V; gz *= P.V;
@@ -565,7 +607,8 @@ __global__ void __launch_bounds__(BS) k_dp(uint32_t n, const float4 *__restrict_
   const float4 xi = X[i];
   const float li = xi.w;
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for_each_neighbor<8>(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
+  // padding entries (j = i): the gradient at d = 0 is 0 and dx = 0
+  for_each_neighbor_w<8>(nbr, i, P.ld, ncount[i], [&](uint32_t j, bool) {
     const float4 xj = X[j];
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float fl = (li + xj.w) * spiky_f(P, dist2(dx, dy, dz));
@@ -596,9 +639,9 @@ __global__ void __launch_bounds__(BS) k_density(uint32_t n, const float4 *__rest
   if (i >= n) return;
   const float4 xi = X[i];
   float rho = 0.0f;
-  for_each_neighbor<8>(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
+  for_each_neighbor_w<8>(nbr, i, P.ld, ncount[i], [&](uint32_t j, bool valid) {
     const float4 xj = X[j];
-    rho += poly6(P, dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z));
+    rho = fmaf(poly6(P, dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z)), valid ? 1.0f : 0.0f, rho);
   });
   rho *= P.m;
   const float4 xo = Xold[i];
@@ -617,7 +660,8 @@ __global__ void __launch_bounds__(BS) k_vorticity(uint32_t n, const float4 *__re
   if (i >= n) return;
   const float4 xi = pos[i], vi = vel[i];
   float wx = 0.0f, wy = 0.0f, wz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
-  for_each_neighbor<8>(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
+  // padding entries (j = i): zero relative velocity and gradient
+  for_each_neighbor_w<8>(nbr, i, P.ld, ncount[i], [&](uint32_t j, bool) {
     const float4 xj = pos[j], vj = vel[j];
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
@@ -644,7 +688,9 @@ __global__ void __launch_bounds__(BS) k_xsph(uint32_t n, const float4 *__restric
   const float4 xi = pos[i], vi = vel[i], ni = nrm[i], wi = om[i];
   const float rhoi = xi.w;
   float ux = 0.0f, uy = 0.0f, uz = 0.0f, ex = 0.0f, ey = 0.0f, ez = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
-  for_each_neighbor<4>(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
+  // padding entries (j = i): zero relative velocity, gradient, cohesion and
+  // normal difference
+  for_each_neighbor_w<4>(nbr, i, P.ld, ncount[i], [&](uint32_t j, bool) {
     const float4 xj = pos[j], vj = vel[j], nj = nrm[j];
     const float wj = nj.w;
     const float rhoj = xj.w;
