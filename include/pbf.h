@@ -48,8 +48,8 @@ typedef struct pbf_ctx pbf_ctx;
  * MAX_PARTICLE_NUM, MAX_NEIGHBOR_COUNT = 60) and the fixed unit-box domain of
  * PBF/shader/common.ts:43-68, all made runtime values. */
 typedef struct {
-  uint32_t capacity;        /* max particles over all scenes (SPH.MAX_PARTICLE_NUM) */
-  uint32_t max_neighbors;   /* per-particle neighbour capacity (SPH.MAX_NEIGHBOR_COUNT); 0 -> 96 */
+  uint32_t capacity;        /* max particles over all scenes (SPH.MAX_PARTICLE_NUM), <= 2^27 - 256 */
+  uint32_t max_neighbors;   /* per-particle neighbour capacity (SPH.MAX_NEIGHBOR_COUNT); 0 -> 96, < 65536 */
   double particle_radius;   /* r */
   double kernel_radius;     /* h; 0 -> 4 r */
   double rest_density;      /* rho0; 0 -> 1000 */

@@ -549,6 +549,8 @@ int upload_scenes(pbf_ctx *c) {
   return PBF_OK;
 }
 
+constexpr uint32_t kMaxCapacity = (1u << 27) - 256u;
+
 int create_ctx(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc *descs, bool batch, pbf_ctx **out) {
   pbf_ctx *c = nullptr;
   if (!params || !out || nscenes == 0) return PBF_E_INVALID;
@@ -561,8 +563,14 @@ int create_ctx(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc 
   if (p.rest_density <= 0.0) p.rest_density = 1000.0;
   if (p.time_step <= 0.0) p.time_step = 1.0 / 300.0;
   if (p.max_neighbors == 0) p.max_neighbors = 96;
+  if (p.max_neighbors > 65535u) {
+    int r = fail(c, PBF_E_INVALID, "max_neighbors must be below 65536"); delete c; return r;
+  }
   if (p.grid_mode == PBF_GRID_HASH && (p.hash_log2 < 4 || p.hash_log2 > 28)) {
     int r = fail(c, PBF_E_INVALID, "hash_log2 must be in [4, 28]"); delete c; return r;
+  }
+  if (p.capacity > kMaxCapacity) {  // 32-bit gather offsets (slot * 16 bytes)
+    int r = fail(c, PBF_E_CAPACITY, "capacity above 2^27 - 256 particles"); delete c; return r;
   }
   c->cfg = {5u, 0.01f, 0.1f, 0.5f, 9.8f};  // src/common/config.ts:22-28
   c->batch = batch;

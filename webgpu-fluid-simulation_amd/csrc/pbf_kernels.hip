@@ -72,6 +72,40 @@ __device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ n
   for (; k < cnt; ++k) f(row[(size_t)k * ld]);
 }
 
+// Largest value among the active lanes (values below 2^16), bit by bit from
+// ballots, as a scalar: lanes that left the kernel early take no part.
+__device__ __forceinline__ uint32_t active_max(uint32_t v) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int b = 15; b >= 0; --b) {
+    const uint32_t t = m | (1u << b);
+    if (__ballot(v >= t)) m = t;
+  }
+  return __builtin_amdgcn_readfirstlane(m);
+}
+
+// Raw buffer gathers (gfx9 descriptor word 3 = 0x00020000): a neighbour's
+// record is addressed by a 32-bit lane offset j * 16 on a scalar base
+// instead of 64-bit per-lane address arithmetic; rows of the neighbour table
+// by a scalar row offset on a per-lane column offset.  Capacities stay below
+// 2^27 slots, so every offset fits 32 bits.
+constexpr int kBufWord3 = 0x00020000;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x3 __attribute__((ext_vector_type(3)));
+struct Gather {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ Gather(const float4 *p, uint32_t n)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float4 *>(p), 0, (int)(n * 16u), kBufWord3)) {}
+  __device__ float4 operator[](uint32_t j) const {
+    const f32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, j * 16u, 0, 0);
+    return make_float4(v.x, v.y, v.z, v.w);
+  }
+  __device__ float3 xyz(uint32_t j) const {
+    const f32x3 v = __builtin_amdgcn_raw_buffer_load_b96(r, j * 16u, 0, 0);
+    return make_float3(v.x, v.y, v.z);
+  }
+};
+
 // Wavefront-uniform walk: every lane runs as many bodies as the largest count
 // among the active lanes of its wavefront -- batches of UNROLL, then one of 4,
 // then singles, each loop test a ballot -- instead of each lane's own
@@ -82,17 +116,51 @@ __device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ n
 // nothing (d = 0: every gradient and difference term is 0; a poly6 sum takes
 // them with weight 0).  Valid entries are visited in walk order, so every sum
 // is the same as the per-lane loop's.
-template <int UNROLL, typename F>
+template <int UNROLL, typename Fetch, typename Use>
 __device__ __forceinline__ void for_each_neighbor_w(const uint32_t *__restrict__ nbr, uint32_t i, uint32_t ld,
-                                                    uint32_t cnt, F &&f) {
-  // largest count among the active lanes, bit by bit from ballots
-  uint32_t m = 0;
+                                                    uint32_t cnt, Fetch &&fetch, Use &&use) {
+  const uint32_t kmax = active_max(cnt);
+  const uint32_t col = i * 4u, rstride = ld * 4u;
+  auto rows = [&](uint32_t k) {  // rows k.. of the table as one buffer
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(nbr + (size_t)k * ld), 0, -1, kBufWord3);
+  };
+  using T = decltype(fetch(0u));
+  uint32_t k = 0;
+  // each batch: the table loads, then every gather, then the bodies
+  if constexpr (UNROLL >= 8) {
+    for (; k + 8 <= kmax; k += 8) {
+      const auto r = rows(k);
+      uint32_t j[8];
+      T d[8];
 #pragma unroll
-  for (int b = 15; b >= 0; --b) {
-    const uint32_t t = m | (1u << b);
-    if (__ballot(cnt >= t)) m = t;
+      for (int q = 0; q < 8; ++q) j[q] = __builtin_amdgcn_raw_buffer_load_b32(r, col, q * rstride, 0);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) d[q] = fetch(j[q]);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) use(d[q], k + q < cnt);
+    }
   }
-  const uint32_t kmax = __builtin_amdgcn_readfirstlane(m);
+  for (; k + 4 <= kmax; k += 4) {
+    const auto r = rows(k);
+    uint32_t j[4];
+    T d[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) j[q] = __builtin_amdgcn_raw_buffer_load_b32(r, col, q * rstride, 0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) d[q] = fetch(j[q]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) use(d[q], k + q < cnt);
+  }
+  for (; k < kmax; ++k) use(fetch(__builtin_amdgcn_raw_buffer_load_b32(rows(k), col, 0, 0)), k < cnt);
+}
+
+// The same walk with 64-bit global loads for the table and a single body
+// that gathers for itself: faster than the split form for the passes that
+// gather two or three records per neighbour (vorticity, XSPH).
+template <int UNROLL, typename F>
+__device__ __forceinline__ void for_each_neighbor_g(const uint32_t *__restrict__ nbr, uint32_t i, uint32_t ld,
+                                                    uint32_t cnt, F &&f) {
+  const uint32_t kmax = active_max(cnt);
   const uint32_t *row = nbr + i;
   uint32_t k = 0;
   if constexpr (UNROLL >= 8) {
@@ -503,8 +571,9 @@ __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__
   float gx = 0.0f, gy = 0.0f, gz = 0.0f, s2 = 0.0f, rho = 0.0f;
   // padding entries (j = i): zero gradient, poly6 taken with weight 0
   // (fma(w, 1, rho) is rho + w exactly)
-  for_each_neighbor_w<4>(nbr, i, P.ld, ncount[i], [&](uint32_t j, bool valid) {
-    const float4 xj = X[j];
+  const Gather GX(X, n);
+  for_each_neighbor_w<4>(nbr, i, P.ld, ncount[i], [&](uint32_t j) { return GX.xyz(j); },
+                         [&](const float3 &xj, bool valid) {
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float d2 = dist2(dx, dy, dz);
     float f = spiky_f(P, d2);
@@ -608,8 +677,8 @@ __global__ void __launch_bounds__(BS) k_dp(uint32_t n, const float4 *__restrict_
   const float li = xi.w;
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   // padding entries (j = i): the gradient at d = 0 is 0 and dx = 0
-  for_each_neighbor_w<8>(nbr, i, P.ld, ncount[i], [&](uint32_t j, bool) {
-    const float4 xj = X[j];
+  const Gather GX(X, n);
+  for_each_neighbor_w<8>(nbr, i, P.ld, ncount[i], [&](uint32_t j) { return GX[j]; }, [&](const float4 &xj, bool) {
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float fl = (li + xj.w) * spiky_f(P, dist2(dx, dy, dz));
     ax = fmaf(fl, dx, ax); ay = fmaf(fl, dy, ay); az = fmaf(fl, dz, az);
@@ -639,8 +708,9 @@ __global__ void __launch_bounds__(BS) k_density(uint32_t n, const float4 *__rest
   if (i >= n) return;
   const float4 xi = X[i];
   float rho = 0.0f;
-  for_each_neighbor_w<8>(nbr, i, P.ld, ncount[i], [&](uint32_t j, bool valid) {
-    const float4 xj = X[j];
+  const Gather GX(X, n);
+  for_each_neighbor_w<8>(nbr, i, P.ld, ncount[i], [&](uint32_t j) { return GX.xyz(j); },
+                         [&](const float3 &xj, bool valid) {
     rho = fmaf(poly6(P, dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z)), valid ? 1.0f : 0.0f, rho);
   });
   rho *= P.m;
@@ -661,7 +731,7 @@ __global__ void __launch_bounds__(BS) k_vorticity(uint32_t n, const float4 *__re
   const float4 xi = pos[i], vi = vel[i];
   float wx = 0.0f, wy = 0.0f, wz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
   // padding entries (j = i): zero relative velocity and gradient
-  for_each_neighbor_w<8>(nbr, i, P.ld, ncount[i], [&](uint32_t j, bool) {
+  for_each_neighbor_g<8>(nbr, i, P.ld, ncount[i], [&](uint32_t j, bool) {
     const float4 xj = pos[j], vj = vel[j];
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
@@ -690,7 +760,7 @@ __global__ void __launch_bounds__(BS) k_xsph(uint32_t n, const float4 *__restric
   float ux = 0.0f, uy = 0.0f, uz = 0.0f, ex = 0.0f, ey = 0.0f, ez = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
   // padding entries (j = i): zero relative velocity, gradient, cohesion and
   // normal difference
-  for_each_neighbor_w<4>(nbr, i, P.ld, ncount[i], [&](uint32_t j, bool) {
+  for_each_neighbor_g<4>(nbr, i, P.ld, ncount[i], [&](uint32_t j, bool) {
     const float4 xj = pos[j], vj = vel[j], nj = nrm[j];
     const float wj = nj.w;
     const float rhoj = xj.w;
