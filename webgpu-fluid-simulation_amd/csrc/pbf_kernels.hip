@@ -100,8 +100,8 @@ struct Gather {
     const f32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, j * 16u, 0, 0);
     return make_float4(v.x, v.y, v.z, v.w);
   }
-  __device__ float3 xyz(uint32_t j) const {
-    const f32x3 v = __builtin_amdgcn_raw_buffer_load_b96(r, j * 16u, 0, 0);
+  __device__ float3 xyz(uint32_t j, uint32_t q = 0) const {  // record j + q, q uniform
+    const f32x3 v = __builtin_amdgcn_raw_buffer_load_b96(r, j * 16u, q * 16u, 0);
     return make_float3(v.x, v.y, v.z);
   }
 };
@@ -452,31 +452,47 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
     constexpr float kReach = 1.001f;
     auto gap = [](int d, float f) { return d < 0 ? f * f : d > 0 ? (1.0f - f) * (1.0f - f) : 0.0f; };
     const uint32_t cbase = sc.cell_base;
-    // Branch-free in the common case: every candidate is stored, accepted
-    // ones in the particle's next LDS row, rejected ones in the spare row R
-    // (overwritten freely); only entries past R leave the LDS.
-    auto test = [&](const float4 &xj, uint32_t j) {
+    // Branch-free: every candidate is written to the particle's next free
+    // LDS row (clamped to the spare row R) and the row is taken only when the
+    // candidate is accepted, so a rejected one is overwritten by the next.
+    // Entries past R are rare; a batch that produced any is tested again
+    // (uniform branch) and those entries go straight to the table.
+    const Gather GX(X0, n);
+    auto test = [&](const float3 &xj, uint32_t j) {
       const float d2 = dist2(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
-      const bool in = d2 <= P.h2;
-      const uint32_t row = in && cnt < (uint32_t)LIST_R ? cnt : (uint32_t)LIST_R;
-      tile[row * LIST_BS + tid] = j;
-      if (in && cnt >= (uint32_t)LIST_R && cnt < P.maxn) nbr[(size_t)cnt * P.ld + i] = j;
-      cnt += in ? 1u : 0u;
+      tile[(cnt < (uint32_t)LIST_R ? cnt : (uint32_t)LIST_R) * LIST_BS + tid] = j;
+      cnt += d2 <= P.h2 ? 1u : 0u;
+    };
+    auto spill = [&](uint32_t j0, uint32_t m, uint32_t k) {  // k: count before the batch
+      for (uint32_t q = 0; q < m; ++q) {
+        const float4 xj = X0[j0 + q];
+        const bool in = dist2(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z) <= P.h2;
+        if (in && k >= (uint32_t)LIST_R && k < P.maxn) nbr[(size_t)k * P.ld + i] = j0 + q;
+        k += in ? 1u : 0u;
+      }
     };
     auto visit = [&](uint32_t t0, uint32_t t1) {
       uint32_t j = t0;
       for (; j + 8 <= t1; j += 8) {  // eight candidate loads in flight
-        float4 c[8];
+        float3 c[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) c[q] = X0[j + q];
+        for (int q = 0; q < 8; ++q) c[q] = GX.xyz(j, q);
+        const uint32_t k0 = cnt;
 #pragma unroll
         for (int q = 0; q < 8; ++q) test(c[q], j + q);
+        if (__ballot(cnt > (uint32_t)LIST_R)) spill(j, 8, k0);
       }
       for (; j + 4 <= t1; j += 4) {
-        const float4 a = X0[j], b = X0[j + 1], c = X0[j + 2], d = X0[j + 3];
+        const float3 a = GX.xyz(j), b = GX.xyz(j, 1), c = GX.xyz(j, 2), d = GX.xyz(j, 3);
+        const uint32_t k0 = cnt;
         test(a, j); test(b, j + 1); test(c, j + 2); test(d, j + 3);
+        if (__ballot(cnt > (uint32_t)LIST_R)) spill(j, 4, k0);
       }
-      for (; j < t1; ++j) test(X0[j], j);
+      for (; j < t1; ++j) {
+        const uint32_t k0 = cnt;
+        test(GX.xyz(j), j);
+        if (__ballot(cnt > (uint32_t)LIST_R)) spill(j, 1, k0);
+      }
     };
     if constexpr (!HASH) {
       // the 18 range bounds are loaded up front so their latencies overlap
