@@ -78,6 +78,30 @@ def test_cell_faces_and_walls():
     _bit_exact(_scene(pos, name="faces"))
 
 
+def test_wide_slabs_mix_table_formats():
+    # Two x-slabs of 64 x 64 cells with 8 particles each: a neighbour one
+    # slab over sits ~32768 slots away, so some wavefronts keep 32-bit
+    # neighbour entries while a small block elsewhere uses 16-bit offsets.
+    r = 0.025
+    g = (np.arange(128) + 0.5) * 2 * r
+    X, Y, Z = np.meshgrid(g[:4] + 0.4, g, g, indexing="ij")
+    wide = np.stack([X.ravel(), Y.ravel(), Z.ravel()], -1)
+    X, Y, Z = np.meshgrid(g[:8] + 4.0, g[:8] + 4.0, g[:8] + 4.0, indexing="ij")
+    small = np.stack([X.ravel(), Y.ravel(), Z.ravel()], -1)
+    pts = np.concatenate([wide, small])
+    pos = np.zeros((len(pts), 4), np.float32)
+    pos[:, :3] = pts + np.random.default_rng(4).uniform(-0.002, 0.002, pts.shape)
+    print(f"[wide slabs] n={len(pos)}", flush=True)
+    sc = _scene(pos, L=6.4, name="slabs", iters=2, field=False, r=r)
+    sim = scenes.make_engine(sc)
+    o = make_oracle(sc)
+    sim.search(sc.substep_dt())
+    o.search(sc.substep_dt())
+    assert np.array_equal(sim.read_neighbors()[1], o.neighbors()[1])
+    sim.close()
+    _bit_exact(sc, steps=1)
+
+
 def test_crowded_cells():
     rng = np.random.default_rng(3)
     pos = np.zeros((600, 4), np.float32)

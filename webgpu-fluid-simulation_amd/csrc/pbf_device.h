@@ -24,6 +24,8 @@ struct Phys {
   float vort, tension;
   uint32_t ld;   // leading dimension of the slot-major neighbour table
   uint32_t maxn; // neighbour capacity per particle
+  uint32_t *nbr16;  // 16-bit neighbour table (offset pairs), slot-major
+  uint32_t *wfmt;   // per wavefront: 1 = 16-bit table, 0 = 32-bit table
   double Kp64, h2_64, V64;
 };
 

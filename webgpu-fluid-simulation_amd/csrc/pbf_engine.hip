@@ -73,6 +73,7 @@ struct pbf_ctx {
   float4 *XA = nullptr, *XB = nullptr, *Xold = nullptr, *B = nullptr, *xs = nullptr, *stage4 = nullptr;
   uint32_t *ids = nullptr, *cellkey = nullptr, *rank = nullptr, *tmp_key = nullptr, *tmp_src = nullptr;
   uint32_t *ncount = nullptr, *nbr = nullptr;
+  uint32_t *nbr16 = nullptr, *wfmt = nullptr;  // 16-bit table and per-wavefront format
   float *lam = nullptr, *lam_sorted = nullptr;
   uint32_t *counts = nullptr, *offsets = nullptr;
   std::vector<uint32_t *> scan_sums, scan_prefix;
@@ -167,6 +168,8 @@ Phys make_phys(const pbf_ctx *c, float dt) {
   P.tension = c->cfg.surface_tension;
   P.ld = c->ld;
   P.maxn = c->maxn;
+  P.nbr16 = c->nbr16;
+  P.wfmt = c->wfmt;
   P.Kp64 = 315.0 / (64.0 * PI * std::pow(h, 9));
   P.h2_64 = h * h;
   P.V64 = V64;
@@ -618,6 +621,8 @@ int create_ctx(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc 
   step(dalloc(&c->ids, N), "alloc"); step(dalloc(&c->cellkey, N), "alloc"); step(dalloc(&c->rank, N), "alloc");
   step(dalloc(&c->tmp_key, N), "alloc"); step(dalloc(&c->tmp_src, N), "alloc"); step(dalloc(&c->ncount, N), "alloc");
   step(dalloc(&c->nbr, (size_t)c->maxn * c->ld), "alloc");
+  step(dalloc(&c->nbr16, (size_t)((c->maxn + 1) / 2) * c->ld), "alloc");
+  step(dalloc(&c->wfmt, (size_t)c->ncap / 64), "alloc");
   step(dalloc(&c->lam, N), "alloc"); step(dalloc(&c->lam_sorted, N), "alloc");
   step(dalloc(&c->counts, (size_t)c->ncells), "alloc");
   step(dalloc(&c->offsets, (size_t)c->ncells + 1), "alloc");
@@ -750,7 +755,7 @@ int pbf_destroy(pbf_ctx *c) {
   drain_events(c);
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
   void *bufs[] = {c->pos, c->vel, c->acc, c->om, c->nrm, c->XA, c->XB, c->Xold, c->B, c->xs, c->stage4, c->ids,
-                  c->cellkey, c->rank, c->tmp_key, c->tmp_src, c->ncount, c->nbr, c->lam, c->lam_sorted, c->counts,
+                  c->cellkey, c->rank, c->tmp_key, c->tmp_src, c->ncount, c->nbr, c->nbr16, c->wfmt, c->lam, c->lam_sorted, c->counts,
                   c->offsets, c->flags, c->d_scenes, c->sid, c->ranges, c->pool, c->ci, c->per_scene, c->hist,
                   c->hist_ctr, c->dbg_counts, c->dbg_ids, c->ids_save, c->lam_acc, c->acc_sorted, c->feats,
                   c->vel_sorted, c->snap_pos, c->snap_vel, c->snap_acc, c->snap_om, c->snap_nrm, c->snap_ids,
@@ -1185,8 +1190,19 @@ int pbf_read_neighbors(pbf_ctx *c, uint32_t *nbr_off, uint32_t *nbr_ids, uint64_
   }
   uint32_t mx = 0;
   for (uint32_t i = 0; i < n; ++i) mx = std::max(mx, cnt[i]);
-  std::vector<uint32_t> table((size_t)mx * c->ld);
-  if (mx) HIPCHK(hipMemcpy(table.data(), c->nbr, table.size() * 4, hipMemcpyDeviceToHost));
+  std::vector<uint32_t> table((size_t)mx * c->ld), table16((size_t)((mx + 1) / 2) * c->ld);
+  std::vector<uint32_t> wfmt((size_t)c->ncap / 64);
+  if (mx) {
+    HIPCHK(hipMemcpy(table.data(), c->nbr, table.size() * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(table16.data(), c->nbr16, table16.size() * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(wfmt.data(), c->wfmt, wfmt.size() * 4, hipMemcpyDeviceToHost));
+  }
+  // entry k of slot s in the wavefront's format (16-bit offsets or slots)
+  auto entry = [&](uint32_t s, uint32_t k) -> uint32_t {
+    if (!wfmt[s >> 6]) return table[(size_t)k * c->ld + s];
+    const uint32_t w = table16[(size_t)(k >> 1) * c->ld + s];
+    return s - 32768u + ((k & 1u) ? (w >> 16) : (w & 0xffffu));
+  };
   std::vector<uint32_t> by_id(n ? n : 1), slot_of(n ? n : 1);
   for (uint32_t i = 0; i < n; ++i) { by_id[ids[i]] = cnt[i]; slot_of[ids[i]] = i; }
   uint64_t tot = 0;
@@ -1201,7 +1217,7 @@ int pbf_read_neighbors(pbf_ctx *c, uint32_t *nbr_off, uint32_t *nbr_ids, uint64_
     uint64_t w = 0;
     for (uint32_t q = 0; q < n; ++q) {
       uint32_t s = slot_of[q];
-      for (uint32_t k = 0; k < cnt[s]; ++k) nbr_ids[w++] = ids[table[(size_t)k * c->ld + s]];
+      for (uint32_t k = 0; k < cnt[s]; ++k) nbr_ids[w++] = ids[entry(s, k)];
     }
   }
   return PBF_OK;

@@ -42,10 +42,33 @@ __device__ __forceinline__ uint32_t slot_index() {
 
 // The neighbour table is slot-major: entry k of particle i sits at
 // nbr[k * ld + i], so one load instruction of a wavefront reads 256
-// contiguous bytes.  Visits the neighbours in walk order.
+// contiguous bytes.  A wavefront whose entries all lie within +-32767 slots
+// of their particles is stored instead as 16-bit offsets j - i + 32768,
+// entries 2p and 2p + 1 packed in word nbr16[p * ld + i] (wfmt[wave] = 1):
+// half the bytes for every walk.  Dense grids list neighbours in ascending
+// slot order, so the first and last entries bound the offsets.
+__device__ __forceinline__ bool wave16(const Phys &P, uint32_t i) {
+  return __builtin_amdgcn_readfirstlane(P.wfmt[i >> 6]) != 0u;
+}
+
+// Per-lane walk (learned-feature and diagnostic passes): visits the
+// neighbours in walk order.
 template <int UNROLL = 4, typename F>
-__device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ nbr, uint32_t i, uint32_t ld,
+__device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ nbr, uint32_t i, const Phys &P,
                                                   uint32_t cnt, F &&f) {
+  const uint32_t ld = P.ld;
+  if (wave16(P, i)) {
+    const uint32_t *row = P.nbr16 + i;
+    const uint32_t base = i - 32768u;
+    uint32_t k = 0;
+    for (; k + 2 <= cnt; k += 2) {
+      const uint32_t w = row[(size_t)(k >> 1) * ld];
+      f(base + (w & 0xffffu));
+      f(base + (w >> 16));
+    }
+    if (k < cnt) f(base + (row[(size_t)(k >> 1) * ld] & 0xffffu));
+    return;
+  }
   const uint32_t *row = nbr + i;
   uint32_t k = 0;
   if constexpr (UNROLL >= 8) {
@@ -117,16 +140,60 @@ struct Gather {
 // them with weight 0).  Valid entries are visited in walk order, so every sum
 // is the same as the per-lane loop's.
 template <int UNROLL, typename Fetch, typename Use>
-__device__ __forceinline__ void for_each_neighbor_w(const uint32_t *__restrict__ nbr, uint32_t i, uint32_t ld,
+__device__ __forceinline__ void for_each_neighbor_w(const uint32_t *__restrict__ nbr, uint32_t i, const Phys &P,
                                                     uint32_t cnt, Fetch &&fetch, Use &&use) {
   const uint32_t kmax = active_max(cnt);
-  const uint32_t col = i * 4u, rstride = ld * 4u;
+  const uint32_t ld = P.ld, col = i * 4u, rstride = ld * 4u;
+  using T = decltype(fetch(0u));
+  uint32_t k = 0;
+  if (wave16(P, i)) {
+    const uint32_t base = i - 32768u;
+    auto pairs = [&](uint32_t k) {  // pair rows k/2.. of the 16-bit table as one buffer
+      return __builtin_amdgcn_make_buffer_rsrc(P.nbr16 + (size_t)(k >> 1) * ld, 0, -1, kBufWord3);
+    };
+    // each batch: the table loads, then every gather, then the bodies
+    if constexpr (UNROLL >= 8) {
+      for (; k + 8 <= kmax; k += 8) {
+        const auto r = pairs(k);
+        uint32_t w[4];
+        T d[8];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[q] = __builtin_amdgcn_raw_buffer_load_b32(r, col, q * rstride, 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          d[2 * q] = fetch(base + (w[q] & 0xffffu));
+          d[2 * q + 1] = fetch(base + (w[q] >> 16));
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) use(d[q], k + q < cnt);
+      }
+    }
+    for (; k + 4 <= kmax; k += 4) {
+      const auto r = pairs(k);
+      uint32_t w[2];
+      T d[4];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) w[q] = __builtin_amdgcn_raw_buffer_load_b32(r, col, q * rstride, 0);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        d[2 * q] = fetch(base + (w[q] & 0xffffu));
+        d[2 * q + 1] = fetch(base + (w[q] >> 16));
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) use(d[q], k + q < cnt);
+    }
+    for (; k + 2 <= kmax; k += 2) {
+      const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(pairs(k), col, 0, 0);
+      const T d0 = fetch(base + (w & 0xffffu)), d1 = fetch(base + (w >> 16));
+      use(d0, k < cnt);
+      use(d1, k + 1 < cnt);
+    }
+    if (k < kmax) use(fetch(base + (__builtin_amdgcn_raw_buffer_load_b32(pairs(k), col, 0, 0) & 0xffffu)), k < cnt);
+    return;
+  }
   auto rows = [&](uint32_t k) {  // rows k.. of the table as one buffer
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(nbr + (size_t)k * ld), 0, -1, kBufWord3);
   };
-  using T = decltype(fetch(0u));
-  uint32_t k = 0;
-  // each batch: the table loads, then every gather, then the bodies
   if constexpr (UNROLL >= 8) {
     for (; k + 8 <= kmax; k += 8) {
       const auto r = rows(k);
@@ -158,11 +225,45 @@ __device__ __forceinline__ void for_each_neighbor_w(const uint32_t *__restrict__
 // that gathers for itself: faster than the split form for the passes that
 // gather two or three records per neighbour (vorticity, XSPH).
 template <int UNROLL, typename F>
-__device__ __forceinline__ void for_each_neighbor_g(const uint32_t *__restrict__ nbr, uint32_t i, uint32_t ld,
+__device__ __forceinline__ void for_each_neighbor_g(const uint32_t *__restrict__ nbr, uint32_t i, const Phys &P,
                                                     uint32_t cnt, F &&f) {
   const uint32_t kmax = active_max(cnt);
-  const uint32_t *row = nbr + i;
+  const uint32_t ld = P.ld;
   uint32_t k = 0;
+  if (wave16(P, i)) {
+    const uint32_t *row = P.nbr16 + i;
+    const uint32_t base = i - 32768u;
+    if constexpr (UNROLL >= 8) {
+      for (; k + 8 <= kmax; k += 8) {
+        uint32_t w[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[q] = row[(size_t)((k >> 1) + q) * ld];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          f(base + (w[q] & 0xffffu), k + 2 * q < cnt);
+          f(base + (w[q] >> 16), k + 2 * q + 1 < cnt);
+        }
+      }
+    }
+    for (; k + 4 <= kmax; k += 4) {
+      uint32_t w[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) w[q] = row[(size_t)((k >> 1) + q) * ld];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        f(base + (w[q] & 0xffffu), k + 2 * q < cnt);
+        f(base + (w[q] >> 16), k + 2 * q + 1 < cnt);
+      }
+    }
+    for (; k + 2 <= kmax; k += 2) {
+      const uint32_t w = row[(size_t)(k >> 1) * ld];
+      f(base + (w & 0xffffu), k < cnt);
+      f(base + (w >> 16), k + 1 < cnt);
+    }
+    if (k < kmax) f(base + (row[(size_t)(k >> 1) * ld] & 0xffffu), k < cnt);
+    return;
+  }
+  const uint32_t *row = nbr + i;
   if constexpr (UNROLL >= 8) {
     for (; k + 8 <= kmax; k += 8) {
       uint32_t j[8];
@@ -434,7 +535,12 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
   const uint32_t tid = threadIdx.x;
   const uint32_t base_slot = xcd_block(blockIdx.x, gridDim.x) * LIST_BS;
   const uint32_t i = base_slot + tid;
-  uint32_t cnt = 0;
+  auto store16 = [&](uint32_t k, uint32_t j) {  // entry k as a 16-bit offset
+    reinterpret_cast<uint16_t *>(P.nbr16)[((size_t)(k >> 1) * P.ld + i) * 2 + (k & 1u)] =
+        (uint16_t)(j - (i - 32768u));
+  };
+  uint32_t cnt = 0, last = i;
+  bool fits16 = true;  // every entry within +-32767 slots of i
   if (i < n) {
     const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
     const float4 xi = X0[i];
@@ -463,11 +569,17 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
       tile[(cnt < (uint32_t)LIST_R ? cnt : (uint32_t)LIST_R) * LIST_BS + tid] = j;
       cnt += d2 <= P.h2 ? 1u : 0u;
     };
+    // spilled entries go to both tables (the wavefront's format is decided
+    // at the end); `last` keeps the latest one
     auto spill = [&](uint32_t j0, uint32_t m, uint32_t k) {  // k: count before the batch
       for (uint32_t q = 0; q < m; ++q) {
         const float4 xj = X0[j0 + q];
         const bool in = dist2(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z) <= P.h2;
-        if (in && k >= (uint32_t)LIST_R && k < P.maxn) nbr[(size_t)k * P.ld + i] = j0 + q;
+        if (in && k >= (uint32_t)LIST_R && k < P.maxn) {
+          nbr[(size_t)k * P.ld + i] = j0 + q;
+          if constexpr (!HASH) store16(k, j0 + q);
+          last = j0 + q;
+        }
         k += in ? 1u : 0u;
       }
     };
@@ -531,7 +643,13 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
           }
     }
     ncount[i] = cnt < P.maxn ? cnt : P.maxn;
+    const uint32_t c = cnt < P.maxn ? cnt : P.maxn;
+    if (c > 0 && c - 1 < (uint32_t)LIST_R) last = tile[(c - 1) * LIST_BS + tid];
+    const uint32_t first = c > 0 ? tile[tid] : i;
+    fits16 = !HASH && first - i + 32768u < 65536u && last - i + 32768u < 65536u;
   }
+  const bool w16 = !HASH && !__ballot(!fits16);  // lanes past n count as fitting
+  if (__lane_id() == 0) P.wfmt[i >> 6] = w16 ? 1u : 0u;
   const uint32_t wm = wave_max_u32(cnt);  // lanes past n hold cnt = 0
   if (__lane_id() == 0 && wm > 0) {
     atomicMax(&flags[1], wm);
@@ -543,8 +661,21 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
   const uint32_t lrows = wcap < (uint32_t)LIST_R ? wcap : (uint32_t)LIST_R;
   if (i < n) {
     const uint32_t c = cnt < P.maxn ? cnt : P.maxn;
-    for (uint32_t k = 0; k < lrows; ++k) nbr[(size_t)k * P.ld + i] = k < c ? tile[k * LIST_BS + tid] : i;
-    for (uint32_t k = c > (uint32_t)LIST_R ? c : (uint32_t)LIST_R; k < wcap; ++k) nbr[(size_t)k * P.ld + i] = i;
+    const uint32_t k1 = c > (uint32_t)LIST_R ? c : (uint32_t)LIST_R;
+    if (w16) {
+      // whole pair words for the LDS rows (R is even), then half-words; the
+      // walks read whole pairs, so padding runs to wcap rounded up to even
+      const uint32_t base = i - 32768u;
+      for (uint32_t k = 0; k < lrows; k += 2) {
+        const uint32_t a = k < c ? tile[k * LIST_BS + tid] : i;
+        const uint32_t b = k + 1 < c ? tile[(k + 1) * LIST_BS + tid] : i;
+        P.nbr16[(size_t)(k >> 1) * P.ld + i] = (a - base) | ((b - base) << 16);
+      }
+      for (uint32_t k = k1; k < ((wcap + 1u) & ~1u); ++k) store16(k, i);
+    } else {
+      for (uint32_t k = 0; k < lrows; ++k) nbr[(size_t)k * P.ld + i] = k < c ? tile[k * LIST_BS + tid] : i;
+      for (uint32_t k = k1; k < wcap; ++k) nbr[(size_t)k * P.ld + i] = i;
+    }
   }
 }
 
@@ -588,7 +719,7 @@ __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__
   // padding entries (j = i): zero gradient, poly6 taken with weight 0
   // (fma(w, 1, rho) is rho + w exactly)
   const Gather GX(X, n);
-  for_each_neighbor_w<4>(nbr, i, P.ld, ncount[i], [&](uint32_t j) { return GX.xyz(j); },
+  for_each_neighbor_w<4>(nbr, i, P, ncount[i], [&](uint32_t j) { return GX.xyz(j); },
                          [&](const float3 &xj, bool valid) {
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float d2 = dist2(dx, dy, dz);
@@ -640,7 +771,7 @@ __global__ void __launch_bounds__(BS) k_lambda_feat(uint32_t n, float4 *X, float
   const float4 vi = lb.vel_prev[i];
   const uint32_t cnt = ncount[i];
   float gx = 0.0f, gy = 0.0f, gz = 0.0f, s2 = 0.0f, rho = 0.0f, dv = 0.0f;
-  for_each_neighbor<2>(nbr, i, P.ld, cnt, [&](uint32_t j) {
+  for_each_neighbor<2>(nbr, i, P, cnt, [&](uint32_t j) {
     const float4 xj = X[j];
     const float4 vj = lb.vel_prev[j];
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -694,7 +825,7 @@ __global__ void __launch_bounds__(BS) k_dp(uint32_t n, const float4 *__restrict_
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   // padding entries (j = i): the gradient at d = 0 is 0 and dx = 0
   const Gather GX(X, n);
-  for_each_neighbor_w<8>(nbr, i, P.ld, ncount[i], [&](uint32_t j) { return GX[j]; }, [&](const float4 &xj, bool) {
+  for_each_neighbor_w<8>(nbr, i, P, ncount[i], [&](uint32_t j) { return GX[j]; }, [&](const float4 &xj, bool) {
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float fl = (li + xj.w) * spiky_f(P, dist2(dx, dy, dz));
     ax = fmaf(fl, dx, ax); ay = fmaf(fl, dy, ay); az = fmaf(fl, dz, az);
@@ -725,7 +856,7 @@ __global__ void __launch_bounds__(BS) k_density(uint32_t n, const float4 *__rest
   const float4 xi = X[i];
   float rho = 0.0f;
   const Gather GX(X, n);
-  for_each_neighbor_w<8>(nbr, i, P.ld, ncount[i], [&](uint32_t j) { return GX.xyz(j); },
+  for_each_neighbor_w<8>(nbr, i, P, ncount[i], [&](uint32_t j) { return GX.xyz(j); },
                          [&](const float3 &xj, bool valid) {
     rho = fmaf(poly6(P, dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z)), valid ? 1.0f : 0.0f, rho);
   });
@@ -747,7 +878,7 @@ __global__ void __launch_bounds__(BS) k_vorticity(uint32_t n, const float4 *__re
   const float4 xi = pos[i], vi = vel[i];
   float wx = 0.0f, wy = 0.0f, wz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
   // padding entries (j = i): zero relative velocity and gradient
-  for_each_neighbor_g<8>(nbr, i, P.ld, ncount[i], [&](uint32_t j, bool) {
+  for_each_neighbor_g<8>(nbr, i, P, ncount[i], [&](uint32_t j, bool) {
     const float4 xj = pos[j], vj = vel[j];
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
@@ -776,7 +907,7 @@ __global__ void __launch_bounds__(BS) k_xsph(uint32_t n, const float4 *__restric
   float ux = 0.0f, uy = 0.0f, uz = 0.0f, ex = 0.0f, ey = 0.0f, ez = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
   // padding entries (j = i): zero relative velocity, gradient, cohesion and
   // normal difference
-  for_each_neighbor_g<4>(nbr, i, P.ld, ncount[i], [&](uint32_t j, bool) {
+  for_each_neighbor_g<4>(nbr, i, P, ncount[i], [&](uint32_t j, bool) {
     const float4 xj = pos[j], vj = vel[j], nj = nrm[j];
     const float wj = nj.w;
     const float rhoj = xj.w;
@@ -818,7 +949,7 @@ __global__ void __launch_bounds__(BS) k_diag(uint32_t n, const float4 *__restric
   if (i >= n) return;
   const float4 xi = pos[i];
   double rho = 0.0;
-  for_each_neighbor(nbr, i, P.ld, ncount[i], [&](uint32_t j) {
+  for_each_neighbor(nbr, i, P, ncount[i], [&](uint32_t j) {
     const float4 xj = pos[j];
     double dx = (double)xi.x - (double)xj.x, dy = (double)xi.y - (double)xj.y, dz = (double)xi.z - (double)xj.z;
     double d2 = (dx * dx + dy * dy) + dz * dz;
