@@ -69,6 +69,7 @@ def cpu_baseline(sc, budget_s: float):
 
 
 WARM_BLOB = os.path.join(ROOT, "webgpu-fluid-simulation_amd", "warm_start", "dam_break.f32")
+RATIO_BLOB = os.path.join(ROOT, "webgpu-fluid-simulation_amd", "warm_start", "dam_break_ratio.f32")
 PROFILE = os.path.join(ROOT, "profiles", "dam_break_1m_kernels_and_traffic.md")
 
 
@@ -86,54 +87,67 @@ def profiled_traffic(pass_name):
 TARGET_ERR = 0.005
 
 
-def warm_start_comparison(sc, substeps, plain_its, learned_its, relax, blob, previous_its=(), previous_relax=0.5):
-    """Mean (over `substeps` substeps) density error and ms/substep of the plain
-    solve at each iteration count, and of the learned warm start with the
-    plain solve as fallback (a learned substep whose mean density error is
-    above the plain solve's at the same count is redone plainly)."""
+def warm_start_comparison(sc, substeps, plain_its, variants, long_iterations=40):
+    """Mean (over `substeps` substeps) density error, ms/substep and trajectory
+    fidelity of the plain solve at each iteration count and of every warm-start
+    variant (name, mode, blob, relax, iteration counts).  Fidelity = the largest
+    deviation of the mean height (world units) and of the kinetic energy
+    (relative) from a `long_iterations` plain run, checked every 20 substeps;
+    timings come from a separate run without checkpoints.  A warm start can
+    lower the density error by pushing the fluid apart, so a row counts as
+    reaching the target only when its deviations stay within twice those of
+    the plain solve that reaches it (plus 0.01 / 0.02)."""
     import learned
+    ref = learned.run(sc, substeps, long_iterations)
     out = {"scene": sc.name, "particles": sc.n, "substeps": substeps, "target_density_error": TARGET_ERR,
-           "plain": [], "learned": []}
+           "reference_iterations": long_iterations, "plain": [], "warm": []}
+
+    def measure(it, mode=None, blob=None, relax=1.0, thr=0.0):
+        t = learned.run(sc, substeps, it, blob, relax, fallback_threshold=thr, every=0, mode=mode)
+        f = learned.run(sc, substeps, it, blob, relax, fallback_threshold=thr, every=20, mode=mode)
+        dh, dk = learned.fidelity(f, ref)
+        return {"iterations": it, "density_error": float(np.mean(t.density_error)),
+                "ms_per_substep": 1e3 * t.seconds / substeps, "height_dev": dh, "kinetic_dev": dk,
+                "fallbacks": t.fallbacks}
+
     plain_err = {}
     for it in plain_its:
-        t = learned.run(sc, substeps, it, every=0)
-        e = float(np.mean(t.density_error))
-        plain_err[it] = e
-        out["plain"].append({"iterations": it, "density_error": e, "ms_per_substep": 1e3 * t.seconds / substeps})
-        log(f"  plain   I={it:2d}: density error {e:.4%}, {1e3 * t.seconds / substeps:.3f} ms/substep")
-    for it in learned_its:
-        thr = plain_err.get(it, TARGET_ERR)
-        try:
-            t = learned.run(sc, substeps, it, blob, relax, fallback_threshold=thr, every=0)
-        except Exception as e:  # noqa: BLE001 - an unstable learned run is a result, not a crash
-            out["learned"].append({"iterations": it, "relax": relax, "error": str(e)})
-            log(f"  learned I={it:2d}: failed: {e}")
-            continue
-        e = float(np.mean(t.density_error))
-        out["learned"].append({"iterations": it, "relax": relax, "density_error": e,
-                               "ms_per_substep": 1e3 * t.seconds / substeps, "fallbacks": t.fallbacks,
-                               "fallback_threshold": thr})
-        log(f"  learned I={it:2d}: density error {e:.4%}, {1e3 * t.seconds / substeps:.3f} ms/substep, "
-            f"{t.fallbacks} of {substeps} substeps fell back")
-    # the temporal warm start (previous substep's lambda sum, no network)
-    from pbf_host import WARM_PREVIOUS
-    out["previous"] = []
-    for it in previous_its:
-        try:
-            t = learned.run(sc, substeps, it, None, previous_relax, every=0, mode=WARM_PREVIOUS)
-        except Exception as e:  # noqa: BLE001
-            out["previous"].append({"iterations": it, "relax": previous_relax, "error": str(e)})
-            continue
-        e = float(np.mean(t.density_error))
-        out["previous"].append({"iterations": it, "relax": previous_relax, "density_error": e,
-                                "ms_per_substep": 1e3 * t.seconds / substeps})
-        log(f"  previous I={it:2d}: density error {e:.4%}, {1e3 * t.seconds / substeps:.3f} ms/substep")
+        r = measure(it)
+        r.pop("fallbacks")
+        plain_err[it] = r["density_error"]
+        out["plain"].append(r)
+        log(f"  plain      I={it:2d}: density error {r['density_error']:.4%}, {r['ms_per_substep']:.3f} ms/substep, "
+            f"height dev {r['height_dev']:.4f}, KE dev {r['kinetic_dev']:.3f}")
     reach = [r for r in out["plain"] if r["density_error"] <= TARGET_ERR]
-    lreach = [r for r in out["learned"] if "density_error" in r and r["density_error"] <= TARGET_ERR]
-    out["plain_iterations_to_target"] = reach[0]["iterations"] if reach else None
-    out["learned_iterations_to_target"] = lreach[0]["iterations"] if lreach else None
-    if reach and lreach:
-        out["speedup_at_target"] = reach[0]["ms_per_substep"] / lreach[0]["ms_per_substep"]
+    base = reach[0] if reach else None
+    for name, mode, blob, relax, its in variants:
+        for it in its:
+            # the learned modes fall back to the plain solve when a substep ends above the
+            # plain solve's error at the same count
+            thr = plain_err.get(it, TARGET_ERR) if name != "previous" else 0.0
+            try:
+                r = measure(it, mode, blob, relax, thr)
+            except Exception as e:  # noqa: BLE001 - an unstable warm start is a result, not a crash
+                out["warm"].append({"solver": name, "iterations": it, "relax": relax, "error": str(e)})
+                log(f"  {name:10s} I={it:2d} relax {relax}: failed: {e}")
+                continue
+            r.update({"solver": name, "relax": relax, "fallback_threshold": thr})
+            r["fidelity_ok"] = bool(base is None or (r["height_dev"] <= 2 * base["height_dev"] + 0.01 and
+                                                     r["kinetic_dev"] <= 2 * base["kinetic_dev"] + 0.02))
+            out["warm"].append(r)
+            log(f"  {name:10s} I={it:2d} relax {relax}: density error {r['density_error']:.4%}, "
+                f"{r['ms_per_substep']:.3f} ms/substep, height dev {r['height_dev']:.4f}, KE dev "
+                f"{r['kinetic_dev']:.3f}, {r['fallbacks']} fallbacks{'' if r['fidelity_ok'] else ', trajectory off'}")
+    out["plain_iterations_to_target"] = base["iterations"] if base else None
+    best = None
+    for r in out["warm"]:
+        if "density_error" in r and r["density_error"] <= TARGET_ERR and r["fidelity_ok"]:
+            if best is None or r["ms_per_substep"] < best["ms_per_substep"]:
+                best = r
+    out["best_warm_start_at_target"] = ({k: best[k] for k in ("solver", "iterations", "relax", "ms_per_substep")}
+                                        if best else None)
+    if base and best:
+        out["speedup_at_target"] = base["ms_per_substep"] / best["ms_per_substep"]
     return out
 
 
@@ -284,14 +298,22 @@ def main():
         log(f"cpu baseline {cpu['value'] / 1e6:.3f} M particle-substeps/s on {cpu['cores']} threads")
 
     warm = None
-    if not args.no_learned and os.path.exists(WARM_BLOB):
+    if not args.no_learned and os.path.exists(WARM_BLOB) and os.path.exists(RATIO_BLOB):
+        from pbf_host import WARM_LEARNED, WARM_PREVIOUS, WARM_RATIO
         blob = np.fromfile(WARM_BLOB, "<f4")
+        rblob = np.fromfile(RATIO_BLOB, "<f4")
         import train_warm_start as tw
-        log("warm start, held-out dam break the network was not trained on:")
-        small = warm_start_comparison(tw.family(tw.EVAL_SHAPE, tw.EVAL_SEED), 150, (2, 3, 4, 5, 6, 8), (2, 3, 4),
-                                      0.5, blob, (2, 3, 4), 0.8)
+        log("warm start, held-out dam break the networks were not trained on:")
+        small = warm_start_comparison(tw.family(tw.EVAL_SHAPE, tw.EVAL_SEED), 150, (2, 3, 4, 5, 6, 8),
+                                      [("learned", WARM_LEARNED, blob, 0.5, (2, 3)),
+                                       ("ratio", WARM_RATIO, rblob, 0.6, (2, 3)),
+                                       ("previous", WARM_PREVIOUS, None, 0.8, (2, 3))])
         log("warm start, the 1M dam break:")
-        big = warm_start_comparison(sc, 100, (4, 8, 16, 32, 64), (4, 8), 0.5, blob, (4, 8, 16, 32), 0.5)
+        big = warm_start_comparison(sc, 100, (16, 32, 48),
+                                    [("learned", WARM_LEARNED, blob, 0.25, (16,)),
+                                     ("ratio", WARM_RATIO, rblob, 0.5, (16, 24)),
+                                     ("previous", WARM_PREVIOUS, None, 0.5, (16, 24)),
+                                     ("previous", WARM_PREVIOUS, None, 0.25, (16, 24))])
         warm = {"held_out_dam_break": small, "dam_break_1m": big}
 
     batched = None

@@ -257,8 +257,14 @@ int pbf_copy_bandwidth(size_t bytes, uint32_t iterations, double *gbps);
  * 15 boundary part of rho~. */
 enum { PBF_MLP_PARAMS = 5313, PBF_MLP_FEATURES = 16 };
 /* PBF_WARM_PREVIOUS: iteration 0 applies relax x the previous substep's
- * lambda sum, without a network (the classic temporal warm start). */
-enum { PBF_WARM_PLAIN = 0, PBF_WARM_LEARNED = 1, PBF_WARM_PREVIOUS = 2 };
+ * lambda sum, without a network (the classic temporal warm start).
+ * PBF_WARM_RATIO: the network's output (y * tsig + tmu) is a per-particle
+ * factor s on that sum (feature 3): iteration 0 applies
+ * clamp(relax s, 0, PBF_WARM_RATIO_MAX relax) x Lambda_prev, and lmin is not
+ * used.  A network with zero weights and tmu = 1 is exactly
+ * PBF_WARM_PREVIOUS. */
+enum { PBF_WARM_PLAIN = 0, PBF_WARM_LEARNED = 1, PBF_WARM_PREVIOUS = 2, PBF_WARM_RATIO = 3 };
+#define PBF_WARM_RATIO_MAX 2.0f
 
 uint32_t pbf_mlp_blob_size(void);
 /* Glorot-uniform weights from a SplitMix64 stream, zero biases, identity
@@ -274,8 +280,9 @@ int pbf_mlp_forward_backward(const float *blob, const float *features, const flo
  * trained weights; loss_per_epoch (epochs floats) may be NULL. */
 int pbf_mlp_train(const float *features, const float *targets, uint64_t n, uint32_t epochs, uint32_t batch,
                   float lr, uint64_t seed, float *blob, float *loss_per_epoch);
-/* Selects the solve: PBF_WARM_PLAIN, or PBF_WARM_LEARNED with the network
- * in blob (NULL keeps the weights already set).  relax in (0, 1] scales the
+/* Selects the solve: PBF_WARM_PLAIN, PBF_WARM_PREVIOUS, or PBF_WARM_LEARNED /
+ * PBF_WARM_RATIO with the network in blob (NULL keeps the weights already
+ * set).  relax in (0, 1] scales the
  * predicted lambda: the later iterations only remove compression, so an
  * over-predicted lambda pushes particles apart for good (kinetic energy);
  * relax < 1 keeps the warm start on the safe side.  fallback_threshold > 0

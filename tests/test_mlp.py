@@ -324,3 +324,59 @@ def test_temporal_warm_start_runs_and_lowers_the_error():
     prev = learned.run(ev, 60, 2, None, 0.8, every=30, mode=WARM_PREVIOUS)
     assert np.all(np.isfinite(prev.height)) and np.all(np.isfinite(prev.kinetic))
     assert prev.density_error.mean() < 0.8 * plain.density_error.mean()
+
+
+@pytest.mark.gpu
+def test_ratio_mode_with_a_constant_network_is_the_temporal_warm_start():
+    """PBF_WARM_RATIO applies clamp(relax s, 0, 2 relax) x Lambda_prev with s the
+    network output; zero weights and tmu = 1 make s = 1, so every substep must
+    be bit-identical to PBF_WARM_PREVIOUS at the same relax."""
+    import train_warm_start as tw
+    from pbf_host import WARM_PREVIOUS, WARM_RATIO
+    blob = np.zeros(mo.BLOB, np.float32)
+    blob[mo.OFF_FSIG:mo.OFF_TMU] = 1.0
+    blob[mo.OFF_TMU] = 1.0
+    blob[mo.OFF_TSIG] = 1.0
+    blob[mo.OFF_LMIN] = -1e30
+    sc = tw.family(tw.EVAL_SHAPE, tw.EVAL_SEED)
+    print(f"[ratio = temporal] n={sc.n}", flush=True)
+    sims = [scenes.make_engine(sc, max_neighbors=128) for _ in range(2)]
+    sims[0].set_warm_start(None, WARM_PREVIOUS, 0.6)
+    sims[1].set_warm_start(blob, WARM_RATIO, 0.6)
+    for _ in range(12):
+        for s in sims:
+            s.step(sc.substep_dt(), 1, 3)
+    a, b = sims[0].read_positions(), sims[1].read_positions()
+    assert np.array_equal(a, b)
+    assert np.array_equal(sims[0].read_velocities(), sims[1].read_velocities())
+    for s in sims:
+        s.close()
+
+
+RATIO_BLOB = os.path.join(os.path.dirname(WARM_BLOB), "dam_break_ratio.f32")
+
+
+@pytest.mark.gpu
+def test_ratio_warm_start_gain_on_held_out_dam_break():
+    """The committed ratio network at 3 iterations (relax 0.6) reaches a mean
+    density error the plain solve needs 5 for, with a trajectory at least as
+    close to a 40-iteration reference run as the plain solve at 5."""
+    import learned
+    import train_warm_start as tw
+    from pbf_host import WARM_RATIO
+    blob = np.fromfile(RATIO_BLOB, "<f4")
+    assert blob.size == mlp_blob_size()
+    ev = tw.family(tw.EVAL_SHAPE, tw.EVAL_SEED)
+    print(f"[ratio gain] n={ev.n}", flush=True)
+    sub = 150
+    ref = learned.run(ev, sub, 40)
+    plain3, plain5 = learned.run(ev, sub, 3), learned.run(ev, sub, 5)
+    warm3 = learned.run(ev, sub, 3, blob, 0.6, mode=WARM_RATIO)
+    e_p3, e_p5, e_w3 = (float(t.density_error.mean()) for t in (plain3, plain5, warm3))
+    print(f"density error plain I=3 {e_p3:.5f}  plain I=5 {e_p5:.5f}  ratio I=3 {e_w3:.5f}", flush=True)
+    assert e_w3 <= 0.005 < e_p3
+    assert e_w3 <= e_p5
+    dh_p, dk_p = learned.fidelity(plain5, ref)
+    dh_w, dk_w = learned.fidelity(warm3, ref)
+    print(f"fidelity plain I=5 ({dh_p:.4f}, {dk_p:.3f})  ratio I=3 ({dh_w:.4f}, {dk_w:.3f})", flush=True)
+    assert dh_w <= dh_p and dk_w <= dk_p

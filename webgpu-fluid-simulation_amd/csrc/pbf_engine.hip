@@ -32,7 +32,7 @@ const char *kPassNames =
 // lambda at iteration 0), training-pair collection (features at iteration
 // 0, lambda summed over all iterations), and the temporal warm start
 // (iteration 0 applies the previous substep's lambda sum).
-enum SubstepMode : uint32_t { M_PLAIN = 0, M_LEARNED = 1, M_COLLECT = 2, M_PREVIOUS = 3 };
+enum SubstepMode : uint32_t { M_PLAIN = 0, M_LEARNED = 1, M_COLLECT = 2, M_PREVIOUS = 3, M_RATIO = 4 };
 
 struct SceneHost {
   double lo[3], hi[3];
@@ -292,9 +292,10 @@ void enqueue_substep(pbf_ctx *c, const LaunchCtx &L, uint32_t iters, bool diag, 
     if (mode != M_PLAIN && it == 0) launch_lambda_features(t, X, c->B, c->nbr, c->ncount, lb);
     else launch_lambda(t, X, c->B, c->nbr, c->ncount, mode != M_PLAIN ? c->lam_acc : nullptr);
     T.end();
-    if (mode == M_LEARNED && it == 0) {
+    if ((mode == M_LEARNED || mode == M_RATIO) && it == 0) {
       t = T.begin(P_MLP, L);
-      launch_mlp_infer(c->stream, c->mlp, c->n, c->feats, X, c->lam_acc, t.ev0, t.ev1);
+      launch_mlp_infer(c->stream, c->mlp, c->n, c->feats, X, c->lam_acc,
+                       mode == M_RATIO ? PBF_WARM_RATIO_MAX * c->relax_prev : 0.0f, t.ev0, t.ev1);
       T.end();
     } else if (mode == M_PREVIOUS && it == 0) {
       launch_warm_previous(L, c->acc_sorted, X, c->lam_acc, c->relax_prev);
@@ -406,7 +407,9 @@ int max_speed(pbf_ctx *c, float dt, double *vmax);
 // error), or either figure is not finite.
 int run_learned_substep(pbf_ctx *c, float dt, uint32_t iters) {
   c->learned_substeps += 1;
-  const uint32_t mode = c->warm_mode == PBF_WARM_PREVIOUS ? M_PREVIOUS : M_LEARNED;
+  const uint32_t mode = c->warm_mode == PBF_WARM_PREVIOUS ? M_PREVIOUS
+                        : c->warm_mode == PBF_WARM_RATIO  ? M_RATIO
+                                                          : M_LEARNED;
   if (!(c->fb_threshold > 0.0)) return run_substep(c, dt, iters, mode, c->record_diag);
   int rc = ensure_snapshot(c);
   if (rc != PBF_OK) return rc;
@@ -937,7 +940,7 @@ int pbf_step(pbf_ctx *c, float dt, uint32_t substeps, uint32_t iterations) {
 int pbf_set_warm_start(pbf_ctx *c, const float *blob, size_t length, uint32_t mode, float relax,
                        double fallback_threshold, uint32_t fallback_iterations) {
   if (!c) return PBF_E_INVALID;
-  if (mode > PBF_WARM_PREVIOUS) return fail(c, PBF_E_INVALID, "unknown warm-start mode");
+  if (mode > PBF_WARM_RATIO) return fail(c, PBF_E_INVALID, "unknown warm-start mode");
   if (!(relax > 0.0f && relax <= 1.0f)) return fail(c, PBF_E_INVALID, "relax must be in (0, 1]");
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -950,7 +953,8 @@ int pbf_set_warm_start(pbf_ctx *c, const float *blob, size_t length, uint32_t mo
     c->relax_set = -1.0f;
     if (rc != PBF_OK) return fail(c, rc, "could not upload the network weights");
   }
-  if (mode == PBF_WARM_LEARNED && !c->mlp) return fail(c, PBF_E_STATE, "learned warm start needs network weights");
+  if ((mode == PBF_WARM_LEARNED || mode == PBF_WARM_RATIO) && !c->mlp)
+    return fail(c, PBF_E_STATE, "learned warm start needs network weights");
   if (c->mlp && relax != c->relax_set) {
     int rc = mlp_engine_relax(c->mlp, relax, c->stream);
     if (rc != PBF_OK) return fail(c, rc, "could not set the warm-start relaxation");

@@ -64,7 +64,7 @@ int mlp_engine_set(void **handle, const float *blob, hipStream_t st);
 int mlp_engine_relax(void *handle, float relax, hipStream_t st);
 void mlp_engine_free(void **handle);
 void launch_mlp_infer(hipStream_t st, void *handle, uint32_t n, const float *feats, float4 *X, float *lam_acc,
-                      hipEvent_t e0, hipEvent_t e1);
+                      float smax, hipEvent_t e0, hipEvent_t e1);
 void launch_copy(hipStream_t st, void *dst, const void *src, size_t bytes);
 void launch_warm_previous(const LaunchCtx &L, const float *acc_prev, float4 *X, float *lam_acc, float relax);
 void launch_speed(const LaunchCtx &L, const float4 *vel, double *ci, const uint32_t *ranges, uint32_t nscenes,

@@ -105,8 +105,11 @@ __device__ __forceinline__ void tile_forward(const MlpDev &M, const float *__res
 
 // Inference: lambda_hat = y * tsig + tmu for every slot; written to X[i].w
 // (the lambda of Jacobi iteration 0) and to the accumulated-lambda array.
+// Ratio mode (smax > 0): the output is a factor s on the previous substep's
+// lambda sum (feature 3), clamped to [0, smax]: lambda = s * Lambda_prev; a
+// network that outputs the constant relax is the temporal warm start.
 __global__ void __launch_bounds__(256) k_mlp_infer(uint32_t n, const float *__restrict__ feats, MlpDev M,
-                                                   float4 *X, float *__restrict__ lam_acc) {
+                                                   float4 *X, float *__restrict__ lam_acc, float smax) {
   const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int l = __lane_id();
@@ -121,8 +124,12 @@ __global__ void __launch_bounds__(256) k_mlp_infer(uint32_t n, const float *__re
       // prediction keeps the plain lambda already in X[p].w
       float *wp = reinterpret_cast<float *>(X) + 4 * (size_t)p + 3;
       float lam = fmaf(T.y, M.misc[OFF_TSIG - OFF_B3], M.misc[OFF_TMU - OFF_B3]);
-      if (!(lam <= 0.0f) || !isfinite(lam)) lam = *wp;
-      lam = fmaxf(lam, M.misc[OFF_LMIN - OFF_B3]);
+      if (smax > 0.0f) {
+        lam = fminf(fmaxf(lam, 0.0f), smax) * feats[(size_t)p * 16 + 3];  // fmaxf(NaN, 0) = 0
+      } else {
+        if (!(lam <= 0.0f) || !isfinite(lam)) lam = *wp;
+        lam = fmaxf(lam, M.misc[OFF_LMIN - OFF_B3]);
+      }
       *wp = lam;
       if (lam_acc) lam_acc[p] = lam;
     }
@@ -404,15 +411,16 @@ void mlp_engine_free(void **handle) {
 }
 
 void launch_mlp_infer(hipStream_t st, void *handle, uint32_t n, const float *feats, float4 *X, float *lam_acc,
-                      hipEvent_t e0, hipEvent_t e1) {
+                      float smax, hipEvent_t e0, hipEvent_t e1) {
   MlpBuffers *B = static_cast<MlpBuffers *>(handle);
   if (!B || !n) return;
   const uint32_t tiles = (n + 31) / 32;
   const uint32_t blocks = std::min<uint32_t>((tiles + 3) / 4, 2048u);
   if (e0 || e1)
-    hipExtLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, e0, e1, 0, n, feats, mlp_view(*B), X, lam_acc);
+    hipExtLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, e0, e1, 0, n, feats, mlp_view(*B), X, lam_acc,
+                          smax);
   else
-    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, n, feats, mlp_view(*B), X, lam_acc);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, n, feats, mlp_view(*B), X, lam_acc, smax);
 }
 
 }  // namespace pbf

@@ -23,7 +23,7 @@ from typing import Callable, List, Optional, Sequence
 
 import numpy as np
 
-from pbf_host import (MLP_FEATURES, WARM_LEARNED, WARM_PLAIN, PBFError, mlp_init, mlp_train)
+from pbf_host import (MLP_FEATURES, WARM_LEARNED, WARM_PLAIN, WARM_RATIO_MAX, PBFError, mlp_init, mlp_train)
 from scenes import Scene, make_engine
 
 OFF_FMU, OFF_FSIG, OFF_TMU, OFF_TSIG, OFF_LMIN = 5313, 5329, 5345, 5346, 5347
@@ -48,15 +48,18 @@ def set_normalisation(blob: np.ndarray, features: np.ndarray, targets: np.ndarra
 
 def collect(scene: Scene, substeps: int, long_iterations: int = 40, iterations: Optional[int] = None,
             blob: Optional[np.ndarray] = None, skip: int = 0, stride: int = 1, relax: float = 1.0,
-            keep: float = 1.0, seed: int = 0, max_neighbors: int = 128, fallback_threshold: float = 0.0):
+            keep: float = 1.0, seed: int = 0, max_neighbors: int = 128, fallback_threshold: float = 0.0,
+            mode: Optional[int] = None):
     """Training pairs along `substeps` substeps of `scene` with the deployed
-    solver (plain, or learned when `blob` is given); `keep` < 1 keeps a
-    seeded random fraction of the particles of every pair."""
+    solver (plain, learned when `blob` is given, or the warm-start `mode`);
+    `keep` < 1 keeps a seeded random fraction of the particles of every pair."""
     rng = np.random.default_rng(seed)
     sim = make_engine(scene, max_neighbors=max_neighbors)
     try:
-        if blob is not None:
-            sim.set_warm_start(blob, WARM_LEARNED, relax, fallback_threshold)
+        if mode is None:
+            mode = WARM_LEARNED if blob is not None else WARM_PLAIN
+        if mode != WARM_PLAIN:
+            sim.set_warm_start(blob, mode, relax, fallback_threshold)
         it = iterations if iterations is not None else scene.iterations
         dt = scene.substep_dt()
         F: List[np.ndarray] = []
@@ -84,6 +87,21 @@ def collect(scene: Scene, substeps: int, long_iterations: int = 40, iterations: 
         return np.concatenate(F), np.concatenate(T)
     finally:
         sim.close()
+
+
+RATIO_MIN_PREV = 1e-3  # |Lambda_prev| below which a pair carries no ratio
+
+
+def ratio_pairs(features: np.ndarray, targets: np.ndarray):
+    """Ratio-mode training pairs: the factor s = Lambda / Lambda_prev (feature
+    3) that turns the previous substep's lambda sum into the long solve's,
+    clipped to [0, WARM_RATIO_MAX]; pairs without a previous sum are dropped."""
+    f = np.asarray(features, np.float32).reshape(-1, MLP_FEATURES)
+    t = np.asarray(targets, np.float64).reshape(-1)
+    prev = f[:, 3].astype(np.float64)
+    ok = prev < -RATIO_MIN_PREV
+    s = np.clip(t[ok] / prev[ok], 0.0, WARM_RATIO_MAX)
+    return f[ok], s.astype(np.float32)
 
 
 @dataclass

@@ -48,7 +48,8 @@ export class PBF {
                       jitter = 0, seed = 0): number {
     return addon.voxelizeMesh(this.handle, positions, index, matrixWorld ?? null, jitter, seed);
   }
-  /** Learned warm start (no reference counterpart): mode 0 plain, 1 network, 2 previous lambda sum. */
+  /** Learned warm start (no reference counterpart): mode 0 plain, 1 network, 2 previous lambda sum,
+   *  3 network factor on the previous lambda sum. */
   public setWarmStart(blob: Float32Array | null, mode = 1, relax = 0.5, fallbackThreshold = 0, fallbackIterations = 0): void {
     addon.setWarmStart(this.handle, blob, mode, relax, fallbackThreshold, fallbackIterations);
   }

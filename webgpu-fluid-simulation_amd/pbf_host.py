@@ -258,7 +258,8 @@ def copy_bandwidth(nbytes: int = 1 << 30, iterations: int = 20) -> float:
 
 MLP_PARAMS = 5313
 MLP_FEATURES = 16
-WARM_PLAIN, WARM_LEARNED, WARM_PREVIOUS = 0, 1, 2
+WARM_PLAIN, WARM_LEARNED, WARM_PREVIOUS, WARM_RATIO = 0, 1, 2, 3
+WARM_RATIO_MAX = 2.0  # clamp of the ratio-mode factor, in units of relax
 
 
 def mlp_blob_size() -> int:

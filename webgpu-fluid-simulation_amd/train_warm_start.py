@@ -2,6 +2,8 @@
 
     python webgpu-fluid-simulation_amd/train_warm_start.py [--out DIR] [--quick]
 
+    python webgpu-fluid-simulation_amd/train_warm_start.py --ratio --iterations 12 --eval-relax 0.5,0.6,0.7
+
 Training data: dam breaks of several block shapes and seeds (r = 0.025,
 h = 4r, 1/60 s frames in 2 substeps, XSPH 0.01, vorticity 0.01, tension
 0.1 -- the physics of the 1M-particle dam break), collected along the
@@ -29,7 +31,7 @@ import numpy as np  # noqa: E402
 
 import learned  # noqa: E402
 import scenes  # noqa: E402
-from pbf_host import PBFError  # noqa: E402
+from pbf_host import WARM_PREVIOUS, WARM_RATIO, PBFError  # noqa: E402
 
 TRAIN_SHAPES = [(16, 16, 8), (20, 20, 10), (24, 16, 12), (28, 24, 14)]
 EVAL_SHAPE = (24, 18, 12)
@@ -67,6 +69,95 @@ def evaluate(ev, sub, blob, long_iterations, plain_its, learned_its, relaxes, lo
     return float(ref.density_error.mean()), rows
 
 
+def evaluate_modes(ev, sub, long_iterations, plain_its, runs, log=print):
+    """Mean density error, ms/substep and fidelity of the plain solve at each
+    count and of each (name, mode, blob, relax, iterations) run."""
+    ref = learned.run(ev, sub, long_iterations)
+    rows = []
+
+    def row(name, it, rx, tr):
+        dh, dk = learned.fidelity(tr, ref)
+        rows.append({"solver": name, "iterations": it, "relax": rx, "density_error": float(tr.density_error.mean()),
+                     "height_dev": dh, "kinetic_dev": dk})
+        log(f"{name:9s} I={it:2d} relax {rx if rx is not None else '-'}: err {rows[-1]['density_error']:.5f}"
+            f"  dh {dh:.4f}  dKE {dk:.3f}")
+
+    for it in plain_its:
+        row("plain", it, None, learned.run(ev, sub, it))
+    for name, mode, blob, rx, its in runs:
+        for it in its:
+            try:
+                row(name, it, rx, learned.run(ev, sub, it, blob, rx, mode=mode))
+            except PBFError as e:
+                rows.append({"solver": name, "iterations": it, "relax": rx, "error": str(e)})
+                log(f"{name} I={it} relax {rx} failed: {e}")
+    return float(ref.density_error.mean()), rows
+
+
+def main_ratio(a, log):
+    """Ratio mode: the network predicts the factor between the long solve's
+    lambda sum and the previous substep's (PBF_WARM_RATIO).  Pairs are
+    collected along the temporal warm start's trajectory, then (DAgger) along
+    the ratio mode's own."""
+    shapes = TRAIN_SHAPES[:2] if a.quick else TRAIN_SHAPES
+    sub = 60 if a.quick else a.substeps
+    t0 = time.time()
+    jobs = [(s, 100 + 7 * k + j, 1.0) for k, s in enumerate(shapes) for j in range(2)]
+    if a.medium and not a.quick:
+        jobs.append((tuple(int(x) for x in a.medium.split(",")), 200, a.medium_keep))
+    F, T = [], []
+
+    def gather(mode, blob, tag):
+        for shape, seed, keep in jobs:
+            f, t = learned.collect(family(shape, seed), sub, a.long, a.iterations, blob, relax=a.relax, keep=keep,
+                                   seed=seed + (1 if blob is not None else 0), mode=mode)
+            fr, sr = learned.ratio_pairs(f, t)
+            log(f"collect ({tag}) {shape} seed {seed}: {len(sr)} ratio pairs of {len(t)}, "
+                f"s mean {sr.mean():.3f} std {sr.std():.3f}")
+            F.append(fr)
+            T.append(sr)
+
+    gather(WARM_PREVIOUS, None, "temporal")
+    res = learned.train(np.concatenate(F), np.concatenate(T), a.epochs)
+    gather(WARM_RATIO, res.blob, "ratio")
+    Fa, Ta = np.concatenate(F), np.concatenate(T)
+    res = learned.train(Fa, Ta, a.epochs)
+    t_train = time.time() - t0
+    log(f"samples {len(Ta)}  loss {res.losses[0]:.4f} -> {res.losses[-1]:.4f}  ({t_train:.1f} s)")
+    relaxes = [float(x) for x in a.eval_relax.split(",")]
+    runs = [("previous", WARM_PREVIOUS, None, rx, (2, 3, 4)) for rx in relaxes] + \
+           [("ratio", WARM_RATIO, res.blob, rx, (2, 3, 4)) for rx in relaxes]
+    ref_err, rows = evaluate_modes(family(EVAL_SHAPE, EVAL_SEED), sub, a.long, (2, 3, 4, 5, 6, 8), runs, log)
+    large = None
+    if a.eval_large and not a.quick:
+        shape = tuple(int(x) for x in a.eval_large.split(","))
+        log(f"-- large evaluation {shape}, {a.large_substeps} substeps")
+        its = tuple(int(x) for x in a.large_iterations.split(","))
+        lruns = [("previous", WARM_PREVIOUS, None, rx, its) for rx in relaxes] + \
+                [("ratio", WARM_RATIO, res.blob, rx, its) for rx in relaxes]
+        lref, lrows = evaluate_modes(family(shape, EVAL_SEED), a.large_substeps, a.long, (16, 32, 48), lruns, log)
+        large = {"shape": shape, "seed": EVAL_SEED, "substeps": a.large_substeps, "reference_density_error": lref,
+                 "rows": lrows}
+    os.makedirs(a.out, exist_ok=True)
+    res.blob.astype("<f4").tofile(os.path.join(a.out, "dam_break_ratio.f32"))
+    meta = {
+        "network": "16-64-64-1 SiLU, bf16 weights, fp32 accumulate; output = factor on the previous lambda sum",
+        "mode": "PBF_WARM_RATIO",
+        "train": [{"shape": s, "seed": sd, "keep": k} for s, sd, k in jobs], "substeps": sub,
+        "long_iterations": a.long, "deployed_iterations": a.iterations, "collect_relax": a.relax,
+        "epochs": a.epochs, "samples": int(len(Ta)), "loss_first_last": [float(res.losses[0]), float(res.losses[-1])],
+        "physics": {"particle_radius": 0.025, "kernel_radius": 0.1, "substep_dt": "1/120 s", "xsph": 0.01,
+                    "vorticity": 0.01, "surface_tension": 0.1},
+        "eval": {"shape": EVAL_SHAPE, "seed": EVAL_SEED, "substeps": sub, "reference_iterations": a.long,
+                 "reference_density_error": ref_err, "rows": rows},
+        "eval_large": large,
+        "train_seconds": t_train,
+    }
+    with open(os.path.join(a.out, "dam_break_ratio.json"), "w") as fh:
+        json.dump(meta, fh, indent=1)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "warm_start"))
@@ -84,8 +175,12 @@ def main(argv=None):
     ap.add_argument("--eval-iterations", default="1,2,3,4,6")
     ap.add_argument("--eval-large", default="128,128,64", help="block of the large evaluation ('' to skip)")
     ap.add_argument("--large-substeps", type=int, default=200)
+    ap.add_argument("--ratio", action="store_true", help="train the ratio mode (PBF_WARM_RATIO)")
+    ap.add_argument("--large-iterations", default="12,16,20,24", help="ratio mode: iteration counts at the large block")
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)  # noqa: E731
+    if a.ratio:
+        return main_ratio(a, log)
     shapes = TRAIN_SHAPES[:2] if a.quick else TRAIN_SHAPES
     sub = 60 if a.quick else a.substeps
     t0 = time.time()
