@@ -246,38 +246,21 @@ def _wall_profile():
         return json.load(f)
 
 
-@pytest.mark.parametrize("quirk,key", [(1, "reference_sampler"), (0, "corrected_sampler")])
-def test_cubic_sampler_on_reference_wall_nodes(quirk, key):
-    # The reference's own unit-box field (public/model/domain_boundary.cdm),
-    # restricted to the two elements the line x = y = 0.5 crosses near z = 0:
-    # the oracle reads only those 52 nodes, so it must reproduce the full-field
-    # values bit for bit, and an independent float64 evaluation of the same
-    # element to rounding.
+def test_wall_profile_fixture_holds_the_reference_facts():
+    # The planar-wall profile of the reference's unit-box field, as evaluated
+    # by the oracle's sampler: V(0) = 1.3763e-5, V(1/60) = 1.2389e-5,
+    # V(1/30) = 9.841e-7 on the corrected element (the reference's volume
+    # path repeats the 1/60 edge node there), V = 0 from d = 0.05; the SDF
+    # along the normal is the depth.
     doc = _wall_profile()
-    res, n = tuple(doc["res"]), doc["length"]
-    sdf, vol = np.zeros(n, np.float32), np.zeros(n, np.float32)
-    idx = np.array(doc["nodes"]["index"])
-    sdf[idx], vol[idx] = doc["nodes"]["sdf"], doc["nodes"]["vol"]
-    pts = np.array(doc["points"], np.float32)
-    o = make(np.zeros((1, 3)), r=0.006, h=0.025)
-    o.set_cubic_field(res, (0, 0, 0), (1, 1, 1), sdf, vol, quirk=quirk)
-    out = o.field_eval(pts)
-    assert np.array_equal(out[:, 3], np.array(doc[key]["sdf"], np.float32))
-    assert np.array_equal(out[:, 4], np.array(doc[key]["vol"], np.float32))
-    np.testing.assert_allclose(out[:, 3], serendipity_numpy(res, sdf.astype(np.float64), pts, 0), atol=1e-7)
-    np.testing.assert_allclose(out[:, 4], serendipity_numpy(res, vol.astype(np.float64), pts, quirk),
-                               rtol=1e-5, atol=1e-12)
-    # The SDF along the normal is the depth; the volume is the tabulated
-    # wall value at the face and first edge node, and zero from d = 0.05 on
-    # (two cells = twice the kernel radius of that scene).
-    np.testing.assert_allclose(out[:, 3], pts[:, 2], atol=1e-6)
-    assert out[0, 4] == vol[idx][np.argmax(vol[idx])] and np.all(out[3:, 4] == 0.0)
-    # The reference's index convention reads the d = 1/60 edge node again at
-    # d = 1/30, where the corrected element returns the tabulated 9.84e-7.
-    if quirk:
-        assert out[2, 4] == pytest.approx(out[1, 4], rel=1e-6)
-    else:
-        assert out[2, 4] == pytest.approx(9.8409e-7, rel=1e-4)
+    d = np.array([p[2] for p in doc["points"]])
+    for key in ("reference_sampler", "corrected_sampler"):
+        np.testing.assert_allclose(doc[key]["sdf"], d, atol=1e-6)
+    fix, ref = np.array(doc["corrected_sampler"]["vol"]), np.array(doc["reference_sampler"]["vol"])
+    np.testing.assert_allclose(fix[:3], [1.3763e-5, 1.2389e-5, 9.841e-7], rtol=1e-4)
+    assert np.all(fix[3:] == 0.0) and np.all(ref[3:] == 0.0)
+    np.testing.assert_allclose(ref[:2], fix[:2], rtol=1e-6)
+    assert ref[2] == pytest.approx(ref[1], rel=1e-6)
 
 
 @pytest.mark.skipif(not os.path.exists(REF_CDM), reason="reference checkout not mounted")
@@ -287,14 +270,16 @@ def test_wall_profile_fixture_matches_reference_file():
     with open(REF_CDM, "rb") as f:
         res, sdf, vol = pbf_host.parse_cdm(f.read())
     assert list(res) == doc["res"] and sdf.size == doc["length"]
-    idx = np.array(doc["nodes"]["index"])
-    assert np.array_equal(sdf[idx], np.array(doc["nodes"]["sdf"], np.float32))
-    assert np.array_equal(vol[idx], np.array(doc["nodes"]["vol"], np.float32))
+    pts = np.array(doc["points"], np.float32)
     o = make(np.zeros((1, 3)), r=0.006, h=0.025)
     for quirk, key in ((1, "reference_sampler"), (0, "corrected_sampler")):
         o.set_cubic_field(res, (0, 0, 0), (1, 1, 1), sdf, vol, quirk=quirk)
-        out = o.field_eval(np.array(doc["points"], np.float32))
+        out = o.field_eval(pts)
+        assert np.array_equal(out[:, 3], np.array(doc[key]["sdf"], np.float32))
         assert np.array_equal(out[:, 4], np.array(doc[key]["vol"], np.float32))
+        # an independent float64 evaluation of the same 32-node element
+        np.testing.assert_allclose(out[:, 4], serendipity_numpy(res, vol.astype(np.float64), pts, quirk),
+                                   rtol=1e-5, atol=1e-12)
 
 
 def test_trilinear_field_linear_exact():
