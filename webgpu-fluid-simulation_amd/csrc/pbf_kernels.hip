@@ -233,11 +233,16 @@ __device__ __forceinline__ void for_each_neighbor_g(const uint32_t *__restrict__
   if (wave16(P, i)) {
     const uint32_t *row = P.nbr16 + i;
     const uint32_t base = i - 32768u;
+    const uint32_t col = i * 4u, rstride = ld * 4u;
+    auto pairs = [&](uint32_t k) {  // pair rows k/2.. of the 16-bit table as one buffer
+      return __builtin_amdgcn_make_buffer_rsrc(P.nbr16 + (size_t)(k >> 1) * ld, 0, -1, kBufWord3);
+    };
     if constexpr (UNROLL >= 8) {
       for (; k + 8 <= kmax; k += 8) {
+        const auto r = pairs(k);
         uint32_t w[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) w[q] = row[(size_t)((k >> 1) + q) * ld];
+        for (int q = 0; q < 4; ++q) w[q] = __builtin_amdgcn_raw_buffer_load_b32(r, col, q * rstride, 0);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           f(base + (w[q] & 0xffffu), k + 2 * q < cnt);
@@ -246,9 +251,10 @@ __device__ __forceinline__ void for_each_neighbor_g(const uint32_t *__restrict__
       }
     }
     for (; k + 4 <= kmax; k += 4) {
+      const auto r = pairs(k);
       uint32_t w[2];
 #pragma unroll
-      for (int q = 0; q < 2; ++q) w[q] = row[(size_t)((k >> 1) + q) * ld];
+      for (int q = 0; q < 2; ++q) w[q] = __builtin_amdgcn_raw_buffer_load_b32(r, col, q * rstride, 0);
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         f(base + (w[q] & 0xffffu), k + 2 * q < cnt);
