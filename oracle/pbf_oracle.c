@@ -532,8 +532,11 @@ static inline void cell_coord(const orc_sim *s, const float *x, int c[3]) {
 
 static inline uint32_t cell_key(const orc_sim *s, int cx, int cy, int cz) {
   if (s->grid_mode == 1) {
-    uint32_t hsh = ((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u);
-    return hsh & (s->ncells - 1u);
+    /* the row-major cell index modulo the table size (2^k): nearby cells
+       land in nearby buckets, so the bucket-sorted particles keep their
+       spatial locality; cells 2^k apart in that index share a bucket */
+    uint32_t lin = ((uint32_t)cx * (uint32_t)s->dims[1] + (uint32_t)cy) * (uint32_t)s->dims[2] + (uint32_t)cz;
+    return lin & (s->ncells - 1u);
   }
   return ((uint32_t)cx * (uint32_t)s->dims[1] + (uint32_t)cy) * (uint32_t)s->dims[2] + (uint32_t)cz;
 }

@@ -102,6 +102,31 @@ def test_wide_slabs_mix_table_formats():
     _bit_exact(sc, steps=1)
 
 
+@pytest.mark.parametrize("hash_log2", [9, 6])
+def test_hashed_grid_wraps_and_collisions(hash_log2):
+    # 20^3 cells in a 2^9-bucket table: the 27 cells around a cell still hash
+    # to distinct buckets (row-major indices differ by at most 421), but the
+    # index wraps the table inside some z-columns (visited cell by cell) and
+    # far cells share buckets.  2^6 buckets: the 27 cells collide too (the
+    # first-occurrence walk without pruning).
+    rng = np.random.default_rng(hash_log2)
+    pos = np.zeros((6000, 4), np.float32)
+    pos[:, :3] = rng.uniform(0.05, 1.95, (6000, 3))
+    sc = _scene(pos, L=2.0, name=f"hash{hash_log2}", field=False)
+    sc.grid_mode, sc.hash_log2 = 1, hash_log2
+    print(f"[hash wrap] 2^{hash_log2} buckets", flush=True)
+    sim = scenes.make_engine(sc)
+    o = make_oracle(sc)
+    sim.search(sc.substep_dt())
+    o.search(sc.substep_dt())
+    gc, go = sim.read_cells()
+    oc, oo = o.cells()
+    assert np.array_equal(gc, oc) and np.array_equal(go, oo)
+    assert np.array_equal(sim.read_neighbors()[1], o.neighbors()[1])
+    sim.close()
+    _bit_exact(sc, steps=1)
+
+
 def test_crowded_cells():
     rng = np.random.default_rng(3)
     pos = np.zeros((600, 4), np.float32)

@@ -37,6 +37,7 @@ struct Scene {
   uint32_t cell_base;
   uint32_t ncells;
   uint32_t hash_mask;  // 0: dense grid
+  uint32_t distinct27;  // the 27 cells around any cell map to distinct buckets (always, dense)
   uint32_t pbase, pcount;
   int32_t field_kind;  // 0 none, 1 cubic 32-node, 2 trilinear
   int32_t fres[3];
@@ -165,11 +166,11 @@ __device__ __forceinline__ int cell_axis(const Scene &sc, float x, int a) {
   return v;
 }
 __device__ __forceinline__ uint32_t cell_local(const Scene &sc, int cx, int cy, int cz) {
-  if (sc.hash_mask) {
-    uint32_t h = ((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u);
-    return h & sc.hash_mask;
-  }
-  return ((uint32_t)cx * (uint32_t)sc.dims[1] + (uint32_t)cy) * (uint32_t)sc.dims[2] + (uint32_t)cz;
+  // hashed grid: the row-major index modulo the table size (2^k), so nearby
+  // cells land in nearby buckets and the bucket-sorted particles keep their
+  // spatial locality; cells 2^k apart in that index share a bucket
+  const uint32_t lin = ((uint32_t)cx * (uint32_t)sc.dims[1] + (uint32_t)cy) * (uint32_t)sc.dims[2] + (uint32_t)cz;
+  return sc.hash_mask ? (lin & sc.hash_mask) : lin;
 }
 
 // ---------------------------------------------------------------------------

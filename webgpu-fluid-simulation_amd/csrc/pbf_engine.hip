@@ -514,6 +514,9 @@ Scene to_device_scene(const pbf_ctx *c, const SceneHost &s, uint64_t field_off) 
   d.cell_base = s.cell_base;
   d.ncells = s.ncells;
   d.hash_mask = c->params.grid_mode == PBF_GRID_HASH ? s.ncells - 1u : 0u;
+  // row-major indices of the 27 cells around a cell differ by at most
+  // D1 D2 + D2 + 1; below the table size they cannot share a bucket
+  d.distinct27 = !d.hash_mask || (uint64_t)s.dims[1] * (uint64_t)s.dims[2] + (uint64_t)s.dims[2] + 1u < s.ncells;
   d.pbase = s.pbase;
   d.pcount = s.count;
   d.field_kind = s.field_kind;

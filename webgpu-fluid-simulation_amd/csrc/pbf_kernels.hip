@@ -46,7 +46,8 @@ __device__ __forceinline__ uint32_t slot_index() {
 // of their particles is stored instead as 16-bit offsets j - i + 32768,
 // entries 2p and 2p + 1 packed in word nbr16[p * ld + i] (wfmt[wave] = 1):
 // half the bytes for every walk.  Dense grids list neighbours in ascending
-// slot order, so the first and last entries bound the offsets.
+// slot order, so the first and last entries bound the offsets; hashed grids
+// take the range over every entry.
 __device__ __forceinline__ bool wave16(const Phys &P, uint32_t i) {
   return __builtin_amdgcn_readfirstlane(P.wfmt[i >> 6]) != 0u;
 }
@@ -552,7 +553,7 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
     reinterpret_cast<uint16_t *>(P.nbr16)[((size_t)(k >> 1) * P.ld + i) * 2 + (k & 1u)] =
         (uint16_t)(j - (i - 32768u));
   };
-  uint32_t cnt = 0, last = i;
+  uint32_t cnt = 0, last = i, smin = i, smax = i;
   bool fits16 = true;  // every entry within +-32767 slots of i
   if (i < n) {
     const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
@@ -583,15 +584,17 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
       cnt += d2 <= P.h2 ? 1u : 0u;
     };
     // spilled entries go to both tables (the wavefront's format is decided
-    // at the end); `last` keeps the latest one
+    // at the end); `last` keeps the latest one, smin / smax their range
     auto spill = [&](uint32_t j0, uint32_t m, uint32_t k) {  // k: count before the batch
       for (uint32_t q = 0; q < m; ++q) {
         const float4 xj = X0[j0 + q];
         const bool in = dist2(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z) <= P.h2;
         if (in && k >= (uint32_t)LIST_R && k < P.maxn) {
           nbr[(size_t)k * P.ld + i] = j0 + q;
-          if constexpr (!HASH) store16(k, j0 + q);
+          store16(k, j0 + q);
           last = j0 + q;
+          smin = smin < j0 + q ? smin : j0 + q;
+          smax = smax > j0 + q ? smax : j0 + q;
         }
         k += in ? 1u : 0u;
       }
@@ -619,27 +622,51 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
         if (__ballot(cnt > (uint32_t)LIST_R)) spill(j, 1, k0);
       }
     };
-    if constexpr (!HASH) {
-      // the 18 range bounds are loaded up front so their latencies overlap
+    if (!HASH || sc.distinct27) {
+      // Each of the 9 columns (x, y) is visited as one slot range over its
+      // cells z-1..z+1 -- contiguous cells, and in a hashed grid contiguous
+      // buckets unless the row-major index wraps the table (then cell by
+      // cell).  With every bucket of the 27 cells distinct (always on a dense
+      // grid), skipping out-of-reach cells leaves the list unchanged.  The 18
+      // range bounds are loaded up front so their latencies overlap.
       const float gz_lo = gap(-1, fz), gz_hi = gap(1, fz);
       uint32_t t0[9], t1[9];
+      int zl[9], zh[9];
+      uint32_t wrap = 0;
 #pragma unroll
       for (int r = 0; r < 9; ++r) {
         const int dx = r / 3 - 1, dy = r % 3 - 1;
         const int nx = cx + dx, ny = cy + dy;
         const float gxy = gap(dx, fx) + gap(dy, fy);
         t0[r] = t1[r] = 0;
+        zl[r] = zh[r] = 0;
         if (nx < 0 || nx >= D0 || ny < 0 || ny >= D1 || gxy > kReach) continue;
         const int zlo = (cz > 0 && gxy + gz_lo <= kReach) ? cz - 1 : cz;
         const int zhi = (cz + 1 < D2 && gxy + gz_hi <= kReach) ? cz + 1 : cz;
-        t0[r] = offsets[cbase + cell_local(sc, nx, ny, zlo)];
-        t1[r] = offsets[cbase + cell_local(sc, nx, ny, zhi) + 1];
+        const uint32_t blo = cell_local(sc, nx, ny, zlo), bhi = cell_local(sc, nx, ny, zhi);
+        if constexpr (HASH) {
+          if (bhi - blo != (uint32_t)(zhi - zlo)) wrap |= 1u << r;
+        }
+        zl[r] = zlo;
+        zh[r] = zhi;
+        t0[r] = offsets[cbase + blo];
+        t1[r] = offsets[cbase + bhi + 1];
       }
 #pragma unroll
-      for (int r = 0; r < 9; ++r) visit(t0[r], t1[r]);
+      for (int r = 0; r < 9; ++r) {
+        if (HASH && (wrap >> r & 1u)) {
+          const int nx = cx + r / 3 - 1, ny = cy + r % 3 - 1;
+          for (int z = zl[r]; z <= zh[r]; ++z) {
+            const uint32_t k = cbase + cell_local(sc, nx, ny, z);
+            visit(offsets[k], offsets[k + 1]);
+          }
+        } else {
+          visit(t0[r], t1[r]);
+        }
+      }
     } else {
-      // hashed buckets are visited at their first occurrence among the 27
-      // cells; skipping a cell could move that occurrence, so no pruning here
+      // buckets may repeat among the 27 cells: each is visited at its first
+      // occurrence; skipping a cell could move that occurrence, so no pruning
       uint32_t seen[27];
       int nseen = 0;
       for (int dx = -1; dx <= 1; ++dx)
@@ -657,11 +684,26 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
     }
     ncount[i] = cnt < P.maxn ? cnt : P.maxn;
     const uint32_t c = cnt < P.maxn ? cnt : P.maxn;
-    if (c > 0 && c - 1 < (uint32_t)LIST_R) last = tile[(c - 1) * LIST_BS + tid];
-    const uint32_t first = c > 0 ? tile[tid] : i;
-    fits16 = !HASH && first - i + 32768u < 65536u && last - i + 32768u < 65536u;
+    uint32_t lo, hi;
+    if constexpr (HASH) {
+      // bucket order is not slot order: the range over every entry
+      lo = smin;
+      hi = smax;
+      const uint32_t rows = c < (uint32_t)LIST_R ? c : (uint32_t)LIST_R;
+      for (uint32_t k = 0; k < rows; ++k) {
+        const uint32_t v = tile[k * LIST_BS + tid];
+        lo = lo < v ? lo : v;
+        hi = hi > v ? hi : v;
+      }
+    } else {
+      // ascending slot order: the first and last entries bound the rest
+      if (c > 0 && c - 1 < (uint32_t)LIST_R) last = tile[(c - 1) * LIST_BS + tid];
+      lo = c > 0 ? tile[tid] : i;
+      hi = last;
+    }
+    fits16 = i - lo <= 32768u && hi - i <= 32767u;
   }
-  const bool w16 = !HASH && !__ballot(!fits16);  // lanes past n count as fitting
+  const bool w16 = !__ballot(!fits16);  // lanes past n count as fitting
   if (__lane_id() == 0) P.wfmt[i >> 6] = w16 ? 1u : 0u;
   const uint32_t wm = wave_max_u32(cnt);  // lanes past n hold cnt = 0
   if (__lane_id() == 0 && wm > 0) {
