@@ -310,7 +310,7 @@ int pbf_collect_training(pbf_ctx *ctx, float dt, uint32_t long_iterations, uint3
  * same under any setting. */
 typedef struct {
   uint32_t walk_block[8];  /* lambda, dp, density, vorticity, xsph, diag, -, -: 64, 128, 192 or 256 */
-  uint32_t list_variant;   /* 0..3: (LDS rows, work-group) = (32,128), (16,128), (32,64), (48,64) */
+  uint32_t list_variant;   /* 0..3: (LDS rows, work-group) = (32,128), (16,128), (32,64), (56,64) */
   uint32_t reserved[7];
 } pbf_tuning;
 int pbf_get_tuning(const pbf_ctx *ctx, pbf_tuning *out);

@@ -1187,7 +1187,7 @@ void launch_build_list(const LaunchCtx &L, const float4 *X0, const uint32_t *off
   switch (L.lvar) {
     case 1: build_list_rb<16, 128>(L, X0, offsets, nbr, ncount, flags); break;
     case 2: build_list_rb<32, 64>(L, X0, offsets, nbr, ncount, flags); break;
-    case 3: build_list_rb<48, 64>(L, X0, offsets, nbr, ncount, flags); break;
+    case 3: build_list_rb<56, 64>(L, X0, offsets, nbr, ncount, flags); break;
     default: build_list_rb<32, 128>(L, X0, offsets, nbr, ncount, flags); break;
   }
 }
