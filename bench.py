@@ -84,6 +84,23 @@ def profiled_traffic(pass_name):
     except OSError:
         return None
     return None
+COUNTERS = os.path.join(ROOT, "profiles", "dam_break_1m_pass_counters.md")
+SIMDS, CLOCK_HZ = 1024, 2.4e9  # MI355X: 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock
+
+
+def profiled_valu(pass_name):
+    """VALU wave-instructions per launch of a pass from the committed
+    rocprofv3 SQ_INSTS_VALU summary, or None."""
+    try:
+        for line in open(COUNTERS):
+            if line.startswith("valu_instructions_per_launch:"):
+                kv = dict(t.split("=") for t in line.split(":", 1)[1].split())
+                return int(kv[pass_name]) if pass_name in kv else None
+    except OSError:
+        return None
+    return None
+
+
 TARGET_ERR = 0.005
 
 
@@ -280,6 +297,13 @@ def main():
                 "traffic_GBs": round(traffic / avg_s / 1e9, 1) if traffic else None,
                 "traffic_source": "profiles/dam_break_1m_kernels_and_traffic.md (rocprofv3 FETCH_SIZE x2, "
                                   "WRITE_SIZE x1 per dispatch)" if traffic else None}
+        valu = profiled_valu(dom)
+        if valu:
+            # the kernel's own ceiling: a wave64 VALU instruction occupies a
+            # 16-lane SIMD for 4 cycles
+            roof["valu"] = {"wave_instructions_per_launch": valu,
+                            "issue_frac": round(valu * 4 / (SIMDS * CLOCK_HZ * avg_s), 3),
+                            "source": "profiles/dam_break_1m_pass_counters.md (rocprofv3 SQ_INSTS_VALU)"}
         for name in per_pass:
             if name in pb:
                 a = pb[name] * N / (per_pass[name]["us_avg"] * 1e-6) / 1e9
