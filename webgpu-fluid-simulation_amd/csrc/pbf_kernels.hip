@@ -771,18 +771,25 @@ __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__
     b = B[i];
   }
   float gx = 0.0f, gy = 0.0f, gz = 0.0f, s2 = 0.0f, rho = 0.0f;
-  // padding entries (j = i): zero gradient, poly6 taken with weight 0
-  // (fma(w, 1, rho) is rho + w exactly)
+  // poly6 and the spiky gradient with their range tests shared (the values
+  // of poly6() and spiky_f()); padding entries (j = i) have a zero gradient
+  // and their poly6 term is not added
   const Gather GX(X, n);
   for_each_neighbor_w<4>(nbr, i, P, ncount[i], [&](uint32_t j) { return GX.xyz(j); },
                          [&](const float3 &xj, bool valid) {
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float d2 = dist2(dx, dy, dz);
-    float f = spiky_f(P, d2);
+    const bool in = d2 <= P.h2;
+    const float r = sqrt_n(d2);
+    const float q = P.h - r;
+    const float fs = div_n(P.Ks * (q * q), r);
+    const float f = (in && d2 > 0.0f) ? fs : 0.0f;
+    const float t = P.h2 - d2;
+    const float w = P.Kp * ((t * t) * t);
     float ax = f * dx, ay = f * dy, az = f * dz;
     gx += ax; gy += ay; gz += az;
     s2 = fmaf(az, az, fmaf(ay, ay, fmaf(ax, ax, s2)));
-    rho = fmaf(poly6(P, d2), valid ? 1.0f : 0.0f, rho);
+    rho += (in && valid) ? w : 0.0f;
   });
   rho *= P.V;
   gx *= P.V; gy *= P.V; gz *= P.V;
