@@ -85,20 +85,23 @@ def profiled_traffic(pass_name):
         return None
     return None
 COUNTERS = os.path.join(ROOT, "profiles", "dam_break_1m_pass_counters.md")
-SIMDS, CLOCK_HZ = 1024, 2.4e9  # MI355X: 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock
+SIMDS = 1024  # MI355X: 256 CUs x 4 SIMDs
 
 
 def profiled_valu(pass_name):
-    """VALU wave-instructions per launch of a pass from the committed
-    rocprofv3 SQ_INSTS_VALU summary, or None."""
+    """(VALU wave-instructions per launch of a pass, engine clock in Hz) from
+    the committed rocprofv3 SQ_INSTS_VALU / GRBM_GUI_ACTIVE summary, or None."""
+    n = clk = None
     try:
         for line in open(COUNTERS):
             if line.startswith("valu_instructions_per_launch:"):
                 kv = dict(t.split("=") for t in line.split(":", 1)[1].split())
-                return int(kv[pass_name]) if pass_name in kv else None
+                n = int(kv[pass_name]) if pass_name in kv else None
+            elif line.startswith("engine_clock_hz:"):
+                clk = float(line.split(":", 1)[1])
     except OSError:
         return None
-    return None
+    return (n, clk) if n and clk else None
 
 
 TARGET_ERR = 0.005
@@ -300,10 +303,11 @@ def main():
         valu = profiled_valu(dom)
         if valu:
             # the kernel's own ceiling: a wave64 VALU instruction occupies a
-            # 16-lane SIMD for 4 cycles
-            roof["valu"] = {"wave_instructions_per_launch": valu,
-                            "issue_frac": round(valu * 4 / (SIMDS * CLOCK_HZ * avg_s), 3),
-                            "source": "profiles/dam_break_1m_pass_counters.md (rocprofv3 SQ_INSTS_VALU)"}
+            # 16-lane SIMD for 4 cycles, at the clock the profile measured
+            roof["valu"] = {"wave_instructions_per_launch": valu[0], "engine_clock_hz": valu[1],
+                            "issue_frac": round(valu[0] * 4 / (SIMDS * valu[1] * avg_s), 3),
+                            "source": "profiles/dam_break_1m_pass_counters.md (rocprofv3 SQ_INSTS_VALU, "
+                                      "GRBM_GUI_ACTIVE)"}
         for name in per_pass:
             if name in pb:
                 a = pb[name] * N / (per_pass[name]["us_avg"] * 1e-6) / 1e9
