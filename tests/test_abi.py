@@ -166,3 +166,31 @@ def test_cubic_field_generator_and_cdm_writer():
     assert res2 == res and np.array_equal(s2, sdf) and np.array_equal(v2, vol)
     with pytest.raises(pbf_host.PBFError):
         pbf_host.write_cdm(res, sdf[:-1], vol[:-1])
+
+
+def test_host_entry_points_reject_non_finite_and_oversized_input():
+    # GPU-free entry points validate what the reference's JS loops would turn
+    # into endless or undefined loops: NaN / inf extents, spacings whose
+    # lattice exceeds any 32-bit count, and non-finite map domains.
+    import ctypes as C
+    lib = pbf_host.load_library()
+    d3 = lambda v: (C.c_double * 3)(*v)  # noqa: E731
+    cnt = C.c_uint32()
+    nan, inf = float("nan"), float("inf")
+    cube = lambda r, mn, mx: lib.pbf_voxelize_cube_host(r, d3(mn), d3(mx), 0.0, 1, 0, None, 0, C.byref(cnt))  # noqa
+    assert cube(0.01, (0, 0, 0), (nan, 1, 1)) == pbf_host.PBF_E_INVALID
+    assert cube(0.01, (0, 0, 0), (inf, 1, 1)) == pbf_host.PBF_E_INVALID
+    assert cube(inf, (0, 0, 0), (1, 1, 1)) == pbf_host.PBF_E_INVALID
+    assert cube(1e-9, (0, 0, 0), (1, 1, 1)) == pbf_host.PBF_E_CAPACITY
+    assert cube(0.01, (0, 0, 0), (-1, 1, 1)) == pbf_host.PBF_OK and cnt.value == 0
+    sph = lambda r, c, R: lib.pbf_voxelize_sphere_host(r, d3(c), R, 0.0, 1, 0, None, 0, C.byref(cnt))  # noqa
+    assert sph(0.01, (0, 0, 0), inf) == pbf_host.PBF_E_INVALID
+    assert sph(0.01, (nan, 0, 0), 0.1) == pbf_host.PBF_E_INVALID
+    assert sph(1e-6, (0, 0, 0), 1.0) == pbf_host.PBF_E_CAPACITY
+    assert sph(0.01, (0, 0, 0), -0.5) == pbf_host.PBF_OK and cnt.value == 0
+    with pytest.raises(pbf_host.PBFError):
+        pbf_host.build_volume_map(0, (4, 4, 4), (0, 0, 0), (nan, 1, 1), 0.1, None, 0.025, where="host")
+    with pytest.raises(pbf_host.PBFError):
+        pbf_host.build_volume_map(0, (4, 4, 4), (0, 0, 0), (1, 1, 1), inf, None, 0.025, where="host")
+    with pytest.raises(pbf_host.PBFError):
+        pbf_host.build_volume_map(1, (4, 4, 4), (0, 0, 0), (1, 1, 1), 0.1, (0.5,) * 8 + (nan,), 0.025, where="host")

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -805,14 +806,28 @@ int pbf_set_config(pbf_ctx *c, const pbf_config *cfg) {
   return PBF_OK;
 }
 
+// Room for `cnt` more staged particles: more than the largest capacity can
+// never be uploaded ('To Many Particles!' at upload), so it fails here before
+// allocating host memory for them.
+static int stage_room(pbf_ctx *c, uint32_t cnt, size_t *at) {
+  *at = c->staged.size();
+  if (*at / 4 + (size_t)cnt > kMaxCapacity) return fail(c, PBF_E_CAPACITY, "To Many Particles!");
+  try {
+    c->staged.resize(*at + 4 * (size_t)cnt);
+  } catch (const std::bad_alloc &) {
+    return fail(c, PBF_E_CAPACITY, "out of host memory for staged particles");
+  }
+  return PBF_OK;
+}
+
 int pbf_voxelize_cube(pbf_ctx *c, const double mn[3], const double mx[3], double jitter, uint64_t seed,
                       uint32_t *added) {
   if (!c || !mn || !mx) return PBF_E_INVALID;
   uint32_t cnt = 0;
   int rc = pbf_voxelize_cube_host(c->params.particle_radius, mn, mx, jitter, seed, 0, nullptr, 0, &cnt);
   if (rc != PBF_OK) return fail(c, rc, "voxelizeCube: invalid arguments");
-  size_t at = c->staged.size();
-  c->staged.resize(at + 4 * (size_t)cnt);
+  size_t at = 0;
+  if ((rc = stage_room(c, cnt, &at)) != PBF_OK) return rc;
   rc = pbf_voxelize_cube_host(c->params.particle_radius, mn, mx, jitter, seed, c->staged_serial, c->staged.data() + at,
                               cnt, &cnt);
   if (rc != PBF_OK) return fail(c, rc, "voxelizeCube failed");
@@ -827,8 +842,8 @@ int pbf_voxelize_sphere(pbf_ctx *c, const double center[3], double radius, doubl
   uint32_t cnt = 0;
   int rc = pbf_voxelize_sphere_host(c->params.particle_radius, center, radius, jitter, seed, 0, nullptr, 0, &cnt);
   if (rc != PBF_OK) return fail(c, rc, "voxelizeSphere: invalid arguments");
-  size_t at = c->staged.size();
-  c->staged.resize(at + 4 * (size_t)cnt);
+  size_t at = 0;
+  if ((rc = stage_room(c, cnt, &at)) != PBF_OK) return rc;
   rc = pbf_voxelize_sphere_host(c->params.particle_radius, center, radius, jitter, seed, c->staged_serial,
                                 c->staged.data() + at, cnt, &cnt);
   if (rc != PBF_OK) return fail(c, rc, "voxelizeSphere failed");
@@ -844,8 +859,8 @@ int pbf_voxelize_mesh(pbf_ctx *c, const float *vertices, uint32_t nverts, const 
   int rc = pbf_voxelize_mesh_host(c->params.particle_radius, vertices, nverts, indices, ntris, world, jitter, seed, 0,
                                   nullptr, 0, &cnt);
   if (rc != PBF_OK) return fail(c, rc, "voxelizeMesh: invalid mesh or matrix");
-  size_t at = c->staged.size();
-  c->staged.resize(at + 4 * (size_t)cnt);
+  size_t at = 0;
+  if ((rc = stage_room(c, cnt, &at)) != PBF_OK) return rc;
   rc = pbf_voxelize_mesh_host(c->params.particle_radius, vertices, nverts, indices, ntris, world, jitter, seed,
                               c->staged_serial, c->staged.data() + at, cnt, &cnt);
   if (rc != PBF_OK) return fail(c, rc, "voxelizeMesh failed");

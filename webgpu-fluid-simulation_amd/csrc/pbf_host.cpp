@@ -147,7 +147,13 @@ struct VolumeEval {
 
   int init(uint32_t kind, const double *shape, const double lo[3], const double hi[3], double kernel_radius,
            double particle_radius) {
-    if (!lo || !hi || kind > 1 || !(kernel_radius > 0.0)) return PBF_E_INVALID;
+    if (!lo || !hi || kind > 1 || !(kernel_radius > 0.0) || !std::isfinite(kernel_radius)) return PBF_E_INVALID;
+    if (!(particle_radius >= 0.0) || !std::isfinite(particle_radius)) return PBF_E_INVALID;
+    for (int a = 0; a < 3; ++a)
+      if (!std::isfinite(lo[a]) || !std::isfinite(hi[a]) || !(hi[a] > lo[a])) return PBF_E_INVALID;
+    if (shape)
+      for (int a = 0; a < (kind == 1 ? 9 : 6); ++a)
+        if (!std::isfinite(shape[a])) return PBF_E_INVALID;
     std::memset(&S, 0, sizeof(S));
     S.kind = kind;
     for (int a = 0; a < 3; ++a) { S.box_lo[a] = lo[a]; S.box_hi[a] = hi[a]; }
@@ -318,18 +324,39 @@ extern "C" int pbf_write_cdm(const int32_t res[3], const float *sdf, const float
   return PBF_OK;
 }
 
+namespace {
+// Lattice points along one axis of a box of this extent (floor(extent /
+// spacing), 0 for an empty box), or -1 when the extent or the spacing is not
+// finite or the count is out of any 32-bit range.
+long lattice_count(double extent, double diam) {
+  if (!std::isfinite(extent) || !std::isfinite(diam) || !(diam > 0.0)) return -1;
+  const double n = std::floor(extent / diam);
+  if (!(n > 0.0)) return 0;
+  if (n > 2147483647.0) return -1;
+  return (long)n;
+}
+
+bool all_finite(const double *v, int n) {
+  for (int a = 0; a < n; ++a)
+    if (!std::isfinite(v[a])) return false;
+  return true;
+}
+}  // namespace
+
 // SPH.voxelizeCube (SPH.ts:99-121): lattice of spacing 2r from `mn`, i (x)
 // outermost, k (z) innermost, f64 arithmetic rounded once to f32; plus the
 // seeded jitter of particle serial0 + q.  out == NULL: count only.
 extern "C" int pbf_voxelize_cube_host(double r, const double mn[3], const double mx[3], double jitter, uint64_t seed,
                                       uint64_t serial0, float *out, uint64_t capacity, uint32_t *count) {
   using namespace pbf;
-  if (!mn || !mx || !(r > 0.0)) return PBF_E_INVALID;
+  if (!mn || !mx || !(r > 0.0) || !all_finite(mn, 3) || !all_finite(mx, 3)) return PBF_E_INVALID;
   const double diam = 2.0 * r;
-  const long nx = (long)std::floor((mx[0] - mn[0]) / diam);
-  const long ny = (long)std::floor((mx[1] - mn[1]) / diam);
-  const long nz = (long)std::floor((mx[2] - mn[2]) / diam);
-  uint64_t total = (nx > 0 && ny > 0 && nz > 0) ? (uint64_t)nx * ny * nz : 0;
+  const long nx = lattice_count(mx[0] - mn[0], diam);
+  const long ny = lattice_count(mx[1] - mn[1], diam);
+  const long nz = lattice_count(mx[2] - mn[2], diam);
+  if (nx < 0 || ny < 0 || nz < 0) return PBF_E_INVALID;
+  if ((double)nx * (double)ny * (double)nz > 4294967295.0) return PBF_E_CAPACITY;
+  const uint64_t total = (uint64_t)nx * (uint64_t)ny * (uint64_t)nz;
   if (count) *count = (uint32_t)total;
   if (!out) return PBF_OK;
   if (total > capacity) return PBF_E_CAPACITY;
@@ -349,9 +376,16 @@ extern "C" int pbf_voxelize_cube_host(double r, const double mn[3], const double
 extern "C" int pbf_voxelize_sphere_host(double r, const double center[3], double radius, double jitter, uint64_t seed,
                                         uint64_t serial0, float *out, uint64_t capacity, uint32_t *count) {
   using namespace pbf;
-  if (!center || !(r > 0.0)) return PBF_E_INVALID;
+  if (!center || !(r > 0.0) || !all_finite(center, 3) || !std::isfinite(radius)) return PBF_E_INVALID;
   const double diam = 2.0 * r;
-  const long n = (long)std::floor(radius / r / 2.0);
+  if (radius < 0.0) {  // the reference's loops from -n to n run empty
+    if (count) *count = 0;
+    return PBF_OK;
+  }
+  const double nd = std::floor(radius / r / 2.0);  // the reference's expression, rounding included
+  if (!std::isfinite(nd)) return PBF_E_INVALID;
+  if (2.0 * nd + 1.0 > 1625.0) return PBF_E_CAPACITY;  // (2n+1)^3 candidates < 2^32
+  const long n = (long)nd;
   const double r2 = radius * radius;
   uint64_t q = 0;
   for (long i = -n; i <= n; ++i)
@@ -507,11 +541,13 @@ extern "C" int pbf_voxelize_mesh_host(double r, const float *vertices, uint32_t 
       hi[a] = std::fmax(hi[a], w[a]);
     }
   }
-  R.build();
   const double diam = 2.0 * r;
-  const long nx = (long)std::floor((hi[0] - lo[0]) / diam);
-  const long ny = (long)std::floor((hi[1] - lo[1]) / diam);
-  const long nz = (long)std::floor((hi[2] - lo[2]) / diam);
+  const long nx = lattice_count(hi[0] - lo[0], diam);
+  const long ny = lattice_count(hi[1] - lo[1], diam);
+  const long nz = lattice_count(hi[2] - lo[2], diam);
+  if (nx < 0 || ny < 0 || nz < 0) return PBF_E_INVALID;  // non-finite vertices or matrix
+  if ((double)nx * (double)ny * (double)nz > 4294967295.0) return PBF_E_CAPACITY;
+  R.build();
   uint64_t q = 0;
   for (long i = 0; i < nx; ++i)
     for (long j = 0; j < ny; ++j)

@@ -99,6 +99,13 @@ extern "C" int pbf_build_volume_map_gpu(uint32_t kind, const double *shape, cons
                                         float *sdf_vol) {
   if (!res || !lo || !hi || !sdf_vol || kind > 1 || !(kernel_radius > 0.0)) return PBF_E_INVALID;
   if (res[0] < 1 || res[1] < 1 || res[2] < 1) return PBF_E_INVALID;
+  if (!std::isfinite(kernel_radius) || !(particle_radius >= 0.0) || !std::isfinite(particle_radius))
+    return PBF_E_INVALID;
+  for (int a = 0; a < 3; ++a)
+    if (!std::isfinite(lo[a]) || !std::isfinite(hi[a]) || !(hi[a] > lo[a])) return PBF_E_INVALID;
+  if (shape)
+    for (int a = 0; a < (kind == 1 ? 9 : 6); ++a)
+      if (!std::isfinite(shape[a])) return PBF_E_INVALID;
   MapArgs A = {};
   A.S.kind = kind;
   for (int a = 0; a < 3; ++a) {
