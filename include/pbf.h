@@ -71,7 +71,7 @@ enum {
 /* Simulation options.  Reference: PBFConfig.setConfig (PBFConfig.ts:31-46)
  * writing the SimulationOptions uniform (src/common/shader.ts:50-57). */
 typedef struct {
-  uint32_t iteration;       /* Jacobi iterations per substep */
+  uint32_t iteration;       /* Jacobi iterations per substep, <= 4096 */
   float xsph;               /* XSPH coefficient */
   float vorticity;          /* vorticity confinement epsilon */
   float surface_tension;    /* Akinci surface-tension gamma */
@@ -180,7 +180,8 @@ int pbf_build_volume_map_gpu(uint32_t kind, const double *shape, const int32_t r
 /* Reference: PBF.run (PBF.ts:398-448) encodes one substep with the
  * compile-time dt and the configured iteration count.  pbf_step runs
  * `substeps` substeps of dt/substeps with `iterations` Jacobi iterations each
- * (0 = the configured count), asynchronously on the context stream. */
+ * (0 = the configured count, at most 4096), asynchronously on the context
+ * stream. */
 int pbf_step(pbf_ctx *ctx, float dt, uint32_t substeps, uint32_t iterations);
 /* Exactly PBF.run: one substep of params.time_step, configured iterations. */
 int pbf_run(pbf_ctx *ctx);

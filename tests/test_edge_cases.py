@@ -170,3 +170,19 @@ def test_capacity_limit():
     with pytest.raises(PBFError, match="To Many Particles!"):
         sim.setParticlePosition(np.zeros((1001, 4), np.float32))
     sim.close()
+
+
+def test_argument_limits():
+    sim = PBF(particle_radius=0.025, kernel_radius=0.1, capacity=256)
+    sim.setParticlePosition(np.zeros((4, 4), np.float32) + 0.5)
+    with pytest.raises(PBFError, match="iterations must be <= 4096"):
+        sim.step(1.0 / 60.0, 1, 5000)
+    with pytest.raises(PBFError, match="dt must be > 0"):
+        sim.step(0.0, 1, 2)
+    with pytest.raises(PBFError, match="iteration must be <= 4096"):
+        sim.setConfig({"iteration": 5000, "XSPH": 0.0, "vorticity": 0.0, "surfaceTension": 0.0, "gravity": 9.8})
+    sim.step(1.0 / 60.0, 1, 2)      # the context is still usable
+    sim.sync()
+    sim.close()
+    with pytest.raises(PBFError):
+        PBF(particle_radius=0.025, kernel_radius=0.1, capacity=1 << 28)

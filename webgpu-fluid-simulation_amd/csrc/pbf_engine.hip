@@ -560,6 +560,7 @@ int upload_scenes(pbf_ctx *c) {
 }
 
 constexpr uint32_t kMaxCapacity = (1u << 27) - 256u;
+constexpr uint32_t kMaxIterations = 4096;  // Jacobi iterations per substep (one graph node pair each)
 
 int create_ctx(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc *descs, bool batch, pbf_ctx **out) {
   pbf_ctx *c = nullptr;
@@ -801,6 +802,7 @@ int pbf_reset(pbf_ctx *c) {
 
 int pbf_set_config(pbf_ctx *c, const pbf_config *cfg) {
   if (!c || !cfg) return PBF_E_INVALID;
+  if (cfg->iteration > kMaxIterations) return fail(c, PBF_E_INVALID, "iteration must be <= 4096");
   c->cfg = *cfg;
   clear_graphs(c);
   return PBF_OK;
@@ -943,6 +945,7 @@ int pbf_clear_boundary(pbf_ctx *c) {
 int pbf_step(pbf_ctx *c, float dt, uint32_t substeps, uint32_t iterations) {
   if (!c) return PBF_E_INVALID;
   if (!(dt > 0.0f) || substeps == 0) return fail(c, PBF_E_INVALID, "dt must be > 0 and substeps >= 1");
+  if (iterations > kMaxIterations) return fail(c, PBF_E_INVALID, "iterations must be <= 4096");
   HIPCHK(hipSetDevice(c->device));
   if (c->n == 0) return PBF_OK;
   const uint32_t iters = iterations ? iterations : c->cfg.iteration;
@@ -1015,6 +1018,8 @@ int pbf_set_tuning(pbf_ctx *c, const pbf_tuning *t) {
 int pbf_autotune(pbf_ctx *c, float dt, uint32_t iterations, uint32_t repeats, pbf_tuning *out) {
   if (!c) return PBF_E_INVALID;
   if (!(dt > 0.0f) || repeats == 0) return fail(c, PBF_E_INVALID, "dt must be > 0 and repeats >= 1");
+  if (iterations > kMaxIterations || repeats > 1000)
+    return fail(c, PBF_E_INVALID, "iterations must be <= 4096 and repeats <= 1000");
   if (c->n == 0) return fail(c, PBF_E_STATE, "no particles to tune on");
   HIPCHK(hipSetDevice(c->device));
   int rc = sync_ctx(c);
@@ -1084,6 +1089,8 @@ int pbf_collect_training(pbf_ctx *c, float dt, uint32_t long_iterations, uint32_
                          float *targets, uint32_t capacity, uint32_t *count) {
   if (!c) return PBF_E_INVALID;
   if (!(dt > 0.0f) || long_iterations == 0) return fail(c, PBF_E_INVALID, "dt must be > 0 and long_iterations >= 1");
+  if (long_iterations > kMaxIterations || iterations > kMaxIterations)
+    return fail(c, PBF_E_INVALID, "iterations must be <= 4096");
   HIPCHK(hipSetDevice(c->device));
   if (count) *count = c->n;
   if (c->n == 0) return PBF_OK;
