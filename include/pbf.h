@@ -198,7 +198,9 @@ enum {
   PBF_FIELD_OMEGA = 4, PBF_FIELD_NORMAL = 5, PBF_FIELD_BOUNDARY = 6, PBF_FIELD_PREDICTED = 7
 };
 /* Any per-particle float4 state in original order (debug readback of
- * PBF.debug, PBF.ts:514-573). LAMBDA is in .x. */
+ * PBF.debug, PBF.ts:514-573). LAMBDA is in .x; NORMAL and VELOCITY have
+ * w = 0; PREDICTED is defined after pbf_search only (a step reuses its
+ * buffer). */
 int pbf_read_field(pbf_ctx *ctx, uint32_t field, float *out4, uint32_t count);
 
 /* Neighbour search results of the last substep (NeighborSearch.debug,

@@ -762,7 +762,7 @@ void orc_substep(orc_sim *s, float dt, int iterations) {
       float K = s->two_rho0 / (rhoi + rhoj);
       float sw = k_poly6(s, d2) * irj;
       ux = fmaf(vx, sw, ux); uy = fmaf(vy, sw, uy); uz = fmaf(vz, sw, uz);
-      float se = (wj * k_spiky_f(s, d2)) * irj;
+      float se = k_spiky_f(s, d2) * (wj * irj); /* |omega_j| / rho_j formed per particle */
       ex = fmaf(se, dx, ex); ey = fmaf(se, dy, ey); ez = fmaf(se, dz, ez);
       float mc = s->m * k_cohesion_f(s, d2);
       tx = fmaf(-K, fmaf(mc, dx, ni[0] - nj[0]), tx);

@@ -392,6 +392,72 @@ def test_free_fall_recursion():
     assert o.get(oracle.VELOCITY)[0, 1] == pytest.approx(v, rel=1e-4)
 
 
+def test_vorticity_normal_xsph_tension_f64():
+    # K13 (vorticityConfinement.ts:46-62) and K14 (XSPH.ts:56-88) evaluated in
+    # float64 from the published kernels (spiky gradient, poly6, Akinci
+    # cohesion) on the oracle's own state and neighbour list after two
+    # substeps of a random blob: a swapped cross product, a lost 1/rho_j, a
+    # wrong Kij or a missing term moves a result by far more than the f32
+    # rounding allowed here.
+    rng = np.random.default_rng(11)
+    r, h = 0.025, 0.1
+    x = rng.uniform(0.35, 0.65, (400, 3))
+    o = make(x, r=r, h=h)
+    xsph, vort, tens = 0.05, 0.2, 0.5
+    o.set_config(xsph, vort, tens, 9.8)
+    dt = float(np.float32(1 / 120))
+    o.substep(dt, 3)
+    o.substep(dt, 3)
+    p = o.get(oracle.POSITION).astype(np.float64)
+    v = o.get(oracle.VELOCITY)[:, :3].astype(np.float64)
+    om, nrm, acc = o.get(oracle.OMEGA), o.get(oracle.NORMAL), o.get(oracle.ACCEL)
+    off, ids = o.neighbors()
+    m = 0.9 * (2 * r) ** 3 * 1000.0
+    ks, kp, kc = -45 / (math.pi * h ** 6), 315 / (64 * math.pi * h ** 9), 32 / (math.pi * h ** 9)
+    x3, rho = p[:, :3], p[:, 3]
+
+    def pairs(i):
+        j = ids[off[i]:off[i + 1]].astype(np.int64)
+        d = x3[i] - x3[j]
+        q = np.sqrt((d * d).sum(1))
+        inr, pos = q <= h, q > 0
+        unit = np.where(pos[:, None], d / np.where(pos, q, 1)[:, None], 0)
+        grad = np.where(inr, ks * (h - q) ** 2, 0)[:, None] * unit
+        w = np.where(inr, kp * (h * h - q * q) ** 3, 0)
+        t = (h - q) ** 3 * q ** 3
+        c = np.where(q > h / 2, t, 2 * t - h ** 6 / 64)
+        coh = (kc * np.where(inr, c, 0))[:, None] * unit
+        return j, grad, w, coh
+
+    n = len(x3)
+    w64, n64 = np.zeros((n, 3)), np.zeros((n, 3))
+    for i in range(n):
+        j, grad, _, _ = pairs(i)
+        vol = (m / rho[j])[:, None]
+        w64[i] = (vol * np.cross(v[j] - v[i], grad)).sum(0)
+        n64[i] = h * (vol * grad).sum(0)
+    wabs = np.linalg.norm(w64, axis=1)
+    a64, eta = np.zeros((n, 3)), np.zeros(n)
+    for i in range(n):
+        j, grad, w, coh = pairs(i)
+        vel = ((v[j] - v[i]) * (w / rho[j])[:, None]).sum(0)
+        e = ((wabs[j] / rho[j])[:, None] * grad).sum(0)
+        K = 2000.0 / (rho[i] + rho[j])
+        ten = -(K[:, None] * (m * coh + (n64[i] - n64[j]))).sum(0)
+        eta[i] = np.linalg.norm(e)
+        eh = e / eta[i] if eta[i] > 0 else 0 * e
+        a64[i] = xsph / dt * m * vel + vort * np.cross(eh, w64[i]) + tens * ten
+    scale = lambda a: np.abs(a).max()
+    assert scale(w64) > 0.1 and scale(n64) > 0.1
+    np.testing.assert_allclose(om[:, :3], w64, atol=2e-5 * scale(w64))
+    np.testing.assert_allclose(om[:, 3], wabs, atol=2e-5 * scale(w64))
+    np.testing.assert_allclose(nrm[:, :3], n64, atol=2e-5 * scale(n64))
+    # eta-hat is ill-conditioned where eta nearly cancels
+    ok = eta > 1e-3 * np.median(eta)
+    assert ok.mean() > 0.95
+    np.testing.assert_allclose(acc[ok, :3], a64[ok], atol=1e-4 * scale(a64))
+
+
 def test_density_error_fp64_brute_force():
     sc = scenes.dam_break(6, 6, 6, map_res=16, name="tiny")
     o = oracle.Oracle(sc.positions, sc.particle_radius, sc.kernel_radius, 1000.0, sc.domain_lo, sc.domain_hi)

@@ -310,10 +310,11 @@ void enqueue_substep(pbf_ctx *c, const LaunchCtx &L, uint32_t iters, bool diag, 
   launch_density(t, X, c->Xold, c->nbr, c->ncount, c->pos, c->vel, c->lam);
   T.end();
   t = T.begin(P_VORT, L);
-  launch_vorticity(t, c->pos, c->vel, c->nbr, c->ncount, c->om, c->nrm);
+  // the XSPH velocity sum goes to the free Jacobi buffer
+  launch_vorticity(t, c->pos, c->vel, c->nbr, c->ncount, c->om, c->nrm, Xn);
   T.end();
   t = T.begin(P_XSPH, L);
-  launch_xsph(t, c->pos, c->vel, c->om, c->nrm, c->nbr, c->ncount, c->acc);
+  launch_xsph(t, c->pos, c->om, c->nrm, Xn, c->nbr, c->ncount, c->acc);
   T.end();
   if (diag) {
     t = T.begin(P_DIAG, L);

@@ -73,9 +73,10 @@ void launch_dp(const LaunchCtx &L, const float4 *X, const float4 *B, const uint3
                float4 *Xn);
 void launch_density(const LaunchCtx &L, const float4 *X, const float4 *Xold, const uint32_t *nbr,
                     const uint32_t *ncount, float4 *pos, float4 *vel, float *lam);
+// visc: the XSPH velocity sum, written by the vorticity pass, read by XSPH
 void launch_vorticity(const LaunchCtx &L, const float4 *pos, const float4 *vel, const uint32_t *nbr,
-                      const uint32_t *ncount, float4 *om, float4 *nrm);
-void launch_xsph(const LaunchCtx &L, const float4 *pos, const float4 *vel, const float4 *om, const float4 *nrm,
+                      const uint32_t *ncount, float4 *om, float4 *nrm, float4 *visc);
+void launch_xsph(const LaunchCtx &L, const float4 *pos, const float4 *om, const float4 *nrm, const float4 *visc,
                  const uint32_t *nbr, const uint32_t *ncount, float4 *acc);
 void launch_diag(const LaunchCtx &L, const float4 *pos, const uint32_t *nbr, const uint32_t *ncount, double *ci,
                  const uint32_t *ranges, uint32_t nscenes, double *per_scene, double *hist, uint32_t hist_cap,

@@ -930,63 +930,65 @@ __global__ void __launch_bounds__(BS) k_density(uint32_t n, const float4 *__rest
   lam[i] = xi.w;
 }
 
-// K13: vorticity and the Akinci surface normal.
+// K13: vorticity, the Akinci surface normal and the XSPH velocity sum (the
+// XSPH sum needs the same relative velocity and distance, so K14 gathers no
+// velocities).
 __global__ void __launch_bounds__(BS) k_vorticity(uint32_t n, const float4 *__restrict__ pos,
                                                   const float4 *__restrict__ vel, const uint32_t *__restrict__ nbr,
                                                   const uint32_t *__restrict__ ncount, float4 *__restrict__ om,
-                                                  float4 *__restrict__ nrm, Phys P) {
+                                                  float4 *__restrict__ nrm, float4 *__restrict__ visc, Phys P) {
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = pos[i], vi = vel[i];
-  float wx = 0.0f, wy = 0.0f, wz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
+  float wx = 0.0f, wy = 0.0f, wz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f;
   // padding entries (j = i): zero relative velocity and gradient
   for_each_neighbor_g<8>(nbr, i, P, ncount[i], [&](uint32_t j, bool) {
     const float4 xj = pos[j], vj = vel[j];
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
+    float d2 = dist2(dx, dy, dz);
     float vol = P.m * vj.w;
-    float f = spiky_f(P, dist2(dx, dy, dz));
+    float f = spiky_f(P, d2);
     float gx = f * dx, gy = f * dy, gz = f * dz;
     float cx = fmaf(uy, gz, -(uz * gy)), cy = fmaf(uz, gx, -(ux * gz)), cz = fmaf(ux, gy, -(uy * gx));
     wx = fmaf(vol, cx, wx); wy = fmaf(vol, cy, wy); wz = fmaf(vol, cz, wz);
     nx = fmaf(vol, gx, nx); ny = fmaf(vol, gy, ny); nz = fmaf(vol, gz, nz);
+    float sw = poly6(P, d2) * vj.w;
+    sx = fmaf(ux, sw, sx); sy = fmaf(uy, sw, sy); sz = fmaf(uz, sw, sz);
   });
   const float wl = sqrtf(dist2(wx, wy, wz));
   om[i] = make_float4(wx, wy, wz, wl);
-  // w carries |omega| so XSPH gathers one float4 less per neighbour
-  nrm[i] = make_float4(P.h * nx, P.h * ny, P.h * nz, wl);
+  // w carries |omega| / rho, the factor K14 applies to each neighbour's
+  // gradient
+  nrm[i] = make_float4(P.h * nx, P.h * ny, P.h * nz, wl * vi.w);
+  visc[i] = make_float4(sx, sy, sz, 0.0f);
 }
 
 // K14: XSPH viscosity, vorticity confinement force and surface tension.
-__global__ void __launch_bounds__(BS) k_xsph(uint32_t n, const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+__global__ void __launch_bounds__(BS) k_xsph(uint32_t n, const float4 *__restrict__ pos,
                                              const float4 *__restrict__ om, const float4 *__restrict__ nrm,
-                                             const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ ncount,
-                                             float4 *__restrict__ acc, Phys P) {
+                                             const float4 *__restrict__ visc, const uint32_t *__restrict__ nbr,
+                                             const uint32_t *__restrict__ ncount, float4 *__restrict__ acc, Phys P) {
   const uint32_t i = slot_index();
   if (i >= n) return;
-  const float4 xi = pos[i], vi = vel[i], ni = nrm[i], wi = om[i];
+  const float4 xi = pos[i], ni = nrm[i], wi = om[i], ui = visc[i];
   const float rhoi = xi.w;
-  float ux = 0.0f, uy = 0.0f, uz = 0.0f, ex = 0.0f, ey = 0.0f, ez = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
-  // padding entries (j = i): zero relative velocity, gradient, cohesion and
-  // normal difference
+  float ex = 0.0f, ey = 0.0f, ez = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
+  // padding entries (j = i): zero gradient, cohesion and normal difference
   for_each_neighbor_g<4>(nbr, i, P, ncount[i], [&](uint32_t j, bool) {
-    const float4 xj = pos[j], vj = vel[j], nj = nrm[j];
-    const float wj = nj.w;
+    const float4 xj = pos[j], nj = nrm[j];
     const float rhoj = xj.w;
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float d2 = dist2(dx, dy, dz);
-    float vx = vj.x - vi.x, vy = vj.y - vi.y, vz = vj.z - vi.z;
-    float irj = vj.w;
     float K = div_n(P.two_rho0, rhoi + rhoj);
-    float sw = poly6(P, d2) * irj;
-    ux = fmaf(vx, sw, ux); uy = fmaf(vy, sw, uy); uz = fmaf(vz, sw, uz);
-    float se = (wj * spiky_f(P, d2)) * irj;
+    float se = spiky_f(P, d2) * nj.w;
     ex = fmaf(se, dx, ex); ey = fmaf(se, dy, ey); ez = fmaf(se, dz, ez);
     float mc = P.m * cohesion_f(P, d2);
     tx = fmaf(-K, fmaf(mc, dx, ni.x - nj.x), tx);
     ty = fmaf(-K, fmaf(mc, dy, ni.y - nj.y), ty);
     tz = fmaf(-K, fmaf(mc, dz, ni.z - nj.z), tz);
   });
+  const float ux = ui.x, uy = ui.y, uz = ui.z;
   float el = sqrtf(dist2(ex, ey, ez));
   float hx = 0.0f, hy = 0.0f, hz = 0.0f;
   if (el > 0.0f) {
@@ -1275,15 +1277,17 @@ void launch_density(const LaunchCtx &L, const float4 *X, const float4 *Xold, con
 }
 
 void launch_vorticity(const LaunchCtx &L, const float4 *pos, const float4 *vel, const uint32_t *nbr,
-                      const uint32_t *ncount, float4 *om, float4 *nrm) {
+                      const uint32_t *ncount, float4 *om, float4 *nrm, float4 *visc) {
   if (!L.n) return;
-  go(L.stream, L.ev0, L.ev1, k_vorticity, dim3(nblocks(L.n, L.bs)), dim3(L.bs), L.n, pos, vel, nbr, ncount, om, nrm, L.P);
+  go(L.stream, L.ev0, L.ev1, k_vorticity, dim3(nblocks(L.n, L.bs)), dim3(L.bs), L.n, pos, vel, nbr, ncount, om, nrm,
+     visc, L.P);
 }
 
-void launch_xsph(const LaunchCtx &L, const float4 *pos, const float4 *vel, const float4 *om, const float4 *nrm,
+void launch_xsph(const LaunchCtx &L, const float4 *pos, const float4 *om, const float4 *nrm, const float4 *visc,
                  const uint32_t *nbr, const uint32_t *ncount, float4 *acc) {
   if (!L.n) return;
-  go(L.stream, L.ev0, L.ev1, k_xsph, dim3(nblocks(L.n, L.bs)), dim3(L.bs), L.n, pos, vel, om, nrm, nbr, ncount, acc, L.P);
+  go(L.stream, L.ev0, L.ev1, k_xsph, dim3(nblocks(L.n, L.bs)), dim3(L.bs), L.n, pos, om, nrm, visc, nbr, ncount, acc,
+     L.P);
 }
 
 void launch_diag(const LaunchCtx &L, const float4 *pos, const uint32_t *nbr, const uint32_t *ncount, double *ci,
