@@ -231,7 +231,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch setting (no autotuning)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path.",
+        print("bench.py measures one GPU: the engine runs on a single GPU and has no multi-GPU path.",
               file=sys.stderr)
         sys.exit(2)
 

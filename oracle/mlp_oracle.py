@@ -1,7 +1,7 @@
 """CPU restatement of the warm-start network (16 -> 64 -> 64 -> 1, SiLU).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the GPU
-network kernels (libpbf's k_mlp_infer / k_mlp_train).  The shipped library
+TEST INFRASTRUCTURE ONLY: imported by tests/, which check the GPU network
+kernels (libpbf's k_mlp_infer / k_mlp_train) against it.  The shipped library
 never loads it.  The network has no counterpart in the reference (which
 solves lambda from zero every substep, PBF.ts:419-431).
 

@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (liboracle.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() (as the
-checker) and bench.py's cpu_baseline leg.  The shipped library never loads it.
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() (to check
+the GPU result) and bench.py's cpu_baseline leg.  The shipped library never loads it.
 """
 from __future__ import annotations
 

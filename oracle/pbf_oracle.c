@@ -1,8 +1,8 @@
 /*
  * pbf_oracle.c -- CPU oracle for the Position Based Fluids substep.
  *
- * TEST INFRASTRUCTURE ONLY: used by tests/, by __graft_entry__.smoke() as the
- * checker and by bench.py's cpu_baseline leg; never linked into the shipped
+ * TEST INFRASTRUCTURE ONLY: used by tests/, by __graft_entry__.smoke() to check
+ * the GPU result and by bench.py's cpu_baseline leg; never linked into the shipped
  * library.  See pbf_oracle.h for the reference files each part restates.
  *
  * Numerics (fixed here because the reference leaves them open):

@@ -2,8 +2,8 @@
  * pbf_oracle.h -- CPU oracle for the Position Based Fluids substep.
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links, includes
- * or calls this code.  It is used by tests/, by __graft_entry__.smoke() as the
- * checker, and by bench.py's cpu_baseline leg.  It is a plain-C restatement of
+ * or calls this code.  It is used by tests/, by __graft_entry__.smoke() to check
+ * the GPU result, and by bench.py's cpu_baseline leg.  It is a plain-C restatement of
  * the reference step (LinzhouLi/WebGPU-Fluid-Simulation):
  *
  *   orchestration     src/simulator/PBF/PBF.ts:398-448

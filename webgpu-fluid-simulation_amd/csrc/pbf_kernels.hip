@@ -1187,7 +1187,7 @@ static void build_list_rb(const LaunchCtx &L, const float4 *X0, const uint32_t *
 }
 
 // List variants (LDS rows R, work-group size B): 0 (32, 128), 1 (16, 128),
-// 2 (32, 64), 3 (48, 64): 16, 8, 8 and 12 KB of LDS.  (Staging each
+// 2 (32, 64), 3 (56, 64): 16.9, 8.7, 8.4 and 14.6 KB of LDS (R + 1 rows).  (Staging each
 // wavefront's nine candidate column ranges in LDS was measured too: 30-60 %
 // slower, the candidate loads are L1 hits and not what bounds this pass.)
 void launch_build_list(const LaunchCtx &L, const float4 *X0, const uint32_t *offsets, uint32_t *nbr, uint32_t *ncount,

@@ -445,7 +445,7 @@ class PBF:
             return
         self._check(self.lib.pbf_run(self.ctx))
 
-    # -- extended API (north-star surface) ----------------------------------
+    # -- extended API: diagnostics, warm starts, batching, tuning ----------
     def step(self, dt: float, substeps: int = 1, iterations: int = 0):
         self._check(self.lib.pbf_step(self.ctx, float(dt), int(substeps), int(iterations)))
 
