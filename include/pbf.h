@@ -224,6 +224,8 @@ int pbf_read_density_history(pbf_ctx *ctx, double *mean_max, uint32_t capacity, 
 
 uint32_t pbf_particle_count(const pbf_ctx *ctx);
 uint32_t pbf_scene_count(const pbf_ctx *ctx);
+/* Message of the context's last failed call; with NULL, the message of the
+ * last failed pbf_create / pbf_batch_create on the calling thread. */
 const char *pbf_last_error(const pbf_ctx *ctx);
 const char *pbf_version(void);
 

@@ -194,3 +194,26 @@ def test_host_entry_points_reject_non_finite_and_oversized_input():
         pbf_host.build_volume_map(0, (4, 4, 4), (0, 0, 0), (1, 1, 1), inf, None, 0.025, where="host")
     with pytest.raises(pbf_host.PBFError):
         pbf_host.build_volume_map(1, (4, 4, 4), (0, 0, 0), (1, 1, 1), 0.1, (0.5,) * 8 + (nan,), 0.025, where="host")
+
+
+def test_create_rejects_bad_parameters_with_a_message():
+    # Validation runs before any device call, so it is checked here; the
+    # message of a failed create is pbf_last_error(NULL).  A dense grid whose
+    # cell count passes 2^31 (4096^3 cells here, which a 32-bit product would
+    # wrap to 0) and non-finite sizes are refused instead of indexed.
+    nan, inf = float("nan"), float("inf")
+    cases = [
+        (dict(particle_radius=nan, kernel_radius=0.1), "particle_radius must be finite"),
+        (dict(particle_radius=0.025, kernel_radius=inf), "must be finite"),
+        (dict(particle_radius=0.025, kernel_radius=0.1, domain_hi=(nan, 1, 1)), "scene domain must be finite"),
+        (dict(particle_radius=0.025, kernel_radius=0.1, domain_hi=(0, 1, 1)), "hi > lo"),
+        (dict(particle_radius=0.025, kernel_radius=0.1, domain_hi=(409.6, 409.6, 409.6)), "too many grid cells"),
+        (dict(particle_radius=0.025, kernel_radius=1e-12, domain_hi=(1, 1, 1), grid_mode=1, hash_log2=20),
+         "too many grid cells"),
+        (dict(particle_radius=0.025, kernel_radius=0.1, max_neighbors=70000), "max_neighbors"),
+        (dict(particle_radius=0.025, kernel_radius=0.1, capacity=1 << 28), "capacity above"),
+    ]
+    for kw, msg in cases:
+        kw.setdefault("capacity", 1024)
+        with pytest.raises(pbf_host.PBFError, match=msg):
+            pbf_host.PBF(**kw)

@@ -120,6 +120,9 @@ struct pbf_ctx {
 
 namespace {
 
+// The message of the last failed create on this thread (pbf_last_error(NULL)).
+thread_local std::string g_create_error;
+
 int fail(pbf_ctx *c, int code, const std::string &msg) {
   if (c) c->err = msg;
   return code;
@@ -489,14 +492,24 @@ int max_speed(pbf_ctx *c, float dt, double *vmax) {
   return PBF_OK;
 }
 
-void compute_grid(const pbf_params &p, SceneHost &s) {
+// Grid of cell size h over the scene box; false when it cannot be indexed
+// (an axis of 2^30 cells or more, or a dense grid of 2^31 cells or more).
+bool compute_grid(const pbf_params &p, SceneHost &s) {
+  uint64_t dense = 1;
   for (int a = 0; a < 3; ++a) {
-    double L = s.hi[a] - s.lo[a];
-    int D = (int)std::ceil(L / p.kernel_radius - 1e-7);
-    s.dims[a] = D < 1 ? 1 : D;
+    const double L = s.hi[a] - s.lo[a];
+    const double D = std::ceil(L / p.kernel_radius - 1e-7);
+    if (!(D < 1073741824.0)) return false;
+    s.dims[a] = D < 1.0 ? 1 : (int)D;
+    dense *= (uint64_t)s.dims[a];
   }
-  if (p.grid_mode == PBF_GRID_HASH) s.ncells = 1u << p.hash_log2;
-  else s.ncells = (uint32_t)s.dims[0] * (uint32_t)s.dims[1] * (uint32_t)s.dims[2];
+  if (p.grid_mode == PBF_GRID_HASH) {
+    s.ncells = 1u << p.hash_log2;
+  } else {
+    if (dense > 0x7fffffffull) return false;
+    s.ncells = (uint32_t)dense;
+  }
+  return true;
 }
 
 Scene to_device_scene(const pbf_ctx *c, const SceneHost &s, uint64_t field_off) {
@@ -565,25 +578,33 @@ constexpr uint32_t kMaxIterations = 4096;  // Jacobi iterations per substep (one
 
 int create_ctx(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc *descs, bool batch, pbf_ctx **out) {
   pbf_ctx *c = nullptr;
-  if (!params || !out || nscenes == 0) return PBF_E_INVALID;
+  g_create_error.clear();
+  if (!params || !out || nscenes == 0) {
+    g_create_error = "params, out and at least one scene are required";
+    return PBF_E_INVALID;
+  }
   *out = nullptr;
   c = new pbf_ctx();
   c->params = *params;
   pbf_params &p = c->params;
-  if (p.particle_radius <= 0.0) { int r = fail(c, PBF_E_INVALID, "particle_radius must be > 0"); delete c; return r; }
+  auto reject = [&](int code, const std::string &msg) {
+    g_create_error = msg;
+    delete c;
+    return code;
+  };
+  if (!(p.particle_radius > 0.0) || !std::isfinite(p.particle_radius))
+    return reject(PBF_E_INVALID, "particle_radius must be finite and > 0");
+  if (!std::isfinite(p.kernel_radius) || !std::isfinite(p.rest_density) || !std::isfinite(p.time_step))
+    return reject(PBF_E_INVALID, "kernel_radius, rest_density and time_step must be finite");
   if (p.kernel_radius <= 0.0) p.kernel_radius = 4.0 * p.particle_radius;
   if (p.rest_density <= 0.0) p.rest_density = 1000.0;
   if (p.time_step <= 0.0) p.time_step = 1.0 / 300.0;
   if (p.max_neighbors == 0) p.max_neighbors = 96;
-  if (p.max_neighbors > 65535u) {
-    int r = fail(c, PBF_E_INVALID, "max_neighbors must be below 65536"); delete c; return r;
-  }
-  if (p.grid_mode == PBF_GRID_HASH && (p.hash_log2 < 4 || p.hash_log2 > 28)) {
-    int r = fail(c, PBF_E_INVALID, "hash_log2 must be in [4, 28]"); delete c; return r;
-  }
-  if (p.capacity > kMaxCapacity) {  // 32-bit gather offsets (slot * 16 bytes)
-    int r = fail(c, PBF_E_CAPACITY, "capacity above 2^27 - 256 particles"); delete c; return r;
-  }
+  if (p.max_neighbors > 65535u) return reject(PBF_E_INVALID, "max_neighbors must be below 65536");
+  if (p.grid_mode == PBF_GRID_HASH && (p.hash_log2 < 4 || p.hash_log2 > 28))
+    return reject(PBF_E_INVALID, "hash_log2 must be in [4, 28]");
+  if (p.capacity > kMaxCapacity)  // 32-bit gather offsets (slot * 16 bytes)
+    return reject(PBF_E_CAPACITY, "capacity above 2^27 - 256 particles");
   c->cfg = {5u, 0.01f, 0.1f, 0.5f, 9.8f};  // src/common/config.ts:22-28
   c->batch = batch;
   uint32_t cell_base = 0, pbase = 0;
@@ -593,21 +614,22 @@ int create_ctx(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc 
     for (int a = 0; a < 3; ++a) {
       s.lo[a] = descs ? descs[k].domain_lo[a] : p.domain_lo[a];
       s.hi[a] = descs ? descs[k].domain_hi[a] : p.domain_hi[a];
-      if (!(s.hi[a] > s.lo[a])) { int r = fail(c, PBF_E_INVALID, "scene domain must have hi > lo"); delete c; return r; }
+      if (!(s.hi[a] > s.lo[a]) || !std::isfinite(s.lo[a]) || !std::isfinite(s.hi[a]))
+        return reject(PBF_E_INVALID, "scene domain must be finite with hi > lo");
     }
     s.count = descs ? descs[k].particle_count : 0;
-    compute_grid(p, s);
+    if (!compute_grid(p, s)) return reject(PBF_E_CAPACITY, "too many grid cells");
     s.cell_base = cell_base;
     s.pbase = pbase;
     total_cells += s.ncells;
     total_particles += s.count;
-    if (total_cells > 0x7fffffffull) { int r = fail(c, PBF_E_CAPACITY, "too many grid cells"); delete c; return r; }
+    if (total_cells > 0x7fffffffull) return reject(PBF_E_CAPACITY, "too many grid cells");
     cell_base += s.ncells;
     pbase += s.count;
     c->scenes.push_back(s);
   }
   if (batch && total_particles > p.capacity) {
-    int r = fail(c, PBF_E_CAPACITY, "To Many Particles!"); delete c; return r;
+    return reject(PBF_E_CAPACITY, "To Many Particles!");
   }
   c->ncells = (uint32_t)total_cells;
   c->ncap = ((p.capacity + 255u) / 256u) * 256u;
@@ -616,7 +638,7 @@ int create_ctx(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc 
   c->maxn = p.max_neighbors;
   c->ld = c->ncap;
   hipError_t e = hipGetDevice(&c->device);
-  if (e != hipSuccess) { int r = fail(c, PBF_E_DEVICE, std::string("hipGetDevice: ") + hipGetErrorString(e)); delete c; return r; }
+  if (e != hipSuccess) return reject(PBF_E_DEVICE, std::string("hipGetDevice: ") + hipGetErrorString(e));
   int rc = PBF_OK;
   auto step = [&](hipError_t err, const char *what) {
     if (rc == PBF_OK && err != hipSuccess) rc = fail(c, PBF_E_DEVICE, std::string(what) + ": " + hipGetErrorString(err));
@@ -669,9 +691,8 @@ int create_ctx(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc 
   }
   if (rc == PBF_OK) rc = upload_scenes(c);
   if (rc != PBF_OK) {
-    std::string msg = c->err;
+    g_create_error = c->err;
     pbf_destroy(c);
-    (void)msg;
     return rc;
   }
   *out = c;
@@ -1310,7 +1331,10 @@ int pbf_read_density_history(pbf_ctx *c, double *mean_max, uint32_t capacity, ui
 
 uint32_t pbf_particle_count(const pbf_ctx *c) { return c ? c->n : 0u; }
 uint32_t pbf_scene_count(const pbf_ctx *c) { return c ? (uint32_t)c->scenes.size() : 0u; }
-const char *pbf_last_error(const pbf_ctx *c) { return c ? c->err.c_str() : "null context"; }
+const char *pbf_last_error(const pbf_ctx *c) {
+  if (c) return c->err.c_str();
+  return g_create_error.empty() ? "null context" : g_create_error.c_str();
+}
 
 int pbf_pass_times(pbf_ctx *c, double *ms, uint32_t *launches, uint32_t capacity, uint32_t *count,
                    const char **names) {

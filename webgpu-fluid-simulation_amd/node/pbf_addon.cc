@@ -150,7 +150,7 @@ napi_value Create(napi_env env, napi_callback_info info) {
   pbf_ctx *c = nullptr;
   int rc = pbf_create(&p, &c);
   if (rc != PBF_OK) {
-    napi_throw_error(env, nullptr, ("pbf_create failed: " + std::to_string(rc)).c_str());
+    napi_throw_error(env, nullptr, pbf_last_error(nullptr));
     return nullptr;
   }
   return wrap_ctx(env, c);
@@ -178,7 +178,7 @@ napi_value BatchCreate(napi_env env, napi_callback_info info) {
   pbf_ctx *c = nullptr;
   int rc = pbf_batch_create(&p, ns, d.data(), &c);
   if (rc != PBF_OK) {
-    napi_throw_error(env, nullptr, ("pbf_batch_create failed: " + std::to_string(rc)).c_str());
+    napi_throw_error(env, nullptr, pbf_last_error(nullptr));
     return nullptr;
   }
   return wrap_ctx(env, c);

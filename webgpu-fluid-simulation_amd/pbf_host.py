@@ -341,7 +341,7 @@ class PBF:
         else:
             rc = self.lib.pbf_create(C.byref(p), C.byref(self.ctx))
         if rc != PBF_OK:
-            raise PBFError(rc, f"pbf_create failed ({rc})")
+            raise PBFError(rc, self.lib.pbf_last_error(None).decode())
         self.pause = True
         self.stepCount = 1
         self.particlePositionArray: list = []
