@@ -194,6 +194,14 @@ def test_host_entry_points_reject_non_finite_and_oversized_input():
         pbf_host.build_volume_map(0, (4, 4, 4), (0, 0, 0), (1, 1, 1), inf, None, 0.025, where="host")
     with pytest.raises(pbf_host.PBFError):
         pbf_host.build_volume_map(1, (4, 4, 4), (0, 0, 0), (1, 1, 1), 0.1, (0.5,) * 8 + (nan,), 0.025, where="host")
+    # resolutions beyond 65536 per axis (the builders' and samplers' index range)
+    big = (C.c_int32 * 3)(2147483647, 1, 1)
+    length = C.c_size_t()
+    assert lib.pbf_build_cubic_field(0, None, big, d3((0, 0, 0)), d3((1, 1, 1)), 0.1, 0.025, None, None, 0,
+                                     C.byref(length)) == pbf_host.PBF_E_INVALID
+    one = (C.c_float * 1)()
+    assert lib.pbf_write_cdm((C.c_int32 * 3)(65537, 1, 1), one, one, 1, None, 0, None) == pbf_host.PBF_E_INVALID
+    assert lib.pbf_build_volume_map(0, None, big, d3((0, 0, 0)), d3((1, 1, 1)), 0.1, 0.025, one) == pbf_host.PBF_E_INVALID
 
 
 def test_create_rejects_bad_parameters_with_a_message():

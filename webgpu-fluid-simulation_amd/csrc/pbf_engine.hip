@@ -909,17 +909,36 @@ int pbf_set_positions(pbf_ctx *c, const float *xyzw, size_t length, uint32_t cou
   return upload_positions(c, xyzw, count);
 }
 
+// A field box the samplers can use: finite, hi > lo on every axis.
+static bool field_box_ok(const double lo[3], const double hi[3]) {
+  for (int a = 0; a < 3; ++a)
+    if (!std::isfinite(lo[a]) || !std::isfinite(hi[a]) || !(hi[a] > lo[a])) return false;
+  return true;
+}
+// The device samplers index a field array with 32-bit unsigned arithmetic.
+constexpr size_t kMaxFieldValues = (size_t)1 << 31;
+constexpr int32_t kMaxFieldRes = 1 << 16;
+
 int pbf_set_cubic_field(pbf_ctx *c, uint32_t scene, const int32_t res[3], const double lo[3], const double hi[3],
                         const float *sdf, const float *vol, size_t length) {
   if (!c || !res || !lo || !hi || !sdf || !vol) return PBF_E_INVALID;
   if (scene >= c->scenes.size()) return fail(c, PBF_E_INVALID, "scene index out of range");
   int r3[3] = {res[0], res[1], res[2]};
-  if (r3[0] < 1 || r3[1] < 1 || r3[2] < 1) return fail(c, PBF_E_FORMAT, "Invalid Boundary Model File!");
+  for (int a = 0; a < 3; ++a)
+    if (r3[a] < 1 || r3[a] > kMaxFieldRes) return fail(c, PBF_E_FORMAT, "Invalid Boundary Model File!");
+  if (!field_box_ok(lo, hi)) return fail(c, PBF_E_INVALID, "field box must be finite with hi > lo");
   size_t fl = cubic_field_length(r3);
+  if (fl >= kMaxFieldValues) return fail(c, PBF_E_CAPACITY, "boundary field too large");
   if (length != fl) return fail(c, PBF_E_FORMAT, "Invalid Boundary Model File!");
   auto &s = c->scenes[scene];
-  s.field.assign(sdf, sdf + fl);
-  s.field.insert(s.field.end(), vol, vol + fl);
+  try {
+    s.field.assign(sdf, sdf + fl);
+    s.field.insert(s.field.end(), vol, vol + fl);
+  } catch (const std::bad_alloc &) {
+    s.field.clear();
+    s.field_kind = PBF_FIELD_NONE;
+    return fail(c, PBF_E_CAPACITY, "out of host memory for the boundary field");
+  }
   s.field_kind = PBF_FIELD_CUBIC;
   for (int a = 0; a < 3; ++a) { s.fres[a] = res[a]; s.flo[a] = lo[a]; s.fhi[a] = hi[a]; }
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -944,10 +963,19 @@ int pbf_set_volume_map(pbf_ctx *c, uint32_t scene, const int32_t res[3], const d
                        const float *sdf_vol) {
   if (!c || !res || !lo || !hi || !sdf_vol) return PBF_E_INVALID;
   if (scene >= c->scenes.size()) return fail(c, PBF_E_INVALID, "scene index out of range");
-  if (res[0] < 1 || res[1] < 1 || res[2] < 1) return fail(c, PBF_E_INVALID, "volume map resolution must be >= 1");
-  size_t nn = (size_t)(res[0] + 1) * (res[1] + 1) * (res[2] + 1);
+  for (int a = 0; a < 3; ++a)
+    if (res[a] < 1 || res[a] > kMaxFieldRes) return fail(c, PBF_E_INVALID, "volume map resolution must be in [1, 65536]");
+  if (!field_box_ok(lo, hi)) return fail(c, PBF_E_INVALID, "field box must be finite with hi > lo");
+  const size_t nn = ((size_t)res[0] + 1) * ((size_t)res[1] + 1) * ((size_t)res[2] + 1);
+  if (2 * nn >= kMaxFieldValues) return fail(c, PBF_E_CAPACITY, "boundary field too large");
   auto &s = c->scenes[scene];
-  s.field.assign(sdf_vol, sdf_vol + 2 * nn);
+  try {
+    s.field.assign(sdf_vol, sdf_vol + 2 * nn);
+  } catch (const std::bad_alloc &) {
+    s.field.clear();
+    s.field_kind = PBF_FIELD_NONE;
+    return fail(c, PBF_E_CAPACITY, "out of host memory for the boundary field");
+  }
   s.field_kind = PBF_FIELD_TRILINEAR;
   for (int a = 0; a < 3; ++a) { s.fres[a] = res[a]; s.flo[a] = lo[a]; s.fhi[a] = hi[a]; }
   HIPCHK(hipStreamSynchronize(c->stream));

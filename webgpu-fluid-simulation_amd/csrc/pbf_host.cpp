@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -23,9 +24,15 @@ uint64_t splitmix64(uint64_t x) {
 
 // Values of one 32-node cubic field: corner nodes (R+1)^3, then the three
 // edge-node sets (src/simulator/boundary/volumeMap.ts:6-11).
-size_t cubic_field_length(const int r[3]) {
-  return (size_t)(r[0] + 1) * (r[1] + 1) * (r[2] + 1) + (size_t)(2 * r[0]) * (r[1] + 1) * (r[2] + 1) +
-         (size_t)(r[0] + 1) * (2 * r[1]) * (r[2] + 1) + (size_t)(r[0] + 1) * (r[1] + 1) * (2 * r[2]);
+// Field resolutions accepted per axis (the device samplers index fields with
+// 32-bit arithmetic; the builders below stay in range with this bound).
+static bool res_ok(const int32_t *r) {
+  return r && r[0] >= 1 && r[1] >= 1 && r[2] >= 1 && r[0] <= 65536 && r[1] <= 65536 && r[2] <= 65536;
+}
+
+size_t cubic_field_length(const int r[3]) {  // r >= 1 (callers check)
+  const size_t x = (size_t)r[0], y = (size_t)r[1], z = (size_t)r[2];
+  return (x + 1) * (y + 1) * (z + 1) + 2 * x * (y + 1) * (z + 1) + (x + 1) * 2 * y * (z + 1) + (x + 1) * (y + 1) * 2 * z;
 }
 
 namespace {
@@ -68,23 +75,34 @@ bool parse_numbers(const Line &l, std::vector<float> &out) {
 // .cdm text: line 0 "Rx Ry Rz", line 1 count, line 2 SDF values, line 3
 // count, line 4 volume values (src/simulator/boundary/volumeMap.ts:23-39).
 int parse_cdm(const char *text, size_t len, int res[3], float **sdf, float **vol, size_t *flen) {
-  auto lines = split_lines(text, len);
-  if (lines.size() < 5) return -1;
-  std::vector<float> head, a, b;
-  if (!parse_numbers(lines[0], head) || head.size() != 3) return -1;
-  for (int k = 0; k < 3; ++k) {
-    if (!(head[k] >= 1.0f && head[k] <= 4096.0f) || head[k] != std::floor(head[k])) return -1;
-    res[k] = (int)head[k];
+  *sdf = *vol = nullptr;
+  try {
+    auto lines = split_lines(text, len);
+    if (lines.size() < 5) return -1;
+    std::vector<float> head, a, b;
+    if (!parse_numbers(lines[0], head) || head.size() != 3) return -1;
+    for (int k = 0; k < 3; ++k) {
+      if (!(head[k] >= 1.0f && head[k] <= 4096.0f) || head[k] != std::floor(head[k])) return -1;
+      res[k] = (int)head[k];
+    }
+    if (!parse_numbers(lines[2], a) || !parse_numbers(lines[4], b)) return -1;
+    size_t fl = cubic_field_length(res);
+    if (a.size() != fl || b.size() != fl) return -1;
+    *sdf = (float *)std::malloc(fl * 4);
+    *vol = (float *)std::malloc(fl * 4);
+    if (!*sdf || !*vol) {
+      std::free(*sdf);
+      std::free(*vol);
+      *sdf = *vol = nullptr;
+      return -1;
+    }
+    std::memcpy(*sdf, a.data(), fl * 4);
+    std::memcpy(*vol, b.data(), fl * 4);
+    *flen = fl;
+    return 0;
+  } catch (const std::bad_alloc &) {  // text too large for host memory: not a valid file here
+    return -1;
   }
-  if (!parse_numbers(lines[2], a) || !parse_numbers(lines[4], b)) return -1;
-  size_t fl = cubic_field_length(res);
-  if (a.size() != fl || b.size() != fl) return -1;
-  *sdf = (float *)std::malloc(fl * 4);
-  *vol = (float *)std::malloc(fl * 4);
-  std::memcpy(*sdf, a.data(), fl * 4);
-  std::memcpy(*vol, b.data(), fl * 4);
-  *flen = fl;
-  return 0;
 }
 
 // Seeded jitter of particle `serial`, axis `axis`: uniform in [-amp, amp].
@@ -236,8 +254,7 @@ void parallel_for(size_t total, F f) {
 extern "C" int pbf_build_volume_map(uint32_t kind, const double *shape, const int32_t res[3], const double lo[3],
                                     const double hi[3], double kernel_radius, double particle_radius,
                                     float *sdf_vol) {
-  if (!res || !sdf_vol) return PBF_E_INVALID;
-  if (res[0] < 1 || res[1] < 1 || res[2] < 1) return PBF_E_INVALID;
+  if (!pbf::res_ok(res) || !sdf_vol) return PBF_E_INVALID;
   VolumeEval E;
   int rc = E.init(kind, shape, lo, hi, kernel_radius, particle_radius);
   if (rc != PBF_OK) return rc;
@@ -261,7 +278,7 @@ extern "C" int pbf_build_cubic_field(uint32_t kind, const double *shape, const i
                                      const double hi[3], double kernel_radius, double particle_radius, float *sdf,
                                      float *vol, size_t capacity, size_t *field_length) {
   using namespace pbf;
-  if (!res || res[0] < 1 || res[1] < 1 || res[2] < 1) return PBF_E_INVALID;
+  if (!res_ok(res)) return PBF_E_INVALID;
   const int R[3] = {res[0], res[1], res[2]};
   const size_t fl = cubic_field_length(R);
   if (field_length) *field_length = fl;
@@ -304,24 +321,28 @@ extern "C" int pbf_build_cubic_field(uint32_t kind, const double *shape, const i
 extern "C" int pbf_write_cdm(const int32_t res[3], const float *sdf, const float *vol, size_t length, char *out,
                              size_t capacity, size_t *written) {
   using namespace pbf;
-  if (!res || !sdf || !vol || res[0] < 1 || res[1] < 1 || res[2] < 1) return PBF_E_INVALID;
+  if (!res_ok(res) || !sdf || !vol) return PBF_E_INVALID;
   const int R[3] = {res[0], res[1], res[2]};
   if (length != cubic_field_length(R)) return PBF_E_LENGTH;
-  std::string t;
-  t.reserve(length * 32 + 64);
-  char buf[64];
-  std::snprintf(buf, sizeof(buf), "%d %d %d\r\n%zu\r\n", R[0], R[1], R[2], length);
-  t += buf;
-  for (size_t q = 0; q < length; ++q) { std::snprintf(buf, sizeof(buf), "%.9g ", (double)sdf[q]); t += buf; }
-  std::snprintf(buf, sizeof(buf), "\r\n%zu\r\n", length);
-  t += buf;
-  for (size_t q = 0; q < length; ++q) { std::snprintf(buf, sizeof(buf), "%.9g ", (double)vol[q]); t += buf; }
-  t += "\r\n";
-  if (written) *written = t.size();
-  if (!out) return PBF_OK;
-  if (capacity < t.size()) return PBF_E_CAPACITY;
-  std::memcpy(out, t.data(), t.size());
-  return PBF_OK;
+  try {  // the text is built in host memory: an allocation failure is a status, not an exception
+    std::string t;
+    t.reserve(length * 32 + 64);
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%d %d %d\r\n%zu\r\n", R[0], R[1], R[2], length);
+    t += buf;
+    for (size_t q = 0; q < length; ++q) { std::snprintf(buf, sizeof(buf), "%.9g ", (double)sdf[q]); t += buf; }
+    std::snprintf(buf, sizeof(buf), "\r\n%zu\r\n", length);
+    t += buf;
+    for (size_t q = 0; q < length; ++q) { std::snprintf(buf, sizeof(buf), "%.9g ", (double)vol[q]); t += buf; }
+    t += "\r\n";
+    if (written) *written = t.size();
+    if (!out) return PBF_OK;
+    if (capacity < t.size()) return PBF_E_CAPACITY;
+    std::memcpy(out, t.data(), t.size());
+    return PBF_OK;
+  } catch (const std::bad_alloc &) {
+    return PBF_E_CAPACITY;
+  }
 }
 
 namespace {

@@ -98,7 +98,8 @@ extern "C" int pbf_build_volume_map_gpu(uint32_t kind, const double *shape, cons
                                         const double hi[3], double kernel_radius, double particle_radius,
                                         float *sdf_vol) {
   if (!res || !lo || !hi || !sdf_vol || kind > 1 || !(kernel_radius > 0.0)) return PBF_E_INVALID;
-  if (res[0] < 1 || res[1] < 1 || res[2] < 1) return PBF_E_INVALID;
+  for (int a = 0; a < 3; ++a)
+    if (res[a] < 1 || res[a] > 65536) return PBF_E_INVALID;
   if (!std::isfinite(kernel_radius) || !(particle_radius >= 0.0) || !std::isfinite(particle_radius))
     return PBF_E_INVALID;
   for (int a = 0; a < 3; ++a)
@@ -157,6 +158,7 @@ extern "C" int pbf_build_volume_map_gpu(uint32_t kind, const double *shape, cons
   const double t2r = h * h - 4.0 * particle_radius * particle_radius;
   A.vcap = t2r > 0.0 ? 0.9 / (A.kp * t2r * t2r * t2r) : 0.0;
   const size_t total = ((size_t)res[0] + 1) * ((size_t)res[1] + 1) * ((size_t)res[2] + 1);
+  if ((total + 255) / 256 > 0x7fffffffu) return PBF_E_CAPACITY;  // one launch, 1-D grid
   double *dqp = nullptr, *dqw = nullptr;
   float *dout = nullptr;
   hipStream_t st = nullptr;
