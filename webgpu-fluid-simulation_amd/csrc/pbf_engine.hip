@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <new>
 #include <string>
 #include <vector>
@@ -576,7 +577,8 @@ int upload_scenes(pbf_ctx *c) {
 constexpr uint32_t kMaxCapacity = (1u << 27) - 256u;
 constexpr uint32_t kMaxIterations = 4096;  // Jacobi iterations per substep (one graph node pair each)
 
-int create_ctx(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc *descs, bool batch, pbf_ctx **out) {
+int create_ctx_body(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc *descs, bool batch, pbf_ctx **out,
+                    pbf_ctx **made) {
   pbf_ctx *c = nullptr;
   g_create_error.clear();
   if (!params || !out || nscenes == 0) {
@@ -585,10 +587,12 @@ int create_ctx(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc 
   }
   *out = nullptr;
   c = new pbf_ctx();
+  *made = c;
   c->params = *params;
   pbf_params &p = c->params;
   auto reject = [&](int code, const std::string &msg) {
     g_create_error = msg;
+    *made = nullptr;
     delete c;
     return code;
   };
@@ -692,11 +696,26 @@ int create_ctx(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc 
   if (rc == PBF_OK) rc = upload_scenes(c);
   if (rc != PBF_OK) {
     g_create_error = c->err;
+    *made = nullptr;
     pbf_destroy(c);
     return rc;
   }
   *out = c;
   return PBF_OK;
+}
+
+// create_ctx_body with host-allocation failures turned into a status; a
+// context already made is released with its device buffers.
+int create_ctx(const pbf_params *params, uint32_t nscenes, const pbf_scene_desc *descs, bool batch, pbf_ctx **out) {
+  pbf_ctx *made = nullptr;
+  try {
+    return create_ctx_body(params, nscenes, descs, batch, out, &made);
+  } catch (const std::exception &) {
+    if (made) pbf_destroy(made);
+    if (out) *out = nullptr;
+    g_create_error = "out of host memory";
+    return PBF_E_CAPACITY;
+  }
 }
 
 int upload_positions(pbf_ctx *c, const float *xyzw, uint32_t count) {
@@ -766,6 +785,16 @@ int read_sorted4(pbf_ctx *c, const float4 *src, const uint32_t *ids, float *out,
 
 // ===========================================================================
 // C ABI
+// Host allocations inside an entry point (staging vectors, graphs, event
+// pools) report failure as a status: nothing unwinds through the C ABI.
+#define PBF_CATCH(ctx)                                                      \
+  catch (const std::bad_alloc &) {                                          \
+    return fail(ctx, PBF_E_CAPACITY, "out of host memory");                 \
+  }                                                                         \
+  catch (const std::exception &ex) {                                        \
+    return fail(ctx, PBF_E_DEVICE, std::string("internal error: ") + ex.what()); \
+  }
+
 extern "C" {
 
 const char *pbf_version(void) { return "pbf-mi355x 1.0 (gfx950)"; }
@@ -802,7 +831,7 @@ int pbf_destroy(pbf_ctx *c) {
   return PBF_OK;
 }
 
-int pbf_reset(pbf_ctx *c) {
+int pbf_reset(pbf_ctx *c) try {
   if (!c) return PBF_E_INVALID;
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -820,15 +849,15 @@ int pbf_reset(pbf_ctx *c) {
   HIPCHK(hipMemset(c->flags, 0, 8));
   HIPCHK(hipMemset(c->hist_ctr, 0, 4));
   return upload_scenes(c);
-}
+} PBF_CATCH(c)
 
-int pbf_set_config(pbf_ctx *c, const pbf_config *cfg) {
+int pbf_set_config(pbf_ctx *c, const pbf_config *cfg) try {
   if (!c || !cfg) return PBF_E_INVALID;
   if (cfg->iteration > kMaxIterations) return fail(c, PBF_E_INVALID, "iteration must be <= 4096");
   c->cfg = *cfg;
   clear_graphs(c);
   return PBF_OK;
-}
+} PBF_CATCH(c)
 
 // Room for `cnt` more staged particles: more than the largest capacity can
 // never be uploaded ('To Many Particles!' at upload), so it fails here before
@@ -877,7 +906,7 @@ int pbf_voxelize_sphere(pbf_ctx *c, const double center[3], double radius, doubl
 }
 
 int pbf_voxelize_mesh(pbf_ctx *c, const float *vertices, uint32_t nverts, const uint32_t *indices, uint32_t ntris,
-                      const double world[16], double jitter, uint64_t seed, uint32_t *added) {
+                      const double world[16], double jitter, uint64_t seed, uint32_t *added) try {
   if (!c) return PBF_E_INVALID;
   uint32_t cnt = 0;
   int rc = pbf_voxelize_mesh_host(c->params.particle_radius, vertices, nverts, indices, ntris, world, jitter, seed, 0,
@@ -891,23 +920,23 @@ int pbf_voxelize_mesh(pbf_ctx *c, const float *vertices, uint32_t nverts, const 
   c->staged_serial += cnt;
   if (added) *added = cnt;
   return PBF_OK;
-}
+} PBF_CATCH(c)
 
 uint32_t pbf_staged_count(const pbf_ctx *c) { return c ? (uint32_t)(c->staged.size() / 4) : 0u; }
 
-int pbf_set_particle_position(pbf_ctx *c) {
+int pbf_set_particle_position(pbf_ctx *c) try {
   if (!c) return PBF_E_INVALID;
   uint32_t count = (uint32_t)(c->staged.size() / 4);
   return pbf_set_positions(c, c->staged.data(), c->staged.size(), count);
-}
+} PBF_CATCH(c)
 
-int pbf_set_positions(pbf_ctx *c, const float *xyzw, size_t length, uint32_t count) {
+int pbf_set_positions(pbf_ctx *c, const float *xyzw, size_t length, uint32_t count) try {
   if (!c) return PBF_E_INVALID;
   if (length != (size_t)count * 4) return fail(c, PBF_E_LENGTH, "Illegal length of Particle Position Buffer!");
   if (count > c->params.capacity) return fail(c, PBF_E_CAPACITY, "To Many Particles!");
   if (count && !xyzw) return PBF_E_INVALID;
   return upload_positions(c, xyzw, count);
-}
+} PBF_CATCH(c)
 
 // A field box the samplers can use: finite, hi > lo on every axis.
 static bool field_box_ok(const double lo[3], const double hi[3]) {
@@ -920,7 +949,7 @@ constexpr size_t kMaxFieldValues = (size_t)1 << 31;
 constexpr int32_t kMaxFieldRes = 1 << 16;
 
 int pbf_set_cubic_field(pbf_ctx *c, uint32_t scene, const int32_t res[3], const double lo[3], const double hi[3],
-                        const float *sdf, const float *vol, size_t length) {
+                        const float *sdf, const float *vol, size_t length) try {
   if (!c || !res || !lo || !hi || !sdf || !vol) return PBF_E_INVALID;
   if (scene >= c->scenes.size()) return fail(c, PBF_E_INVALID, "scene index out of range");
   int r3[3] = {res[0], res[1], res[2]};
@@ -943,9 +972,9 @@ int pbf_set_cubic_field(pbf_ctx *c, uint32_t scene, const int32_t res[3], const 
   for (int a = 0; a < 3; ++a) { s.fres[a] = res[a]; s.flo[a] = lo[a]; s.fhi[a] = hi[a]; }
   HIPCHK(hipStreamSynchronize(c->stream));
   return upload_scenes(c);
-}
+} PBF_CATCH(c)
 
-int pbf_set_boundary_data(pbf_ctx *c, const char *text, size_t length) {
+int pbf_set_boundary_data(pbf_ctx *c, const char *text, size_t length) try {
   if (!c || !text) return PBF_E_INVALID;
   int res[3];
   float *sdf = nullptr, *vol = nullptr;
@@ -957,10 +986,10 @@ int pbf_set_boundary_data(pbf_ctx *c, const char *text, size_t length) {
   free(sdf);
   free(vol);
   return rc;
-}
+} PBF_CATCH(c)
 
 int pbf_set_volume_map(pbf_ctx *c, uint32_t scene, const int32_t res[3], const double lo[3], const double hi[3],
-                       const float *sdf_vol) {
+                       const float *sdf_vol) try {
   if (!c || !res || !lo || !hi || !sdf_vol) return PBF_E_INVALID;
   if (scene >= c->scenes.size()) return fail(c, PBF_E_INVALID, "scene index out of range");
   for (int a = 0; a < 3; ++a)
@@ -980,9 +1009,9 @@ int pbf_set_volume_map(pbf_ctx *c, uint32_t scene, const int32_t res[3], const d
   for (int a = 0; a < 3; ++a) { s.fres[a] = res[a]; s.flo[a] = lo[a]; s.fhi[a] = hi[a]; }
   HIPCHK(hipStreamSynchronize(c->stream));
   return upload_scenes(c);
-}
+} PBF_CATCH(c)
 
-int pbf_clear_boundary(pbf_ctx *c) {
+int pbf_clear_boundary(pbf_ctx *c) try {
   if (!c) return PBF_E_INVALID;
   HIPCHK(hipStreamSynchronize(c->stream));
   for (auto &s : c->scenes) {
@@ -990,9 +1019,9 @@ int pbf_clear_boundary(pbf_ctx *c) {
     s.field.clear();
   }
   return upload_scenes(c);
-}
+} PBF_CATCH(c)
 
-int pbf_step(pbf_ctx *c, float dt, uint32_t substeps, uint32_t iterations) {
+int pbf_step(pbf_ctx *c, float dt, uint32_t substeps, uint32_t iterations) try {
   if (!c) return PBF_E_INVALID;
   if (!(dt > 0.0f) || substeps == 0) return fail(c, PBF_E_INVALID, "dt must be > 0 and substeps >= 1");
   if (iterations > kMaxIterations) return fail(c, PBF_E_INVALID, "iterations must be <= 4096");
@@ -1006,10 +1035,10 @@ int pbf_step(pbf_ctx *c, float dt, uint32_t substeps, uint32_t iterations) {
     if (rc != PBF_OK) return rc;
   }
   return PBF_OK;
-}
+} PBF_CATCH(c)
 
 int pbf_set_warm_start(pbf_ctx *c, const float *blob, size_t length, uint32_t mode, float relax,
-                       double fallback_threshold, uint32_t fallback_iterations) {
+                       double fallback_threshold, uint32_t fallback_iterations) try {
   if (!c) return PBF_E_INVALID;
   if (mode > PBF_WARM_RATIO) return fail(c, PBF_E_INVALID, "unknown warm-start mode");
   if (!(relax > 0.0f && relax <= 1.0f)) return fail(c, PBF_E_INVALID, "relax must be in (0, 1]");
@@ -1045,7 +1074,7 @@ int pbf_set_warm_start(pbf_ctx *c, const float *blob, size_t length, uint32_t mo
   clear_graphs(c);
   HIPCHK(hipStreamSynchronize(c->stream));
   return PBF_OK;
-}
+} PBF_CATCH(c)
 
 int pbf_get_tuning(const pbf_ctx *c, pbf_tuning *out) {
   if (!c || !out) return PBF_E_INVALID;
@@ -1065,7 +1094,7 @@ int pbf_set_tuning(pbf_ctx *c, const pbf_tuning *t) {
   return PBF_OK;
 }
 
-int pbf_autotune(pbf_ctx *c, float dt, uint32_t iterations, uint32_t repeats, pbf_tuning *out) {
+int pbf_autotune(pbf_ctx *c, float dt, uint32_t iterations, uint32_t repeats, pbf_tuning *out) try {
   if (!c) return PBF_E_INVALID;
   if (!(dt > 0.0f) || repeats == 0) return fail(c, PBF_E_INVALID, "dt must be > 0 and repeats >= 1");
   if (iterations > kMaxIterations || repeats > 1000)
@@ -1126,7 +1155,7 @@ int pbf_autotune(pbf_ctx *c, float dt, uint32_t iterations, uint32_t repeats, pb
   clear_graphs(c);
   if (out) *out = best;
   return PBF_OK;
-}
+} PBF_CATCH(c)
 
 int pbf_warm_start_stats(const pbf_ctx *c, uint64_t *learned_substeps, uint64_t *fallbacks) {
   if (!c) return PBF_E_INVALID;
@@ -1136,7 +1165,7 @@ int pbf_warm_start_stats(const pbf_ctx *c, uint64_t *learned_substeps, uint64_t 
 }
 
 int pbf_collect_training(pbf_ctx *c, float dt, uint32_t long_iterations, uint32_t iterations, float *features,
-                         float *targets, uint32_t capacity, uint32_t *count) {
+                         float *targets, uint32_t capacity, uint32_t *count) try {
   if (!c) return PBF_E_INVALID;
   if (!(dt > 0.0f) || long_iterations == 0) return fail(c, PBF_E_INVALID, "dt must be > 0 and long_iterations >= 1");
   if (long_iterations > kMaxIterations || iterations > kMaxIterations)
@@ -1172,29 +1201,29 @@ int pbf_collect_training(pbf_ctx *c, float dt, uint32_t long_iterations, uint32_
   const uint32_t iters = iterations ? iterations : c->cfg.iteration;
   if (c->warm_mode != PBF_WARM_PLAIN) return run_learned_substep(c, dt, iters);
   return run_substep(c, dt, iters, M_COLLECT, c->record_diag);
-}
+} PBF_CATCH(c)
 
-int pbf_run(pbf_ctx *c) {
+int pbf_run(pbf_ctx *c) try {
   if (!c) return PBF_E_INVALID;
   return pbf_step(c, (float)c->params.time_step, 1, c->cfg.iteration ? c->cfg.iteration : 1);
-}
+} PBF_CATCH(c)
 
 int pbf_sync(pbf_ctx *c) {
   if (!c) return PBF_E_INVALID;
   return sync_ctx(c);
 }
 
-int pbf_read_positions(pbf_ctx *c, float *xyzw, uint32_t count) {
+int pbf_read_positions(pbf_ctx *c, float *xyzw, uint32_t count) try {
   if (!c || (!xyzw && count)) return PBF_E_INVALID;
   return read_sorted4(c, c->pos, c->ids, xyzw, count);
-}
+} PBF_CATCH(c)
 
-int pbf_read_velocities(pbf_ctx *c, float *xyzw, uint32_t count) {
+int pbf_read_velocities(pbf_ctx *c, float *xyzw, uint32_t count) try {
   if (!c || (!xyzw && count)) return PBF_E_INVALID;
   return read_sorted4(c, c->vel, c->ids, xyzw, count, 1);
-}
+} PBF_CATCH(c)
 
-int pbf_read_field(pbf_ctx *c, uint32_t field, float *out4, uint32_t count) {
+int pbf_read_field(pbf_ctx *c, uint32_t field, float *out4, uint32_t count) try {
   if (!c || (!out4 && count)) return PBF_E_INVALID;
   const uint32_t *ids = c->last_was_search ? c->dbg_ids : c->ids;
   switch (field) {
@@ -1216,11 +1245,11 @@ int pbf_read_field(pbf_ctx *c, uint32_t field, float *out4, uint32_t count) {
     }
     default: return fail(c, PBF_E_INVALID, "unknown field");
   }
-}
+} PBF_CATCH(c)
 
 uint32_t pbf_num_cells(const pbf_ctx *c) { return c ? c->ncells : 0u; }
 
-int pbf_search(pbf_ctx *c, float dt) {
+int pbf_search(pbf_ctx *c, float dt) try {
   if (!c) return PBF_E_INVALID;
   if (!(dt > 0.0f)) return fail(c, PBF_E_INVALID, "dt must be > 0");
   HIPCHK(hipSetDevice(c->device));
@@ -1237,9 +1266,9 @@ int pbf_search(pbf_ctx *c, float dt) {
   c->have_dbg_counts = true;
   c->have_lists = true;
   return sync_ctx(c);
-}
+} PBF_CATCH(c)
 
-int pbf_read_cells(pbf_ctx *c, uint32_t *counts, uint32_t *offsets) {
+int pbf_read_cells(pbf_ctx *c, uint32_t *counts, uint32_t *offsets) try {
   if (!c) return PBF_E_INVALID;
   int rc = sync_ctx(c);
   if (rc != PBF_OK) return rc;
@@ -1254,9 +1283,9 @@ int pbf_read_cells(pbf_ctx *c, uint32_t *counts, uint32_t *offsets) {
     }
   }
   return PBF_OK;
-}
+} PBF_CATCH(c)
 
-int pbf_read_neighbors(pbf_ctx *c, uint32_t *nbr_off, uint32_t *nbr_ids, uint64_t capacity, uint64_t *pairs) {
+int pbf_read_neighbors(pbf_ctx *c, uint32_t *nbr_off, uint32_t *nbr_ids, uint64_t capacity, uint64_t *pairs) try {
   if (!c) return PBF_E_INVALID;
   int rc = sync_ctx(c);
   if (rc != PBF_OK) return rc;
@@ -1300,9 +1329,9 @@ int pbf_read_neighbors(pbf_ctx *c, uint32_t *nbr_off, uint32_t *nbr_ids, uint64_
     }
   }
   return PBF_OK;
-}
+} PBF_CATCH(c)
 
-int pbf_density_error(pbf_ctx *c, uint32_t scene, double *mean, double *maxv) {
+int pbf_density_error(pbf_ctx *c, uint32_t scene, double *mean, double *maxv) try {
   if (!c) return PBF_E_INVALID;
   int rc = sync_ctx(c);
   if (rc != PBF_OK) return rc;
@@ -1326,7 +1355,7 @@ int pbf_density_error(pbf_ctx *c, uint32_t scene, double *mean, double *maxv) {
     if (maxv) *maxv = m;
   }
   return PBF_OK;
-}
+} PBF_CATCH(c)
 
 int pbf_record_density_error(pbf_ctx *c, uint32_t enable) {
   if (!c) return PBF_E_INVALID;
@@ -1336,7 +1365,7 @@ int pbf_record_density_error(pbf_ctx *c, uint32_t enable) {
   return PBF_OK;
 }
 
-int pbf_read_density_history(pbf_ctx *c, double *mean_max, uint32_t capacity, uint32_t *count) {
+int pbf_read_density_history(pbf_ctx *c, double *mean_max, uint32_t capacity, uint32_t *count) try {
   if (!c) return PBF_E_INVALID;
   int rc = sync_ctx(c);
   if (rc != PBF_OK) return rc;
@@ -1355,7 +1384,7 @@ int pbf_read_density_history(pbf_ctx *c, double *mean_max, uint32_t capacity, ui
   }
   if (count) *count = take;
   return PBF_OK;
-}
+} PBF_CATCH(c)
 
 uint32_t pbf_particle_count(const pbf_ctx *c) { return c ? c->n : 0u; }
 uint32_t pbf_scene_count(const pbf_ctx *c) { return c ? (uint32_t)c->scenes.size() : 0u; }
@@ -1365,7 +1394,7 @@ const char *pbf_last_error(const pbf_ctx *c) {
 }
 
 int pbf_pass_times(pbf_ctx *c, double *ms, uint32_t *launches, uint32_t capacity, uint32_t *count,
-                   const char **names) {
+                   const char **names) try {
   if (!c) return PBF_E_INVALID;
   int rc = sync_ctx(c);
   if (rc != PBF_OK) return rc;
@@ -1379,7 +1408,7 @@ int pbf_pass_times(pbf_ctx *c, double *ms, uint32_t *launches, uint32_t capacity
   if (count) *count = P_NPASS;
   if (names) *names = kPassNames;
   return PBF_OK;
-}
+} PBF_CATCH(c)
 
 int pbf_copy_bandwidth(size_t bytes, uint32_t iterations, double *gbps) {
   pbf_ctx *c = nullptr;

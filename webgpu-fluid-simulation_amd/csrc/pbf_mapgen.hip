@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <exception>
 #include <vector>
 
 #include "../../include/pbf.h"
@@ -96,7 +97,7 @@ __global__ void __launch_bounds__(256) k_volume_map(MapArgs A, const double *__r
 
 extern "C" int pbf_build_volume_map_gpu(uint32_t kind, const double *shape, const int32_t res[3], const double lo[3],
                                         const double hi[3], double kernel_radius, double particle_radius,
-                                        float *sdf_vol) {
+                                        float *sdf_vol) try {
   if (!res || !lo || !hi || !sdf_vol || kind > 1 || !(kernel_radius > 0.0)) return PBF_E_INVALID;
   for (int a = 0; a < 3; ++a)
     if (res[a] < 1 || res[a] > 65536) return PBF_E_INVALID;
@@ -182,4 +183,6 @@ extern "C" int pbf_build_volume_map_gpu(uint32_t kind, const double *shape, cons
   if (dout) (void)hipFree(dout);
   if (st) (void)hipStreamDestroy(st);
   return rc;
+} catch (const std::exception &) {  // host vectors of the quadrature: a status, not an exception
+  return PBF_E_CAPACITY;
 }
