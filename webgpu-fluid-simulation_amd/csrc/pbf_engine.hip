@@ -1318,14 +1318,19 @@ int pbf_read_neighbors(pbf_ctx *c, uint32_t *nbr_off, uint32_t *nbr_ids, uint64_
     if (nbr_off) nbr_off[q] = (uint32_t)tot;
     tot += by_id[q];
   }
-  if (nbr_off) nbr_off[n] = (uint32_t)tot;
   if (pairs) *pairs = tot;
+  if (nbr_off && tot > 0xffffffffull) return fail(c, PBF_E_CAPACITY, "neighbour offsets exceed 32 bits");
+  if (nbr_off) nbr_off[n] = (uint32_t)tot;
   if (nbr_ids) {
     if (capacity < tot) return fail(c, PBF_E_CAPACITY, "neighbour id buffer too small");
     uint64_t w = 0;
     for (uint32_t q = 0; q < n; ++q) {
       uint32_t s = slot_of[q];
-      for (uint32_t k = 0; k < cnt[s]; ++k) nbr_ids[w++] = ids[entry(s, k)];
+      for (uint32_t k = 0; k < cnt[s]; ++k) {
+        const uint32_t e = entry(s, k);
+        if (e >= n) return fail(c, PBF_E_STATE, "neighbour table entry out of range");
+        nbr_ids[w++] = ids[e];
+      }
     }
   }
   return PBF_OK;
