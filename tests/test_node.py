@@ -31,6 +31,21 @@ def test_addon_exports_reference_api():
 
 
 @needs_node
+def test_failed_create_throws_the_engine_message():
+    # parameter validation runs before any device call: the thrown Error
+    # carries pbf_last_error(NULL); NaN counts saturate and are refused
+    js = ("const {PBF}=require(%r); const out=[];"
+          "for (const p of [{particleRadius: NaN}, {particleRadius: 0.025, capacity: NaN},"
+          "                 {particleRadius: 0.025, gridMode: 7}]) {"
+          "  try { new PBF(p); out.push('created'); } catch (e) { out.push(e.message); } }"
+          "console.log(JSON.stringify(out))") % os.path.join(NODE_DIR, "PBF.js")
+    out = json.loads(subprocess.run([NODE, "-e", js], capture_output=True, text=True, check=True).stdout)
+    assert "particle_radius must be finite" in out[0]
+    assert "capacity above" in out[1]
+    assert "grid_mode must be" in out[2]
+
+
+@needs_node
 @pytest.mark.gpu
 def test_node_cube_drop_equals_python(tmp_path):
     import scenes

@@ -605,6 +605,8 @@ int create_ctx_body(const pbf_params *params, uint32_t nscenes, const pbf_scene_
   if (p.time_step <= 0.0) p.time_step = 1.0 / 300.0;
   if (p.max_neighbors == 0) p.max_neighbors = 96;
   if (p.max_neighbors > 65535u) return reject(PBF_E_INVALID, "max_neighbors must be below 65536");
+  if (p.grid_mode != PBF_GRID_DENSE && p.grid_mode != PBF_GRID_HASH)
+    return reject(PBF_E_INVALID, "grid_mode must be PBF_GRID_DENSE or PBF_GRID_HASH");
   if (p.grid_mode == PBF_GRID_HASH && (p.hash_log2 < 4 || p.hash_log2 > 28))
     return reject(PBF_E_INVALID, "hash_log2 must be in [4, 28]");
   if (p.capacity > kMaxCapacity)  // 32-bit gather offsets (slot * 16 bytes)

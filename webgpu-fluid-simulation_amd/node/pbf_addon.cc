@@ -9,6 +9,7 @@
 #include <node_api.h>
 
 #include <cstdint>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -113,10 +114,16 @@ napi_value new_float_array(napi_env env, size_t n, float **data) {
   return arr;
 }
 
+// A JS number as uint32; NaN, negative or too large values become 0xffffffff,
+// which every uint32 parameter of pbf_params rejects or ignores.
+uint32_t as_u32(double v) { return (v >= 0.0 && v <= 4294967295.0) ? (uint32_t)v : 0xffffffffu; }
+// A JS number as a uint64 seed: NaN and negative values are 0, huge ones saturate.
+uint64_t as_u64(double v) { return v >= 18446744073709549568.0 ? ~0ull : (v >= 0.0 ? (uint64_t)v : 0ull); }
+
 void read_params(napi_env env, napi_value o, pbf_params &p) {
   std::memset(&p, 0, sizeof(p));
-  p.capacity = (uint32_t)prop(env, o, "capacity", 1 << 17);
-  p.max_neighbors = (uint32_t)prop(env, o, "maxNeighbors", 0);
+  p.capacity = as_u32(prop(env, o, "capacity", 1 << 17));
+  p.max_neighbors = as_u32(prop(env, o, "maxNeighbors", 0));
   p.particle_radius = prop(env, o, "particleRadius", 0.006);
   p.kernel_radius = prop(env, o, "kernelRadius", 0.025);
   p.rest_density = prop(env, o, "restDensity", 1000.0);
@@ -128,9 +135,9 @@ void read_params(napi_env env, napi_value o, pbf_params &p) {
   napi_has_named_property(env, o, "domainHi", &has);
   if (has) { napi_value v; napi_get_named_property(env, o, "domainHi", &v); vec3(env, v, hi); }
   for (int a = 0; a < 3; ++a) { p.domain_lo[a] = lo[a]; p.domain_hi[a] = hi[a]; }
-  p.grid_mode = (uint32_t)prop(env, o, "gridMode", PBF_GRID_DENSE);
-  p.hash_log2 = (uint32_t)prop(env, o, "hashLog2", 22);
-  p.flags = (uint32_t)prop(env, o, "flags", 0);
+  p.grid_mode = as_u32(prop(env, o, "gridMode", PBF_GRID_DENSE));
+  p.hash_log2 = as_u32(prop(env, o, "hashLog2", 22));
+  p.flags = as_u32(prop(env, o, "flags", 0));
 }
 
 void finalize_ctx(napi_env, void *data, void *) { pbf_destroy(static_cast<pbf_ctx *>(data)); }
@@ -163,7 +170,10 @@ napi_value BatchCreate(napi_env env, napi_callback_info info) {
   pbf_params p;
   read_params(env, a.v[0], p);
   uint32_t ns = 0;
-  napi_get_array_length(env, a.v[1], &ns);
+  if (napi_get_array_length(env, a.v[1], &ns) != napi_ok || ns == 0 || ns > (1u << 20)) {
+    napi_throw_range_error(env, nullptr, "scenes must be an array of 1 to 2^20 scene descriptions");
+    return nullptr;
+  }
   std::vector<pbf_scene_desc> d(ns);
   for (uint32_t k = 0; k < ns; ++k) {
     napi_value s, v;
@@ -173,7 +183,7 @@ napi_value BatchCreate(napi_env env, napi_callback_info info) {
     vec3(env, v, d[k].domain_lo);
     napi_get_named_property(env, s, "domainHi", &v);
     vec3(env, v, d[k].domain_hi);
-    d[k].particle_count = (uint32_t)prop(env, s, "count", 0);
+    d[k].particle_count = as_u32(prop(env, s, "count", 0));
   }
   pbf_ctx *c = nullptr;
   int rc = pbf_batch_create(&p, ns, d.data(), &c);
@@ -205,7 +215,7 @@ napi_value Reset(napi_env env, napi_callback_info info) {
 napi_value SetConfig(napi_env env, napi_callback_info info) {
   CTX_OR_RETURN(0);
   pbf_config cfg;
-  cfg.iteration = (uint32_t)prop(env, a.v[1], "iteration", 5);
+  cfg.iteration = as_u32(prop(env, a.v[1], "iteration", 5));
   cfg.xsph = (float)prop(env, a.v[1], "XSPH", 0.01);
   cfg.vorticity = (float)prop(env, a.v[1], "vorticity", 0.1);
   cfg.surface_tension = (float)prop(env, a.v[1], "surfaceTension", 0.5);
@@ -220,7 +230,7 @@ napi_value VoxelizeCube(napi_env env, napi_callback_info info) {
   vec3(env, a.v[1], mn);
   vec3(env, a.v[2], mx);
   double jitter = a.n > 3 ? num(env, a.v[3]) : 0.0;
-  uint64_t seed = a.n > 4 ? (uint64_t)num(env, a.v[4]) : 0;
+  uint64_t seed = a.n > 4 ? as_u64(num(env, a.v[4])) : 0;
   uint32_t added = 0;
   CHECK(pbf_voxelize_cube(c, mn, mx, jitter, seed, &added));
   napi_value r;
@@ -234,7 +244,7 @@ napi_value VoxelizeSphere(napi_env env, napi_callback_info info) {
   vec3(env, a.v[1], ctr);
   double radius = num(env, a.v[2]);
   double jitter = a.n > 3 ? num(env, a.v[3]) : 0.0;
-  uint64_t seed = a.n > 4 ? (uint64_t)num(env, a.v[4]) : 0;
+  uint64_t seed = a.n > 4 ? as_u64(num(env, a.v[4])) : 0;
   uint32_t added = 0;
   CHECK(pbf_voxelize_sphere(c, ctr, radius, jitter, seed, &added));
   napi_value r;
@@ -289,7 +299,7 @@ napi_value VoxelizeMesh(napi_env env, napi_callback_info info) {
     mp = m;
   }
   double jitter = a.n > 4 ? num(env, a.v[4]) : 0.0;
-  uint64_t seed = a.n > 5 ? (uint64_t)num(env, a.v[5]) : 0;
+  uint64_t seed = a.n > 5 ? as_u64(num(env, a.v[5])) : 0;
   uint32_t added = 0;
   CHECK(pbf_voxelize_mesh(c, vp, (uint32_t)(vlen / 3), idx.data(), (uint32_t)(idx.size() / 3), mp, jitter, seed,
                           &added));
@@ -306,10 +316,10 @@ napi_value SetWarmStart(napi_env env, napi_callback_info info) {
   napi_valuetype t = napi_undefined;
   napi_typeof(env, a.v[1], &t);
   if (t == napi_object && !float_array(env, a.v[1], &bp, &blen)) return nullptr;
-  uint32_t mode = (uint32_t)num(env, a.v[2], 1);
+  uint32_t mode = as_u32(num(env, a.v[2], 1));
   float relax = (float)num(env, a.v[3], 1.0);
   double thr = num(env, a.v[4], 0.0);
-  uint32_t fb = (uint32_t)num(env, a.v[5], 0);
+  uint32_t fb = as_u32(num(env, a.v[5], 0));
   CHECK(pbf_set_warm_start(c, bp, blen, mode, relax, thr, fb));
   return undefined(env);
 }
@@ -332,8 +342,8 @@ napi_value tuning_object(napi_env env, const pbf_tuning &t) {
 napi_value Autotune(napi_env env, napi_callback_info info) {
   CTX_OR_RETURN(0);
   pbf_tuning t;
-  CHECK(pbf_autotune(c, (float)num(env, a.v[1], 1.0 / 300.0), (uint32_t)num(env, a.v[2], 0),
-                     (uint32_t)num(env, a.v[3], 3), &t));
+  CHECK(pbf_autotune(c, (float)num(env, a.v[1], 1.0 / 300.0), as_u32(num(env, a.v[2], 0)),
+                     as_u32(num(env, a.v[3], 3)), &t));
   return tuning_object(env, t);
 }
 
@@ -344,9 +354,9 @@ napi_value SetTuning(napi_env env, napi_callback_info info) {
   CHECK(pbf_get_tuning(c, &t));
   for (uint32_t k = 0; k < 6; ++k) {
     napi_value e;
-    if (napi_get_element(env, a.v[1], k, &e) == napi_ok) t.walk_block[k] = (uint32_t)num(env, e, 256);
+    if (napi_get_element(env, a.v[1], k, &e) == napi_ok) t.walk_block[k] = as_u32(num(env, e, 256));
   }
-  t.list_variant = (uint32_t)num(env, a.v[2], 0);
+  t.list_variant = as_u32(num(env, a.v[2], 0));
   CHECK(pbf_set_tuning(c, &t));
   return undefined(env);
 }
@@ -357,7 +367,7 @@ napi_value SetParticlePosition(napi_env env, napi_callback_info info) {
     float *d;
     size_t len;
     if (!float_array(env, a.v[1], &d, &len)) return nullptr;
-    uint32_t count = (uint32_t)num(env, a.v[2]);
+    uint32_t count = as_u32(num(env, a.v[2]));
     CHECK(pbf_set_positions(c, d, len, count));
   } else {
     CHECK(pbf_set_particle_position(c));
@@ -380,7 +390,7 @@ napi_value SetBoundaryData(napi_env env, napi_callback_info info) {
 
 napi_value SetVolumeMap(napi_env env, napi_callback_info info) {
   CTX_OR_RETURN(0);
-  uint32_t scene = (uint32_t)num(env, a.v[1]);
+  uint32_t scene = as_u32(num(env, a.v[1]));
   double r[3], lo[3], hi[3];
   vec3(env, a.v[2], r);
   vec3(env, a.v[3], lo);
@@ -388,8 +398,13 @@ napi_value SetVolumeMap(napi_env env, napi_callback_info info) {
   float *d;
   size_t len;
   if (!float_array(env, a.v[5], &d, &len)) return nullptr;
+  for (int k = 0; k < 3; ++k)
+    if (!(r[k] >= 1.0 && r[k] <= 65536.0) || r[k] != std::floor(r[k])) {
+      napi_throw_range_error(env, nullptr, "volume map resolution must be integers in [1, 65536]");
+      return nullptr;
+    }
   int32_t res[3] = {(int32_t)r[0], (int32_t)r[1], (int32_t)r[2]};
-  size_t need = 2 * (size_t)(res[0] + 1) * (res[1] + 1) * (res[2] + 1);
+  size_t need = 2 * ((size_t)res[0] + 1) * ((size_t)res[1] + 1) * ((size_t)res[2] + 1);
   if (len != need) {
     napi_throw_range_error(env, nullptr, "volume map length must be 2 (res+1)^3");
     return nullptr;
@@ -401,8 +416,8 @@ napi_value SetVolumeMap(napi_env env, napi_callback_info info) {
 napi_value Step(napi_env env, napi_callback_info info) {
   CTX_OR_RETURN(0);
   float dt = (float)num(env, a.v[1], 1.0 / 300.0);
-  uint32_t sub = a.n > 2 ? (uint32_t)num(env, a.v[2], 1) : 1;
-  uint32_t it = a.n > 3 ? (uint32_t)num(env, a.v[3], 0) : 0;
+  uint32_t sub = a.n > 2 ? as_u32(num(env, a.v[2], 1)) : 1;
+  uint32_t it = a.n > 3 ? as_u32(num(env, a.v[3], 0)) : 0;
   CHECK(pbf_step(c, dt, sub, it));
   return undefined(env);
 }
@@ -421,7 +436,7 @@ napi_value Sync(napi_env env, napi_callback_info info) {
 
 napi_value ReadField(napi_env env, napi_callback_info info) {
   CTX_OR_RETURN(0);
-  uint32_t field = (uint32_t)num(env, a.v[1]);
+  uint32_t field = as_u32(num(env, a.v[1]));
   uint32_t n = pbf_particle_count(c);
   float *d = nullptr;
   napi_value arr = new_float_array(env, 4 * (size_t)n, &d);
@@ -432,7 +447,7 @@ napi_value ReadField(napi_env env, napi_callback_info info) {
 
 napi_value DensityError(napi_env env, napi_callback_info info) {
   CTX_OR_RETURN(0);
-  uint32_t scene = a.n > 1 ? (uint32_t)num(env, a.v[1], 4294967295.0) : 0xFFFFFFFFu;
+  uint32_t scene = a.n > 1 ? as_u32(num(env, a.v[1], 4294967295.0)) : 0xFFFFFFFFu;
   double m = 0, x = 0;
   CHECK(pbf_density_error(c, scene, &m, &x));
   napi_value arr, vm, vx;
