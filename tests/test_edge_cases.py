@@ -186,3 +186,21 @@ def test_argument_limits():
     sim.close()
     with pytest.raises(PBFError):
         PBF(particle_radius=0.025, kernel_radius=0.1, capacity=1 << 28)
+
+
+def test_boundary_field_limits():
+    # field resolutions beyond the samplers' 32-bit index range and
+    # non-finite or empty field boxes are refused; the context keeps working
+    sim = PBF(particle_radius=0.025, kernel_radius=0.1, capacity=256)
+    sim.setParticlePosition(np.zeros((4, 4), np.float32) + 0.5)
+    one = np.zeros(2 * 2 * 2 * 2, np.float32)
+    with pytest.raises(PBFError, match="resolution"):
+        sim.setVolumeMap((70000, 1, 1), (0, 0, 0), (1, 1, 1), one)
+    with pytest.raises(PBFError, match="field box"):
+        sim.setVolumeMap((1, 1, 1), (0, 0, 0), (float("nan"), 1, 1), one)
+    with pytest.raises(PBFError, match="field box"):
+        sim.setVolumeMap((1, 1, 1), (0, 0, 0), (0, 1, 1), one)
+    sim.setVolumeMap((1, 1, 1), (0, 0, 0), (1, 1, 1), one)
+    sim.step(1.0 / 60.0, 1, 2)
+    sim.sync()
+    sim.close()
