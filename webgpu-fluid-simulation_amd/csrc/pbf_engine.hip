@@ -352,8 +352,11 @@ int run_substep(pbf_ctx *c, float dt, uint32_t iters, uint32_t mode, bool diag) 
     enqueue_substep(c, L, iters, diag, T, mode);
     hipError_t le = hipGetLastError();
     hipError_t ee = hipStreamEndCapture(c->stream, &graph);
-    if (le != hipSuccess) return fail(c, PBF_E_DEVICE, std::string("launch during capture: ") + hipGetErrorString(le));
-    if (ee != hipSuccess) return fail(c, PBF_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+    if (le != hipSuccess || ee != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      if (le != hipSuccess) return fail(c, PBF_E_DEVICE, std::string("launch during capture: ") + hipGetErrorString(le));
+      return fail(c, PBF_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+    }
     hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (ie != hipSuccess) return fail(c, PBF_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
