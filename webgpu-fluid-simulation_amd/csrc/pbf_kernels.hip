@@ -809,6 +809,11 @@ __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__
   if constexpr (ACC) lam_acc[i] += lam;
 }
 
+struct FeatRec {
+  float3 x;  // neighbour position
+  float4 v;  // neighbour velocity of the previous substep
+};
+
 // Iteration 0 of the learned and data-generation modes: the lambda pass plus
 // the 16 features of the warm-start network (previous-substep lambdas,
 // density, constraint gradient terms, boundary, velocity divergence).
@@ -833,17 +838,20 @@ __global__ void __launch_bounds__(BS) k_lambda_feat(uint32_t n, float4 *X, float
   const float4 vi = lb.vel_prev[i];
   const uint32_t cnt = ncount[i];
   float gx = 0.0f, gy = 0.0f, gz = 0.0f, s2 = 0.0f, rho = 0.0f, dv = 0.0f;
-  for_each_neighbor<2>(nbr, i, P, cnt, [&](uint32_t j) {
-    const float4 xj = X[j];
-    const float4 vj = lb.vel_prev[j];
-    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+  // the lambda pass's walk (same order, so the same lambda bit for bit) with
+  // the previous velocity gathered too; padding entries (j = i) have a zero
+  // gradient and velocity difference, and their poly6 term is not added
+  const Gather GX(X, n), GV(lb.vel_prev, n);
+  for_each_neighbor_w<4>(nbr, i, P, cnt, [&](uint32_t j) { return FeatRec{GX.xyz(j), GV[j]}; },
+                         [&](const FeatRec &r, bool valid) {
+    float dx = xi.x - r.x.x, dy = xi.y - r.x.y, dz = xi.z - r.x.z;
     float d2 = dist2(dx, dy, dz);
     float f = spiky_f(P, d2);
     float ax = f * dx, ay = f * dy, az = f * dz;
     gx += ax; gy += ay; gz += az;
     s2 = fmaf(az, az, fmaf(ay, ay, fmaf(ax, ax, s2)));
-    rho += poly6(P, d2);
-    dv = fmaf(vj.x - vi.x, ax, fmaf(vj.y - vi.y, ay, fmaf(vj.z - vi.z, az, dv)));
+    rho += valid ? poly6(P, d2) : 0.0f;
+    dv = fmaf(r.v.x - vi.x, ax, fmaf(r.v.y - vi.y, ay, fmaf(r.v.z - vi.z, az, dv)));
   });
   rho *= P.V;
   gx *= P.V; gy *= P.V; gz *= P.V;

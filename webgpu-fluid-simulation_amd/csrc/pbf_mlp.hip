@@ -44,7 +44,14 @@ __device__ __forceinline__ short f2bf(float f) {
   u += 0x7FFFu + ((u >> 16) & 1u);
   return (short)(u >> 16);
 }
-__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + expf(-z)); }
+// Logistic function with the hardware exp2 and reciprocal (each about 1 ulp):
+// the network is checked against its CPU model to 2e-3, far above this, and the
+// library-wide correctly rounded division would make the 64 activations per
+// lane and tile, not the matrix products, the cost of a tile.  Large |z|
+// saturates to 0 or 1 (exp2 -> inf or 0).
+__device__ __forceinline__ float sigm(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * -1.44269504088896341f));
+}
 __device__ __forceinline__ int crow(int i, int h) { return 8 * (i >> 2) + 4 * h + (i & 3); }
 
 // Forward pass of one 32-particle tile.  Lane l: particle l % 32, half h = l / 32.
@@ -63,7 +70,9 @@ __device__ __forceinline__ void tile_forward(const MlpDev &M, const float *__res
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int f = ks * 8 + h * 4 + r;
-      float v = valid ? (feats[(size_t)p * 16 + f] - M.misc[OFF_FMU - OFF_B3 + f]) / M.misc[OFF_FSIG - OFF_B3 + f] : 0.0f;
+      float v = valid ? (feats[(size_t)p * 16 + f] - M.misc[OFF_FMU - OFF_B3 + f]) *
+                            __builtin_amdgcn_rcpf(M.misc[OFF_FSIG - OFF_B3 + f])
+                      : 0.0f;
       T.xb[ks][r] = f2bf(v);
     }
   float part = 0.0f;
@@ -108,27 +117,108 @@ __device__ __forceinline__ void tile_forward(const MlpDev &M, const float *__res
 // Ratio mode (smax > 0): the output is a factor s on the previous substep's
 // lambda sum (feature 3), clamped to [0, smax]: lambda = s * Lambda_prev; a
 // network that outputs the constant relax is the temporal warm start.
+// The parameters one lane of an inference wavefront uses, loaded once before
+// its tile loop (they are the same for every tile): the A operands of both
+// layers and the biases / output weights of its accumulator rows.
+struct InferConst {
+  s4 a1[2][2], a2[2][8];
+  float b1[2][16], b2[2][16], w3[2][16];
+  float fmu[8], fisig[8];
+};
+
+__device__ __forceinline__ void load_infer_const(const MlpDev &M, InferConst &K) {
+  const int l = __lane_id(), h = l >> 5, c = l & 31;
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt) {
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) K.a1[mt][ks] = *reinterpret_cast<const s4 *>(M.W1T + (mt * 32 + c) * 16 + ks * 8 + h * 4);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) K.a2[mt][q] = *reinterpret_cast<const s4 *>(M.W2T + (mt * 32 + c) * 64 + q * 8 + h * 4);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int hid = mt * 32 + crow(i, h);
+      K.b1[mt][i] = M.b1[hid];
+      K.b2[mt][i] = M.b2[hid];
+      K.w3[mt][i] = M.w3[hid];
+    }
+  }
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int f = ks * 8 + h * 4 + r;
+      K.fmu[ks * 4 + r] = M.misc[OFF_FMU - OFF_B3 + f];
+      K.fisig[ks * 4 + r] = __builtin_amdgcn_rcpf(M.misc[OFF_FSIG - OFF_B3 + f]);
+    }
+}
+
+// tile_forward's arithmetic with the parameters already in registers; returns
+// the network output (normalised units) of this lane's particle.
+__device__ __forceinline__ float tile_forward_infer(const InferConst &K, float b3, const float *__restrict__ feats,
+                                                    uint32_t p, bool valid) {
+  const int l = __lane_id(), h = l >> 5;
+  s4 xb[2];
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks) {
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (valid) v = *reinterpret_cast<const float4 *>(feats + (size_t)p * 16 + ks * 8 + h * 4);
+    xb[ks][0] = f2bf(valid ? (v.x - K.fmu[ks * 4 + 0]) * K.fisig[ks * 4 + 0] : 0.0f);
+    xb[ks][1] = f2bf(valid ? (v.y - K.fmu[ks * 4 + 1]) * K.fisig[ks * 4 + 1] : 0.0f);
+    xb[ks][2] = f2bf(valid ? (v.z - K.fmu[ks * 4 + 2]) * K.fisig[ks * 4 + 2] : 0.0f);
+    xb[ks][3] = f2bf(valid ? (v.w - K.fmu[ks * 4 + 3]) * K.fisig[ks * 4 + 3] : 0.0f);
+  }
+  s4 a1b[8];
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt) {
+    f16v acc = {0};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(K.a1[mt][ks], xb[ks], acc, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const float z = acc[i] + K.b1[mt][i];
+      a1b[mt * 4 + (i >> 2)][i & 3] = f2bf(z * sigm(z));
+    }
+  }
+  float part = 0.0f;
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt) {
+    f16v acc = {0};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(K.a2[mt][q], a1b[q], acc, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const float z = acc[i] + K.b2[mt][i];
+      part = fmaf(K.w3[mt][i], z * sigm(z), part);
+    }
+  }
+  part += __shfl_xor(part, 32, 64);
+  return part + b3;
+}
+
 __global__ void __launch_bounds__(256) k_mlp_infer(uint32_t n, const float *__restrict__ feats, MlpDev M,
                                                    float4 *X, float *__restrict__ lam_acc, float smax) {
   const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int l = __lane_id();
   const uint32_t tiles = (n + 31) / 32;
+  InferConst K;
+  load_infer_const(M, K);
+  const float b3 = M.misc[0], tsig = M.misc[OFF_TSIG - OFF_B3], tmu = M.misc[OFF_TMU - OFF_B3];
+  const float lmin = M.misc[OFF_LMIN - OFF_B3];
   for (uint32_t t = wave; t < tiles; t += waves) {
     const uint32_t p = t * 32 + (l & 31);
     const bool valid = p < n;
-    TileAct T;
-    tile_forward(M, feats, p, valid, T);
+    const float y = tile_forward_infer(K, b3, feats, p, valid);
     if (valid && l < 32) {
       // lambda = -C / (den + eps) is never positive; a non-finite or positive
       // prediction keeps the plain lambda already in X[p].w
       float *wp = reinterpret_cast<float *>(X) + 4 * (size_t)p + 3;
-      float lam = fmaf(T.y, M.misc[OFF_TSIG - OFF_B3], M.misc[OFF_TMU - OFF_B3]);
+      float lam = fmaf(y, tsig, tmu);
       if (smax > 0.0f) {
         lam = fminf(fmaxf(lam, 0.0f), smax) * feats[(size_t)p * 16 + 3];  // fmaxf(NaN, 0) = 0
       } else {
         if (!(lam <= 0.0f) || !isfinite(lam)) lam = *wp;
-        lam = fmaxf(lam, M.misc[OFF_LMIN - OFF_B3]);
+        lam = fmaxf(lam, lmin);
       }
       *wp = lam;
       if (lam_acc) lam_acc[p] = lam;
@@ -415,7 +505,9 @@ void launch_mlp_infer(hipStream_t st, void *handle, uint32_t n, const float *fea
   MlpBuffers *B = static_cast<MlpBuffers *>(handle);
   if (!B || !n) return;
   const uint32_t tiles = (n + 31) / 32;
-  const uint32_t blocks = std::min<uint32_t>((tiles + 3) / 4, 2048u);
+  // one resident round (2 wavefronts per SIMD at this kernel's register use,
+  // 256 CUs): each wavefront loads the parameters once for all its tiles
+  const uint32_t blocks = std::min<uint32_t>((tiles + 3) / 4, 512u);
   if (e0 || e1)
     hipExtLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, e0, e1, 0, n, feats, mlp_view(*B), X, lam_acc,
                           smax);
