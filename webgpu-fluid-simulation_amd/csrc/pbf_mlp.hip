@@ -38,12 +38,9 @@ enum : uint32_t {
   OFF_FMU = 5313, OFF_FSIG = 5329, OFF_TMU = 5345, OFF_TSIG = 5346, OFF_LMIN = 5347, BLOB = 5348, NPARAM = 5313
 };
 
-// float -> bfloat16, round to nearest even
-__device__ __forceinline__ short f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
-}
+// float -> bfloat16, round to nearest even (gfx950's v_cvt_pk_bf16_f32; the
+// same bits as adding 0x7fff + lsb and truncating, for every finite value)
+__device__ __forceinline__ short f2bf(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 // Logistic function with the hardware exp2 and reciprocal (each about 1 ulp):
 // the network is checked against its CPU model to 2e-3, far above this, and the
 // library-wide correctly rounded division would make the 64 activations per
