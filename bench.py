@@ -24,7 +24,7 @@ sys.path[:0] = [os.path.join(ROOT, "webgpu-fluid-simulation_amd"), os.path.join(
 import numpy as np  # noqa: E402
 
 import scenes  # noqa: E402
-from pbf_host import PBF_FLAG_TIMING, copy_bandwidth  # noqa: E402
+from pbf_host import PBF_FLAG_TIMING, PBF_GRID_HASH, copy_bandwidth  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
@@ -68,6 +68,17 @@ def cpu_baseline(sc, budget_s: float):
                       f"{el:.1f} s, OpenMP {threads} threads"}
 
 
+# measured scenes: constructor and description
+WORKLOADS = {
+    "dam_break_1m": (scenes.dam_break_1m,
+                     "dam break: 1,048,576 particles (128x128x64), r=0.025, h=0.1, 12.8^3 box, 128^3 trilinear "
+                     "box volume map, dt=1/60 x 2 substeps, 4 Jacobi iterations, XSPH 0.01, vorticity 0.01, "
+                     "surface tension 0.1"),
+    "obstacle_drop_4m": (scenes.obstacle_drop_4m,
+                         "obstacle drop: 4,194,304 particles (128x256x128) onto sphere + torus, r=0.025, h=0.1, "
+                         "hashed grid of 2^22 buckets, 256^3 trilinear volume map, dt=1/60 x 2 substeps, "
+                         "4 Jacobi iterations, XSPH 0.01"),
+}
 WARM_BLOB = os.path.join(ROOT, "webgpu-fluid-simulation_amd", "warm_start", "dam_break.f32")
 RATIO_BLOB = os.path.join(ROOT, "webgpu-fluid-simulation_amd", "warm_start", "dam_break_ratio.f32")
 PROFILE = os.path.join(ROOT, "profiles", "dam_break_1m_kernels_and_traffic.md")
@@ -229,17 +240,26 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the plain vs learned warm-start comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-scene batched measurement")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch setting (no autotuning)")
+    ap.add_argument("--workload", default="dam_break_1m", choices=sorted(WORKLOADS),
+                    help="the measured scene (the default is the headline workload; the other is the "
+                         "hashed-grid parity configuration, measured without the warm-start and batch sections)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine runs on a single GPU and has no multi-GPU path.",
               file=sys.stderr)
         sys.exit(2)
 
-    sc = scenes.dam_break_1m()
+    headline = args.workload == "dam_break_1m"
+    if not headline:
+        args.no_learned = args.no_batch = True
+    sc = WORKLOADS[args.workload][0]()
     N = sc.n
     B = 1 if sc.field is not None else 0
-    C = int(np.prod([int(np.ceil((sc.domain_hi[a] - sc.domain_lo[a]) / sc.kernel_radius - 1e-7))
-                     for a in range(3)]))
+    if sc.grid_mode == PBF_GRID_HASH:
+        C = 1 << sc.hash_log2
+    else:
+        C = int(np.prod([int(np.ceil((sc.domain_hi[a] - sc.domain_lo[a]) / sc.kernel_radius - 1e-7))
+                         for a in range(3)]))
     log(f"scene {sc.name}: N={N} cells={C} I={sc.iterations} substeps/step={sc.substeps}")
     # Timed region 1 (value): the engine's default mode, one hipGraph replay
     # per substep.  Timed region 2 (roofline): the same steps from the same
@@ -293,14 +313,14 @@ def main():
         avg_s = per_pass[dom]["ms_total"] / per_pass[dom]["launches"] / 1e3
         bytes_per_launch = pb[dom] * N
         ach = bytes_per_launch / avg_s / 1e9
-        traffic = profiled_traffic(dom)
+        traffic = profiled_traffic(dom) if headline else None  # the committed profiles cover the headline scene
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": round(avg_s * 1e6, 2),
                 "traffic_GBs": round(traffic / avg_s / 1e9, 1) if traffic else None,
                 "traffic_source": "profiles/dam_break_1m_kernels_and_traffic.md (rocprofv3 FETCH_SIZE x2, "
                                   "WRITE_SIZE x1 per dispatch)" if traffic else None}
-        valu = profiled_valu(dom)
+        valu = profiled_valu(dom) if headline else None
         if valu:
             # the kernel's own ceiling: a wave64 VALU instruction occupies a
             # 16-lane SIMD for 4 cycles, at the clock the profile measured
@@ -361,9 +381,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded lattice dam break)",
-        "config": {"workload": "dam break: 1,048,576 particles (128x128x64), r=0.025, h=0.1, "
-                               "12.8^3 box, 128^3 trilinear box volume map, dt=1/60 x 2 substeps, 4 Jacobi "
-                               "iterations, XSPH 0.01, vorticity 0.01, surface tension 0.1",
+        "config": {"workload": WORKLOADS[args.workload][1],
                    "particles": N, "substeps_per_step": sc.substeps, "iterations": sc.iterations,
                    "grid_cells": C, "parallelism": "single GPU"},
         "roofline": roof,
