@@ -5,6 +5,7 @@ import math
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -225,3 +226,11 @@ def test_create_rejects_bad_parameters_with_a_message():
         kw.setdefault("capacity", 1024)
         with pytest.raises(pbf_host.PBFError, match=msg):
             pbf_host.PBF(**kw)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # the engine is single-GPU: bench.py --gpus N > 1 exits with a message
+    # before touching a device
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
