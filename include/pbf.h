@@ -208,6 +208,14 @@ int pbf_read_field(pbf_ctx *ctx, uint32_t field, float *out4, uint32_t count);
  * counts/offsets may be NULL.  Neighbour lists are in original particle ids,
  * ordered as walked (27 cells x-major, ascending id inside a cell). */
 uint32_t pbf_num_cells(const pbf_ctx *ctx);
+/* GPU-free: 1 when, on a hashed grid of dims[0] x dims[1] x dims[2] cells
+ * with 2^hash_log2 buckets (bucket = row-major cell index mod 2^hash_log2),
+ * the 27 cells around every cell lie in 27 different buckets; 0 otherwise.
+ * The neighbour search takes its pruned per-column walk only then, and
+ * otherwise visits each repeated bucket once, at its first occurrence in the
+ * x-major 27-cell order (the reference's dense grid, neighborShader.ts:64-154,
+ * never repeats a cell).  No reference counterpart. */
+int pbf_hash_buckets_distinct(const int32_t dims[3], uint32_t hash_log2);
 int pbf_read_cells(pbf_ctx *ctx, uint32_t *counts, uint32_t *offsets);
 int pbf_read_neighbors(pbf_ctx *ctx, uint32_t *nbr_off, uint32_t *nbr_ids, uint64_t capacity, uint64_t *pairs);
 /* Predict + neighbour search only; the state is not advanced. */

@@ -234,3 +234,43 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+
+
+def _brute_distinct(dims, hash_log2):
+    # every cell's 27-cell neighbourhood (inside the grid), bucket = row-major
+    # index mod 2^k; distinct iff no neighbourhood repeats a bucket
+    D0, D1, D2 = dims
+    x, y, z = np.meshgrid(np.arange(D0), np.arange(D1), np.arange(D2), indexing="ij")
+    x, y, z = x.ravel(), y.ravel(), z.ravel()
+    buckets = []
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                nx, ny, nz = x + dx, y + dy, z + dz
+                ok = (nx >= 0) & (ny >= 0) & (nz >= 0) & (nx < D0) & (ny < D1) & (nz < D2)
+                b = ((nx * D1 + ny) * D2 + nz) & ((1 << hash_log2) - 1)
+                buckets.append(np.where(ok, b, -1 - np.arange(27)[len(buckets)]))
+    B = np.sort(np.stack(buckets, 1), 1)
+    return not np.any((B[:, 1:] == B[:, :-1]) & (B[:, 1:] >= 0))
+
+
+@pytest.mark.parametrize("dims,hash_log2", [
+    ((32, 32, 32), 11), ((32, 32, 32), 12), ((32, 32, 32), 10), ((20, 20, 20), 9), ((20, 20, 20), 6),
+    ((16, 16, 16), 9), ((16, 16, 16), 10), ((1, 8, 64), 9), ((3, 5, 7), 4), ((40, 24, 16), 10),
+    ((12, 12, 12), 8), ((2, 2, 2), 4), ((64, 64, 64), 13)])
+def test_hash_bucket_distinctness_matches_enumeration(dims, hash_log2):
+    # The pruned list walk is taken only when this predicate holds; a wrong
+    # predicate visits a repeated bucket twice and lists neighbours twice.
+    # 32^3 / 2^11: cells x-1 and x+1 are 2 * 1024 apart in the row-major
+    # index, one table length.
+    assert pbf_host.hash_buckets_distinct(dims, hash_log2) == _brute_distinct(dims, hash_log2), (dims, hash_log2)
+
+
+def test_hash_bucket_collision_cases_named():
+    assert not pbf_host.hash_buckets_distinct((32, 32, 32), 11)
+    assert not pbf_host.hash_buckets_distinct((64, 64, 64), 13)
+    assert pbf_host.hash_buckets_distinct((20, 20, 20), 9)
+    # 20^3 in 2^6 buckets: 400 and 20 are 16 and 20 mod 64, and no
+    # 16 a + 20 b + c with a, b, c in [-2, 2] but zero is a multiple of 64
+    assert pbf_host.hash_buckets_distinct((20, 20, 20), 6)
+    assert not pbf_host.hash_buckets_distinct((20, 20, 20), 4)   # 400 = 25 * 16

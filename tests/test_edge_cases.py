@@ -7,7 +7,7 @@ import pytest
 
 import scenes
 from pairs import gpu_fields, make_oracle, oracle_fields, ulp_diff
-from pbf_host import PBF, PBFError
+from pbf_host import PBF, PBFError, hash_buckets_distinct
 
 pytestmark = pytest.mark.gpu
 
@@ -102,19 +102,24 @@ def test_wide_slabs_mix_table_formats():
     _bit_exact(sc, steps=1)
 
 
-@pytest.mark.parametrize("hash_log2", [9, 6])
-def test_hashed_grid_wraps_and_collisions(hash_log2):
-    # 20^3 cells in a 2^9-bucket table: the 27 cells around a cell still hash
-    # to distinct buckets (row-major indices differ by at most 421), but the
-    # index wraps the table inside some z-columns (visited cell by cell) and
-    # far cells share buckets.  2^6 buckets: the 27 cells collide too (the
-    # first-occurrence walk without pruning).
+@pytest.mark.parametrize("L,hash_log2,distinct", [(2.0, 9, True), (2.0, 4, False), (3.2, 11, False),
+                                                   (3.2, 12, True)])
+def test_hashed_grid_wraps_and_collisions(L, hash_log2, distinct):
+    # 20^3 cells (L = 2) in 2^9 buckets: the 27 cells around a cell hash to
+    # distinct buckets, but the index wraps the table inside some z-columns
+    # (visited cell by cell) and far cells share buckets.  2^4 buckets: cells
+    # one x-column apart (400 = 25 * 16) share a bucket, so the walk visits
+    # each bucket at its first occurrence.  32^3 cells (L = 3.2) in 2^11
+    # buckets: cells x - 1 and x + 1 are 2 * 1024 apart in the row-major index,
+    # one table length -- the same bucket, visited once.
+    assert hash_buckets_distinct((round(L / 0.1),) * 3, hash_log2) == distinct
     rng = np.random.default_rng(hash_log2)
-    pos = np.zeros((6000, 4), np.float32)
-    pos[:, :3] = rng.uniform(0.05, 1.95, (6000, 3))
-    sc = _scene(pos, L=2.0, name=f"hash{hash_log2}", field=False)
+    n = 6000 if L == 2.0 else 16000
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rng.uniform(0.05, L - 0.05, (n, 3))
+    sc = _scene(pos, L=L, name=f"hash{hash_log2}", field=False)
     sc.grid_mode, sc.hash_log2 = 1, hash_log2
-    print(f"[hash wrap] 2^{hash_log2} buckets", flush=True)
+    print(f"[hash wrap] {round(L / 0.1)}^3 cells, 2^{hash_log2} buckets", flush=True)
     sim = scenes.make_engine(sc)
     o = make_oracle(sc)
     sim.search(sc.substep_dt())
@@ -122,7 +127,13 @@ def test_hashed_grid_wraps_and_collisions(hash_log2):
     gc, go = sim.read_cells()
     oc, oo = o.cells()
     assert np.array_equal(gc, oc) and np.array_equal(go, oo)
-    assert np.array_equal(sim.read_neighbors()[1], o.neighbors()[1])
+    goff, gids = sim.read_neighbors()
+    ooff, oids = o.neighbors()
+    assert np.array_equal(goff, ooff) and np.array_equal(gids, oids)
+    # no particle lists a neighbour twice
+    for q in range(0, n, 97):
+        row = gids[goff[q]:goff[q + 1]]
+        assert len(np.unique(row)) == len(row)
     sim.close()
     _bit_exact(sc, steps=1)
 

@@ -231,3 +231,22 @@ def test_generated_obstacle_cdm_runs_at_parity():
     for k in g:
         assert ulp_diff(g[k], c[k]) == 0, (k, ulp_diff(g[k], c[k]))
     sim.close()
+
+
+def test_graph_cache_eviction_with_many_substep_shapes():
+    """More substep shapes than the context caches executable graphs for (16),
+    launched back to back without a host synchronisation: every eviction
+    waits for the stream, and the results equal direct launches."""
+    sc = scenes.dam_break(12, 12, 8, map_res=24, name="graph_cycle")
+    a = scenes.make_engine(sc)
+    b = scenes.make_engine(sc, flags=PBF_FLAG_NO_GRAPH)
+    dt = sc.substep_dt()
+    shapes = [(dt * (1.0 + 0.01 * (k % 5)), 1 + k % 7) for k in range(24)] * 2
+    print(f"[graph cycle] {len(set(shapes))} shapes, {len(shapes)} substeps", flush=True)
+    for sdt, it in shapes:
+        a.step(sdt, 1, it)
+        b.step(sdt, 1, it)
+    assert np.array_equal(a.read_positions(), b.read_positions())
+    assert np.array_equal(a.read_velocities(), b.read_velocities())
+    a.close()
+    b.close()

@@ -50,7 +50,10 @@ struct SceneHost {
 struct GraphEntry {
   uint32_t dt_bits, iters, diag, mode;
   hipGraphExec_t exec;
+  uint64_t used;  // launch serial of its latest replay
 };
+
+constexpr size_t kGraphCache = 16;  // executable graphs kept per context
 
 struct PendingEvent {
   int pass;
@@ -97,6 +100,7 @@ struct pbf_ctx {
   bool last_was_search = false, have_dbg_counts = false, have_lists = false;
   // graphs and timing
   std::vector<GraphEntry> graphs;
+  uint64_t graph_serial = 0, graph_evictions = 0;
   std::vector<hipEvent_t> event_pool;
   std::vector<PendingEvent> pending;
   double pass_ms[P_NPASS] = {0};
@@ -181,9 +185,27 @@ Phys make_phys(const pbf_ctx *c, float dt) {
   return P;
 }
 
+// Destroying an executable graph that is still running on the stream frees
+// what its in-flight launch uses (and what a profiler's completion callbacks
+// read), so every destroy waits for the stream first.
 void clear_graphs(pbf_ctx *c) {
+  if (c->graphs.empty()) return;
+  (void)hipStreamSynchronize(c->stream);
   for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
   c->graphs.clear();
+}
+
+// A full cache drops its least recently used executable, after the stream
+// has drained (the one just launched may be that one's last user).
+void evict_graph(pbf_ctx *c) {
+  if (c->graphs.empty()) return;
+  size_t old = 0;
+  for (size_t k = 1; k < c->graphs.size(); ++k)
+    if (c->graphs[k].used < c->graphs[old].used) old = k;
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipGraphExecDestroy(c->graphs[old].exec);
+  c->graphs.erase(c->graphs.begin() + (long)old);
+  c->graph_evictions += 1;
 }
 
 LaunchCtx make_launch(pbf_ctx *c, float dt) {
@@ -344,9 +366,13 @@ int run_substep(pbf_ctx *c, float dt, uint32_t iters, uint32_t mode, bool diag) 
   uint32_t bits;
   std::memcpy(&bits, &dt, 4);
   hipGraphExec_t exec = nullptr;
+  GraphEntry *hit = nullptr;
   for (auto &g : c->graphs)
-    if (g.dt_bits == bits && g.iters == iters && g.diag == (uint32_t)diag && g.mode == mode) exec = g.exec;
-  if (!exec) {
+    if (g.dt_bits == bits && g.iters == iters && g.diag == (uint32_t)diag && g.mode == mode) hit = &g;
+  if (hit) {
+    exec = hit->exec;
+    hit->used = ++c->graph_serial;
+  } else {
     hipGraph_t graph = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     enqueue_substep(c, L, iters, diag, T, mode);
@@ -360,8 +386,8 @@ int run_substep(pbf_ctx *c, float dt, uint32_t iters, uint32_t mode, bool diag) 
     hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (ie != hipSuccess) return fail(c, PBF_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
-    if (c->graphs.size() >= 16) clear_graphs(c);
-    c->graphs.push_back({bits, iters, (uint32_t)diag, mode, exec});
+    if (c->graphs.size() >= kGraphCache) evict_graph(c);
+    c->graphs.push_back({bits, iters, (uint32_t)diag, mode, exec, ++c->graph_serial});
   }
   HIPCHK(hipGraphLaunch(exec, c->stream));
   return PBF_OK;
@@ -533,9 +559,9 @@ Scene to_device_scene(const pbf_ctx *c, const SceneHost &s, uint64_t field_off) 
   d.cell_base = s.cell_base;
   d.ncells = s.ncells;
   d.hash_mask = c->params.grid_mode == PBF_GRID_HASH ? s.ncells - 1u : 0u;
-  // row-major indices of the 27 cells around a cell differ by at most
-  // D1 D2 + D2 + 1; below the table size they cannot share a bucket
-  d.distinct27 = !d.hash_mask || (uint64_t)s.dims[1] * (uint64_t)s.dims[2] + (uint64_t)s.dims[2] + 1u < s.ncells;
+  // the list build's pruned column walk needs the 27 cells around every
+  // cell in 27 different buckets (always true on a dense grid)
+  d.distinct27 = !d.hash_mask || pbf_hash_buckets_distinct(s.dims, c->params.hash_log2);
   d.pbase = s.pbase;
   d.pcount = s.count;
   d.field_kind = s.field_kind;

@@ -609,3 +609,23 @@ extern "C" int pbf_parse_cdm(const char *text, size_t length, int32_t res[3], fl
   std::free(b);
   return rc;
 }
+
+// A hashed grid stores cell (x, y, z) in bucket ((x D1 + y) D2 + z) mod 2^k.
+// Two of the 27 cells around a cell differ in that row-major index by
+// a D1 D2 + b D2 + c with a, b, c in [-2, 2] (|a| < D0, |b| < D1, |c| < D2,
+// since both cells lie in the grid), so the 27 buckets are distinct for every
+// cell exactly when no such nonzero difference is a multiple of 2^k.
+extern "C" int pbf_hash_buckets_distinct(const int32_t dims[3], uint32_t hash_log2) {
+  if (!dims || dims[0] < 1 || dims[1] < 1 || dims[2] < 1 || hash_log2 > 62) return 0;
+  const uint64_t mask = (1ull << hash_log2) - 1ull;
+  const int64_t D1 = dims[1], D2 = dims[2];
+  for (int a = -2; a <= 2; ++a)
+    for (int b = -2; b <= 2; ++b)
+      for (int c = -2; c <= 2; ++c) {
+        if (a == 0 && b == 0 && c == 0) continue;
+        if (std::abs(a) >= dims[0] || std::abs(b) >= dims[1] || std::abs(c) >= dims[2]) continue;
+        const int64_t d = (int64_t)a * D1 * D2 + (int64_t)b * D2 + c;
+        if (((uint64_t)d & mask) == 0) return 0;
+      }
+  return 1;
+}

@@ -100,6 +100,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "pbf_read_positions": (I, [P, pf, U32]), "pbf_read_velocities": (I, [P, pf, U32]),
         "pbf_read_field": (I, [P, U32, pf, U32]),
         "pbf_num_cells": (U32, [P]),
+        "pbf_hash_buckets_distinct": (I, [C.POINTER(C.c_int32), U32]),
         "pbf_read_cells": (I, [P, pu, pu]),
         "pbf_read_neighbors": (I, [P, pu, pu, U64, C.POINTER(U64)]),
         "pbf_search": (I, [P, F]),
@@ -143,6 +144,12 @@ def _fptr(a: np.ndarray):
 
 def _uptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def hash_buckets_distinct(dims, hash_log2: int) -> bool:
+    """True when the 27 cells around every cell of a dims grid hashed into
+    2^hash_log2 buckets land in 27 different buckets (GPU-free)."""
+    return bool(load_library().pbf_hash_buckets_distinct(_i3(dims), int(hash_log2)))
 
 
 _map_on_gpu = None
