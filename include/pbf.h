@@ -233,7 +233,8 @@ int pbf_read_density_history(pbf_ctx *ctx, double *mean_max, uint32_t capacity, 
 uint32_t pbf_particle_count(const pbf_ctx *ctx);
 uint32_t pbf_scene_count(const pbf_ctx *ctx);
 /* Message of the context's last failed call; with NULL, the message of the
- * last failed pbf_create / pbf_batch_create on the calling thread. */
+ * last failed pbf_create / pbf_batch_create / pbf_mlp_forward_backward_noalloc
+ * on the calling thread. */
 const char *pbf_last_error(const pbf_ctx *ctx);
 const char *pbf_version(void);
 
@@ -288,6 +289,22 @@ int pbf_mlp_init(uint64_t seed, float *blob);
  * mean((y - (target - tmu) / tsig)^2).  Any output pointer may be NULL. */
 int pbf_mlp_forward_backward(const float *blob, const float *features, const float *targets, uint32_t n, float *y,
                              float *grad, float *loss);
+/* pbf_mlp_forward_backward without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight training
+ * loops).  No counterpart in the reference.  Every pointer is DEVICE memory
+ * of the current GPU: blob (pbf_mlp_blob_size() floats), features (16 n),
+ * targets (n); outputs y (n), grad (PBF_MLP_PARAMS), loss (1), each may be
+ * NULL.  Enqueued on `stream` (a hipStream_t; NULL = the default stream) and
+ * asynchronous: the outputs are ready when the stream reaches that point.
+ * The packed bf16 weights and the gradient / loss accumulators live in a
+ * fixed 57,344-byte array per GPU that the library holds until the process
+ * ends; calls that overlap on one GPU share it and race, so serialise them
+ * (one stream, or an event between streams).  n must be in
+ * [1, PBF_MLP_NOALLOC_MAX_SAMPLES]; errors are returned as codes with the
+ * message in pbf_last_error(NULL). */
+#define PBF_MLP_NOALLOC_MAX_SAMPLES 2147483648u
+int pbf_mlp_forward_backward_noalloc(const float *blob, const float *features, const float *targets, uint32_t n,
+                                     float *y, float *grad, float *loss, void *stream);
 /* Adam (beta 0.9 / 0.999, eps 1e-8) over shuffled mini-batches on the GPU;
  * blob holds the initial network (and its normalisation) and receives the
  * trained weights; loss_per_epoch (epochs floats) may be NULL. */

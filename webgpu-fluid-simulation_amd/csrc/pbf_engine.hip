@@ -128,6 +128,12 @@ namespace {
 // The message of the last failed create on this thread (pbf_last_error(NULL)).
 thread_local std::string g_create_error;
 
+}  // namespace
+
+void pbf::set_thread_error(const std::string &msg) { g_create_error = msg; }
+
+namespace {
+
 int fail(pbf_ctx *c, int code, const std::string &msg) {
   if (c) c->err = msg;
   return code;

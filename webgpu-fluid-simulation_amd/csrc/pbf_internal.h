@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "pbf_device.h"
 
 namespace pbf {
@@ -86,6 +88,10 @@ void launch_unsort1(hipStream_t st, uint32_t n, const float *src, const uint32_t
 void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v);
 void launch_iota(hipStream_t st, uint32_t *p, uint32_t n);
 void launch_copy_probe(hipStream_t st, const void *src, void *dst, size_t bytes);
+
+// Message of the last failed context-free call on this thread
+// (pbf_last_error(NULL)).
+void set_thread_error(const std::string &msg);
 
 // Host utilities (pbf_host.cpp).
 uint64_t splitmix64(uint64_t x);

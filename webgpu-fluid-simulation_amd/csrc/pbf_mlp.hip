@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -223,7 +224,8 @@ __global__ void __launch_bounds__(256) k_mlp_infer(uint32_t n, const float *__re
   }
 }
 
-// Training: forward + backward over a batch (`idx` selects the samples),
+// Training: forward + backward over a batch (`idx` selects the samples, or
+// NULL for samples 0 .. nb-1),
 // MSE in normalised target units; gradients summed into `grad` (NPARAM
 // floats, scaled by inv_batch) and the loss into loss[0].
 constexpr int TW = 4;  // waves per block
@@ -249,7 +251,7 @@ __global__ void __launch_bounds__(256) k_mlp_train(uint32_t nb, const uint32_t *
   for (uint32_t t = wave; t < tiles; t += waves) {
     const uint32_t q = t * 32 + c;
     const bool valid = q < nb;
-    const uint32_t p = valid ? idx[q] : 0u;
+    const uint32_t p = valid ? (idx ? idx[q] : q) : 0u;  // no index array: sample q
     TileAct T;
     tile_forward(M, feats, p, valid, T);
     const float tn = valid ? (targ[p] - M.misc[OFF_TMU - OFF_B3]) / M.misc[OFF_TSIG - OFF_B3] : 0.0f;
@@ -399,6 +401,33 @@ __global__ void k_mlp_pack(const float *__restrict__ w, short *__restrict__ W1T,
     f32[q] = w[OFF_B1 + q];
   }
 }
+
+__global__ void k_mlp_zero(float *__restrict__ p, uint32_t n) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) p[q] = 0.0f;
+}
+
+__global__ void k_mlp_copy_out(const float *__restrict__ gacc, const float *__restrict__ lacc,
+                               float *__restrict__ grad, float *__restrict__ loss) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (grad && q < NPARAM) grad[q] = gacc[q];
+  if (loss && q == 0) loss[0] = lacc[0];
+}
+
+// Scratch of the allocation-free forward + backward: the bf16 packings of W1
+// (transposed) and W2 (both layouts), the float32 vectors b1 .. lmin, the
+// gradient accumulator and the loss (byte offsets below; 57,344 bytes per
+// GPU, independent of the sample count).  Host code reaches it only through
+// the per-device address from hipGetSymbolAddress.
+constexpr size_t kScrW1T = 0;
+constexpr size_t kScrW2T = kScrW1T + 1024 * 2;
+constexpr size_t kScrW2N = kScrW2T + 4096 * 2;
+constexpr size_t kScrF32 = kScrW2N + 4096 * 2;
+constexpr size_t kScrGrad = (kScrF32 + (size_t)(BLOB - OFF_B1) * 4 + 255) & ~(size_t)255;
+constexpr size_t kScrLoss = kScrGrad + (size_t)NPARAM * 4;
+constexpr size_t kScrBytes = (kScrLoss + 4 + 255) & ~(size_t)255;
+static_assert(kScrBytes == 57344, "scratch size documented in include/pbf.h");
+static __device__ __align__(256) unsigned char scratch[kScrBytes];
 
 }  // namespace pbf
 
@@ -581,6 +610,34 @@ void launch_train(TrainState &S, uint32_t nb, const uint32_t *idx, float inv_bat
                      (const float *)S.targ, mlp_view(S.B), inv_batch, S.grad, S.loss, yout);
 }
 
+// Address of the scratch array on the current device, looked up once per
+// device (each GPU holds its own copy of a __device__ array).
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+unsigned char *g_scratch_dev[kMaxDevices] = {};
+
+int mlp_fail(int code, const std::string &msg) {
+  set_thread_error(msg);
+  return code;
+}
+
+int scratch_on_device(unsigned char **out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return mlp_fail(PBF_E_DEVICE, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return mlp_fail(PBF_E_DEVICE, "device index above 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return mlp_fail(PBF_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<unsigned char *>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return PBF_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -611,6 +668,39 @@ int pbf_mlp_forward_backward(const float *blob, const float *features, const flo
   TCHK(hipMemcpyAsync(&l, S.loss, 4, hipMemcpyDeviceToHost, S.st));
   TCHK(hipStreamSynchronize(S.st));
   if (loss) *loss = l;
+  return PBF_OK;
+}
+
+int pbf_mlp_forward_backward_noalloc(const float *blob, const float *features, const float *targets, uint32_t n,
+                                     float *y, float *grad, float *loss, void *stream) {
+  if (!blob || !features || !targets || n == 0)
+    return mlp_fail(PBF_E_INVALID, "blob, features and targets are required and n must be >= 1");
+  if (n > PBF_MLP_NOALLOC_MAX_SAMPLES)
+    return mlp_fail(PBF_E_CAPACITY, "n above PBF_MLP_NOALLOC_MAX_SAMPLES (2^31 samples)");
+  unsigned char *scr = nullptr;
+  int rc = scratch_on_device(&scr);
+  if (rc != PBF_OK) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  MlpBuffers B;  // views into the scratch, nothing allocated
+  B.W1T = reinterpret_cast<short *>(scr + kScrW1T);
+  B.W2T = reinterpret_cast<short *>(scr + kScrW2T);
+  B.W2N = reinterpret_cast<short *>(scr + kScrW2N);
+  B.f32 = reinterpret_cast<float *>(scr + kScrF32);
+  float *gacc = reinterpret_cast<float *>(scr + kScrGrad), *lacc = reinterpret_cast<float *>(scr + kScrLoss);
+  // the scratch keeps the previous call's contents: repack, then clear the
+  // accumulators (gradient and loss are adjacent) with a kernel, which a
+  // captured graph replays correctly
+  hipLaunchKernelGGL(k_mlp_pack, dim3(17), dim3(256), 0, st, blob, B.W1T, B.W2T, B.W2N, B.f32);
+  hipLaunchKernelGGL(k_mlp_zero, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, st, gacc, (uint32_t)(NPARAM + 1));
+  const uint32_t tiles = (n + 31) / 32;
+  const uint32_t blocks = std::max<uint32_t>(1u, std::min<uint32_t>((tiles + TW - 1) / TW, 512u));
+  hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(64 * TW), 0, st, n, (const uint32_t *)nullptr, features,
+                     targets, mlp_view(B), 1.0f / (float)n, gacc, lacc, y);
+  if (grad || loss)
+    hipLaunchKernelGGL(k_mlp_copy_out, dim3((NPARAM + 255) / 256), dim3(256), 0, st, (const float *)gacc,
+                       (const float *)lacc, grad, loss);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return mlp_fail(PBF_E_DEVICE, std::string("launch: ") + hipGetErrorString(e));
   return PBF_OK;
 }
 

@@ -115,6 +115,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "pbf_mlp_init": (I, [U64, pf]),
         "pbf_mlp_forward_backward": (I, [pf, pf, pf, U32, pf, pf, pf]),
         "pbf_mlp_train": (I, [pf, pf, U64, U32, U32, F, U64, pf, pf]),
+        "pbf_mlp_forward_backward_noalloc": (I, [P, P, P, U32, P, P, P, P]),
         "pbf_set_warm_start": (I, [P, pf, S, U32, F, D, U32]),
         "pbf_warm_start_stats": (I, [P, C.POINTER(U64), C.POINTER(U64)]),
         "pbf_collect_training": (I, [P, F, U32, U32, pf, pf, U32, pu]),
