@@ -44,6 +44,7 @@ def lib():
             "orc_clear_field": (None, [P]),
             "orc_substep": (None, [P, F, I]), "orc_search_only": (None, [P, F]),
             "orc_get": (None, [P, I, pf]), "orc_count": (I, [P]), "orc_num_cells": (C.c_uint32, [P]),
+            "orc_set_velocity": (None, [P, pf]),
             "orc_grid_dims": (None, [P, pi]),
             "orc_get_cells": (None, [P, pu, pu]), "orc_num_pairs": (C.c_uint64, [P]),
             "orc_get_neighbors": (None, [P, pu, pu]),
@@ -128,6 +129,11 @@ class Oracle:
         out = np.zeros((self.n, 4), np.float32)
         self.L.orc_get(self.s, field, _f(out))
         return out
+
+    def set_velocity(self, v):
+        a = np.zeros((self.n, 4), np.float32)
+        a[:, :3] = np.asarray(v, np.float32).reshape(-1, 3)
+        self.L.orc_set_velocity(self.s, _f(a))
 
     def cells(self):
         nc = int(self.L.orc_num_cells(self.s))

@@ -785,6 +785,13 @@ void orc_substep(orc_sim *s, float dt, int iterations) {
 
 /* ------------------------------------------------------------------ */
 /* Accessors.                                                          */
+void orc_set_velocity(orc_sim *s, const float *v4) {
+  for (int i = 0; i < s->n; ++i) {
+    for (int c = 0; c < 3; ++c) s->v[4 * i + c] = v4[4 * i + c];
+    s->v[4 * i + 3] = 0.0f;
+  }
+}
+
 void orc_get(const orc_sim *s, int field, float *out4) {
   const float *src = NULL;
   size_t n4 = (size_t)s->n * 4;

@@ -76,6 +76,9 @@ void orc_substep(orc_sim *s, float dt, int iterations);
 void orc_search_only(orc_sim *s, float dt);
 
 void orc_get(const orc_sim *s, int field, float *out4);
+/* Sets every particle's velocity (xyz of n float4 rows, id order): the
+ * state the next substep's predict (timeIntegration.ts:20-31) starts from. */
+void orc_set_velocity(orc_sim *s, const float *v4);
 int orc_count(const orc_sim *s);
 uint32_t orc_num_cells(const orc_sim *s);
 void orc_grid_dims(const orc_sim *s, int dims[3]);

@@ -500,3 +500,139 @@ def test_oracle_is_deterministic_across_thread_counts(tmp_path):
         subprocess.run([sys.executable, "-c", code, str(f)], check=True, env=env)
         outs.append(np.load(f))
     assert np.array_equal(outs[0], outs[1])
+
+
+# --------------------------------------------------------------------------
+# K13 / K14 invariants (vorticityConfinement.ts:46-62, XSPH.ts:56-88): facts
+# that hold whatever the summation order, each chosen so that a plausible
+# misreading of the shaders (v_j for v_j - v_i, a swapped cross product, a
+# lost m / rho_j or K_ij, a flipped normal difference) fails it.
+def _lattice(n, spacing, origin):
+    g = origin + spacing * np.arange(n)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], -1)
+
+
+def test_uniform_velocity_gives_zero_vorticity_and_xsph_exactly():
+    # Dyadic lattice (r = 2^-7, spacing 2^-6), dt = 1/64 and g = 4 make
+    # every predicted position exact: x* = x - 2^-10 in y.  The rest lattice
+    # has rho~ = 0.9088 < 1, so lambda = 0 and every particle gets exactly
+    # v = (0, -1/16, 0): then omega = sum m/rho_j (v_j - v_i) x grad W = 0,
+    # the XSPH sum of (v_j - v_i) W is 0 and the vorticity force
+    # eps (eta^ x omega) is 0, bit for bit.
+    r = 2.0 ** -7
+    x = _lattice(10, 2 * r, 0.25)
+    o = make(x, r=r)
+    o.set_config(0.05, 0.2, 0.0, 4.0)
+    o.substep(1.0 / 64.0, 3)
+    v = o.get(oracle.VELOCITY)[:, :3]
+    assert np.all(v[:, 1] == -1.0 / 16.0) and np.all(v[:, 0] == 0) and np.all(v[:, 2] == 0)
+    assert np.all(o.get(oracle.POSITION)[:, :3] == (x + [0, -2.0 ** -10, 0]).astype(np.float32))
+    assert np.all(o.get(oracle.OMEGA) == 0.0)
+    assert np.all(o.get(oracle.ACCEL)[:, :3] == 0.0)
+    n = o.get(oracle.NORMAL)[:, :3]
+    assert np.abs(n).max() > 0.1          # the block's faces do have normals
+
+
+def test_rigid_rotation_gives_omega_along_the_rotation_axis():
+    # v = Omega x (x - c) on a 12^3 lattice (spacing 2r, h = 4r).  For an
+    # interior particle, with x_ij = x_i - x_j and grad W_ij = f(r) x_ij
+    # (f = -45/(pi h^6) (h - r)^2 / r < 0):
+    #   (v_j - v_i) x grad W_ij = f (Omega r^2 - x_ij (x_ij . Omega)),
+    # and on a cubic lattice sum_j f x_ij x_ij^T = (1/3) sum_j f r^2 I, so
+    #   omega_i = S Omega,  S = (2/3) sum_j (m / rho_j) f(r_ij) r_ij^2 < 0.
+    # Swapping the cross product flips the sign; dropping m / rho_j changes S.
+    r, h = 0.025, 0.1
+    x = _lattice(12, 2 * r, 0.3)
+    c = x.mean(0)
+    Om = np.array([0.3, 1.0, -0.5])
+    o = make(x, r=r, hi=(2, 2, 2))
+    o.set_config(0.0, 0.0, 0.0, 0.0)
+    o.set_velocity(np.cross(Om, x - c))
+    o.substep(1e-3, 2)
+    p = o.get(oracle.POSITION).astype(np.float64)
+    om = o.get(oracle.OMEGA)[:, :3].astype(np.float64)
+    m = 0.9 * (2 * r) ** 3 * 1000.0
+    ks = -45.0 / (math.pi * h ** 6)
+    idx = np.arange(12)
+    I, J, K = np.meshgrid(idx, idx, idx, indexing="ij")
+    inner = ((I >= 2) & (I <= 9) & (J >= 2) & (J <= 9) & (K >= 2) & (K <= 9)).ravel()
+    checked = 0
+    for i in np.nonzero(inner)[0]:
+        d = p[i, :3] - p[:, :3]
+        q = np.sqrt((d * d).sum(1))
+        nb = (q <= h) & (q > 0)
+        S = (2.0 / 3.0) * np.sum(m / p[nb, 3] * ks * (h - q[nb]) ** 2 / q[nb] * q[nb] ** 2)
+        assert S < 0
+        assert np.linalg.norm(om[i] - S * Om) <= 2e-3 * np.linalg.norm(S * Om), (i, om[i], S * Om)
+        checked += 1
+    assert checked == 8 ** 3
+
+
+def test_normals_vanish_inside_and_point_into_the_fluid_at_a_free_surface():
+    # n_i = h sum m / rho_j grad W_ij: zero by symmetry inside a lattice; on
+    # the top face every neighbour lies below (x_ij . y > 0, f < 0), so n
+    # points down, into the fluid.
+    r = 0.025
+    x = _lattice(10, 2 * r, 0.3)
+    o = make(x, r=r, hi=(2, 2, 2))
+    o.set_config(0.0, 0.0, 0.0, 0.0)
+    o.substep(1e-3, 2)
+    n = o.get(oracle.NORMAL)[:, :3].astype(np.float64)
+    idx = np.arange(10)
+    I, J, K = np.meshgrid(idx, idx, idx, indexing="ij")
+    I, J, K = I.ravel(), J.ravel(), K.ravel()
+    top = (J == 9) & (I >= 2) & (I <= 7) & (K >= 2) & (K <= 7)
+    inner = (I >= 2) & (I <= 7) & (J >= 2) & (J <= 7) & (K >= 2) & (K <= 7)
+    ny = n[top, 1]
+    # (the lattice coordinates 0.3 + 0.05 k are rounded to float32: 1e-5)
+    assert np.all(ny < 0) and np.all(np.abs(n[top][:, [0, 2]]).max(1) < 1e-5 * np.abs(ny))
+    assert np.abs(n[inner]).max() < 1e-5 * np.abs(ny).min()
+
+
+def test_surface_tension_forces_sum_to_zero_over_an_isolated_cloud():
+    # tension_i = -sum_j K_ij (m C(x_ij) + (n_i - n_j)): K_ij is symmetric,
+    # the cohesion vector and n_i - n_j are antisymmetric, so the forces of
+    # an isolated cloud (equal masses) cancel (momentum is conserved).
+    rng = np.random.default_rng(5)
+    r = 0.025
+    x = rng.uniform(0.4, 0.6, (300, 3))
+    o = make(x, r=r, hi=(2, 2, 2))
+    o.set_config(0.0, 0.0, 0.5, 9.8)
+    o.substep(1.0 / 240.0, 3)
+    o.substep(1.0 / 240.0, 3)
+    a = o.get(oracle.ACCEL)[:, :3].astype(np.float64)
+    assert np.abs(a).sum() > 1.0
+    assert np.abs(a.sum(0)).max() <= 1e-5 * np.abs(a).sum(0).max()
+
+
+@pytest.mark.parametrize("d", [0.07, 0.03])
+def test_two_particle_tension_closed_form(d):
+    # Two particles at distance d (both branches of the Akinci cohesion
+    # spline, Akinci et al. 2013 eq. 2): with x^ = (x_i - x_j) / d,
+    #   rho = m (W(0) + W(d)),  K = 2 rho0 / (2 rho),
+    #   n_i - n_j = 2 h (m / rho) spiky(d) x^,
+    #   a_i = -gamma K (m C(d) + 2 h (m / rho) spiky(d)) x^.
+    r, h, gamma = 0.025, 0.1, 1.0
+    u = np.array([1.0, 2.0, 2.0]) / 3.0
+    xi = np.array([0.5, 0.5, 0.5])
+    pts = np.array([xi, xi - d * u], np.float32)
+    o = make(pts.astype(np.float64), r=r, hi=(2, 2, 2))
+    o.set_config(0.0, 0.0, gamma, 0.0)
+    o.substep(1e-3, 1)
+    p = o.get(oracle.POSITION).astype(np.float64)
+    a = o.get(oracle.ACCEL)[:, :3].astype(np.float64)
+    dv = p[0, :3] - p[1, :3]
+    dd = np.linalg.norm(dv)
+    xh = dv / dd
+    m = 0.9 * (2 * r) ** 3 * 1000.0
+    W = lambda q: 315.0 / (64 * math.pi * h ** 9) * (h * h - q * q) ** 3
+    rho = m * (W(0.0) + W(dd))
+    assert p[0, 3] == pytest.approx(rho, rel=1e-5)
+    t = (h - dd) ** 3 * dd ** 3
+    Cd = 32.0 / (math.pi * h ** 9) * (t if 2 * dd > h else 2 * t - h ** 6 / 64)
+    spiky = -45.0 / (math.pi * h ** 6) * (h - dd) ** 2
+    K = 2000.0 / (2 * rho)
+    want = -gamma * K * (m * Cd + 2 * h * (m / rho) * spiky) * xh
+    np.testing.assert_allclose(a[0], want, rtol=2e-5, atol=1e-6 * np.abs(want).max())
+    np.testing.assert_allclose(a[1], -want, rtol=2e-5, atol=1e-6 * np.abs(want).max())
