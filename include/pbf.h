@@ -278,6 +278,14 @@ enum { PBF_MLP_PARAMS = 5313, PBF_MLP_FEATURES = 16 };
  * used.  A network with zero weights and tmu = 1 is exactly
  * PBF_WARM_PREVIOUS. */
 enum { PBF_WARM_PLAIN = 0, PBF_WARM_LEARNED = 1, PBF_WARM_PREVIOUS = 2, PBF_WARM_RATIO = 3 };
+/* OR'ed into a mode: projected accumulation.  Iterations 1.. take their
+ * increment from the unclamped constraint, -(rho~ - 1) / (den + eps), add it
+ * to the substep's lambda sum, keep that sum <= 0, and apply the change of
+ * the sum (accumulated multipliers clamped as a whole, as in Catto's
+ * warm-started sequential impulses).  A warm start that pushed particles
+ * apart is pulled back by the later iterations, never past a zero sum.  With
+ * PBF_WARM_PLAIN it is the cold-started solve in that form. */
+#define PBF_WARM_PROJECT 0x100u
 #define PBF_WARM_RATIO_MAX 2.0f
 
 uint32_t pbf_mlp_blob_size(void);
@@ -310,6 +318,13 @@ int pbf_mlp_forward_backward_noalloc(const float *blob, const float *features, c
  * trained weights; loss_per_epoch (epochs floats) may be NULL. */
 int pbf_mlp_train(const float *features, const float *targets, uint64_t n, uint32_t epochs, uint32_t batch,
                   float lr, uint64_t seed, float *blob, float *loss_per_epoch);
+/* pbf_mlp_train with an asymmetric loss: a sample whose prediction is above
+ * its target weighs over_weight (> 0) times its squared error, one below it
+ * weighs 1.  With over_weight > 1 the network learns a value on the low side
+ * of each target's spread (an expectile), which for a factor on a negative
+ * lambda sum means a warm start that errs towards pushing too little. */
+int pbf_mlp_train_weighted(const float *features, const float *targets, uint64_t n, uint32_t epochs, uint32_t batch,
+                           float lr, uint64_t seed, float over_weight, float *blob, float *loss_per_epoch);
 /* Selects the solve: PBF_WARM_PLAIN, PBF_WARM_PREVIOUS, or PBF_WARM_LEARNED /
  * PBF_WARM_RATIO with the network in blob (NULL keeps the weights already
  * set).  relax in (0, 1] scales the

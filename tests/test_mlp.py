@@ -261,6 +261,37 @@ def test_fallback_replaces_a_rejected_substep_with_the_plain_solve():
 
 
 @pytest.mark.gpu
+def test_device_side_fallback_graph_replay_equals_direct_launches():
+    """The learned substep, its device-side check and the gated restore +
+    plain re-solve are one graph: replayed with eager reads between the
+    replays, with a threshold that keeps some substeps and rejects others,
+    it gives the same states and the same rejection count as direct
+    launches."""
+    from pbf_host import PBF_FLAG_NO_GRAPH, WARM_PREVIOUS
+    sc = _small_scene()
+    dt = sc.substep_dt()
+    probe = scenes.make_engine(sc)
+    errs = []
+    for _ in range(10):
+        probe.step(dt, 1, 2)
+        errs.append(probe.density_error()[0])
+    thr = float(np.median(errs))           # some substeps above it, some below
+    print(f"[device fallback] n={sc.n} threshold {thr:.4g}", flush=True)
+    a = scenes.make_engine(sc)
+    b = scenes.make_engine(sc, flags=PBF_FLAG_NO_GRAPH)
+    for e in (a, b):
+        e.set_warm_start(None, WARM_PREVIOUS, 1.0, fallback_threshold=thr, fallback_iterations=4)
+    for k in range(10):
+        a.step(dt, 1, 2)
+        b.step(dt, 1, 2)
+        assert np.array_equal(a.read_positions(), b.read_positions()), k
+    assert np.array_equal(a.read_velocities(), b.read_velocities())
+    sa, sb = a.warm_start_stats(), b.warm_start_stats()
+    print(f"[device fallback] stats graph {sa} direct {sb}", flush=True)
+    assert sa == sb and sa[0] == 10 and 0 < sa[1] < 10
+
+
+@pytest.mark.gpu
 def test_warm_start_argument_errors():
     sc = _small_scene()
     s = scenes.make_engine(sc)

@@ -35,6 +35,11 @@ const char *kPassNames =
 // 0, lambda summed over all iterations), and the temporal warm start
 // (iteration 0 applies the previous substep's lambda sum).
 enum SubstepMode : uint32_t { M_PLAIN = 0, M_LEARNED = 1, M_COLLECT = 2, M_PREVIOUS = 3, M_RATIO = 4 };
+// OR'ed into a substep mode: iterations 1.. project the accumulated lambda
+// (the learned / collection modes only).
+constexpr uint32_t M_PROJECT = 16;
+// Graph-key bit of a learned substep followed by its device-side check.
+constexpr uint32_t M_FALLBACK = 32;
 
 struct SceneHost {
   double lo[3], hi[3];
@@ -48,7 +53,7 @@ struct SceneHost {
 };
 
 struct GraphEntry {
-  uint32_t dt_bits, iters, diag, mode;
+  uint32_t dt_bits, iters, diag, mode, aux;
   hipGraphExec_t exec;
   uint64_t used;  // launch serial of its latest replay
 };
@@ -107,7 +112,8 @@ struct pbf_ctx {
   uint32_t pass_count[P_NPASS] = {0};
   // learned warm start
   void *mlp = nullptr;           // network weights on the device
-  uint32_t warm_mode = M_PLAIN;  // M_PLAIN or M_LEARNED
+  uint32_t warm_mode = PBF_WARM_PLAIN;  // PBF_WARM_*
+  bool warm_project = false;            // PBF_WARM_PROJECT
   double fb_threshold = 0.0;     // mean density error that rejects a learned substep (0: never)
   uint32_t fb_iters = 0;         // iterations of the plain re-solve
   float relax_set = -1.0f;       // scale applied to the network's lambda
@@ -118,7 +124,11 @@ struct pbf_ctx {
   float4 *snap_pos = nullptr, *snap_vel = nullptr, *snap_acc = nullptr, *snap_om = nullptr, *snap_nrm = nullptr;
   uint32_t *snap_ids = nullptr;
   float *snap_lam = nullptr, *snap_lacc = nullptr;
-  uint64_t learned_substeps = 0, fallbacks = 0;
+  uint64_t learned_substeps = 0;
+  // device-side fallback decision: the gate word, the rejection counter, the
+  // threshold, and the per-scene speed reduction
+  uint32_t *gate = nullptr, *fb_count = nullptr;
+  double *fb_thr = nullptr, *per_scene2 = nullptr;
   // launch tuning
   pbf_tuning tuning = {{256, 256, 256, 256, 256, 256, 256, 256}, 0, {0, 0, 0, 0, 0, 0, 0}};
 };
@@ -188,6 +198,7 @@ Phys make_phys(const pbf_ctx *c, float dt) {
   P.Kp64 = 315.0 / (64.0 * PI * std::pow(h, 9));
   P.h2_64 = h * h;
   P.V64 = V64;
+  P.gate = nullptr;
   return P;
 }
 
@@ -316,14 +327,16 @@ void enqueue_search(pbf_ctx *c, const LaunchCtx &L, PassTimer &T, bool dbg, uint
 }
 
 // The launch sequence of one substep (PBF.ts:398-448).
-void enqueue_substep(pbf_ctx *c, const LaunchCtx &L, uint32_t iters, bool diag, PassTimer &T, uint32_t mode) {
+void enqueue_substep(pbf_ctx *c, const LaunchCtx &L, uint32_t iters, bool diag, PassTimer &T, uint32_t mode_bits) {
+  const bool project = (mode_bits & M_PROJECT) != 0;
+  const uint32_t mode = mode_bits & ~M_PROJECT;
   enqueue_search(c, L, T, false, mode);
   float4 *X = c->XA, *Xn = c->XB;
   LearnBufs lb{c->lam_acc, c->acc_sorted, c->lam_sorted, c->vel_sorted, c->feats};
   for (uint32_t it = 0; it < iters; ++it) {
     LaunchCtx t = T.begin(P_LAMBDA, L);
     if (mode != M_PLAIN && it == 0) launch_lambda_features(t, X, c->B, c->nbr, c->ncount, lb);
-    else launch_lambda(t, X, c->B, c->nbr, c->ncount, mode != M_PLAIN ? c->lam_acc : nullptr);
+    else launch_lambda(t, X, c->B, c->nbr, c->ncount, mode != M_PLAIN ? c->lam_acc : nullptr, project);
     T.end();
     if ((mode == M_LEARNED || mode == M_RATIO) && it == 0) {
       t = T.begin(P_MLP, L);
@@ -356,7 +369,10 @@ void enqueue_substep(pbf_ctx *c, const LaunchCtx &L, uint32_t iters, bool diag, 
   }
 }
 
-int run_substep(pbf_ctx *c, float dt, uint32_t iters, uint32_t mode, bool diag) {
+// Runs an enqueue sequence once: directly, or as a cached hipGraph per
+// (dt, iterations, diagnostics, mode, aux) shape.
+template <typename Enqueue>
+int run_enqueued(pbf_ctx *c, float dt, uint32_t iters, bool diag, uint32_t mode, uint32_t aux, Enqueue &&enq) {
   LaunchCtx L = make_launch(c, dt);
   const bool timing = (c->params.flags & PBF_FLAG_TIMING) != 0;
   const bool use_graph = !(c->params.flags & PBF_FLAG_NO_GRAPH) && !timing;
@@ -365,7 +381,7 @@ int run_substep(pbf_ctx *c, float dt, uint32_t iters, uint32_t mode, bool diag) 
   c->have_dbg_counts = false;
   c->have_lists = true;
   if (!use_graph) {
-    enqueue_substep(c, L, iters, diag, T, mode);
+    enq(L, T);
     HIPCHK(hipGetLastError());
     return PBF_OK;
   }
@@ -374,14 +390,14 @@ int run_substep(pbf_ctx *c, float dt, uint32_t iters, uint32_t mode, bool diag) 
   hipGraphExec_t exec = nullptr;
   GraphEntry *hit = nullptr;
   for (auto &g : c->graphs)
-    if (g.dt_bits == bits && g.iters == iters && g.diag == (uint32_t)diag && g.mode == mode) hit = &g;
+    if (g.dt_bits == bits && g.iters == iters && g.diag == (uint32_t)diag && g.mode == mode && g.aux == aux) hit = &g;
   if (hit) {
     exec = hit->exec;
     hit->used = ++c->graph_serial;
   } else {
     hipGraph_t graph = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_substep(c, L, iters, diag, T, mode);
+    enq(L, T);
     hipError_t le = hipGetLastError();
     hipError_t ee = hipStreamEndCapture(c->stream, &graph);
     if (le != hipSuccess || ee != hipSuccess) {
@@ -393,10 +409,15 @@ int run_substep(pbf_ctx *c, float dt, uint32_t iters, uint32_t mode, bool diag) 
     (void)hipGraphDestroy(graph);
     if (ie != hipSuccess) return fail(c, PBF_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
     if (c->graphs.size() >= kGraphCache) evict_graph(c);
-    c->graphs.push_back({bits, iters, (uint32_t)diag, mode, exec, ++c->graph_serial});
+    c->graphs.push_back({bits, iters, (uint32_t)diag, mode, aux, exec, ++c->graph_serial});
   }
   HIPCHK(hipGraphLaunch(exec, c->stream));
   return PBF_OK;
+}
+
+int run_substep(pbf_ctx *c, float dt, uint32_t iters, uint32_t mode, bool diag) {
+  return run_enqueued(c, dt, iters, diag, mode, 0u,
+                      [&](const LaunchCtx &L, PassTimer &T) { enqueue_substep(c, L, iters, diag, T, mode); });
 }
 
 // Buffers of the learned and collection modes (allocated on first use).
@@ -428,54 +449,49 @@ int ensure_snapshot(pbf_ctx *c) {
 
 // Copies the state carried from one substep to the next (slot order), and
 // the readable omega / normal fields.
-void copy_state(pbf_ctx *c, bool save) {
+void copy_state(pbf_ctx *c, bool save, const uint32_t *gate = nullptr) {
   const size_t n = c->n;
   struct { void *live, *snap; size_t bytes; } t[] = {
       {c->pos, c->snap_pos, 16 * n}, {c->vel, c->snap_vel, 16 * n},   {c->acc, c->snap_acc, 16 * n},
       {c->ids, c->snap_ids, 4 * n},  {c->lam, c->snap_lam, 4 * n},    {c->lam_acc, c->snap_lacc, 4 * n},
       {c->om, c->snap_om, 16 * n},   {c->nrm, c->snap_nrm, 16 * n}};
   for (auto &e : t)
-    if (e.live && e.snap) launch_copy(c->stream, save ? e.snap : e.live, save ? e.live : e.snap, e.bytes);
+    if (e.live && e.snap) launch_copy(c->stream, save ? e.snap : e.live, save ? e.live : e.snap, e.bytes, gate);
 }
 
-int mean_density_error(pbf_ctx *c, double *mean);
-int max_speed(pbf_ctx *c, float dt, double *vmax);
-
-// One learned substep; when a rejection threshold is set, the substep is
-// checked and replaced by a plain solve if its mean density error is above
-// the threshold or a particle moved more than h in it (|v| dt > h: an
-// over-predicted lambda pushes particles apart, which lowers the density
-// error), or either figure is not finite.
+// One learned substep.  With a rejection threshold set, the check runs on the
+// device in the same launch sequence (graph): the state is saved, the learned
+// substep runs, the FP64 mean density error and the largest |v| dt / h are
+// reduced, k_fallback_gate decides, and the restore and a plain re-solve
+// (the collection mode: the plain solve that also keeps the lambda sum the
+// next learned substep reads) run behind that decision -- every one of their
+// kernels returns at once when the substep is kept.  No host synchronisation.
 int run_learned_substep(pbf_ctx *c, float dt, uint32_t iters) {
   c->learned_substeps += 1;
-  const uint32_t mode = c->warm_mode == PBF_WARM_PREVIOUS ? M_PREVIOUS
-                        : c->warm_mode == PBF_WARM_RATIO  ? M_RATIO
-                                                          : M_LEARNED;
+  const uint32_t mode = (c->warm_mode == PBF_WARM_PREVIOUS ? M_PREVIOUS
+                         : c->warm_mode == PBF_WARM_RATIO  ? M_RATIO
+                         : c->warm_mode == PBF_WARM_PLAIN  ? M_COLLECT
+                                                           : M_LEARNED) |
+                        (c->warm_project ? M_PROJECT : 0u);
   if (!(c->fb_threshold > 0.0)) return run_substep(c, dt, iters, mode, c->record_diag);
   int rc = ensure_snapshot(c);
   if (rc != PBF_OK) return rc;
-  copy_state(c, true);
-  rc = run_substep(c, dt, iters, mode, false);
-  if (rc != PBF_OK) return rc;
-  double mean = 0.0, vmax = 0.0;
-  rc = mean_density_error(c, &mean);
-  if (rc == PBF_OK) rc = max_speed(c, dt, &vmax);
-  if (rc != PBF_OK) return rc;
-  if (!(mean <= c->fb_threshold) || !(vmax <= 1.0)) {
-    c->fallbacks += 1;
-    copy_state(c, false);
-    // the collection mode is the plain solve that also keeps the lambda sum,
-    // which the next learned substep reads
-    rc = run_substep(c, dt, c->fb_iters ? c->fb_iters : iters, M_COLLECT, false);
-    if (rc != PBF_OK) return rc;
-  }
-  if (c->record_diag) {
-    LaunchCtx L = make_launch(c, dt);
-    launch_diag(L, c->pos, c->nbr, c->ncount, c->ci, c->ranges, (uint32_t)c->scenes.size(), c->per_scene, c->hist,
-                c->hist_cap, c->hist_ctr);
-    HIPCHK(hipGetLastError());
-  }
-  return PBF_OK;
+  const uint32_t fb_iters = c->fb_iters ? c->fb_iters : iters;
+  const uint32_t ns = (uint32_t)c->scenes.size();
+  return run_enqueued(c, dt, iters, c->record_diag, mode | M_FALLBACK, fb_iters, [&](const LaunchCtx &L, PassTimer &T) {
+    copy_state(c, true);
+    enqueue_substep(c, L, iters, false, T, mode);
+    launch_diag(L, c->pos, c->nbr, c->ncount, c->ci, c->ranges, ns, c->per_scene, nullptr, 0, nullptr);
+    launch_speed(L, c->vel, c->ci, c->ranges, ns, c->per_scene2);
+    launch_fallback_gate(c->stream, c->per_scene, c->per_scene2, ns, c->n, c->fb_thr, c->gate, c->fb_count);
+    copy_state(c, false, c->gate);
+    LaunchCtx G = L;
+    G.P.gate = c->gate;
+    enqueue_substep(c, G, fb_iters, false, T, M_COLLECT);
+    if (c->record_diag)
+      launch_diag(L, c->pos, c->nbr, c->ncount, c->ci, c->ranges, ns, c->per_scene, c->hist, c->hist_cap,
+                  c->hist_ctr);
+  });
 }
 
 int check_flags(pbf_ctx *c) {
@@ -496,36 +512,6 @@ int sync_ctx(pbf_ctx *c) {
   HIPCHK(hipStreamSynchronize(c->stream));
   drain_events(c);
   return check_flags(c);
-}
-
-// Mean density error over all scenes of the last completed substep.
-int mean_density_error(pbf_ctx *c, double *mean) {
-  const uint32_t ns = (uint32_t)c->scenes.size();
-  LaunchCtx L = make_launch(c, (float)c->params.time_step);
-  launch_diag(L, c->pos, c->nbr, c->ncount, c->ci, c->ranges, ns, c->per_scene, nullptr, 0, nullptr);
-  HIPCHK(hipGetLastError());
-  std::vector<double> ps(2 * (size_t)ns);
-  HIPCHK(hipMemcpyAsync(ps.data(), c->per_scene, ps.size() * 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  double t = 0.0;
-  for (uint32_t k = 0; k < ns; ++k) t += ps[2 * k];
-  *mean = c->n ? t / (double)c->n : 0.0;
-  return PBF_OK;
-}
-
-// Largest |v| dt / h over all particles.
-int max_speed(pbf_ctx *c, float dt, double *vmax) {
-  const uint32_t ns = (uint32_t)c->scenes.size();
-  LaunchCtx L = make_launch(c, dt);
-  launch_speed(L, c->vel, c->ci, c->ranges, ns, c->per_scene);
-  HIPCHK(hipGetLastError());
-  std::vector<double> ps(2 * (size_t)ns);
-  HIPCHK(hipMemcpyAsync(ps.data(), c->per_scene, ps.size() * 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  double m = 0.0;
-  for (uint32_t k = 0; k < ns; ++k) m = ps[2 * k + 1] > m || ps[2 * k + 1] != ps[2 * k + 1] ? ps[2 * k + 1] : m;
-  *vmax = m;
-  return PBF_OK;
 }
 
 // Grid of cell size h over the scene box; false when it cannot be indexed
@@ -706,6 +692,10 @@ int create_ctx_body(const pbf_params *params, uint32_t nscenes, const pbf_scene_
   step(dalloc(&c->ranges, 2 * (size_t)nscenes), "alloc");
   step(dalloc(&c->ci, N), "alloc");
   step(dalloc(&c->per_scene, 2 * (size_t)nscenes), "alloc");
+  step(dalloc(&c->per_scene2, 2 * (size_t)nscenes), "alloc");
+  step(dalloc(&c->gate, 1), "alloc");
+  step(dalloc(&c->fb_count, 1), "alloc");
+  step(dalloc(&c->fb_thr, 1), "alloc");
   step(dalloc(&c->hist, 2 * (size_t)c->hist_cap), "alloc");
   step(dalloc(&c->hist_ctr, 1), "alloc");
   // scan workspace: one level per factor of 2048
@@ -728,6 +718,9 @@ int create_ctx_body(const pbf_params *params, uint32_t nscenes, const pbf_scene_
     step(hipMemset(c->offsets, 0, ((size_t)c->ncells + 1) * 4), "hipMemset");
     step(hipMemset(c->flags, 0, 8), "hipMemset");
     step(hipMemset(c->hist_ctr, 0, 4), "hipMemset");
+    step(hipMemset(c->gate, 0, 4), "hipMemset");
+    step(hipMemset(c->fb_count, 0, 4), "hipMemset");
+    step(hipMemset(c->fb_thr, 0, 8), "hipMemset");
     step(hipMemset(c->ncount, 0, N * 4), "hipMemset");
   }
   if (rc == PBF_OK) rc = upload_scenes(c);
@@ -855,7 +848,7 @@ int pbf_destroy(pbf_ctx *c) {
                   c->offsets, c->flags, c->d_scenes, c->sid, c->ranges, c->pool, c->ci, c->per_scene, c->hist,
                   c->hist_ctr, c->dbg_counts, c->dbg_ids, c->ids_save, c->lam_acc, c->acc_sorted, c->feats,
                   c->vel_sorted, c->snap_pos, c->snap_vel, c->snap_acc, c->snap_om, c->snap_nrm, c->snap_ids,
-                  c->snap_lam, c->snap_lacc};
+                  c->snap_lam, c->snap_lacc, c->per_scene2, c->gate, c->fb_count, c->fb_thr};
   mlp_engine_free(&c->mlp);
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -1067,8 +1060,8 @@ int pbf_step(pbf_ctx *c, float dt, uint32_t substeps, uint32_t iterations) try {
   const uint32_t iters = iterations ? iterations : c->cfg.iteration;
   const float sdt = (float)((double)dt / (double)substeps);
   for (uint32_t s = 0; s < substeps; ++s) {
-    int rc = c->warm_mode != PBF_WARM_PLAIN ? run_learned_substep(c, sdt, iters)
-                                            : run_substep(c, sdt, iters, M_PLAIN, c->record_diag);
+    int rc = c->warm_mode != PBF_WARM_PLAIN || c->warm_project ? run_learned_substep(c, sdt, iters)
+                                                                : run_substep(c, sdt, iters, M_PLAIN, c->record_diag);
     if (rc != PBF_OK) return rc;
   }
   return PBF_OK;
@@ -1077,6 +1070,8 @@ int pbf_step(pbf_ctx *c, float dt, uint32_t substeps, uint32_t iterations) try {
 int pbf_set_warm_start(pbf_ctx *c, const float *blob, size_t length, uint32_t mode, float relax,
                        double fallback_threshold, uint32_t fallback_iterations) try {
   if (!c) return PBF_E_INVALID;
+  const bool project = (mode & PBF_WARM_PROJECT) != 0;
+  mode &= ~(uint32_t)PBF_WARM_PROJECT;
   if (mode > PBF_WARM_RATIO) return fail(c, PBF_E_INVALID, "unknown warm-start mode");
   if (!(relax > 0.0f && relax <= 1.0f)) return fail(c, PBF_E_INVALID, "relax must be in (0, 1]");
   HIPCHK(hipSetDevice(c->device));
@@ -1099,14 +1094,16 @@ int pbf_set_warm_start(pbf_ctx *c, const float *blob, size_t length, uint32_t mo
   }
   if (fallback_threshold < 0.0 || !std::isfinite(fallback_threshold))
     return fail(c, PBF_E_INVALID, "fallback_threshold must be finite and >= 0");
-  if (mode != PBF_WARM_PLAIN) {
+  if (mode != PBF_WARM_PLAIN || project) {
     int rc = ensure_learn_buffers(c);
     if (rc != PBF_OK) return rc;
   }
   c->relax_prev = relax;
   if (mode != c->warm_mode && c->lam_acc) HIPCHK(hipMemsetAsync(c->lam_acc, 0, (size_t)c->ncap * 4, c->stream));
   c->warm_mode = mode;
+  c->warm_project = project;
   c->fb_threshold = fallback_threshold;
+  HIPCHK(hipMemcpyAsync(c->fb_thr, &c->fb_threshold, 8, hipMemcpyHostToDevice, c->stream));
   c->fb_iters = fallback_iterations;
   clear_graphs(c);
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1197,7 +1194,14 @@ int pbf_autotune(pbf_ctx *c, float dt, uint32_t iterations, uint32_t repeats, pb
 int pbf_warm_start_stats(const pbf_ctx *c, uint64_t *learned_substeps, uint64_t *fallbacks) {
   if (!c) return PBF_E_INVALID;
   if (learned_substeps) *learned_substeps = c->learned_substeps;
-  if (fallbacks) *fallbacks = c->fallbacks;
+  if (fallbacks) {
+    // counted on the device by the fallback decision
+    uint32_t f = 0;
+    if (hipSetDevice(c->device) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess ||
+        hipMemcpy(&f, c->fb_count, 4, hipMemcpyDeviceToHost) != hipSuccess)
+      return PBF_E_DEVICE;
+    *fallbacks = f;
+  }
   return PBF_OK;
 }
 
@@ -1236,7 +1240,7 @@ int pbf_collect_training(pbf_ctx *c, float dt, uint32_t long_iterations, uint32_
   // ... and the state then advances by one substep of the selected solver
   copy_state(c, false);
   const uint32_t iters = iterations ? iterations : c->cfg.iteration;
-  if (c->warm_mode != PBF_WARM_PLAIN) return run_learned_substep(c, dt, iters);
+  if (c->warm_mode != PBF_WARM_PLAIN || c->warm_project) return run_learned_substep(c, dt, iters);
   return run_substep(c, dt, iters, M_COLLECT, c->record_diag);
 } PBF_CATCH(c)
 

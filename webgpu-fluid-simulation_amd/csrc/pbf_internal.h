@@ -47,7 +47,8 @@ constexpr uint32_t kListVariants = 4;
 void launch_predict_bin(const LaunchCtx &L, const float4 *pos, const float4 *vel, const float4 *acc, float4 *xs,
                         uint32_t *cellkey, uint32_t *rank, uint32_t *counts);
 void launch_scan(hipStream_t st, uint32_t *in, uint32_t n, uint32_t *out, int zero_in, uint32_t *const *sums,
-                 uint32_t *const *prefix, int level, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                 uint32_t *const *prefix, int level, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
+                 const uint32_t *gate = nullptr);
 void launch_scatter(const LaunchCtx &L, const uint32_t *cellkey, const uint32_t *rank, const uint32_t *offsets,
                     const uint32_t *ids, uint32_t *tmp_key, uint32_t *tmp_src);
 void launch_settle(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *tmp_src, const uint32_t *cellkey,
@@ -56,7 +57,7 @@ void launch_settle(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *
 void launch_build_list(const LaunchCtx &L, const float4 *X0, const uint32_t *offsets, uint32_t *nbr, uint32_t *ncount,
                        uint32_t *flags);
 void launch_lambda(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr, const uint32_t *ncount,
-                   float *lam_acc = nullptr);
+                   float *lam_acc = nullptr, bool project = false);
 void launch_lambda_features(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr, const uint32_t *ncount,
                             const LearnBufs &lb);
 void launch_settle_extra(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *tmp_src, const uint32_t *cellkey,
@@ -67,7 +68,9 @@ int mlp_engine_relax(void *handle, float relax, hipStream_t st);
 void mlp_engine_free(void **handle);
 void launch_mlp_infer(hipStream_t st, void *handle, uint32_t n, const float *feats, float4 *X, float *lam_acc,
                       float smax, hipEvent_t e0, hipEvent_t e1);
-void launch_copy(hipStream_t st, void *dst, const void *src, size_t bytes);
+void launch_copy(hipStream_t st, void *dst, const void *src, size_t bytes, const uint32_t *gate = nullptr);
+void launch_fallback_gate(hipStream_t st, const double *err, const double *speed, uint32_t nscenes, uint32_t n,
+                          const double *threshold, uint32_t *gate, uint32_t *count);
 void launch_warm_previous(const LaunchCtx &L, const float *acc_prev, float4 *X, float *lam_acc, float relax);
 void launch_speed(const LaunchCtx &L, const float4 *vel, double *ci, const uint32_t *ranges, uint32_t nscenes,
                   double *per_scene);

@@ -336,6 +336,7 @@ __global__ void __launch_bounds__(BS) k_predict_bin(uint32_t n, const float4 *__
                                                     uint32_t *__restrict__ rank, uint32_t *__restrict__ counts,
                                                     Scene s0, const Scene *__restrict__ scenes,
                                                     const uint32_t *__restrict__ sid, Phys P) {
+  if (gated_off(P.gate)) return;
   const uint32_t i = slot_index();
   const bool active = i < n;
   uint32_t key = 0;
@@ -362,7 +363,8 @@ constexpr uint32_t SCAN_E = BS * SCAN_V;  // 2048 elements per block
 __device__ __forceinline__ int pad32(int i) { return i + (i >> 5); }
 
 __global__ void __launch_bounds__(BS) k_scan_reduce(const uint32_t *__restrict__ in, uint32_t n,
-                                                    uint32_t *__restrict__ sums) {
+                                                    uint32_t *__restrict__ sums, const uint32_t *gate) {
+  if (gated_off(gate)) return;
   __shared__ uint32_t sh[BS / 64];
   const uint32_t base = blockIdx.x * SCAN_E + threadIdx.x * SCAN_V;
   uint32_t s = 0;
@@ -388,7 +390,8 @@ __global__ void __launch_bounds__(BS) k_scan_reduce(const uint32_t *__restrict__
 // out[0..n] = exclusive scan of in[0..n) (+ prefix[block]); out[n] = total.
 __global__ void __launch_bounds__(BS) k_scan_down(uint32_t *__restrict__ in, uint32_t n,
                                                   const uint32_t *__restrict__ prefix, uint32_t *__restrict__ out,
-                                                  int zero_in) {
+                                                  int zero_in, const uint32_t *gate) {
+  if (gated_off(gate)) return;
   __shared__ uint32_t sh[BS + BS / 32];
   const int t = threadIdx.x;
   const uint32_t base = blockIdx.x * SCAN_E + (uint32_t)t * SCAN_V;
@@ -459,7 +462,8 @@ __global__ void __launch_bounds__(BS) k_scan_down(uint32_t *__restrict__ in, uin
 __global__ void __launch_bounds__(BS) k_scatter(uint32_t n, const uint32_t *__restrict__ cellkey,
                                                 const uint32_t *__restrict__ rank, const uint32_t *__restrict__ offsets,
                                                 const uint32_t *__restrict__ ids, uint32_t *__restrict__ tmp_key,
-                                                uint32_t *__restrict__ tmp_src) {
+                                                uint32_t *__restrict__ tmp_src, const uint32_t *gate) {
+  if (gated_off(gate)) return;
   const uint32_t i = slot_index();
   if (i >= n) return;
   uint32_t s = offsets[cellkey[i]] + rank[i];
@@ -475,7 +479,9 @@ __global__ void __launch_bounds__(BS) k_settle(uint32_t n, const uint32_t *__res
                                                const uint32_t *__restrict__ offsets, const float4 *__restrict__ xs,
                                                const float4 *__restrict__ pos, const float *__restrict__ lam_in,
                                                float4 *__restrict__ X0, float4 *__restrict__ Xold,
-                                               uint32_t *__restrict__ ids, float *__restrict__ lam_out) {
+                                               uint32_t *__restrict__ ids, float *__restrict__ lam_out,
+                                               const uint32_t *gate) {
+  if (gated_off(gate)) return;
   const uint32_t s = slot_index();
   if (s >= n) return;
   const uint32_t p = tmp_src[s], key = tmp_key[s];
@@ -497,7 +503,9 @@ __global__ void __launch_bounds__(BS) k_settle_extra(uint32_t n, const uint32_t 
                                                      const uint32_t *__restrict__ cellkey,
                                                      const uint32_t *__restrict__ offsets,
                                                      const float *__restrict__ acc_in, float *__restrict__ acc_out,
-                                                     const float4 *__restrict__ vel_in, float4 *__restrict__ vel_out) {
+                                                     const float4 *__restrict__ vel_in, float4 *__restrict__ vel_out,
+                                                     const uint32_t *gate) {
+  if (gated_off(gate)) return;
   const uint32_t s = slot_index();
   if (s >= n) return;
   const uint32_t p = tmp_src[s], key = tmp_key[s];
@@ -524,7 +532,8 @@ __global__ void __launch_bounds__(BS) k_warm_previous(uint32_t n, const float *_
 }
 
 __global__ void __launch_bounds__(BS) k_copy_words(uint32_t *__restrict__ dst, const uint32_t *__restrict__ src,
-                                                   size_t n) {
+                                                   size_t n, const uint32_t *gate) {
+  if (gated_off(gate)) return;
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) dst[i] = src[i];
@@ -545,6 +554,7 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
                                                         uint32_t *__restrict__ flags, Scene s0,
                                                         const Scene *__restrict__ scenes,
                                                         const uint32_t *__restrict__ sid, Phys P) {
+  if (gated_off(P.gate)) return;
   __shared__ uint32_t tile[(LIST_R + 1) * LIST_BS];  // row LIST_R: spare
   const uint32_t tid = threadIdx.x;
   const uint32_t base_slot = xcd_block(blockIdx.x, gridDim.x) * LIST_BS;
@@ -743,6 +753,7 @@ __global__ void __launch_bounds__(BS) k_boundary(uint32_t n, const float4 *__res
                                                  const float *__restrict__ pool, Scene s0,
                                                  const Scene *__restrict__ scenes, const uint32_t *__restrict__ sid,
                                                  Phys P) {
+  if (gated_off(P.gate)) return;
   const uint32_t i = slot_index();
   if (i >= n) return;
   const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
@@ -752,13 +763,18 @@ __global__ void __launch_bounds__(BS) k_boundary(uint32_t n, const float4 *__res
 
 // K9: density constraint and lambda, lambda to X[i].w.  FK: 0 no boundary,
 // 1 boundary samples already in B (k_boundary), 2 trilinear maps sampled here.
-// ACC: also add lambda to the per-substep sum (learned / data-generation modes).
-template <bool MULTI, int FK, bool ACC = false>
+// ACC: 0 plain; 1 also add lambda to the per-substep sum (learned / data-
+// generation modes); 2 projected accumulation: the increment comes from the
+// unclamped constraint rho~ - 1, the substep's sum stays <= 0, and X[i].w
+// receives the change of the sum (an over-pushed warm start is pulled back,
+// never past a zero sum).
+template <bool MULTI, int FK, int ACC = 0>
 __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__restrict__ B,
                                                const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ ncount,
                                                const float *__restrict__ pool, Scene s0,
                                                const Scene *__restrict__ scenes, const uint32_t *__restrict__ sid,
                                                Phys P, float *__restrict__ lam_acc) {
+  if (gated_off(P.gate)) return;
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = X[i];
@@ -802,11 +818,18 @@ __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__
     rho = fmaf(b.w, poly6(P, d2), rho);
   }
   float c = rho - 1.0f;
-  float C = c > 0.0f ? c : 0.0f;
   float den = fmaf(gz, gz, fmaf(gy, gy, fmaf(gx, gx, s2)));
-  const float lam = (-C) / (den + P.eps_lambda);
-  reinterpret_cast<float *>(X)[4 * (size_t)i + 3] = lam;
-  if constexpr (ACC) lam_acc[i] += lam;
+  if constexpr (ACC == 2) {
+    const float l0 = lam_acc[i];
+    const float l1 = fminf(l0 + (-c) / (den + P.eps_lambda), 0.0f);
+    reinterpret_cast<float *>(X)[4 * (size_t)i + 3] = l1 - l0;
+    lam_acc[i] = l1;
+  } else {
+    float C = c > 0.0f ? c : 0.0f;
+    const float lam = (-C) / (den + P.eps_lambda);
+    reinterpret_cast<float *>(X)[4 * (size_t)i + 3] = lam;
+    if constexpr (ACC == 1) lam_acc[i] += lam;
+  }
 }
 
 struct FeatRec {
@@ -824,6 +847,7 @@ __global__ void __launch_bounds__(BS) k_lambda_feat(uint32_t n, float4 *X, float
                                                     const float *__restrict__ pool, Scene s0,
                                                     const Scene *__restrict__ scenes,
                                                     const uint32_t *__restrict__ sid, Phys P, LearnBufs lb) {
+  if (gated_off(P.gate)) return;
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = X[i];
@@ -888,6 +912,7 @@ __global__ void __launch_bounds__(BS) k_dp(uint32_t n, const float4 *__restrict_
                                            const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ ncount,
                                            float4 *__restrict__ Xn, Scene s0, const Scene *__restrict__ scenes,
                                            const uint32_t *__restrict__ sid, Phys P) {
+  if (gated_off(P.gate)) return;
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = X[i];
@@ -921,6 +946,7 @@ __global__ void __launch_bounds__(BS) k_density(uint32_t n, const float4 *__rest
                                                 const float4 *__restrict__ Xold, const uint32_t *__restrict__ nbr,
                                                 const uint32_t *__restrict__ ncount, float4 *__restrict__ pos,
                                                 float4 *__restrict__ vel, float *__restrict__ lam, Phys P) {
+  if (gated_off(P.gate)) return;
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = X[i];
@@ -945,6 +971,7 @@ __global__ void __launch_bounds__(BS) k_vorticity(uint32_t n, const float4 *__re
                                                   const float4 *__restrict__ vel, const uint32_t *__restrict__ nbr,
                                                   const uint32_t *__restrict__ ncount, float4 *__restrict__ om,
                                                   float4 *__restrict__ nrm, float4 *__restrict__ visc, Phys P) {
+  if (gated_off(P.gate)) return;
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = pos[i], vi = vel[i];
@@ -977,6 +1004,7 @@ __global__ void __launch_bounds__(BS) k_xsph(uint32_t n, const float4 *__restric
                                              const float4 *__restrict__ om, const float4 *__restrict__ nrm,
                                              const float4 *__restrict__ visc, const uint32_t *__restrict__ nbr,
                                              const uint32_t *__restrict__ ncount, float4 *__restrict__ acc, Phys P) {
+  if (gated_off(P.gate)) return;
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = pos[i], ni = nrm[i], wi = om[i], ui = visc[i];
@@ -1017,6 +1045,7 @@ __global__ void __launch_bounds__(BS) k_diag(uint32_t n, const float4 *__restric
                                              const float *__restrict__ pool, Scene s0,
                                              const Scene *__restrict__ scenes, const uint32_t *__restrict__ sid,
                                              Phys P, double *__restrict__ ci) {
+  if (gated_off(P.gate)) return;
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = pos[i];
@@ -1083,6 +1112,30 @@ __global__ void __launch_bounds__(BS) k_speed(uint32_t n, const float4 *__restri
   if (i >= n) return;
   const float4 v = vel[i];
   ci[i] = (double)(sqrtf(dist2(v.x, v.y, v.z)) * scale);
+}
+
+// The learned substep's check, on the device: gate = 1 (redo the substep
+// plainly) when the mean density error over all scenes is above the
+// threshold, a particle moved more than h (|v| dt / h > 1: an over-pushed
+// warm start lowers the density error by spreading the fluid), or either
+// figure is not finite; the rejections are counted.
+__global__ void k_fallback_gate(const double *__restrict__ err, const double *__restrict__ speed,
+                                uint32_t nscenes, uint32_t n, const double *__restrict__ threshold,
+                                uint32_t *__restrict__ gate, uint32_t *__restrict__ count) {
+  if (threadIdx.x != 0) return;
+  double t = 0.0, v = 0.0;
+  bool finite = true;
+  for (uint32_t s = 0; s < nscenes; ++s) {
+    t += err[2 * s];
+    // a NaN speed drops out of the max but not out of the sum
+    finite = finite && speed[2 * s] == speed[2 * s];
+    const double m = speed[2 * s + 1];
+    v = m > v ? m : v;
+  }
+  const double mean = n ? t / (double)n : 0.0;
+  const bool bad = !finite || !(mean <= *threshold) || !(v <= 1.0);
+  *gate = bad ? 1u : 0u;
+  if (bad) *count += 1u;
 }
 
 // Sum the per-scene totals into history[ctr % cap] = (mean, max) over all.
@@ -1154,23 +1207,23 @@ void launch_predict_bin(const LaunchCtx &L, const float4 *pos, const float4 *vel
 }
 
 void launch_scan(hipStream_t st, uint32_t *in, uint32_t n, uint32_t *out, int zero_in, uint32_t *const *sums,
-                 uint32_t *const *prefix, int level, hipEvent_t e0, hipEvent_t e1) {
+                 uint32_t *const *prefix, int level, hipEvent_t e0, hipEvent_t e1, const uint32_t *gate) {
   const dim3 b(BS);
   if (n <= SCAN_E) {
-    go(st, e0, e1, k_scan_down, dim3(1), b, in, n, (const uint32_t *)nullptr, out, zero_in);
+    go(st, e0, e1, k_scan_down, dim3(1), b, in, n, (const uint32_t *)nullptr, out, zero_in, gate);
     return;
   }
   uint32_t nb = (n + SCAN_E - 1) / SCAN_E;
-  go(st, e0, nullptr, k_scan_reduce, dim3(nb), b, (const uint32_t *)in, n, sums[level]);
-  launch_scan(st, sums[level], nb, prefix[level], 0, sums, prefix, level + 1, nullptr, nullptr);
-  go(st, nullptr, e1, k_scan_down, dim3(nb), b, in, n, (const uint32_t *)prefix[level], out, zero_in);
+  go(st, e0, nullptr, k_scan_reduce, dim3(nb), b, (const uint32_t *)in, n, sums[level], gate);
+  launch_scan(st, sums[level], nb, prefix[level], 0, sums, prefix, level + 1, nullptr, nullptr, gate);
+  go(st, nullptr, e1, k_scan_down, dim3(nb), b, in, n, (const uint32_t *)prefix[level], out, zero_in, gate);
 }
 
 void launch_scatter(const LaunchCtx &L, const uint32_t *cellkey, const uint32_t *rank, const uint32_t *offsets,
                     const uint32_t *ids, uint32_t *tmp_key, uint32_t *tmp_src) {
   if (!L.n) return;
   go(L.stream, L.ev0, L.ev1, k_scatter, dim3(nblocks(L.n, BS)), dim3(BS), L.n, cellkey, rank, offsets, ids, tmp_key,
-     tmp_src);
+     tmp_src, L.P.gate);
 }
 
 void launch_settle(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *tmp_src, const uint32_t *cellkey,
@@ -1178,7 +1231,7 @@ void launch_settle(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *
                    float4 *Xold, uint32_t *ids, float *lam_out) {
   if (!L.n) return;
   go(L.stream, L.ev0, L.ev1, k_settle, dim3(nblocks(L.n, BS)), dim3(BS), L.n, tmp_key, tmp_src, cellkey, offsets, xs,
-     pos, lam_in, X0, Xold, ids, lam_out);
+     pos, lam_in, X0, Xold, ids, lam_out, L.P.gate);
 }
 
 template <int R, int B>
@@ -1215,8 +1268,9 @@ static void fk_dispatch(const LaunchCtx &L, hipEvent_t e0, hipEvent_t e1, dim3 g
   else if (L.fmode == 1) go(L.stream, e0, e1, Sel<MULTI, 1>::fn(), g, dim3(L.bs), args...);
   else go(L.stream, e0, e1, Sel<MULTI, 0>::fn(), g, dim3(L.bs), args...);
 }
-template <bool M, int F> struct SelLambda { static auto fn() { return k_lambda<M, F, false>; } };
-template <bool M, int F> struct SelLambdaAcc { static auto fn() { return k_lambda<M, F, true>; } };
+template <bool M, int F> struct SelLambda { static auto fn() { return k_lambda<M, F, 0>; } };
+template <bool M, int F> struct SelLambdaAcc { static auto fn() { return k_lambda<M, F, 1>; } };
+template <bool M, int F> struct SelLambdaProj { static auto fn() { return k_lambda<M, F, 2>; } };
 template <bool M, int F> struct SelLambdaFeat { static auto fn() { return k_lambda_feat<M, F>; } };
 template <bool M, int F> struct SelDp { static auto fn() { return k_dp<M, F>; } };
 template <bool M, int F> struct SelDiag { static auto fn() { return k_diag<M, F>; } };
@@ -1229,11 +1283,14 @@ static hipEvent_t boundary_pass(const LaunchCtx &L, float4 *X, float4 *B, dim3 g
 }
 
 void launch_lambda(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr, const uint32_t *ncount,
-                   float *lam_acc) {
+                   float *lam_acc, bool project) {
   if (!L.n) return;
   hipEvent_t e0 = boundary_pass(L, X, B, dim3(nblocks(L.n, BS)));
   const dim3 g(nblocks(L.n, L.bs));
-  if (lam_acc) {
+  if (lam_acc && project) {
+    if (L.multi) fk_dispatch<true, SelLambdaProj>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lam_acc);
+    else fk_dispatch<false, SelLambdaProj>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lam_acc);
+  } else if (lam_acc) {
     if (L.multi) fk_dispatch<true, SelLambdaAcc>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lam_acc);
     else fk_dispatch<false, SelLambdaAcc>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lam_acc);
   } else {
@@ -1256,7 +1313,7 @@ void launch_settle_extra(const LaunchCtx &L, const uint32_t *tmp_key, const uint
                          float4 *vel_out) {
   if (!L.n) return;
   go(L.stream, nullptr, nullptr, k_settle_extra, dim3(nblocks(L.n, BS)), dim3(BS), L.n, tmp_key, tmp_src, cellkey,
-     offsets, acc_in, acc_out, vel_in, vel_out);
+     offsets, acc_in, acc_out, vel_in, vel_out, L.P.gate);
 }
 
 void launch_warm_previous(const LaunchCtx &L, const float *acc_prev, float4 *X, float *lam_acc, float relax) {
@@ -1264,10 +1321,16 @@ void launch_warm_previous(const LaunchCtx &L, const float *acc_prev, float4 *X, 
   go(L.stream, nullptr, nullptr, k_warm_previous, dim3(nblocks(L.n, BS)), dim3(BS), L.n, acc_prev, X, lam_acc, relax);
 }
 
-void launch_copy(hipStream_t st, void *dst, const void *src, size_t bytes) {
+void launch_copy(hipStream_t st, void *dst, const void *src, size_t bytes, const uint32_t *gate) {
   size_t n = bytes / 4;
   uint32_t grid = (uint32_t)std::min<size_t>((n + BS - 1) / BS, 4096u);
-  if (n) hipLaunchKernelGGL(k_copy_words, dim3(grid), dim3(BS), 0, st, (uint32_t *)dst, (const uint32_t *)src, n);
+  if (n)
+    hipLaunchKernelGGL(k_copy_words, dim3(grid), dim3(BS), 0, st, (uint32_t *)dst, (const uint32_t *)src, n, gate);
+}
+
+void launch_fallback_gate(hipStream_t st, const double *err, const double *speed, uint32_t nscenes, uint32_t n,
+                          const double *threshold, uint32_t *gate, uint32_t *count) {
+  hipLaunchKernelGGL(k_fallback_gate, dim3(1), dim3(64), 0, st, err, speed, nscenes, n, threshold, gate, count);
 }
 
 void launch_dp(const LaunchCtx &L, const float4 *X, const float4 *B, const uint32_t *nbr, const uint32_t *ncount,

@@ -235,7 +235,8 @@ struct WaveLds {
 
 __global__ void __launch_bounds__(256) k_mlp_train(uint32_t nb, const uint32_t *__restrict__ idx,
                                                    const float *__restrict__ feats, const float *__restrict__ targ,
-                                                   MlpDev M, float inv_batch, float *__restrict__ grad,
+                                                   MlpDev M, float inv_batch, float over_w,
+                                                   float *__restrict__ grad,
                                                    float *__restrict__ loss, float *__restrict__ yout) {
   __shared__ float gacc[NPARAM];
   __shared__ WaveLds wl[TW];
@@ -257,8 +258,10 @@ __global__ void __launch_bounds__(256) k_mlp_train(uint32_t nb, const uint32_t *
     const float tn = valid ? (targ[p] - M.misc[OFF_TMU - OFF_B3]) / M.misc[OFF_TSIG - OFF_B3] : 0.0f;
     const float e = valid ? T.y - tn : 0.0f;
     if (yout && valid && h == 0) yout[q] = T.y;
-    if (h == 0) lsum = fmaf(e, e, lsum);
-    const float dy = 2.0f * e * inv_batch;
+    // over-predictions (e > 0) weigh over_w in the loss
+    const float wgt = e > 0.0f ? over_w : 1.0f;
+    if (h == 0) lsum = fmaf(wgt * e, e, lsum);
+    const float dy = 2.0f * (wgt * e) * inv_batch;
     // b3 and w3
     float db3 = h == 0 ? dy : 0.0f;
     // dz2 = w3 dy silu'(z2); dw3 = a2 dy; db2 = dz2
@@ -603,11 +606,12 @@ int train_setup(TrainState &S, const float *features, const float *targets, uint
   return PBF_OK;
 }
 
-void launch_train(TrainState &S, uint32_t nb, const uint32_t *idx, float inv_batch, float *yout) {
+void launch_train(TrainState &S, uint32_t nb, const uint32_t *idx, float inv_batch, float *yout,
+                  float over_w = 1.0f) {
   const uint32_t tiles = (nb + 31) / 32;
   const uint32_t blocks = std::max<uint32_t>(1u, std::min<uint32_t>((tiles + TW - 1) / TW, 512u));
   hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(64 * TW), 0, S.st, nb, idx, (const float *)S.feats,
-                     (const float *)S.targ, mlp_view(S.B), inv_batch, S.grad, S.loss, yout);
+                     (const float *)S.targ, mlp_view(S.B), inv_batch, over_w, S.grad, S.loss, yout);
 }
 
 // Address of the scratch array on the current device, looked up once per
@@ -695,7 +699,7 @@ int pbf_mlp_forward_backward_noalloc(const float *blob, const float *features, c
   const uint32_t tiles = (n + 31) / 32;
   const uint32_t blocks = std::max<uint32_t>(1u, std::min<uint32_t>((tiles + TW - 1) / TW, 512u));
   hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(64 * TW), 0, st, n, (const uint32_t *)nullptr, features,
-                     targets, mlp_view(B), 1.0f / (float)n, gacc, lacc, y);
+                     targets, mlp_view(B), 1.0f / (float)n, 1.0f, gacc, lacc, y);
   if (grad || loss)
     hipLaunchKernelGGL(k_mlp_copy_out, dim3((NPARAM + 255) / 256), dim3(256), 0, st, (const float *)gacc,
                        (const float *)lacc, grad, loss);
@@ -706,7 +710,13 @@ int pbf_mlp_forward_backward_noalloc(const float *blob, const float *features, c
 
 int pbf_mlp_train(const float *features, const float *targets, uint64_t n, uint32_t epochs, uint32_t batch,
                   float lr, uint64_t seed, float *blob, float *loss_per_epoch) {
+  return pbf_mlp_train_weighted(features, targets, n, epochs, batch, lr, seed, 1.0f, blob, loss_per_epoch);
+}
+
+int pbf_mlp_train_weighted(const float *features, const float *targets, uint64_t n, uint32_t epochs, uint32_t batch,
+                           float lr, uint64_t seed, float over_weight, float *blob, float *loss_per_epoch) {
   if (!features || !targets || !blob || n == 0 || batch == 0 || n > 0xFFFFFFFFull) return PBF_E_INVALID;
+  if (!(over_weight > 0.0f) || !std::isfinite(over_weight)) return PBF_E_INVALID;
   TrainState S;
   int rc = train_setup(S, features, targets, n, blob);
   if (rc != PBF_OK) return rc;
@@ -726,7 +736,7 @@ int pbf_mlp_train(const float *features, const float *targets, uint64_t n, uint3
     for (uint64_t b = 0; b < nbatches; ++b) {
       const uint32_t start = (uint32_t)(b * batch);
       const uint32_t nb = (uint32_t)std::min<uint64_t>(batch, n - start);
-      launch_train(S, nb, S.idx + start, 1.0f / (float)nb, nullptr);
+      launch_train(S, nb, S.idx + start, 1.0f / (float)nb, nullptr, over_weight);
       ++t;
       const float b1c = 1.0f - std::pow(0.9f, (float)t), b2c = 1.0f - std::pow(0.999f, (float)t);
       hipLaunchKernelGGL(k_mlp_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, S.st, S.B.w, S.m, S.v, S.grad, lr,

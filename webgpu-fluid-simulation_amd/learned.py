@@ -111,11 +111,13 @@ class TrainResult:
 
 
 def train(features: np.ndarray, targets: np.ndarray, epochs: int = 20, batch: int = 4096, lr: float = 2e-3,
-          seed: int = 7, blob: Optional[np.ndarray] = None) -> TrainResult:
-    """Fit the network; a given blob is fine-tuned (its normalisation kept)."""
+          seed: int = 7, blob: Optional[np.ndarray] = None, over_weight: float = 1.0) -> TrainResult:
+    """Fit the network; a given blob is fine-tuned (its normalisation kept).
+    over_weight > 1: over-predictions cost more (a low-side expectile)."""
     if blob is None:
         blob = set_normalisation(mlp_init(seed), features, targets)
-    out, losses = mlp_train(features, targets, blob, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    out, losses = mlp_train(features, targets, blob, epochs=epochs, batch=batch, lr=lr, seed=seed,
+                            over_weight=over_weight)
     return TrainResult(out, losses)
 
 

@@ -115,6 +115,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "pbf_mlp_init": (I, [U64, pf]),
         "pbf_mlp_forward_backward": (I, [pf, pf, pf, U32, pf, pf, pf]),
         "pbf_mlp_train": (I, [pf, pf, U64, U32, U32, F, U64, pf, pf]),
+        "pbf_mlp_train_weighted": (I, [pf, pf, U64, U32, U32, F, U64, F, pf, pf]),
         "pbf_mlp_forward_backward_noalloc": (I, [P, P, P, U32, P, P, P, P]),
         "pbf_set_warm_start": (I, [P, pf, S, U32, F, D, U32]),
         "pbf_warm_start_stats": (I, [P, C.POINTER(U64), C.POINTER(U64)]),
@@ -267,6 +268,7 @@ def copy_bandwidth(nbytes: int = 1 << 30, iterations: int = 20) -> float:
 MLP_PARAMS = 5313
 MLP_FEATURES = 16
 WARM_PLAIN, WARM_LEARNED, WARM_PREVIOUS, WARM_RATIO = 0, 1, 2, 3
+WARM_PROJECT = 0x100  # OR-ed into a mode: projected accumulation of lambda
 WARM_RATIO_MAX = 2.0  # clamp of the ratio-mode factor, in units of relax
 
 
@@ -303,15 +305,16 @@ def mlp_forward_backward(blob: np.ndarray, features: np.ndarray, targets: np.nda
 
 
 def mlp_train(features: np.ndarray, targets: np.ndarray, blob: np.ndarray, epochs: int = 10, batch: int = 4096,
-              lr: float = 1e-3, seed: int = 0):
-    """Adam on the GPU; returns (trained blob, mean loss per epoch)."""
+              lr: float = 1e-3, seed: int = 0, over_weight: float = 1.0):
+    """Adam on the GPU; returns (trained blob, mean loss per epoch).
+    over_weight > 1 weighs over-predictions more (pbf_mlp_train_weighted)."""
     lib = load_library()
     blob = np.array(blob, np.float32, copy=True)
     f = np.ascontiguousarray(features, np.float32).reshape(-1, MLP_FEATURES)
     t = np.ascontiguousarray(targets, np.float32).reshape(-1)
     losses = np.zeros(max(int(epochs), 1), np.float32)
-    rc = lib.pbf_mlp_train(_fptr(f), _fptr(t), f.shape[0], int(epochs), int(batch), float(lr), int(seed),
-                           _fptr(blob), _fptr(losses))
+    rc = lib.pbf_mlp_train_weighted(_fptr(f), _fptr(t), f.shape[0], int(epochs), int(batch), float(lr), int(seed),
+                                    float(over_weight), _fptr(blob), _fptr(losses))
     if rc != PBF_OK:
         raise PBFError(rc, "pbf_mlp_train failed")
     return blob, losses[: int(epochs)]
