@@ -286,6 +286,11 @@ enum { PBF_WARM_PLAIN = 0, PBF_WARM_LEARNED = 1, PBF_WARM_PREVIOUS = 2, PBF_WARM
  * apart is pulled back by the later iterations, never past a zero sum.  With
  * PBF_WARM_PLAIN it is the cold-started solve in that form. */
 #define PBF_WARM_PROJECT 0x100u
+/* OR'ed into a warm mode: before iteration 0 applies it, the warm lambda
+ * field is replaced by its poly6-weighted (Shepard) average over each
+ * particle's neighbours, so its gradient -- the first displacement -- is
+ * free of particle-scale noise. */
+#define PBF_WARM_SMOOTH 0x200u
 #define PBF_WARM_RATIO_MAX 2.0f
 
 uint32_t pbf_mlp_blob_size(void);
@@ -331,10 +336,14 @@ int pbf_mlp_train_weighted(const float *features, const float *targets, uint64_t
  * predicted lambda: the later iterations only remove compression, so an
  * over-predicted lambda pushes particles apart for good (kinetic energy);
  * relax < 1 keeps the warm start on the safe side.  fallback_threshold > 0
- * checks each learned substep: when its mean density error is above the
- * threshold, a particle moved more than h in it, or either is not finite,
- * the substep is undone and solved plainly with fallback_iterations (0: the
- * same count). */
+ * checks each learned substep on the device, in the substep's own launch
+ * sequence, and the host reads the one-word decision after it: when its
+ * mean density error is above the
+ * threshold, its mechanical energy (kinetic + potential, summed in FP64) rose
+ * by more than 0.1 % of the kinetic energy it started from, a particle moved
+ * more than 2 h in it, or any of these is not finite, the substep is undone
+ * and solved plainly with fallback_iterations (0: the same count).  The
+ * rejections are counted on the device (pbf_warm_start_stats). */
 int pbf_set_warm_start(pbf_ctx *ctx, const float *blob, size_t length, uint32_t mode, float relax,
                        double fallback_threshold, uint32_t fallback_iterations);
 int pbf_warm_start_stats(const pbf_ctx *ctx, uint64_t *learned_substeps, uint64_t *fallbacks);

@@ -27,13 +27,7 @@ struct Phys {
   uint32_t *nbr16;  // 16-bit neighbour table (offset pairs), slot-major
   uint32_t *wfmt;   // per wavefront: 1 = 16-bit table, 0 = 32-bit table
   double Kp64, h2_64, V64;
-  const uint32_t *gate;  // nullptr, or a device word: the launch does nothing while it is 0
 };
-
-// A substep enqueued behind a device-side decision runs only when the
-// decision word is nonzero (every kernel of it returns at once otherwise);
-// nullptr: always run.  The word is uniform, so this is one scalar load.
-__device__ __forceinline__ bool gated_off(const uint32_t *gate) { return gate && *gate == 0u; }
 
 // One scene: clamp box, grid, particle range and boundary field.
 struct Scene {

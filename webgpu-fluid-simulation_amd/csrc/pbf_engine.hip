@@ -40,6 +40,10 @@ enum SubstepMode : uint32_t { M_PLAIN = 0, M_LEARNED = 1, M_COLLECT = 2, M_PREVI
 constexpr uint32_t M_PROJECT = 16;
 // Graph-key bit of a learned substep followed by its device-side check.
 constexpr uint32_t M_FALLBACK = 32;
+// Graph-key bit of the plain re-solve of a rejected learned substep.
+constexpr uint32_t M_RESOLVE = 128;
+// OR'ed into a warm mode: the warm lambda field is smoothed over neighbours.
+constexpr uint32_t M_SMOOTH = 64;
 
 struct SceneHost {
   double lo[3], hi[3];
@@ -59,6 +63,7 @@ struct GraphEntry {
 };
 
 constexpr size_t kGraphCache = 16;  // executable graphs kept per context
+constexpr uint32_t kMaxReduceBlocks = 64;  // blocks per scene of a per-scene reduction
 
 struct PendingEvent {
   int pass;
@@ -114,6 +119,8 @@ struct pbf_ctx {
   void *mlp = nullptr;           // network weights on the device
   uint32_t warm_mode = PBF_WARM_PLAIN;  // PBF_WARM_*
   bool warm_project = false;            // PBF_WARM_PROJECT
+  bool warm_smooth = false;             // PBF_WARM_SMOOTH
+  float *lam_tmp = nullptr;             // smoothed warm lambda
   double fb_threshold = 0.0;     // mean density error that rejects a learned substep (0: never)
   uint32_t fb_iters = 0;         // iterations of the plain re-solve
   float relax_set = -1.0f;       // scale applied to the network's lambda
@@ -125,10 +132,15 @@ struct pbf_ctx {
   uint32_t *snap_ids = nullptr;
   float *snap_lam = nullptr, *snap_lacc = nullptr;
   uint64_t learned_substeps = 0;
-  // device-side fallback decision: the gate word, the rejection counter, the
+  // device-side fallback decision: the decision word (pinned host memory and
+  // its device address), the rejection counter, the
   // threshold, and the per-scene speed reduction
-  uint32_t *gate = nullptr, *fb_count = nullptr;
-  double *fb_thr = nullptr, *per_scene2 = nullptr;
+  void *decision = nullptr;
+  uint32_t *decision_dev = nullptr, *fb_count = nullptr;
+  double *fb_thr = nullptr, *per_scene2 = nullptr, *energy0 = nullptr, *energy1 = nullptr;
+  // per-scene reductions: blocks per scene (about 16 k slots each) and partials
+  uint32_t red_k = 1;
+  double *partial = nullptr;
   // launch tuning
   pbf_tuning tuning = {{256, 256, 256, 256, 256, 256, 256, 256}, 0, {0, 0, 0, 0, 0, 0, 0}};
 };
@@ -198,7 +210,6 @@ Phys make_phys(const pbf_ctx *c, float dt) {
   P.Kp64 = 315.0 / (64.0 * PI * std::pow(h, 9));
   P.h2_64 = h * h;
   P.V64 = V64;
-  P.gate = nullptr;
   return P;
 }
 
@@ -242,6 +253,8 @@ LaunchCtx make_launch(pbf_ctx *c, float dt) {
   L.sid = c->sid;
   L.pool = c->pool;
   L.P = make_phys(c, dt);
+  L.red_k = c->red_k;
+  L.partial = c->partial;
   return L;
 }
 
@@ -328,8 +341,8 @@ void enqueue_search(pbf_ctx *c, const LaunchCtx &L, PassTimer &T, bool dbg, uint
 
 // The launch sequence of one substep (PBF.ts:398-448).
 void enqueue_substep(pbf_ctx *c, const LaunchCtx &L, uint32_t iters, bool diag, PassTimer &T, uint32_t mode_bits) {
-  const bool project = (mode_bits & M_PROJECT) != 0;
-  const uint32_t mode = mode_bits & ~M_PROJECT;
+  const bool project = (mode_bits & M_PROJECT) != 0, smooth = (mode_bits & M_SMOOTH) != 0;
+  const uint32_t mode = mode_bits & ~(M_PROJECT | M_SMOOTH);
   enqueue_search(c, L, T, false, mode);
   float4 *X = c->XA, *Xn = c->XB;
   LearnBufs lb{c->lam_acc, c->acc_sorted, c->lam_sorted, c->vel_sorted, c->feats};
@@ -346,6 +359,8 @@ void enqueue_substep(pbf_ctx *c, const LaunchCtx &L, uint32_t iters, bool diag, 
     } else if (mode == M_PREVIOUS && it == 0) {
       launch_warm_previous(L, c->acc_sorted, X, c->lam_acc, c->relax_prev);
     }
+    if (smooth && it == 0 && (mode == M_LEARNED || mode == M_RATIO || mode == M_PREVIOUS))
+      launch_smooth_warm(L, X, c->nbr, c->ncount, c->lam_tmp, c->lam_acc);
     t = T.begin(P_DP, L);
     launch_dp(t, X, c->B, c->nbr, c->ncount, Xn);
     T.end();
@@ -428,6 +443,7 @@ int ensure_learn_buffers(pbf_ctx *c) {
   HIPCHK(dalloc(&c->acc_sorted, N));
   HIPCHK(dalloc(&c->feats, 16 * N));
   HIPCHK(dalloc(&c->vel_sorted, N));
+  HIPCHK(dalloc(&c->lam_tmp, N));
   HIPCHK(hipMemsetAsync(c->lam_acc, 0, N * 4, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return PBF_OK;
@@ -449,49 +465,64 @@ int ensure_snapshot(pbf_ctx *c) {
 
 // Copies the state carried from one substep to the next (slot order), and
 // the readable omega / normal fields.
-void copy_state(pbf_ctx *c, bool save, const uint32_t *gate = nullptr) {
+void copy_state(pbf_ctx *c, bool save) {
   const size_t n = c->n;
   struct { void *live, *snap; size_t bytes; } t[] = {
       {c->pos, c->snap_pos, 16 * n}, {c->vel, c->snap_vel, 16 * n},   {c->acc, c->snap_acc, 16 * n},
       {c->ids, c->snap_ids, 4 * n},  {c->lam, c->snap_lam, 4 * n},    {c->lam_acc, c->snap_lacc, 4 * n},
       {c->om, c->snap_om, 16 * n},   {c->nrm, c->snap_nrm, 16 * n}};
   for (auto &e : t)
-    if (e.live && e.snap) launch_copy(c->stream, save ? e.snap : e.live, save ? e.live : e.snap, e.bytes, gate);
+    if (e.live && e.snap) launch_copy(c->stream, save ? e.snap : e.live, save ? e.live : e.snap, e.bytes);
 }
 
 // One learned substep.  With a rejection threshold set, the check runs on the
-// device in the same launch sequence (graph): the state is saved, the learned
-// substep runs, the FP64 mean density error and the largest |v| dt / h are
-// reduced, k_fallback_gate decides, and the restore and a plain re-solve
-// (the collection mode: the plain solve that also keeps the lambda sum the
-// next learned substep reads) run behind that decision -- every one of their
-// kernels returns at once when the substep is kept.  No host synchronisation.
+// device in the substep's own launch sequence (graph): the energy is summed,
+// the state saved, the learned substep runs, the FP64 mean density error, the
+// largest |v| dt / h and the energy are reduced, and k_fallback_gate writes
+// its decision to pinned host memory.  The host reads that one word once the
+// stream has drained and, for a rejected substep, launches the restore and
+// the plain re-solve (the collection mode: the plain solve that also keeps
+// the lambda sum the next learned substep reads).  Predicating the re-solve's
+// kernels on a device word instead (no host wait) was measured slower at
+// 1M particles: its ~100 empty launches over every slot cost 0.25 ms a
+// substep, more than the wait.
 int run_learned_substep(pbf_ctx *c, float dt, uint32_t iters) {
   c->learned_substeps += 1;
   const uint32_t mode = (c->warm_mode == PBF_WARM_PREVIOUS ? M_PREVIOUS
                          : c->warm_mode == PBF_WARM_RATIO  ? M_RATIO
                          : c->warm_mode == PBF_WARM_PLAIN  ? M_COLLECT
                                                            : M_LEARNED) |
-                        (c->warm_project ? M_PROJECT : 0u);
+                        (c->warm_project ? M_PROJECT : 0u) | (c->warm_smooth ? M_SMOOTH : 0u);
   if (!(c->fb_threshold > 0.0)) return run_substep(c, dt, iters, mode, c->record_diag);
   int rc = ensure_snapshot(c);
   if (rc != PBF_OK) return rc;
   const uint32_t fb_iters = c->fb_iters ? c->fb_iters : iters;
   const uint32_t ns = (uint32_t)c->scenes.size();
-  return run_enqueued(c, dt, iters, c->record_diag, mode | M_FALLBACK, fb_iters, [&](const LaunchCtx &L, PassTimer &T) {
+  rc = run_enqueued(c, dt, iters, false, mode | M_FALLBACK, 0u, [&](const LaunchCtx &L, PassTimer &T) {
+    launch_energy(L, c->pos, c->vel, c->ranges, ns, c->energy0);
     copy_state(c, true);
     enqueue_substep(c, L, iters, false, T, mode);
     launch_diag(L, c->pos, c->nbr, c->ncount, c->ci, c->ranges, ns, c->per_scene, nullptr, 0, nullptr);
     launch_speed(L, c->vel, c->ci, c->ranges, ns, c->per_scene2);
-    launch_fallback_gate(c->stream, c->per_scene, c->per_scene2, ns, c->n, c->fb_thr, c->gate, c->fb_count);
-    copy_state(c, false, c->gate);
-    LaunchCtx G = L;
-    G.P.gate = c->gate;
-    enqueue_substep(c, G, fb_iters, false, T, M_COLLECT);
-    if (c->record_diag)
-      launch_diag(L, c->pos, c->nbr, c->ncount, c->ci, c->ranges, ns, c->per_scene, c->hist, c->hist_cap,
-                  c->hist_ctr);
+    launch_energy(L, c->pos, c->vel, c->ranges, ns, c->energy1);
+    launch_fallback_gate(c->stream, c->per_scene, c->per_scene2, c->energy0, c->energy1, ns, c->n, c->fb_thr,
+                         c->decision_dev, c->fb_count);
   });
+  if (rc != PBF_OK) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (*static_cast<volatile uint32_t *>(c->decision)) {
+    rc = run_enqueued(c, dt, fb_iters, false, M_COLLECT | M_RESOLVE, 0u, [&](const LaunchCtx &L, PassTimer &T) {
+      copy_state(c, false);
+      enqueue_substep(c, L, fb_iters, false, T, M_COLLECT);
+    });
+    if (rc != PBF_OK) return rc;
+  }
+  if (c->record_diag) {
+    LaunchCtx L = make_launch(c, dt);
+    launch_diag(L, c->pos, c->nbr, c->ncount, c->ci, c->ranges, ns, c->per_scene, c->hist, c->hist_cap, c->hist_ctr);
+    HIPCHK(hipGetLastError());
+  }
+  return PBF_OK;
 }
 
 int check_flags(pbf_ctx *c) {
@@ -564,6 +595,9 @@ Scene to_device_scene(const pbf_ctx *c, const SceneHost &s, uint64_t field_off) 
 
 // Rebuild the device scene table and the boundary-field pool.
 int upload_scenes(pbf_ctx *c) {
+  uint32_t most = 0;
+  for (auto &s : c->scenes) most = std::max(most, s.count);
+  c->red_k = std::min<uint32_t>(kMaxReduceBlocks, std::max<uint32_t>(1u, (most + 16383u) / 16384u));
   size_t total = 0;
   for (auto &s : c->scenes) total += s.field.size();
   std::vector<uint64_t> offs;
@@ -693,7 +727,14 @@ int create_ctx_body(const pbf_params *params, uint32_t nscenes, const pbf_scene_
   step(dalloc(&c->ci, N), "alloc");
   step(dalloc(&c->per_scene, 2 * (size_t)nscenes), "alloc");
   step(dalloc(&c->per_scene2, 2 * (size_t)nscenes), "alloc");
-  step(dalloc(&c->gate, 1), "alloc");
+  step(dalloc(&c->energy0, 2 * (size_t)nscenes), "alloc");
+  step(dalloc(&c->partial, 2 * (size_t)nscenes * kMaxReduceBlocks), "alloc");
+  step(hipHostMalloc(&c->decision, 4, hipHostMallocMapped), "hipHostMalloc");
+  if (rc == PBF_OK) {
+    *static_cast<uint32_t *>(c->decision) = 0u;
+    step(hipHostGetDevicePointer((void **)&c->decision_dev, c->decision, 0), "hipHostGetDevicePointer");
+  }
+  step(dalloc(&c->energy1, 2 * (size_t)nscenes), "alloc");
   step(dalloc(&c->fb_count, 1), "alloc");
   step(dalloc(&c->fb_thr, 1), "alloc");
   step(dalloc(&c->hist, 2 * (size_t)c->hist_cap), "alloc");
@@ -718,7 +759,6 @@ int create_ctx_body(const pbf_params *params, uint32_t nscenes, const pbf_scene_
     step(hipMemset(c->offsets, 0, ((size_t)c->ncells + 1) * 4), "hipMemset");
     step(hipMemset(c->flags, 0, 8), "hipMemset");
     step(hipMemset(c->hist_ctr, 0, 4), "hipMemset");
-    step(hipMemset(c->gate, 0, 4), "hipMemset");
     step(hipMemset(c->fb_count, 0, 4), "hipMemset");
     step(hipMemset(c->fb_thr, 0, 8), "hipMemset");
     step(hipMemset(c->ncount, 0, N * 4), "hipMemset");
@@ -848,8 +888,10 @@ int pbf_destroy(pbf_ctx *c) {
                   c->offsets, c->flags, c->d_scenes, c->sid, c->ranges, c->pool, c->ci, c->per_scene, c->hist,
                   c->hist_ctr, c->dbg_counts, c->dbg_ids, c->ids_save, c->lam_acc, c->acc_sorted, c->feats,
                   c->vel_sorted, c->snap_pos, c->snap_vel, c->snap_acc, c->snap_om, c->snap_nrm, c->snap_ids,
-                  c->snap_lam, c->snap_lacc, c->per_scene2, c->gate, c->fb_count, c->fb_thr};
+                  c->snap_lam, c->snap_lacc, c->per_scene2, c->fb_count, c->fb_thr, c->lam_tmp,
+                  c->energy0, c->energy1, c->partial};
   mlp_engine_free(&c->mlp);
+  if (c->decision) (void)hipHostFree(c->decision);
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto p : c->scan_sums)
@@ -1070,8 +1112,8 @@ int pbf_step(pbf_ctx *c, float dt, uint32_t substeps, uint32_t iterations) try {
 int pbf_set_warm_start(pbf_ctx *c, const float *blob, size_t length, uint32_t mode, float relax,
                        double fallback_threshold, uint32_t fallback_iterations) try {
   if (!c) return PBF_E_INVALID;
-  const bool project = (mode & PBF_WARM_PROJECT) != 0;
-  mode &= ~(uint32_t)PBF_WARM_PROJECT;
+  const bool project = (mode & PBF_WARM_PROJECT) != 0, smooth = (mode & PBF_WARM_SMOOTH) != 0;
+  mode &= ~(uint32_t)(PBF_WARM_PROJECT | PBF_WARM_SMOOTH);
   if (mode > PBF_WARM_RATIO) return fail(c, PBF_E_INVALID, "unknown warm-start mode");
   if (!(relax > 0.0f && relax <= 1.0f)) return fail(c, PBF_E_INVALID, "relax must be in (0, 1]");
   HIPCHK(hipSetDevice(c->device));
@@ -1102,6 +1144,7 @@ int pbf_set_warm_start(pbf_ctx *c, const float *blob, size_t length, uint32_t mo
   if (mode != c->warm_mode && c->lam_acc) HIPCHK(hipMemsetAsync(c->lam_acc, 0, (size_t)c->ncap * 4, c->stream));
   c->warm_mode = mode;
   c->warm_project = project;
+  c->warm_smooth = smooth;
   c->fb_threshold = fallback_threshold;
   HIPCHK(hipMemcpyAsync(c->fb_thr, &c->fb_threshold, 8, hipMemcpyHostToDevice, c->stream));
   c->fb_iters = fallback_iterations;

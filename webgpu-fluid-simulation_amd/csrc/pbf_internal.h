@@ -40,6 +40,8 @@ struct LaunchCtx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // pass timing: start / stop on the dispatch packets
   uint32_t bs = 256;   // work-group size of the neighbour walks (64..256, a multiple of 64)
   uint32_t lvar = 0;   // neighbour-list kernel variant (LDS rows x work-group size)
+  uint32_t red_k = 1;  // blocks per scene of the per-scene reductions
+  double *partial = nullptr;  // their stage-1 partials: 2 x scenes x red_k doubles
 };
 
 constexpr uint32_t kListVariants = 4;
@@ -47,8 +49,7 @@ constexpr uint32_t kListVariants = 4;
 void launch_predict_bin(const LaunchCtx &L, const float4 *pos, const float4 *vel, const float4 *acc, float4 *xs,
                         uint32_t *cellkey, uint32_t *rank, uint32_t *counts);
 void launch_scan(hipStream_t st, uint32_t *in, uint32_t n, uint32_t *out, int zero_in, uint32_t *const *sums,
-                 uint32_t *const *prefix, int level, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
-                 const uint32_t *gate = nullptr);
+                 uint32_t *const *prefix, int level, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_scatter(const LaunchCtx &L, const uint32_t *cellkey, const uint32_t *rank, const uint32_t *offsets,
                     const uint32_t *ids, uint32_t *tmp_key, uint32_t *tmp_src);
 void launch_settle(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *tmp_src, const uint32_t *cellkey,
@@ -68,9 +69,14 @@ int mlp_engine_relax(void *handle, float relax, hipStream_t st);
 void mlp_engine_free(void **handle);
 void launch_mlp_infer(hipStream_t st, void *handle, uint32_t n, const float *feats, float4 *X, float *lam_acc,
                       float smax, hipEvent_t e0, hipEvent_t e1);
-void launch_copy(hipStream_t st, void *dst, const void *src, size_t bytes, const uint32_t *gate = nullptr);
-void launch_fallback_gate(hipStream_t st, const double *err, const double *speed, uint32_t nscenes, uint32_t n,
-                          const double *threshold, uint32_t *gate, uint32_t *count);
+void launch_copy(hipStream_t st, void *dst, const void *src, size_t bytes);
+void launch_energy(const LaunchCtx &L, const float4 *pos, const float4 *vel, const uint32_t *ranges,
+                   uint32_t nscenes, double *out);
+void launch_fallback_gate(hipStream_t st, const double *err, const double *speed, const double *e0, const double *e1,
+                          uint32_t nscenes, uint32_t n, const double *threshold, uint32_t *decision,
+                          uint32_t *count);
+void launch_smooth_warm(const LaunchCtx &L, float4 *X, const uint32_t *nbr, const uint32_t *ncount, float *tmp,
+                        float *lam_acc);
 void launch_warm_previous(const LaunchCtx &L, const float *acc_prev, float4 *X, float *lam_acc, float relax);
 void launch_speed(const LaunchCtx &L, const float4 *vel, double *ci, const uint32_t *ranges, uint32_t nscenes,
                   double *per_scene);

@@ -336,7 +336,6 @@ __global__ void __launch_bounds__(BS) k_predict_bin(uint32_t n, const float4 *__
                                                     uint32_t *__restrict__ rank, uint32_t *__restrict__ counts,
                                                     Scene s0, const Scene *__restrict__ scenes,
                                                     const uint32_t *__restrict__ sid, Phys P) {
-  if (gated_off(P.gate)) return;
   const uint32_t i = slot_index();
   const bool active = i < n;
   uint32_t key = 0;
@@ -363,8 +362,7 @@ constexpr uint32_t SCAN_E = BS * SCAN_V;  // 2048 elements per block
 __device__ __forceinline__ int pad32(int i) { return i + (i >> 5); }
 
 __global__ void __launch_bounds__(BS) k_scan_reduce(const uint32_t *__restrict__ in, uint32_t n,
-                                                    uint32_t *__restrict__ sums, const uint32_t *gate) {
-  if (gated_off(gate)) return;
+                                                    uint32_t *__restrict__ sums) {
   __shared__ uint32_t sh[BS / 64];
   const uint32_t base = blockIdx.x * SCAN_E + threadIdx.x * SCAN_V;
   uint32_t s = 0;
@@ -390,8 +388,7 @@ __global__ void __launch_bounds__(BS) k_scan_reduce(const uint32_t *__restrict__
 // out[0..n] = exclusive scan of in[0..n) (+ prefix[block]); out[n] = total.
 __global__ void __launch_bounds__(BS) k_scan_down(uint32_t *__restrict__ in, uint32_t n,
                                                   const uint32_t *__restrict__ prefix, uint32_t *__restrict__ out,
-                                                  int zero_in, const uint32_t *gate) {
-  if (gated_off(gate)) return;
+                                                  int zero_in) {
   __shared__ uint32_t sh[BS + BS / 32];
   const int t = threadIdx.x;
   const uint32_t base = blockIdx.x * SCAN_E + (uint32_t)t * SCAN_V;
@@ -462,8 +459,7 @@ __global__ void __launch_bounds__(BS) k_scan_down(uint32_t *__restrict__ in, uin
 __global__ void __launch_bounds__(BS) k_scatter(uint32_t n, const uint32_t *__restrict__ cellkey,
                                                 const uint32_t *__restrict__ rank, const uint32_t *__restrict__ offsets,
                                                 const uint32_t *__restrict__ ids, uint32_t *__restrict__ tmp_key,
-                                                uint32_t *__restrict__ tmp_src, const uint32_t *gate) {
-  if (gated_off(gate)) return;
+                                                uint32_t *__restrict__ tmp_src) {
   const uint32_t i = slot_index();
   if (i >= n) return;
   uint32_t s = offsets[cellkey[i]] + rank[i];
@@ -479,9 +475,7 @@ __global__ void __launch_bounds__(BS) k_settle(uint32_t n, const uint32_t *__res
                                                const uint32_t *__restrict__ offsets, const float4 *__restrict__ xs,
                                                const float4 *__restrict__ pos, const float *__restrict__ lam_in,
                                                float4 *__restrict__ X0, float4 *__restrict__ Xold,
-                                               uint32_t *__restrict__ ids, float *__restrict__ lam_out,
-                                               const uint32_t *gate) {
-  if (gated_off(gate)) return;
+                                               uint32_t *__restrict__ ids, float *__restrict__ lam_out) {
   const uint32_t s = slot_index();
   if (s >= n) return;
   const uint32_t p = tmp_src[s], key = tmp_key[s];
@@ -503,9 +497,7 @@ __global__ void __launch_bounds__(BS) k_settle_extra(uint32_t n, const uint32_t 
                                                      const uint32_t *__restrict__ cellkey,
                                                      const uint32_t *__restrict__ offsets,
                                                      const float *__restrict__ acc_in, float *__restrict__ acc_out,
-                                                     const float4 *__restrict__ vel_in, float4 *__restrict__ vel_out,
-                                                     const uint32_t *gate) {
-  if (gated_off(gate)) return;
+                                                     const float4 *__restrict__ vel_in, float4 *__restrict__ vel_out) {
   const uint32_t s = slot_index();
   if (s >= n) return;
   const uint32_t p = tmp_src[s], key = tmp_key[s];
@@ -531,9 +523,39 @@ __global__ void __launch_bounds__(BS) k_warm_previous(uint32_t n, const float *_
   lam_acc[i] = lam;
 }
 
+// Spatially smoothed warm start: the warm lambda in X.w replaced by its
+// Shepard average over the particle's neighbours (poly6 weights, self
+// included), so the first Jacobi displacement -- the gradient of that field --
+// carries no particle-scale noise from a lambda field of the previous
+// arrangement.  Two kernels: the average into tmp, then tmp into X.w and the
+// substep's sum.
+__global__ void __launch_bounds__(BS) k_smooth_warm(uint32_t n, const float4 *__restrict__ X,
+                                                    const uint32_t *__restrict__ nbr,
+                                                    const uint32_t *__restrict__ ncount, Phys P,
+                                                    float *__restrict__ tmp) {
+  const uint32_t i = slot_index();
+  if (i >= n) return;
+  const float4 xi = X[i];
+  float sw = 0.0f, sl = 0.0f;
+  for_each_neighbor(nbr, i, P, ncount[i], [&](uint32_t j) {
+    const float4 xj = X[j];
+    const float w = poly6(P, dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z));
+    sw += w;
+    sl = fmaf(w, xj.w, sl);
+  });
+  tmp[i] = sw > 0.0f ? sl / sw : xi.w;
+}
+__global__ void __launch_bounds__(BS) k_apply_warm(uint32_t n, const float *__restrict__ tmp, float4 *X,
+                                                   float *__restrict__ lam_acc) {
+  const uint32_t i = slot_index();
+  if (i >= n) return;
+  const float lam = tmp[i];
+  reinterpret_cast<float *>(X)[4 * (size_t)i + 3] = lam;
+  lam_acc[i] = lam;
+}
+
 __global__ void __launch_bounds__(BS) k_copy_words(uint32_t *__restrict__ dst, const uint32_t *__restrict__ src,
-                                                   size_t n, const uint32_t *gate) {
-  if (gated_off(gate)) return;
+                                                   size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) dst[i] = src[i];
@@ -554,7 +576,6 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
                                                         uint32_t *__restrict__ flags, Scene s0,
                                                         const Scene *__restrict__ scenes,
                                                         const uint32_t *__restrict__ sid, Phys P) {
-  if (gated_off(P.gate)) return;
   __shared__ uint32_t tile[(LIST_R + 1) * LIST_BS];  // row LIST_R: spare
   const uint32_t tid = threadIdx.x;
   const uint32_t base_slot = xcd_block(blockIdx.x, gridDim.x) * LIST_BS;
@@ -753,7 +774,6 @@ __global__ void __launch_bounds__(BS) k_boundary(uint32_t n, const float4 *__res
                                                  const float *__restrict__ pool, Scene s0,
                                                  const Scene *__restrict__ scenes, const uint32_t *__restrict__ sid,
                                                  Phys P) {
-  if (gated_off(P.gate)) return;
   const uint32_t i = slot_index();
   if (i >= n) return;
   const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
@@ -774,7 +794,6 @@ __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__
                                                const float *__restrict__ pool, Scene s0,
                                                const Scene *__restrict__ scenes, const uint32_t *__restrict__ sid,
                                                Phys P, float *__restrict__ lam_acc) {
-  if (gated_off(P.gate)) return;
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = X[i];
@@ -847,7 +866,6 @@ __global__ void __launch_bounds__(BS) k_lambda_feat(uint32_t n, float4 *X, float
                                                     const float *__restrict__ pool, Scene s0,
                                                     const Scene *__restrict__ scenes,
                                                     const uint32_t *__restrict__ sid, Phys P, LearnBufs lb) {
-  if (gated_off(P.gate)) return;
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = X[i];
@@ -912,7 +930,6 @@ __global__ void __launch_bounds__(BS) k_dp(uint32_t n, const float4 *__restrict_
                                            const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ ncount,
                                            float4 *__restrict__ Xn, Scene s0, const Scene *__restrict__ scenes,
                                            const uint32_t *__restrict__ sid, Phys P) {
-  if (gated_off(P.gate)) return;
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = X[i];
@@ -946,7 +963,6 @@ __global__ void __launch_bounds__(BS) k_density(uint32_t n, const float4 *__rest
                                                 const float4 *__restrict__ Xold, const uint32_t *__restrict__ nbr,
                                                 const uint32_t *__restrict__ ncount, float4 *__restrict__ pos,
                                                 float4 *__restrict__ vel, float *__restrict__ lam, Phys P) {
-  if (gated_off(P.gate)) return;
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = X[i];
@@ -971,7 +987,6 @@ __global__ void __launch_bounds__(BS) k_vorticity(uint32_t n, const float4 *__re
                                                   const float4 *__restrict__ vel, const uint32_t *__restrict__ nbr,
                                                   const uint32_t *__restrict__ ncount, float4 *__restrict__ om,
                                                   float4 *__restrict__ nrm, float4 *__restrict__ visc, Phys P) {
-  if (gated_off(P.gate)) return;
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = pos[i], vi = vel[i];
@@ -1004,7 +1019,6 @@ __global__ void __launch_bounds__(BS) k_xsph(uint32_t n, const float4 *__restric
                                              const float4 *__restrict__ om, const float4 *__restrict__ nrm,
                                              const float4 *__restrict__ visc, const uint32_t *__restrict__ nbr,
                                              const uint32_t *__restrict__ ncount, float4 *__restrict__ acc, Phys P) {
-  if (gated_off(P.gate)) return;
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = pos[i], ni = nrm[i], wi = om[i], ui = visc[i];
@@ -1045,7 +1059,6 @@ __global__ void __launch_bounds__(BS) k_diag(uint32_t n, const float4 *__restric
                                              const float *__restrict__ pool, Scene s0,
                                              const Scene *__restrict__ scenes, const uint32_t *__restrict__ sid,
                                              Phys P, double *__restrict__ ci) {
-  if (gated_off(P.gate)) return;
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = pos[i];
@@ -1076,33 +1089,66 @@ __global__ void __launch_bounds__(BS) k_diag(uint32_t n, const float4 *__restric
   ci[i] = c > 0.0 ? c : 0.0;
 }
 
-// block s reduces slots [ranges[2s], ranges[2s]+ranges[2s+1]); writes sum, max.
-// With hist != nullptr the totals over all blocks are also appended to a ring.
-__global__ void __launch_bounds__(1024) k_diag_reduce(const double *__restrict__ ci, const uint32_t *__restrict__ ranges,
-                                                      double *__restrict__ out) {
-  __shared__ double ssum[16], smax[16];
-  const uint32_t s = blockIdx.x;
+// Per-scene reductions in two deterministic stages.  Stage 1: block (k, s)
+// of a (K, nscenes) grid reduces the slots b + k 1024 + t, stride K 1024, of
+// scene s (slots [b, b + count) with b = ranges[2s], count = ranges[2s+1])
+// into partial[(s K + k) 2 + {0, 1}]; stage 2 combines the K partials of a
+// scene in order.  KIND 0: (sum, max) of ci; KIND 1: (sum 0.5 |v|^2,
+// sum g y) from the state (gravity is (0, -g, 0)).
+template <int KIND>
+__global__ void __launch_bounds__(1024) k_reduce_stage1(const double *__restrict__ ci, const float4 *__restrict__ pos,
+                                                        const float4 *__restrict__ vel, float g,
+                                                        const uint32_t *__restrict__ ranges,
+                                                        double *__restrict__ partial) {
+  __shared__ double sa[16], sb[16];
+  const uint32_t k = blockIdx.x, K = gridDim.x, s = blockIdx.y;
   const uint32_t b = ranges[2 * s], cnt = ranges[2 * s + 1];
-  double sum = 0.0, mx = 0.0;
-  for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
-    double v = ci[b + k];
-    sum += v;
-    mx = v > mx ? v : mx;
+  double a = 0.0, m = 0.0;
+  for (uint32_t q = k * 1024u + threadIdx.x; q < cnt; q += K * 1024u) {
+    if constexpr (KIND == 0) {
+      const double v = ci[b + q];
+      a += v;
+      m = v > m ? v : m;
+    } else {
+      const float4 v = vel[b + q];
+      a += 0.5 * ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z);
+      m += (double)g * (double)pos[b + q].y;
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
-    sum += __shfl_xor(sum, o, 64);
-    double w = __shfl_xor(mx, o, 64);
-    mx = w > mx ? w : mx;
+    a += __shfl_xor(a, o, 64);
+    const double w = __shfl_xor(m, o, 64);
+    if constexpr (KIND == 0) m = w > m ? w : m;
+    else m += w;
   }
-  if (__lane_id() == 0) { ssum[threadIdx.x >> 6] = sum; smax[threadIdx.x >> 6] = mx; }
+  if (__lane_id() == 0) { sa[threadIdx.x >> 6] = a; sb[threadIdx.x >> 6] = m; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double t = 0.0, m = 0.0;
-    for (uint32_t w = 0; w < blockDim.x / 64; ++w) { t += ssum[w]; m = smax[w] > m ? smax[w] : m; }
-    out[2 * s] = t;
-    out[2 * s + 1] = m;
+    double t = 0.0, u = 0.0;
+    for (uint32_t w = 0; w < blockDim.x / 64; ++w) {
+      t += sa[w];
+      if constexpr (KIND == 0) u = sb[w] > u ? sb[w] : u;
+      else u += sb[w];
+    }
+    partial[2 * ((size_t)s * K + k)] = t;
+    partial[2 * ((size_t)s * K + k) + 1] = u;
   }
+}
+
+template <int KIND>
+__global__ void k_reduce_stage2(const double *__restrict__ partial, uint32_t K, double *__restrict__ out) {
+  if (threadIdx.x != 0) return;
+  const uint32_t s = blockIdx.x;
+  double t = 0.0, u = 0.0;
+  for (uint32_t k = 0; k < K; ++k) {
+    t += partial[2 * ((size_t)s * K + k)];
+    const double v = partial[2 * ((size_t)s * K + k) + 1];
+    if constexpr (KIND == 0) u = v > u ? v : u;
+    else u += v;
+  }
+  out[2 * s] = t;
+  out[2 * s + 1] = u;
 }
 
 // |v| dt / h per particle (the learned solve's rejection test).
@@ -1114,16 +1160,23 @@ __global__ void __launch_bounds__(BS) k_speed(uint32_t n, const float4 *__restri
   ci[i] = (double)(sqrtf(dist2(v.x, v.y, v.z)) * scale);
 }
 
-// The learned substep's check, on the device: gate = 1 (redo the substep
-// plainly) when the mean density error over all scenes is above the
-// threshold, a particle moved more than h (|v| dt / h > 1: an over-pushed
-// warm start lowers the density error by spreading the fluid), or either
-// figure is not finite; the rejections are counted.
+// The learned substep's check, on the device: *decision = 1 (redo the
+// substep plainly) when
+//   - the mean density error over all scenes is above the threshold,
+//   - the mechanical energy (kinetic + potential, all scenes) rose by more
+//     than 0.1 % of the kinetic energy it started from: the plain solve
+//     dissipates energy every substep, while an over-pushed warm start lowers
+//     the density error by turning the push into motion,
+//   - a particle moved more than 2 h (|v| dt / h > 2: an explosion), or
+//   - any of these figures is not finite;
+// the rejections are counted.  `decision` is pinned host memory the host
+// reads after the stream has reached this kernel.
 __global__ void k_fallback_gate(const double *__restrict__ err, const double *__restrict__ speed,
-                                uint32_t nscenes, uint32_t n, const double *__restrict__ threshold,
-                                uint32_t *__restrict__ gate, uint32_t *__restrict__ count) {
+                                const double *__restrict__ e0, const double *__restrict__ e1, uint32_t nscenes,
+                                uint32_t n, const double *__restrict__ threshold, uint32_t *__restrict__ decision,
+                                uint32_t *__restrict__ count) {
   if (threadIdx.x != 0) return;
-  double t = 0.0, v = 0.0;
+  double t = 0.0, v = 0.0, k0 = 0.0, E0 = 0.0, E1 = 0.0;
   bool finite = true;
   for (uint32_t s = 0; s < nscenes; ++s) {
     t += err[2 * s];
@@ -1131,10 +1184,14 @@ __global__ void k_fallback_gate(const double *__restrict__ err, const double *__
     finite = finite && speed[2 * s] == speed[2 * s];
     const double m = speed[2 * s + 1];
     v = m > v ? m : v;
+    k0 += e0[2 * s];
+    E0 += e0[2 * s] + e0[2 * s + 1];
+    E1 += e1[2 * s] + e1[2 * s + 1];
   }
   const double mean = n ? t / (double)n : 0.0;
-  const bool bad = !finite || !(mean <= *threshold) || !(v <= 1.0);
-  *gate = bad ? 1u : 0u;
+  finite = finite && E1 - E1 == 0.0;
+  const bool bad = !finite || !(mean <= *threshold) || !(v <= 2.0) || E1 - E0 > 1e-3 * k0;
+  *decision = bad ? 1u : 0u;
   if (bad) *count += 1u;
 }
 
@@ -1207,23 +1264,23 @@ void launch_predict_bin(const LaunchCtx &L, const float4 *pos, const float4 *vel
 }
 
 void launch_scan(hipStream_t st, uint32_t *in, uint32_t n, uint32_t *out, int zero_in, uint32_t *const *sums,
-                 uint32_t *const *prefix, int level, hipEvent_t e0, hipEvent_t e1, const uint32_t *gate) {
+                 uint32_t *const *prefix, int level, hipEvent_t e0, hipEvent_t e1) {
   const dim3 b(BS);
   if (n <= SCAN_E) {
-    go(st, e0, e1, k_scan_down, dim3(1), b, in, n, (const uint32_t *)nullptr, out, zero_in, gate);
+    go(st, e0, e1, k_scan_down, dim3(1), b, in, n, (const uint32_t *)nullptr, out, zero_in);
     return;
   }
   uint32_t nb = (n + SCAN_E - 1) / SCAN_E;
-  go(st, e0, nullptr, k_scan_reduce, dim3(nb), b, (const uint32_t *)in, n, sums[level], gate);
-  launch_scan(st, sums[level], nb, prefix[level], 0, sums, prefix, level + 1, nullptr, nullptr, gate);
-  go(st, nullptr, e1, k_scan_down, dim3(nb), b, in, n, (const uint32_t *)prefix[level], out, zero_in, gate);
+  go(st, e0, nullptr, k_scan_reduce, dim3(nb), b, (const uint32_t *)in, n, sums[level]);
+  launch_scan(st, sums[level], nb, prefix[level], 0, sums, prefix, level + 1, nullptr, nullptr);
+  go(st, nullptr, e1, k_scan_down, dim3(nb), b, in, n, (const uint32_t *)prefix[level], out, zero_in);
 }
 
 void launch_scatter(const LaunchCtx &L, const uint32_t *cellkey, const uint32_t *rank, const uint32_t *offsets,
                     const uint32_t *ids, uint32_t *tmp_key, uint32_t *tmp_src) {
   if (!L.n) return;
   go(L.stream, L.ev0, L.ev1, k_scatter, dim3(nblocks(L.n, BS)), dim3(BS), L.n, cellkey, rank, offsets, ids, tmp_key,
-     tmp_src, L.P.gate);
+     tmp_src);
 }
 
 void launch_settle(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *tmp_src, const uint32_t *cellkey,
@@ -1231,7 +1288,7 @@ void launch_settle(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *
                    float4 *Xold, uint32_t *ids, float *lam_out) {
   if (!L.n) return;
   go(L.stream, L.ev0, L.ev1, k_settle, dim3(nblocks(L.n, BS)), dim3(BS), L.n, tmp_key, tmp_src, cellkey, offsets, xs,
-     pos, lam_in, X0, Xold, ids, lam_out, L.P.gate);
+     pos, lam_in, X0, Xold, ids, lam_out);
 }
 
 template <int R, int B>
@@ -1313,7 +1370,7 @@ void launch_settle_extra(const LaunchCtx &L, const uint32_t *tmp_key, const uint
                          float4 *vel_out) {
   if (!L.n) return;
   go(L.stream, nullptr, nullptr, k_settle_extra, dim3(nblocks(L.n, BS)), dim3(BS), L.n, tmp_key, tmp_src, cellkey,
-     offsets, acc_in, acc_out, vel_in, vel_out, L.P.gate);
+     offsets, acc_in, acc_out, vel_in, vel_out);
 }
 
 void launch_warm_previous(const LaunchCtx &L, const float *acc_prev, float4 *X, float *lam_acc, float relax) {
@@ -1321,16 +1378,25 @@ void launch_warm_previous(const LaunchCtx &L, const float *acc_prev, float4 *X, 
   go(L.stream, nullptr, nullptr, k_warm_previous, dim3(nblocks(L.n, BS)), dim3(BS), L.n, acc_prev, X, lam_acc, relax);
 }
 
-void launch_copy(hipStream_t st, void *dst, const void *src, size_t bytes, const uint32_t *gate) {
+void launch_copy(hipStream_t st, void *dst, const void *src, size_t bytes) {
   size_t n = bytes / 4;
   uint32_t grid = (uint32_t)std::min<size_t>((n + BS - 1) / BS, 4096u);
-  if (n)
-    hipLaunchKernelGGL(k_copy_words, dim3(grid), dim3(BS), 0, st, (uint32_t *)dst, (const uint32_t *)src, n, gate);
+  if (n) hipLaunchKernelGGL(k_copy_words, dim3(grid), dim3(BS), 0, st, (uint32_t *)dst, (const uint32_t *)src, n);
 }
 
-void launch_fallback_gate(hipStream_t st, const double *err, const double *speed, uint32_t nscenes, uint32_t n,
-                          const double *threshold, uint32_t *gate, uint32_t *count) {
-  hipLaunchKernelGGL(k_fallback_gate, dim3(1), dim3(64), 0, st, err, speed, nscenes, n, threshold, gate, count);
+void launch_smooth_warm(const LaunchCtx &L, float4 *X, const uint32_t *nbr, const uint32_t *ncount, float *tmp,
+                        float *lam_acc) {
+  if (!L.n) return;
+  go(L.stream, nullptr, nullptr, k_smooth_warm, dim3(nblocks(L.n, BS)), dim3(BS), L.n, (const float4 *)X, nbr, ncount,
+     L.P, tmp);
+  go(L.stream, nullptr, nullptr, k_apply_warm, dim3(nblocks(L.n, BS)), dim3(BS), L.n, (const float *)tmp, X, lam_acc);
+}
+
+void launch_fallback_gate(hipStream_t st, const double *err, const double *speed, const double *e0, const double *e1,
+                          uint32_t nscenes, uint32_t n, const double *threshold, uint32_t *decision,
+                          uint32_t *count) {
+  hipLaunchKernelGGL(k_fallback_gate, dim3(1), dim3(64), 0, st, err, speed, e0, e1, nscenes, n, threshold, decision,
+                     count);
 }
 
 void launch_dp(const LaunchCtx &L, const float4 *X, const float4 *B, const uint32_t *nbr, const uint32_t *ncount,
@@ -1361,6 +1427,14 @@ void launch_xsph(const LaunchCtx &L, const float4 *pos, const float4 *om, const 
      L.P);
 }
 
+template <int KIND>
+static void reduce_scenes(const LaunchCtx &L, const double *ci, const float4 *pos, const float4 *vel, float g,
+                          const uint32_t *ranges, uint32_t nscenes, double *out, hipEvent_t e1 = nullptr) {
+  go(L.stream, nullptr, nullptr, k_reduce_stage1<KIND>, dim3(L.red_k, nscenes), dim3(1024), ci, pos, vel, g, ranges,
+     L.partial);
+  go(L.stream, nullptr, e1, k_reduce_stage2<KIND>, dim3(nscenes), dim3(64), (const double *)L.partial, L.red_k, out);
+}
+
 void launch_diag(const LaunchCtx &L, const float4 *pos, const uint32_t *nbr, const uint32_t *ncount, double *ci,
                  const uint32_t *ranges, uint32_t nscenes, double *per_scene, double *hist, uint32_t hist_cap,
                  uint32_t *hist_ctr) {
@@ -1368,7 +1442,7 @@ void launch_diag(const LaunchCtx &L, const float4 *pos, const uint32_t *nbr, con
   const dim3 g(nblocks(L.n, L.bs));
   if (L.multi) fk_dispatch<true, SelDiag>(L, L.ev0, nullptr, g, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
   else fk_dispatch<false, SelDiag>(L, L.ev0, nullptr, g, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
-  go(L.stream, nullptr, hist ? nullptr : L.ev1, k_diag_reduce, dim3(nscenes), dim3(1024), (const double *)ci, ranges, per_scene);
+  reduce_scenes<0>(L, ci, nullptr, nullptr, 0.0f, ranges, nscenes, per_scene, hist ? nullptr : L.ev1);
   if (hist)
     go(L.stream, nullptr, L.ev1, k_diag_history, dim3(1), dim3(64), (const double *)per_scene, nscenes, L.n, hist,
        hist_cap, hist_ctr);
@@ -1378,7 +1452,13 @@ void launch_speed(const LaunchCtx &L, const float4 *vel, double *ci, const uint3
                   double *per_scene) {
   if (!L.n) return;
   hipLaunchKernelGGL(k_speed, dim3(nblocks(L.n, BS)), dim3(BS), 0, L.stream, L.n, vel, L.P.dt / L.P.h, ci);
-  hipLaunchKernelGGL(k_diag_reduce, dim3(nscenes), dim3(1024), 0, L.stream, (const double *)ci, ranges, per_scene);
+  reduce_scenes<0>(L, ci, nullptr, nullptr, 0.0f, ranges, nscenes, per_scene);
+}
+
+void launch_energy(const LaunchCtx &L, const float4 *pos, const float4 *vel, const uint32_t *ranges,
+                   uint32_t nscenes, double *out) {
+  if (!L.n) return;
+  reduce_scenes<1>(L, nullptr, pos, vel, -L.P.gy, ranges, nscenes, out);
 }
 
 void launch_unsort4(hipStream_t st, uint32_t n, const float4 *src, const uint32_t *ids, float4 *dst, int clear_w) {

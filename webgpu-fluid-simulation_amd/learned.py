@@ -132,7 +132,7 @@ class Trajectory:
 
 def run(scene: Scene, substeps: int, iterations: int, blob: Optional[np.ndarray] = None, relax: float = 1.0,
         fallback_threshold: float = 0.0, every: int = 20, max_neighbors: int = 128,
-        mode: Optional[int] = None) -> Trajectory:
+        mode: Optional[int] = None, fallback_iterations: int = 0) -> Trajectory:
     """Density error per substep (FP64 diagnostics on the GPU) and, every
     `every` substeps, the mean height and kinetic energy of the fluid
     (every = 0: no checkpoints, and the wall time of the substeps).  mode:
@@ -142,7 +142,7 @@ def run(scene: Scene, substeps: int, iterations: int, blob: Optional[np.ndarray]
         if mode is None:
             mode = WARM_LEARNED if blob is not None else WARM_PLAIN
         if mode != WARM_PLAIN:
-            sim.set_warm_start(blob, mode, relax, fallback_threshold)
+            sim.set_warm_start(blob, mode, relax, fallback_threshold, fallback_iterations)
         sim.record_density_error(True)
         dt = scene.substep_dt()
         hs, ks = [], []
