@@ -126,3 +126,41 @@ def test_ratio_warm_start_1m_fallback_is_the_plain_solve():
         assert np.array_equal(plain.read_field(k), warm.read_field(k)), k
     plain.close()
     warm.close()
+
+
+DEEP_BLOB = os.path.join(PKG, "warm_start", "deep_ratio.f32")
+
+
+@pytest.mark.timeout(300)
+def test_deep_ratio_network_gain_1m():
+    """The committed deep-fluid ratio network (trained on other, smaller
+    columns) on the 1M dam break, 100 substeps, fallback on: at 20 Jacobi
+    iterations it reaches a mean density error <= 0.5 % (the plain solve
+    needs 44; at 20 it stays near 1.1 %) with a trajectory as close to a
+    200-iteration run as the plain solve at 44; the same warm start without
+    the network (previous lambda sum, same smoothing, relax and fallback)
+    does not reach it."""
+    import learned
+    from pbf_host import WARM_PREVIOUS, WARM_RATIO, WARM_SMOOTH
+    sc = scenes.dam_break_1m()
+    sub = 100
+    print(f"[deep ratio 1M] n={sc.n}", flush=True)
+    blob = np.fromfile(DEEP_BLOB, "<f4")
+    ref = learned.run(sc, sub, 200)
+    p44 = learned.run(sc, sub, 44)
+    p20 = learned.run(sc, sub, 20, every=0)
+    thr = float(p44.density_error.max())
+    net = learned.run(sc, sub, 20, blob, 0.5, fallback_threshold=thr, mode=WARM_RATIO | WARM_SMOOTH,
+                      fallback_iterations=44)
+    prev = learned.run(sc, sub, 20, None, 0.5, fallback_threshold=thr, mode=WARM_PREVIOUS | WARM_SMOOTH,
+                       fallback_iterations=44, every=0)
+    e44, e20, enet, eprev = (float(t.density_error.mean()) for t in (p44, p20, net, prev))
+    dh44, dk44 = learned.fidelity(p44, ref)
+    dh, dk = learned.fidelity(net, ref)
+    print(f"[deep ratio 1M] plain 44 {e44:.4%} plain 20 {e20:.4%} network 20 {enet:.4%} ({net.fallbacks} fallbacks) "
+          f"previous 20 {eprev:.4%} ({prev.fallbacks}); fidelity net h {dh:.4f} KE {dk:.3f}, plain 44 h {dh44:.4f} "
+          f"KE {dk44:.3f}", flush=True)
+    assert e44 <= 0.005 < e20
+    assert enet <= 0.005 and net.fallbacks <= 10
+    assert dh <= 2 * dh44 + 0.005 and dk <= 2 * dk44 + 0.01
+    assert eprev > 0.005 and eprev > enet

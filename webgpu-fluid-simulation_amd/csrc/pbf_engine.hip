@@ -464,15 +464,17 @@ int ensure_snapshot(pbf_ctx *c) {
 }
 
 // Copies the state carried from one substep to the next (slot order), and
-// the readable omega / normal fields.
-void copy_state(pbf_ctx *c, bool save) {
+// with `readable` the omega / normal fields a caller can read back (a substep
+// that follows the restore recomputes them).
+void copy_state(pbf_ctx *c, bool save, bool readable = true) {
   const size_t n = c->n;
   struct { void *live, *snap; size_t bytes; } t[] = {
       {c->pos, c->snap_pos, 16 * n}, {c->vel, c->snap_vel, 16 * n},   {c->acc, c->snap_acc, 16 * n},
       {c->ids, c->snap_ids, 4 * n},  {c->lam, c->snap_lam, 4 * n},    {c->lam_acc, c->snap_lacc, 4 * n},
       {c->om, c->snap_om, 16 * n},   {c->nrm, c->snap_nrm, 16 * n}};
-  for (auto &e : t)
-    if (e.live && e.snap) launch_copy(c->stream, save ? e.snap : e.live, save ? e.live : e.snap, e.bytes);
+  for (size_t k = 0; k < (readable ? 8u : 6u); ++k)
+    if (t[k].live && t[k].snap)
+      launch_copy(c->stream, save ? t[k].snap : t[k].live, save ? t[k].live : t[k].snap, t[k].bytes);
 }
 
 // One learned substep.  With a rejection threshold set, the check runs on the
@@ -500,7 +502,7 @@ int run_learned_substep(pbf_ctx *c, float dt, uint32_t iters) {
   const uint32_t ns = (uint32_t)c->scenes.size();
   rc = run_enqueued(c, dt, iters, false, mode | M_FALLBACK, 0u, [&](const LaunchCtx &L, PassTimer &T) {
     launch_energy(L, c->pos, c->vel, c->ranges, ns, c->energy0);
-    copy_state(c, true);
+    copy_state(c, true, false);
     enqueue_substep(c, L, iters, false, T, mode);
     launch_diag(L, c->pos, c->nbr, c->ncount, c->ci, c->ranges, ns, c->per_scene, nullptr, 0, nullptr);
     launch_speed(L, c->vel, c->ci, c->ranges, ns, c->per_scene2);
@@ -512,7 +514,7 @@ int run_learned_substep(pbf_ctx *c, float dt, uint32_t iters) {
   HIPCHK(hipStreamSynchronize(c->stream));
   if (*static_cast<volatile uint32_t *>(c->decision)) {
     rc = run_enqueued(c, dt, fb_iters, false, M_COLLECT | M_RESOLVE, 0u, [&](const LaunchCtx &L, PassTimer &T) {
-      copy_state(c, false);
+      copy_state(c, false, false);
       enqueue_substep(c, L, fb_iters, false, T, M_COLLECT);
     });
     if (rc != PBF_OK) return rc;
