@@ -288,6 +288,34 @@ def batched_measure(substeps: int, warmup: int, tune: bool):
     return out
 
 
+def warm_start_profile():
+    """One context running the learned path's kernels (feature pass, network
+    inference, smoothing, the device-side check) on every particle of the 1M
+    dam break for 20 substeps, then the training kernels (k_mlp_train,
+    k_mlp_adam, k_mlp_pack) for 2 epochs of 64 mini-batches; the training
+    kernels' work does not depend on the values, so the pairs are synthetic."""
+    import learned
+    from pbf_host import WARM_RATIO, WARM_SMOOTH
+    sc = WORKLOADS["dam_break_1m"][0]()
+    blob = np.fromfile(DEEP_BLOB, "<f4")
+    sim = scenes.make_engine(sc, max_neighbors=128)
+    try:
+        sim.set_warm_start(blob, WARM_RATIO | WARM_SMOOTH, 0.5, 0.0094, 44)
+        dt = sc.substep_dt()
+        for _ in range(20):
+            sim.step(dt, 1, 20)
+        sim.sync()
+        substeps, fallbacks = sim.warm_start_stats()
+    finally:
+        sim.close()
+    rng = np.random.default_rng(5)
+    n = 262144
+    res = learned.train(rng.standard_normal((n, 16)).astype(np.float32),
+                        (rng.standard_normal(n) * 0.01).astype(np.float32), epochs=2, batch=4096)
+    return {"mode": "warm_start_profile", "particles": sc.n, "learned_substeps": substeps, "fallbacks": fallbacks,
+            "iterations": 20, "training_pairs": n, "epochs": 2, "final_loss": float(res.losses[-1])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -302,7 +330,14 @@ def main():
     ap.add_argument("--workload", default="dam_break_1m", choices=sorted(WORKLOADS),
                     help="the measured scene (the default is the headline workload; the other is the "
                          "hashed-grid parity configuration, measured without the warm-start and batch sections)")
+    ap.add_argument("--warm-start-profile", action="store_true",
+                    help="only run the warm-start network's kernels for a kernel trace: 20 substeps of the 1M dam "
+                         "break with the deep ratio network (20 iterations, fallback on), then 2 epochs of "
+                         "training on synthetic pairs; prints one JSON line")
     args = ap.parse_args()
+    if args.warm_start_profile:
+        print(json.dumps(warm_start_profile()))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine runs on a single GPU and has no multi-GPU path.",
               file=sys.stderr)

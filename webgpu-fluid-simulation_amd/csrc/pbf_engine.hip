@@ -328,12 +328,13 @@ void enqueue_search(pbf_ctx *c, const LaunchCtx &L, PassTimer &T, bool dbg, uint
   launch_scatter(t, c->cellkey, c->rank, c->offsets, c->ids, c->tmp_key, c->tmp_src);
   T.end();
   t = T.begin(P_SETTLE, L);
+  // the binning ranks are consumed by the scatter: their buffer takes the
+  // final slots
   launch_settle(t, c->tmp_key, c->tmp_src, c->cellkey, c->offsets, c->XB, c->pos, c->lam, c->XA, c->Xold, c->ids,
-                c->lam_sorted);
+                c->lam_sorted, c->rank);
   T.end();
   if (mode != M_PLAIN)
-    launch_settle_extra(L, c->tmp_key, c->tmp_src, c->cellkey, c->offsets, c->lam_acc, c->acc_sorted, c->vel,
-                        c->vel_sorted);
+    launch_settle_extra(L, c->tmp_src, c->rank, c->lam_acc, c->acc_sorted, c->vel, c->vel_sorted);
   t = T.begin(P_LIST, L);
   launch_build_list(t, c->XA, c->offsets, c->nbr, c->ncount, c->flags);
   T.end();

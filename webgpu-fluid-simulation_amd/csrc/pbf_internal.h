@@ -52,18 +52,18 @@ void launch_scan(hipStream_t st, uint32_t *in, uint32_t n, uint32_t *out, int ze
                  uint32_t *const *prefix, int level, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_scatter(const LaunchCtx &L, const uint32_t *cellkey, const uint32_t *rank, const uint32_t *offsets,
                     const uint32_t *ids, uint32_t *tmp_key, uint32_t *tmp_src);
+// dest[s]: the final slot of sorted slot s (read by launch_settle_extra)
 void launch_settle(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *tmp_src, const uint32_t *cellkey,
                    const uint32_t *offsets, const float4 *xs, const float4 *pos, const float *lam_in, float4 *X0,
-                   float4 *Xold, uint32_t *ids, float *lam_out);
+                   float4 *Xold, uint32_t *ids, float *lam_out, uint32_t *dest);
 void launch_build_list(const LaunchCtx &L, const float4 *X0, const uint32_t *offsets, uint32_t *nbr, uint32_t *ncount,
                        uint32_t *flags);
 void launch_lambda(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr, const uint32_t *ncount,
                    float *lam_acc = nullptr, bool project = false);
 void launch_lambda_features(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr, const uint32_t *ncount,
                             const LearnBufs &lb);
-void launch_settle_extra(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *tmp_src, const uint32_t *cellkey,
-                         const uint32_t *offsets, const float *acc_in, float *acc_out, const float4 *vel_in,
-                         float4 *vel_out);
+void launch_settle_extra(const LaunchCtx &L, const uint32_t *tmp_src, const uint32_t *dest, const float *acc_in,
+                         float *acc_out, const float4 *vel_in, float4 *vel_out);
 int mlp_engine_set(void **handle, const float *blob, hipStream_t st);
 int mlp_engine_relax(void *handle, float relax, hipStream_t st);
 void mlp_engine_free(void **handle);

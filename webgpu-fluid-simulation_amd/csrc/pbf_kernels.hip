@@ -467,45 +467,54 @@ __global__ void __launch_bounds__(BS) k_scatter(uint32_t n, const uint32_t *__re
   tmp_src[s] = i;
 }
 
+// Number of keys in [o, e) below `key`, four per load (aligned uint4) between
+// a scalar head and tail: the ranges are short and L1-resident, so the loop
+// is bound by load issue, and crowded buckets (hashed grids) make it long.
+__device__ __forceinline__ uint32_t rank_below(const uint32_t *__restrict__ keys, uint32_t o, uint32_t e,
+                                               uint32_t key) {
+  uint32_t r = 0, t = o;
+  for (; t < e && (t & 3u); ++t) r += keys[t] < key ? 1u : 0u;
+  const uint4 *q = reinterpret_cast<const uint4 *>(keys + t);
+  const uint32_t nv = (e - t) >> 2;
+  for (uint32_t v = 0; v < nv; ++v) {
+    const uint4 w = q[v];
+    r += (w.x < key ? 1u : 0u) + (w.y < key ? 1u : 0u) + (w.z < key ? 1u : 0u) + (w.w < key ? 1u : 0u);
+  }
+  for (t += 4u * nv; t < e; ++t) r += keys[t] < key ? 1u : 0u;
+  return r;
+}
+
 // Stable order inside each cell (ascending particle id), then move the
-// particle data into its final slot.
+// particle data into its final slot; dest[s] = that slot, for k_settle_extra.
 __global__ void __launch_bounds__(BS) k_settle(uint32_t n, const uint32_t *__restrict__ tmp_key,
                                                const uint32_t *__restrict__ tmp_src,
                                                const uint32_t *__restrict__ cellkey,
                                                const uint32_t *__restrict__ offsets, const float4 *__restrict__ xs,
                                                const float4 *__restrict__ pos, const float *__restrict__ lam_in,
                                                float4 *__restrict__ X0, float4 *__restrict__ Xold,
-                                               uint32_t *__restrict__ ids, float *__restrict__ lam_out) {
+                                               uint32_t *__restrict__ ids, float *__restrict__ lam_out,
+                                               uint32_t *__restrict__ dest) {
   const uint32_t s = slot_index();
   if (s >= n) return;
   const uint32_t p = tmp_src[s], key = tmp_key[s];
   const uint32_t c = cellkey[p];
-  const uint32_t o = offsets[c], e = offsets[c + 1];
-  uint32_t r = 0;
-  for (uint32_t t = o; t < e; ++t) r += tmp_key[t] < key ? 1u : 0u;
-  const uint32_t f = o + r;
+  const uint32_t f = offsets[c] + rank_below(tmp_key, offsets[c], offsets[c + 1], key);
   X0[f] = xs[p];
   Xold[f] = pos[p];
   ids[f] = key;
   if (lam_out) lam_out[f] = lam_in[p];
+  dest[s] = f;
 }
 
 // Learned / data-generation modes: carry the previous substep's lambda sum and
-// velocity along with the reorder (same stable rank as k_settle).
-__global__ void __launch_bounds__(BS) k_settle_extra(uint32_t n, const uint32_t *__restrict__ tmp_key,
-                                                     const uint32_t *__restrict__ tmp_src,
-                                                     const uint32_t *__restrict__ cellkey,
-                                                     const uint32_t *__restrict__ offsets,
+// velocity along with the reorder (the slots k_settle chose).
+__global__ void __launch_bounds__(BS) k_settle_extra(uint32_t n, const uint32_t *__restrict__ tmp_src,
+                                                     const uint32_t *__restrict__ dest,
                                                      const float *__restrict__ acc_in, float *__restrict__ acc_out,
                                                      const float4 *__restrict__ vel_in, float4 *__restrict__ vel_out) {
   const uint32_t s = slot_index();
   if (s >= n) return;
-  const uint32_t p = tmp_src[s], key = tmp_key[s];
-  const uint32_t c = cellkey[p];
-  const uint32_t o = offsets[c], e = offsets[c + 1];
-  uint32_t r = 0;
-  for (uint32_t t = o; t < e; ++t) r += tmp_key[t] < key ? 1u : 0u;
-  const uint32_t f = o + r;
+  const uint32_t p = tmp_src[s], f = dest[s];
   acc_out[f] = acc_in[p];
   vel_out[f] = vel_in[p];
 }
@@ -1285,10 +1294,10 @@ void launch_scatter(const LaunchCtx &L, const uint32_t *cellkey, const uint32_t 
 
 void launch_settle(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *tmp_src, const uint32_t *cellkey,
                    const uint32_t *offsets, const float4 *xs, const float4 *pos, const float *lam_in, float4 *X0,
-                   float4 *Xold, uint32_t *ids, float *lam_out) {
+                   float4 *Xold, uint32_t *ids, float *lam_out, uint32_t *dest) {
   if (!L.n) return;
   go(L.stream, L.ev0, L.ev1, k_settle, dim3(nblocks(L.n, BS)), dim3(BS), L.n, tmp_key, tmp_src, cellkey, offsets, xs,
-     pos, lam_in, X0, Xold, ids, lam_out);
+     pos, lam_in, X0, Xold, ids, lam_out, dest);
 }
 
 template <int R, int B>
@@ -1365,12 +1374,11 @@ void launch_lambda_features(const LaunchCtx &L, float4 *X, float4 *B, const uint
   else fk_dispatch<false, SelLambdaFeat>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lb);
 }
 
-void launch_settle_extra(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *tmp_src, const uint32_t *cellkey,
-                         const uint32_t *offsets, const float *acc_in, float *acc_out, const float4 *vel_in,
-                         float4 *vel_out) {
+void launch_settle_extra(const LaunchCtx &L, const uint32_t *tmp_src, const uint32_t *dest, const float *acc_in,
+                         float *acc_out, const float4 *vel_in, float4 *vel_out) {
   if (!L.n) return;
-  go(L.stream, nullptr, nullptr, k_settle_extra, dim3(nblocks(L.n, BS)), dim3(BS), L.n, tmp_key, tmp_src, cellkey,
-     offsets, acc_in, acc_out, vel_in, vel_out);
+  go(L.stream, nullptr, nullptr, k_settle_extra, dim3(nblocks(L.n, BS)), dim3(BS), L.n, tmp_src, dest, acc_in, acc_out,
+     vel_in, vel_out);
 }
 
 void launch_warm_previous(const LaunchCtx &L, const float *acc_prev, float4 *X, float *lam_acc, float relax) {
