@@ -149,6 +149,15 @@ int pbf_set_volume_map(pbf_ctx *ctx, uint32_t scene, const int32_t res[3], const
                        const float *sdf_vol);
 int pbf_clear_boundary(pbf_ctx *ctx);
 
+/* OR'ed into the `kind` of the map builders below: store the reference's
+ * own boundary volume instead of this library's definition -- the one its
+ * public/model/*.cdm files hold (read by volumeMap.ts:23-39, sampled by
+ * boundaryVolume.ts:40-51): V(x) = 0.8 sum over the 16^3 Gauss-Legendre
+ * points xi of [-h, h]^3 with |xi| <= h of w(xi) gamma(sdf(x + xi)), gamma = 1
+ * in the solid, the cubic-spline ratio W(d) / W(0) of support h for
+ * 0 < d < h, 0 beyond and for points outside the field's box [lo, hi]
+ * (Bender et al. 2019 volume maps).  particle_radius is not used. */
+#define PBF_VOLUME_REFERENCE 0x100u
 /* Host-side map builder for analytic solids: kind 0 = inside of the box
  * [lo, hi] is fluid (walls outside; shape, if given, = box lo[3], hi[3]),
  * kind 1 = that box plus a sphere and a torus (shape[0..2] sphere centre,
@@ -160,10 +169,11 @@ int pbf_clear_boundary(pbf_ctx *ctx);
  * No reference counterpart: the reference ships precomputed .cdm maps. */
 int pbf_build_volume_map(uint32_t kind, const double *shape, const int32_t res[3], const double lo[3],
                          const double hi[3], double kernel_radius, double particle_radius, float *sdf_vol);
-/* The same definition sampled at the nodes of the 32-node cubic element in
+/* The same definitions sampled at the nodes of the 32-node cubic element in
  * the .cdm layout (corner, x-, y-, z-edge node sets) over [lo, hi]: a
- * reference-format field for any analytic solid and resolution.  sdf / vol
- * NULL: field_length only. */
+ * reference-format field for any analytic solid and resolution (with
+ * PBF_VOLUME_REFERENCE in kind, the reference's own volume values).
+ * sdf / vol NULL: field_length only. */
 int pbf_build_cubic_field(uint32_t kind, const double *shape, const int32_t res[3], const double lo[3],
                           const double hi[3], double kernel_radius, double particle_radius, float *sdf, float *vol,
                           size_t capacity, size_t *field_length);

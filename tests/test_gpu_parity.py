@@ -209,12 +209,13 @@ def test_neighbour_overflow_is_reported():
 
 def test_generated_obstacle_cdm_runs_at_parity():
     """A reference-format field built by pbf_build_cubic_field (sphere + torus
-    in the unit box, 24^3 elements), passed as .cdm text through
-    setBoundaryData, steps bit-identically to the oracle's cubic sampler."""
-    from pbf_host import build_cubic_field, parse_cdm, write_cdm
+    in the unit box, 24^3 elements, the reference's own volume definition),
+    passed as .cdm text through setBoundaryData, steps bit-identically to the
+    oracle's cubic sampler."""
+    from pbf_host import VOLUME_REFERENCE, build_cubic_field, parse_cdm, write_cdm
     res = (24, 24, 24)
     shape = [0.5, 0.45, 0.5, 0.12, 0.5, 0.45, 0.5, 0.25, 0.05]
-    sdf, vol = build_cubic_field(1, res, (0, 0, 0), (1, 1, 1), scenes.REF_H, shape, scenes.REF_R)
+    sdf, vol = build_cubic_field(VOLUME_REFERENCE | 1, res, (0, 0, 0), (1, 1, 1), scenes.REF_H, shape)
     text = write_cdm(res, sdf, vol)
     _, s2, v2 = parse_cdm(text)
     fld = scenes.Field("cubic", res, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), sdf=s2, vol=v2)

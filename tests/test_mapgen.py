@@ -18,6 +18,9 @@ pytestmark = pytest.mark.gpu
     (0, (24, 20, 16), (0.0, 0.0, 0.0), (1.6, 1.6, 1.2), 0.1, 0.025, None),
     (0, (20, 20, 20), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.025, 0.0, None),
     (1, (32, 32, 32), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.05, 0.0125, [0.5, 0.5, 0.5, 0.2, 0.5, 0.5, 0.5, 0.3, 0.08]),
+    # the reference's own volume definition (PBF_VOLUME_REFERENCE = 0x100)
+    (0x100, (20, 20, 20), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.025, 0.0, None),
+    (0x101, (24, 24, 24), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.025, 0.0, [0.5, 0.45, 0.5, 0.12, 0.5, 0.45, 0.5, 0.25, 0.05]),
 ])
 def test_gpu_map_matches_host_builder(kind, res, lo, hi, h, r, shape):
     print(f"[mapgen] kind={kind} res={res}", flush=True)

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/pbf.h"
+#include "pbf_quadrature.h"
 
 namespace pbf {
 
@@ -162,9 +163,14 @@ struct VolumeEval {
   pbf::Solid S;
   double h, r, kp, cellv, ballv, vcap;
   std::vector<double> qp, qw;
+  bool ref = false;                  // PBF_VOLUME_REFERENCE
+  double map_lo[3], map_hi[3];       // the field's box
+  std::vector<double> gp, gw;        // its quadrature
 
-  int init(uint32_t kind, const double *shape, const double lo[3], const double hi[3], double kernel_radius,
+  int init(uint32_t kind_flags, const double *shape, const double lo[3], const double hi[3], double kernel_radius,
            double particle_radius) {
+    ref = (kind_flags & PBF_VOLUME_REFERENCE) != 0;
+    const uint32_t kind = kind_flags & ~(uint32_t)PBF_VOLUME_REFERENCE;
     if (!lo || !hi || kind > 1 || !(kernel_radius > 0.0) || !std::isfinite(kernel_radius)) return PBF_E_INVALID;
     if (!(particle_radius >= 0.0) || !std::isfinite(particle_radius)) return PBF_E_INVALID;
     for (int a = 0; a < 3; ++a)
@@ -186,6 +192,8 @@ struct VolumeEval {
     }
     h = kernel_radius;
     r = particle_radius;
+    for (int a = 0; a < 3; ++a) { map_lo[a] = lo[a]; map_hi[a] = hi[a]; }
+    if (ref) pbf::reference_volume_quadrature(h, gp, gw);
     const int M = 16;
     const double PI = 3.141592653589793;
     kp = 315.0 / (64.0 * PI * std::pow(h, 9));
@@ -208,10 +216,33 @@ struct VolumeEval {
     return PBF_OK;
   }
 
+  // The reference's definition (pbf_quadrature.h): gamma of the SDF at the
+  // quadrature points inside the field's box, in the points' order.
+  double reference_volume(const double p[3], double d) const {
+    if (d >= 2.0 * h) return 0.0;
+    double v = 0.0;
+    for (size_t t = 0, u = 0; t < gp.size(); t += 3, ++u) {
+      const double y[3] = {p[0] + gp[t], p[1] + gp[t + 1], p[2] + gp[t + 2]};
+      if (y[0] < map_lo[0] || y[0] > map_hi[0] || y[1] < map_lo[1] || y[1] > map_hi[1] || y[2] < map_lo[2] ||
+          y[2] > map_hi[2])
+        continue;
+      const double phi = S.sdf(y);
+      double g;
+      if (phi <= 0.0) g = 1.0;
+      else if (phi < h) {
+        const double q = phi / h;
+        g = q <= 0.5 ? (6.0 * q * q * q - 6.0 * q * q) + 1.0 : 2.0 * ((1.0 - q) * (1.0 - q) * (1.0 - q));
+      } else g = 0.0;
+      v += gw[u] * g;
+    }
+    return v;
+  }
+
   void eval(const double p[3], float &sdf, float &vol) const {
     double d = S.sdf(p);
     double v;
-    if (d >= h) v = 0.0;
+    if (ref) v = reference_volume(p, d);
+    else if (d >= h) v = 0.0;
     else {
       double vin = 0.0, win = 0.0;
       if (d <= -h) { vin = ballv; win = 1.0; }

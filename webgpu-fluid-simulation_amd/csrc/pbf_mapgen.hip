@@ -14,6 +14,9 @@
 #include <vector>
 
 #include "../../include/pbf.h"
+#include "pbf_quadrature.h"
+
+using pbf::reference_volume_quadrature;
 
 namespace {
 
@@ -50,7 +53,29 @@ struct MapArgs {
   int res[3];
   double h, r, kp, cellv, ballv, vcap;
   uint32_t nq;
+  int refv;  // PBF_VOLUME_REFERENCE: qp / qw hold the reference quadrature
 };
+
+// The reference's boundary volume (pbf_quadrature.h), in the host's order.
+__device__ double reference_volume(const MapArgs &A, const double *__restrict__ qp, const double *__restrict__ qw,
+                                   const double p[3], double d) {
+  if (d >= 2.0 * A.h) return 0.0;
+  double v = 0.0;
+  for (uint32_t u = 0; u < A.nq; ++u) {
+    const double y[3] = {p[0] + qp[3 * u], p[1] + qp[3 * u + 1], p[2] + qp[3 * u + 2]};
+    if (y[0] < A.lo[0] || y[0] > A.hi[0] || y[1] < A.lo[1] || y[1] > A.hi[1] || y[2] < A.lo[2] || y[2] > A.hi[2])
+      continue;
+    const double phi = solid_sdf(A.S, y);
+    double g;
+    if (phi <= 0.0) g = 1.0;
+    else if (phi < A.h) {
+      const double q = phi / A.h;
+      g = q <= 0.5 ? (6.0 * q * q * q - 6.0 * q * q) + 1.0 : 2.0 * ((1.0 - q) * (1.0 - q) * (1.0 - q));
+    } else g = 0.0;
+    v += qw[u] * g;
+  }
+  return v;
+}
 
 __global__ void __launch_bounds__(256) k_volume_map(MapArgs A, const double *__restrict__ qp,
                                                     const double *__restrict__ qw, float *__restrict__ out,
@@ -64,7 +89,9 @@ __global__ void __launch_bounds__(256) k_volume_map(MapArgs A, const double *__r
                  A.lo[2] + (A.hi[2] - A.lo[2]) * (double)k / A.res[2]};
   const double d = solid_sdf(A.S, p);
   double v;
-  if (d >= A.h) {
+  if (A.refv) {
+    v = reference_volume(A, qp, qw, p, d);
+  } else if (d >= A.h) {
     v = 0.0;
   } else {
     double vin = 0.0, win = 0.0;
@@ -98,6 +125,8 @@ __global__ void __launch_bounds__(256) k_volume_map(MapArgs A, const double *__r
 extern "C" int pbf_build_volume_map_gpu(uint32_t kind, const double *shape, const int32_t res[3], const double lo[3],
                                         const double hi[3], double kernel_radius, double particle_radius,
                                         float *sdf_vol) try {
+  const int refv = (kind & PBF_VOLUME_REFERENCE) ? 1 : 0;
+  kind &= ~(uint32_t)PBF_VOLUME_REFERENCE;
   if (!res || !lo || !hi || !sdf_vol || kind > 1 || !(kernel_radius > 0.0)) return PBF_E_INVALID;
   for (int a = 0; a < 3; ++a)
     if (res[a] < 1 || res[a] > 65536) return PBF_E_INVALID;
@@ -153,6 +182,8 @@ extern "C" int pbf_build_volume_map_gpu(uint32_t kind, const double *shape, cons
           qw.push_back(A.kp * t * t * t);
         }
       }
+  A.refv = refv;
+  if (refv) reference_volume_quadrature(h, qp, qw);
   A.nq = (uint32_t)qw.size();
   A.cellv = std::pow(2.0 * h / M, 3);
   A.ballv = 4.0 / 3.0 * 3.141592653589793 * h * h * h;

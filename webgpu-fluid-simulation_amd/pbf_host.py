@@ -270,6 +270,7 @@ MLP_FEATURES = 16
 WARM_PLAIN, WARM_LEARNED, WARM_PREVIOUS, WARM_RATIO = 0, 1, 2, 3
 WARM_PROJECT = 0x100  # OR-ed into a mode: projected accumulation of lambda
 WARM_SMOOTH = 0x200   # OR-ed into a mode: the warm lambda field smoothed over neighbours
+VOLUME_REFERENCE = 0x100  # OR-ed into a map-builder kind: the reference's own boundary volume
 WARM_RATIO_MAX = 2.0  # clamp of the ratio-mode factor, in units of relax
 
 

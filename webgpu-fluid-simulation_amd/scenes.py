@@ -19,7 +19,8 @@ from typing import List, Optional
 
 import numpy as np
 
-from pbf_host import PBF, PBF_GRID_DENSE, PBF_GRID_HASH, build_volume_map, load_library
+from pbf_host import (PBF, PBF_GRID_DENSE, PBF_GRID_HASH, VOLUME_REFERENCE, build_cubic_field, build_volume_map,
+                      load_library)
 
 SEED = 0x5EED
 
@@ -245,17 +246,12 @@ def cubic_field_from(fn_sdf, fn_vol, res=(20, 20, 20), lo=(0.0, 0.0, 0.0), hi=(1
 
 
 def unit_box_cdm_field(res=(20, 20, 20), h=REF_H) -> Field:
-    """Unit-box walls in the .cdm layout: SDF = min distance to the six faces
-    (matches public/model/domain_boundary.cdm to 3.5e-7), volume = spherical
-    cap of the kernel ball beyond the nearest wall."""
-    def sdf(x, y, z):
-        return np.minimum.reduce([x, 1 - x, y, 1 - y, z, 1 - z])
-
-    def vol(x, y, z):
-        d = np.clip(sdf(x, y, z), -h, h)
-        return np.pi * (h - d) ** 2 * (2 * h + d) / 3.0
-
-    return cubic_field_from(sdf, vol, res)
+    """Unit-box walls in the .cdm layout, built by pbf_build_cubic_field with
+    the reference's own volume definition (PBF_VOLUME_REFERENCE): the SDF is
+    the distance to the nearest face and the volume reproduces
+    public/model/domain_boundary.cdm's (tests/test_volume_definition.py)."""
+    sdf, vol = build_cubic_field(VOLUME_REFERENCE, res, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), h)
+    return Field("cubic", tuple(res), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), sdf=sdf, vol=vol)
 
 
 def cdm_text(fld: Field) -> str:
