@@ -24,7 +24,7 @@ sys.path[:0] = [os.path.join(ROOT, "webgpu-fluid-simulation_amd"), os.path.join(
 import numpy as np  # noqa: E402
 
 import scenes  # noqa: E402
-from pbf_host import PBF_FLAG_TIMING, PBF_GRID_HASH, copy_bandwidth  # noqa: E402
+from pbf_host import PBF_FLAG_NO_GRAPH, PBF_FLAG_TIMING, PBF_GRID_HASH, copy_bandwidth  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
@@ -84,35 +84,35 @@ DEEP_BLOB = os.path.join(ROOT, "webgpu-fluid-simulation_amd", "warm_start", "dee
 PROFILE = os.path.join(ROOT, "profiles", "dam_break_1m_kernels_and_traffic.md")
 
 
-def profiled_traffic(pass_name):
-    """HBM bytes per launch of a pass from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (scaled by the copy-kernel factors), or None."""
-    try:
-        for line in open(PROFILE):
-            if line.startswith("traffic_bytes_per_launch:"):
-                kv = dict(t.split("=") for t in line.split(":", 1)[1].split())
-                return int(kv[pass_name]) if pass_name in kv else None
-    except OSError:
-        return None
-    return None
 COUNTERS = os.path.join(ROOT, "profiles", "dam_break_1m_pass_counters.md")
 SIMDS = 1024  # MI355X: 256 CUs x 4 SIMDs
 
 
-def profiled_valu(pass_name):
-    """(VALU wave-instructions per launch of a pass, engine clock in Hz) from
-    the committed rocprofv3 SQ_INSTS_VALU / GRBM_GUI_ACTIVE summary, or None."""
-    n = clk = None
-    try:
-        for line in open(COUNTERS):
-            if line.startswith("valu_instructions_per_launch:"):
-                kv = dict(t.split("=") for t in line.split(":", 1)[1].split())
-                n = int(kv[pass_name]) if pass_name in kv else None
-            elif line.startswith("engine_clock_hz:"):
-                clk = float(line.split(":", 1)[1])
-    except OSError:
-        return None
-    return (n, clk) if n and clk else None
+def profile_lines():
+    """The machine lines of the committed rocprofv3 summaries of the headline
+    scene: {key: {pass: value}} for traffic_bytes_per_launch (FETCH_SIZE and
+    WRITE_SIZE scaled by the copy-kernel factors), avg_launch_us,
+    valu_instructions_per_launch (SQ_INSTS_VALU), engine_clock_hz
+    (GRBM_GUI_ACTIVE cycles / traced duration), ta_busy_frac and bound, plus
+    {"dominant_pass": name} (the pass with the largest total time in the
+    kernel trace)."""
+    out = {}
+    for path in (PROFILE, COUNTERS):
+        try:
+            lines = open(path).read().splitlines()
+        except OSError:
+            continue
+        for line in lines:
+            key, sep, rest = line.partition(":")
+            if not sep or " " in key:
+                continue
+            if key == "dominant_pass":
+                out[key] = rest.strip()
+            elif key in ("traffic_bytes_per_launch", "avg_launch_us", "valu_instructions_per_launch",
+                         "engine_clock_hz", "ta_busy_frac", "bound"):
+                kv = dict(t.split("=", 1) for t in rest.split())
+                out[key] = {k: (v if key == "bound" else float(v)) for k, v in kv.items()}
+    return out
 
 
 TARGET_ERR = 0.005
@@ -330,6 +330,9 @@ def main():
     ap.add_argument("--workload", default="dam_break_1m", choices=sorted(WORKLOADS),
                     help="the measured scene (the default is the headline workload; the other is the "
                          "hashed-grid parity configuration, measured without the warm-start and batch sections)")
+    ap.add_argument("--direct", action="store_true",
+                    help="launch the passes directly instead of replaying a hipGraph per substep (for counter "
+                         "passes under rocprofv3 --pmc)")
     ap.add_argument("--warm-start-profile", action="store_true",
                     help="only run the warm-start network's kernels for a kernel trace: 20 substeps of the 1M dam "
                          "break with the deep ratio network (20 iterations, fallback on), then 2 epochs of "
@@ -359,7 +362,7 @@ def main():
     # per substep.  Timed region 2 (roofline): the same steps from the same
     # initial state with per-pass HIP events on the kernels' own dispatch
     # packets, which needs direct launches.
-    sim = scenes.make_engine(sc)
+    sim = scenes.make_engine(sc, flags=PBF_FLAG_NO_GRAPH if args.direct else 0)
     tuning = None
     if not args.no_tune:
         tuning = sim.autotune(sc.substep_dt(), sc.iterations, 3)
@@ -403,25 +406,40 @@ def main():
         for name, (ms, cnt) in passes.items():
             if cnt:
                 per_pass[name] = {"ms_total": round(ms, 3), "launches": int(cnt), "us_avg": round(1e3 * ms / cnt, 2)}
-        dom = max((k for k in per_pass if k in pb), key=lambda k: per_pass[k]["ms_total"])
+        prof = profile_lines() if headline else {}  # the committed profiles cover the headline scene
+        dom = prof.get("dominant_pass")
+        if dom not in per_pass or dom not in pb:
+            dom = max((k for k in per_pass if k in pb), key=lambda k: per_pass[k]["ms_total"])
         avg_s = per_pass[dom]["ms_total"] / per_pass[dom]["launches"] / 1e3
         bytes_per_launch = pb[dom] * N
         ach = bytes_per_launch / avg_s / 1e9
-        traffic = profiled_traffic(dom) if headline else None  # the committed profiles cover the headline scene
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+        traffic = prof.get("traffic_bytes_per_launch", {}).get(dom)
+        traffic = int(traffic) if traffic else None
+        # the limiter the counters name for this pass (valu, ta or hbm); the
+        # achieved/peak pair stays the algorithmic-bytes HBM fraction
+        roof = {"bound": prof.get("bound", {}).get(dom, "hbm"), "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": round(avg_s * 1e6, 2),
                 "traffic_GBs": round(traffic / avg_s / 1e9, 1) if traffic else None,
+                "traffic_frac": round(traffic / avg_s / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+                "dominant_source": "profiles/dam_break_1m_kernels_and_traffic.md (largest total time in the "
+                                   "rocprofv3 kernel trace)" if prof.get("dominant_pass") == dom else
+                                   "largest total time over this run's pass events",
                 "traffic_source": "profiles/dam_break_1m_kernels_and_traffic.md (rocprofv3 FETCH_SIZE x2, "
                                   "WRITE_SIZE x1 per dispatch)" if traffic else None}
-        valu = profiled_valu(dom) if headline else None
-        if valu:
-            # the kernel's own ceiling: a wave64 VALU instruction occupies a
-            # 16-lane SIMD for 4 cycles, at the clock the profile measured
-            roof["valu"] = {"wave_instructions_per_launch": valu[0], "engine_clock_hz": valu[1],
-                            "issue_frac": round(valu[0] * 4 / (SIMDS * valu[1] * avg_s), 3),
+        vi = prof.get("valu_instructions_per_launch", {}).get(dom)
+        clk = prof.get("engine_clock_hz", {}).get(dom)
+        if vi and clk:
+            # a wave64 VALU instruction occupies a 16-lane SIMD for 4 cycles;
+            # the clock is the profile's GRBM_GUI_ACTIVE cycles over the traced
+            # duration, which counts the dispatch's start and end too, so the
+            # fraction is a lower bound
+            roof["valu"] = {"wave_instructions_per_launch": int(vi), "engine_clock_hz": clk,
+                            "issue_frac": round(vi * 4 / (SIMDS * clk * avg_s), 3),
                             "source": "profiles/dam_break_1m_pass_counters.md (rocprofv3 SQ_INSTS_VALU, "
                                       "GRBM_GUI_ACTIVE)"}
+        if dom in prof.get("ta_busy_frac", {}):
+            roof["ta_busy_frac"] = prof["ta_busy_frac"][dom]
         for name in per_pass:
             if name in pb:
                 a = pb[name] * N / (per_pass[name]["us_avg"] * 1e-6) / 1e9
