@@ -205,13 +205,23 @@ int pbf_read_velocities(pbf_ctx *ctx, float *xyzw, uint32_t count);
 
 enum {
   PBF_FIELD_POSITION = 0, PBF_FIELD_VELOCITY = 1, PBF_FIELD_ACCEL = 2, PBF_FIELD_LAMBDA = 3,
-  PBF_FIELD_OMEGA = 4, PBF_FIELD_NORMAL = 5, PBF_FIELD_BOUNDARY = 6, PBF_FIELD_PREDICTED = 7
+  PBF_FIELD_OMEGA = 4, PBF_FIELD_NORMAL = 5, PBF_FIELD_BOUNDARY = 6, PBF_FIELD_PREDICTED = 7,
+  PBF_FIELD_LAMBDA_SUM = 8
 };
 /* Any per-particle float4 state in original order (debug readback of
- * PBF.debug, PBF.ts:514-573). LAMBDA is in .x; NORMAL and VELOCITY have
- * w = 0; PREDICTED is defined after pbf_search only (a step reuses its
- * buffer). */
+ * PBF.debug, PBF.ts:514-573). LAMBDA (the last iteration's lambda) and
+ * LAMBDA_SUM (the substep's lambda sum that the warm-start modes carry; zero
+ * before any warm-start substep) are in .x; NORMAL and VELOCITY have w = 0;
+ * PREDICTED is defined after pbf_search only (a step reuses its buffer). */
 int pbf_read_field(pbf_ctx *ctx, uint32_t field, float *out4, uint32_t count);
+/* Checkpoint upload, the inverse of pbf_read_field for the state a substep
+ * carries into the next: VELOCITY and ACCEL (w ignored), LAMBDA and
+ * LAMBDA_SUM (.x).  in4: `count` float4 rows in the original particle order,
+ * count = the particle count.  A snapshot {positions, VELOCITY, ACCEL, LAMBDA,
+ * LAMBDA_SUM} read after a step and written back after pbf_set_positions
+ * resumes the run bit for bit.  The reference keeps this state GPU-resident
+ * with no upload path (PBF.ts:71-89 resets it); no reference counterpart. */
+int pbf_write_field(pbf_ctx *ctx, uint32_t field, const float *in4, uint32_t count);
 
 /* Neighbour search results of the last substep (NeighborSearch.debug,
  * neighborSearch.ts:303-378).  Cells are indexed globally over all scenes.

@@ -330,8 +330,11 @@ void enqueue_search(pbf_ctx *c, const LaunchCtx &L, PassTimer &T, bool dbg, uint
   t = T.begin(P_SETTLE, L);
   // the binning ranks are consumed by the scatter: their buffer takes the
   // final slots
+  // the previous lambdas and the slot map only feed the learned and
+  // data-generation modes' extra state (k_settle_extra, the features)
+  const bool extra = mode != M_PLAIN;
   launch_settle(t, c->tmp_key, c->tmp_src, c->cellkey, c->offsets, c->XB, c->pos, c->lam, c->XA, c->Xold, c->ids,
-                c->lam_sorted, c->rank);
+                extra ? c->lam_sorted : nullptr, extra ? c->rank : nullptr);
   T.end();
   if (mode != M_PLAIN)
     launch_settle_extra(L, c->tmp_src, c->rank, c->lam_acc, c->acc_sorted, c->vel, c->vel_sorted);
@@ -1321,17 +1324,47 @@ int pbf_read_field(pbf_ctx *c, uint32_t field, float *out4, uint32_t count) try 
     case PBF_FIELD_NORMAL: return read_sorted4(c, c->nrm, c->ids, out4, count, 1);
     case PBF_FIELD_BOUNDARY: return read_sorted4(c, c->B, c->ids, out4, count);
     case PBF_FIELD_PREDICTED: return read_sorted4(c, c->XA, ids, out4, count);
-    case PBF_FIELD_LAMBDA: {
+    case PBF_FIELD_LAMBDA:
+    case PBF_FIELD_LAMBDA_SUM: {
       if (count != c->n) return fail(c, PBF_E_LENGTH, "readback count must equal the particle count");
       int rc = sync_ctx(c);
       if (rc != PBF_OK) return rc;
-      launch_unsort1(c->stream, c->n, c->lam, c->ids, c->stage4);
+      if (field == PBF_FIELD_LAMBDA_SUM && !c->lam_acc) {  // no warm-start state yet: a zero sum
+        std::memset(out4, 0, (size_t)c->n * 16);
+        return PBF_OK;
+      }
+      launch_unsort1(c->stream, c->n, field == PBF_FIELD_LAMBDA ? c->lam : c->lam_acc, c->ids, c->stage4);
       HIPCHK(hipMemcpyAsync(out4, c->stage4, (size_t)c->n * 16, hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
       return PBF_OK;
     }
     default: return fail(c, PBF_E_INVALID, "unknown field");
   }
+} PBF_CATCH(c)
+
+int pbf_write_field(pbf_ctx *c, uint32_t field, const float *in4, uint32_t count) try {
+  if (!c || (!in4 && count)) return PBF_E_INVALID;
+  if (field != PBF_FIELD_VELOCITY && field != PBF_FIELD_ACCEL && field != PBF_FIELD_LAMBDA &&
+      field != PBF_FIELD_LAMBDA_SUM)
+    return fail(c, PBF_E_INVALID, "only VELOCITY, ACCEL, LAMBDA and LAMBDA_SUM can be written");
+  if (count != c->n) return fail(c, PBF_E_LENGTH, "upload count must equal the particle count");
+  HIPCHK(hipSetDevice(c->device));
+  int rc = sync_ctx(c);
+  if (rc != PBF_OK) return rc;
+  if (c->n == 0) return PBF_OK;
+  if (field == PBF_FIELD_LAMBDA_SUM && (rc = ensure_learn_buffers(c)) != PBF_OK) return rc;
+  // the staging rows are written by the host copy, then gathered into slot
+  // order through the id map (slot i holds particle ids[i])
+  HIPCHK(hipMemcpyAsync(c->stage4, in4, (size_t)c->n * 16, hipMemcpyHostToDevice, c->stream));
+  switch (field) {
+    case PBF_FIELD_VELOCITY: launch_sort4(c->stream, c->n, c->stage4, c->ids, c->vel, 1); break;
+    case PBF_FIELD_ACCEL: launch_sort4(c->stream, c->n, c->stage4, c->ids, c->acc, 1); break;
+    case PBF_FIELD_LAMBDA: launch_sort1(c->stream, c->n, c->stage4, c->ids, c->lam); break;
+    default: launch_sort1(c->stream, c->n, c->stage4, c->ids, c->lam_acc); break;
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PBF_OK;
 } PBF_CATCH(c)
 
 uint32_t pbf_num_cells(const pbf_ctx *c) { return c ? c->ncells : 0u; }

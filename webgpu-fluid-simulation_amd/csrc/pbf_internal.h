@@ -94,6 +94,8 @@ void launch_diag(const LaunchCtx &L, const float4 *pos, const uint32_t *nbr, con
                  uint32_t *hist_ctr);
 void launch_unsort4(hipStream_t st, uint32_t n, const float4 *src, const uint32_t *ids, float4 *dst, int clear_w);
 void launch_unsort1(hipStream_t st, uint32_t n, const float *src, const uint32_t *ids, float4 *dst);
+void launch_sort4(hipStream_t st, uint32_t n, const float4 *src, const uint32_t *ids, float4 *dst, int clear_w);
+void launch_sort1(hipStream_t st, uint32_t n, const float4 *src, const uint32_t *ids, float *dst);
 void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v);
 void launch_iota(hipStream_t st, uint32_t *p, uint32_t n);
 void launch_copy_probe(hipStream_t st, const void *src, void *dst, size_t bytes);

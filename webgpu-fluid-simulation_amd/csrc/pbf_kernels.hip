@@ -485,7 +485,8 @@ __device__ __forceinline__ uint32_t rank_below(const uint32_t *__restrict__ keys
 }
 
 // Stable order inside each cell (ascending particle id), then move the
-// particle data into its final slot; dest[s] = that slot, for k_settle_extra.
+// particle data into its final slot; dest[s] = that slot, for k_settle_extra
+// (lam_out and dest may be null: the plain solve needs neither).
 __global__ void __launch_bounds__(BS) k_settle(uint32_t n, const uint32_t *__restrict__ tmp_key,
                                                const uint32_t *__restrict__ tmp_src,
                                                const uint32_t *__restrict__ cellkey,
@@ -503,7 +504,7 @@ __global__ void __launch_bounds__(BS) k_settle(uint32_t n, const uint32_t *__res
   Xold[f] = pos[p];
   ids[f] = key;
   if (lam_out) lam_out[f] = lam_in[p];
-  dest[s] = f;
+  if (dest) dest[s] = f;
 }
 
 // Learned / data-generation modes: carry the previous substep's lambda sum and
@@ -1236,6 +1237,23 @@ __global__ void __launch_bounds__(BS) k_unsort1(uint32_t n, const float *__restr
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[ids[i]] = make_float4(src[i], 0.0f, 0.0f, 0.0f);
 }
+// The inverse gathers (checkpoint upload): slot i takes the caller-order row
+// of its particle id; a float4 row, or the .x of one for a scalar field.
+__global__ void __launch_bounds__(BS) k_sort4(uint32_t n, const float4 *__restrict__ src,
+                                              const uint32_t *__restrict__ ids, float4 *__restrict__ dst,
+                                              int clear_w) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    float4 v = src[ids[i]];
+    if (clear_w) v.w = 0.0f;
+    dst[i] = v;
+  }
+}
+__global__ void __launch_bounds__(BS) k_sort1(uint32_t n, const float4 *__restrict__ src,
+                                              const uint32_t *__restrict__ ids, float *__restrict__ dst) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[ids[i]].x;
+}
 __global__ void __launch_bounds__(BS) k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -1471,6 +1489,12 @@ void launch_energy(const LaunchCtx &L, const float4 *pos, const float4 *vel, con
 
 void launch_unsort4(hipStream_t st, uint32_t n, const float4 *src, const uint32_t *ids, float4 *dst, int clear_w) {
   if (n) hipLaunchKernelGGL(k_unsort4, dim3(nblocks(n, BS)), dim3(BS), 0, st, n, src, ids, dst, clear_w);
+}
+void launch_sort4(hipStream_t st, uint32_t n, const float4 *src, const uint32_t *ids, float4 *dst, int clear_w) {
+  if (n) hipLaunchKernelGGL(k_sort4, dim3(nblocks(n, BS)), dim3(BS), 0, st, n, src, ids, dst, clear_w);
+}
+void launch_sort1(hipStream_t st, uint32_t n, const float4 *src, const uint32_t *ids, float *dst) {
+  if (n) hipLaunchKernelGGL(k_sort1, dim3(nblocks(n, BS)), dim3(BS), 0, st, n, src, ids, dst);
 }
 void launch_unsort1(hipStream_t st, uint32_t n, const float *src, const uint32_t *ids, float4 *dst) {
   if (n) hipLaunchKernelGGL(k_unsort1, dim3(nblocks(n, BS)), dim3(BS), 0, st, n, src, ids, dst);

@@ -24,7 +24,9 @@ PBF_E_INVALID, PBF_E_CAPACITY, PBF_E_LENGTH, PBF_E_FORMAT, PBF_E_DEVICE, PBF_E_O
 PBF_GRID_DENSE, PBF_GRID_HASH = 0, 1
 PBF_FLAG_NO_GRAPH, PBF_FLAG_CUBIC_FIXED, PBF_FLAG_TIMING = 1, 2, 4
 FIELDS = {"position": 0, "velocity": 1, "accel": 2, "lambda": 3, "omega": 4, "normal": 5,
-          "boundary": 6, "predicted": 7}
+          "boundary": 6, "predicted": 7, "lambda_sum": 8}
+# the state a substep carries into the next (positions aside)
+CHECKPOINT_FIELDS = ("velocity", "accel", "lambda", "lambda_sum")
 
 
 class PBFError(RuntimeError):
@@ -99,6 +101,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "pbf_step": (I, [P, F, U32, U32]), "pbf_run": (I, [P]), "pbf_sync": (I, [P]),
         "pbf_read_positions": (I, [P, pf, U32]), "pbf_read_velocities": (I, [P, pf, U32]),
         "pbf_read_field": (I, [P, U32, pf, U32]),
+        "pbf_write_field": (I, [P, U32, pf, U32]),
         "pbf_num_cells": (U32, [P]),
         "pbf_hash_buckets_distinct": (I, [C.POINTER(C.c_int32), U32]),
         "pbf_read_cells": (I, [P, pu, pu]),
@@ -485,6 +488,32 @@ class PBF:
         out = np.zeros((n, 4), np.float32)
         self._check(self.lib.pbf_read_field(self.ctx, FIELDS[name], _fptr(out), n))
         return out
+
+    def write_field(self, name: str, data: np.ndarray):
+        """Checkpoint upload (pbf_write_field): velocity / accel rows (x, y, z),
+        lambda / lambda_sum values, in the original particle order."""
+        n = int(self.lib.pbf_particle_count(self.ctx))
+        a = np.asarray(data, np.float32)
+        buf = np.zeros((n, 4), np.float32)
+        if name in ("lambda", "lambda_sum"):
+            buf[:, 0] = a[:, 0] if a.ndim == 2 else a.reshape(n)
+        else:
+            buf[:, :3] = a.reshape(n, -1)[:, :3]
+        self._check(self.lib.pbf_write_field(self.ctx, FIELDS[name], _fptr(buf), n))
+
+    def checkpoint(self) -> dict:
+        """Everything a later context needs to resume this run bit for bit:
+        positions plus the carried per-particle state, in the original order."""
+        snap = {"positions": self.read_positions()}
+        for name in CHECKPOINT_FIELDS:
+            snap[name] = self.read_field(name)
+        return snap
+
+    def resume(self, snap: dict):
+        """Load a checkpoint() taken from a context with the same configuration."""
+        self.setParticlePosition(snap["positions"])
+        for name in CHECKPOINT_FIELDS:
+            self.write_field(name, snap[name])
 
     def read_cells(self):
         nc = int(self.lib.pbf_num_cells(self.ctx))
