@@ -88,16 +88,19 @@ COUNTERS = os.path.join(ROOT, "profiles", "dam_break_1m_pass_counters.md")
 SIMDS = 1024  # MI355X: 256 CUs x 4 SIMDs
 
 
-def profile_lines():
-    """The machine lines of the committed rocprofv3 summaries of the headline
+PROFILE_4M = os.path.join(ROOT, "profiles", "obstacle_drop_4m_kernels.md")
+
+
+def profile_lines(workload="dam_break_1m"):
+    """The machine lines of the committed rocprofv3 summaries of a measured
     scene: {key: {pass: value}} for traffic_bytes_per_launch (FETCH_SIZE and
-    WRITE_SIZE scaled by the copy-kernel factors), avg_launch_us,
-    valu_instructions_per_launch (SQ_INSTS_VALU), engine_clock_hz
-    (GRBM_GUI_ACTIVE cycles / traced duration), ta_busy_frac and bound, plus
-    {"dominant_pass": name} (the pass with the largest total time in the
-    kernel trace)."""
+    WRITE_SIZE scaled by the copy-kernel factors), avg_launch_us, and for the
+    headline scene valu_instructions_per_launch (SQ_INSTS_VALU),
+    engine_clock_hz (GRBM_GUI_ACTIVE cycles / traced duration), ta_busy_frac
+    and bound, plus {"dominant_pass": name} (the pass with the largest total
+    time in the kernel trace)."""
     out = {}
-    for path in (PROFILE, COUNTERS):
+    for path in ((PROFILE, COUNTERS) if workload == "dam_break_1m" else (PROFILE_4M,)):
         try:
             lines = open(path).read().splitlines()
         except OSError:
@@ -406,7 +409,7 @@ def main():
         for name, (ms, cnt) in passes.items():
             if cnt:
                 per_pass[name] = {"ms_total": round(ms, 3), "launches": int(cnt), "us_avg": round(1e3 * ms / cnt, 2)}
-        prof = profile_lines() if headline else {}  # the committed profiles cover the headline scene
+        prof = profile_lines(args.workload)
         dom = prof.get("dominant_pass")
         if dom not in per_pass or dom not in pb:
             dom = max((k for k in per_pass if k in pb), key=lambda k: per_pass[k]["ms_total"])
@@ -425,8 +428,9 @@ def main():
                 "dominant_source": "profiles/dam_break_1m_kernels_and_traffic.md (largest total time in the "
                                    "rocprofv3 kernel trace)" if prof.get("dominant_pass") == dom else
                                    "largest total time over this run's pass events",
-                "traffic_source": "profiles/dam_break_1m_kernels_and_traffic.md (rocprofv3 FETCH_SIZE x2, "
-                                  "WRITE_SIZE x1 per dispatch)" if traffic else None}
+                "traffic_source": ("profiles/dam_break_1m_kernels_and_traffic.md" if headline else
+                                   "profiles/obstacle_drop_4m_kernels.md") +
+                                  " (rocprofv3 FETCH_SIZE x2, WRITE_SIZE x1 per dispatch)" if traffic else None}
         vi = prof.get("valu_instructions_per_launch", {}).get(dom)
         clk = prof.get("engine_clock_hz", {}).get(dom)
         if vi and clk:
