@@ -59,7 +59,12 @@ typedef struct {
   uint32_t grid_mode;       /* PBF_GRID_DENSE (cell = h) or PBF_GRID_HASH */
   uint32_t hash_log2;       /* hash table size 2^hash_log2 (PBF_GRID_HASH) */
   uint32_t flags;           /* PBF_FLAG_* */
-  uint32_t reserved[5];
+  uint32_t cell_slabs;      /* 1, 2 or 4 (0 -> 4): a cell's particles are kept ordered by their
+                               z-slab of height h / cell_slabs, then by id; the neighbour list
+                               follows that order (the reference's order inside a cell is its
+                               atomic ranks, neighborShader.ts:30) and its build skips the slabs
+                               of the cells above and below that lie out of reach */
+  uint32_t reserved[4];
 } pbf_params;
 
 enum {

@@ -19,7 +19,7 @@ POSITION, VELOCITY, ACCEL, LAMBDA, OMEGA, NORMAL, BOUNDARY, PREDICTED = range(8)
 class OrcParams(C.Structure):
     _fields_ = [("particle_radius", C.c_double), ("kernel_radius", C.c_double), ("rest_density", C.c_double),
                 ("domain_lo", C.c_double * 3), ("domain_hi", C.c_double * 3), ("grid_mode", C.c_int),
-                ("hash_log2", C.c_int)]
+                ("hash_log2", C.c_int), ("cell_slabs", C.c_int)]
 
 
 _lib = None
@@ -77,7 +77,7 @@ class Oracle:
     """CPU restatement of one PBF scene (particles kept in id order)."""
 
     def __init__(self, xyzw, particle_radius=0.006, kernel_radius=0.025, rest_density=1000.0,
-                 domain_lo=(0, 0, 0), domain_hi=(1, 1, 1), grid_mode=0, hash_log2=22):
+                 domain_lo=(0, 0, 0), domain_hi=(1, 1, 1), grid_mode=0, hash_log2=22, cell_slabs=0):
         self.L = lib()
         p = OrcParams()
         p.particle_radius = particle_radius
@@ -87,6 +87,7 @@ class Oracle:
         p.domain_hi = _d3(domain_hi)
         p.grid_mode = grid_mode
         p.hash_log2 = hash_log2
+        p.cell_slabs = cell_slabs  # z-slabs ordering a cell's particles (0: the library default, 4)
         a = np.ascontiguousarray(np.asarray(xyzw, np.float32).reshape(-1, 4))
         self.n = a.shape[0]
         self.s = self.L.orc_create(C.byref(p), self.n, _f(a))

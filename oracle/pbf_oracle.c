@@ -14,7 +14,8 @@
  *   - spiky / cohesion gradients are f(r) * x_ij with one division by r;
  *   - sin() in the wall clamp is the Cephes single-precision sinf algorithm
  *     (S. L. Moshier), evaluated with plain float operations;
- *   - neighbour order: 27 cells x-major, then ascending particle id.
+ *   - neighbour order: 27 cells x-major, then by z-slab inside the cell
+ *     (cell_slabs slabs of height h / cell_slabs), then ascending particle id.
  */
 #include "pbf_oracle.h"
 
@@ -31,6 +32,7 @@ struct orc_sim {
   float lo[3], hi[3], inv_cell;
   int dims[3];
   int grid_mode, hash_log2;
+  int slabs;      /* z-slabs per cell (order inside a cell) */
   uint32_t ncells;
   double Kp64, h2_64;
   /* config (src/common/shader.ts:50-57) */
@@ -46,6 +48,7 @@ struct orc_sim {
   float *x, *v, *a, *lam, *om, *nrm, *bnd, *xs, *xi, *xn;
   /* search */
   uint32_t *pcell, *cnt, *off, *sorted;
+  uint8_t *pslab;
   uint32_t *noff, *nbr;
   uint64_t nbr_cap, npairs;
 };
@@ -439,6 +442,7 @@ orc_sim *orc_create(const orc_params *p, int n, const float *xyzw) {
   s->inv_cell = (float)(1.0 / s->h64);
   s->grid_mode = p->grid_mode;
   s->hash_log2 = p->hash_log2;
+  s->slabs = (p->cell_slabs == 1 || p->cell_slabs == 2) ? p->cell_slabs : 4;
   if (s->grid_mode == 1)
     s->ncells = 1u << p->hash_log2;
   else
@@ -454,6 +458,7 @@ orc_sim *orc_create(const orc_params *p, int n, const float *xyzw) {
     s->x[4 * i + 3] = 0.0f;
   }
   s->pcell = (uint32_t *)calloc((size_t)(n > 0 ? n : 1), 4);
+  s->pslab = (uint8_t *)calloc((size_t)(n > 0 ? n : 1), 1);
   s->sorted = (uint32_t *)calloc((size_t)(n > 0 ? n : 1), 4);
   s->cnt = (uint32_t *)calloc(s->ncells, 4);
   s->off = (uint32_t *)calloc((size_t)s->ncells + 1, 4);
@@ -466,7 +471,7 @@ orc_sim *orc_create(const orc_params *p, int n, const float *xyzw) {
 void orc_destroy(orc_sim *s) {
   if (!s) return;
   free(s->x); free(s->v); free(s->a); free(s->lam); free(s->om); free(s->nrm); free(s->bnd);
-  free(s->xs); free(s->xi); free(s->xn); free(s->pcell); free(s->sorted); free(s->cnt); free(s->off);
+  free(s->xs); free(s->xi); free(s->xn); free(s->pcell); free(s->pslab); free(s->sorted); free(s->cnt); free(s->off);
   free(s->noff); free(s->nbr); free(s->fsdf); free(s->fvol); free(s->fsv);
   free(s);
 }
@@ -556,6 +561,17 @@ static void predict(orc_sim *s, float dt) {
   }
 }
 
+/* z-slab of a position inside its cell: floor(S * fz), fz = (z - lo) / h -
+   cz in float (the cell's own coordinate, clamped like the cell), clamped to
+   [0, S). */
+static inline int z_slab(const orc_sim *s, const float *x, int cz) {
+  float fz = (x[2] - s->lo[2]) * s->inv_cell - (float)cz;
+  int k = (int)floorf(fz * (float)s->slabs);
+  if (k < 0) k = 0;
+  if (k > s->slabs - 1) k = s->slabs - 1;
+  return k;
+}
+
 static void search(orc_sim *s) {
   int n = s->n;
   uint32_t C = s->ncells;
@@ -565,17 +581,22 @@ static void search(orc_sim *s) {
     cell_coord(s, s->xs + 4 * i, c);
     uint32_t k = cell_key(s, c[0], c[1], c[2]);
     s->pcell[i] = k;
+    s->pslab[i] = (uint8_t)z_slab(s, s->xs + 4 * i, c[2]);
     s->cnt[k]++;
   }
   s->off[0] = 0;
   for (uint32_t c = 0; c < C; ++c) s->off[c + 1] = s->off[c] + s->cnt[c];
   {
+    /* inside a cell (bucket): by z-slab, then ascending id */
     uint32_t *fill = (uint32_t *)malloc((size_t)C * 4);
     memcpy(fill, s->off, (size_t)C * 4);
-    for (int i = 0; i < n; ++i) s->sorted[fill[s->pcell[i]]++] = (uint32_t)i;
+    for (int z = 0; z < s->slabs; ++z)
+      for (int i = 0; i < n; ++i)
+        if (s->pslab[i] == z) s->sorted[fill[s->pcell[i]]++] = (uint32_t)i;
     free(fill);
   }
-  /* neighbour counts then lists (27 cells x-major, ascending id in a cell) */
+  /* neighbour counts then lists (27 cells x-major; inside a cell the order
+     above) */
   uint32_t *ncount = (uint32_t *)calloc((size_t)n + 1, 4);
   float h2 = s->h2;
   for (int pass = 0; pass < 2; ++pass) {

@@ -21,7 +21,11 @@
  *   XSPH + tension    src/simulator/PBF/shader/XSPH.ts:31-92
  *
  * Particles stay in input (id) order; the neighbour list of particle i walks
- * the 27 cells x-major (dx outer, dz inner) and, inside a cell, ascending id.
+ * the 27 cells x-major (dx outer, dz inner) and, inside a cell, by z-slab
+ * (floor(S * frac(z / h)), S = cell_slabs), then ascending id.  The reference
+ * leaves the order inside a cell to its atomic ranks (neighborShader.ts:30),
+ * so any fixed order is as faithful; this one lets the GPU list build skip
+ * the far slabs of the cells above and below.
  * Every floating-point expression is written out operation by operation and
  * the file is compiled with -ffp-contract=off, so results are reproducible.
  */
@@ -43,6 +47,7 @@ typedef struct {
   double domain_hi[3];
   int grid_mode;            /* 0 dense, 1 spatial hash */
   int hash_log2;            /* table size 2^hash_log2 when grid_mode == 1 */
+  int cell_slabs;           /* z-slabs per cell ordering a cell's particles: 1, 2 or 4 (0 -> 4) */
 } orc_params;
 
 /* field ids for orc_get */

@@ -5,9 +5,9 @@ import oracle
 import scenes
 
 
-def make_oracle(sc: scenes.Scene) -> oracle.Oracle:
+def make_oracle(sc: scenes.Scene, cell_slabs: int = 0) -> oracle.Oracle:
     o = oracle.Oracle(sc.positions, sc.particle_radius, sc.kernel_radius, 1000.0, sc.domain_lo, sc.domain_hi,
-                      sc.grid_mode, sc.hash_log2)
+                      sc.grid_mode, sc.hash_log2, cell_slabs)
     c = sc.config
     o.set_config(c["XSPH"], c["vorticity"], c["surfaceTension"], c["gravity"])
     f = sc.field

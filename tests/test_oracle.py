@@ -18,11 +18,11 @@ import scenes
 F32 = np.float32
 
 
-def make(points, r=0.025, h=None, lo=(0, 0, 0), hi=(1, 1, 1), grid_mode=0, hash_log2=22):
+def make(points, r=0.025, h=None, lo=(0, 0, 0), hi=(1, 1, 1), grid_mode=0, hash_log2=22, cell_slabs=0):
     h = 4 * r if h is None else h
     p = np.zeros((len(points), 4), np.float32)
     p[:, :3] = points
-    return oracle.Oracle(p, r, h, 1000.0, lo, hi, grid_mode, hash_log2)
+    return oracle.Oracle(p, r, h, 1000.0, lo, hi, grid_mode, hash_log2, cell_slabs)
 
 
 # --------------------------------------------------------------------------
@@ -122,24 +122,38 @@ def test_neighbor_sets_vs_brute_force(grid_mode, hash_log2):
         assert set(got) == want[i], i
 
 
-def test_neighbor_order_and_cells():
+@pytest.mark.parametrize("slabs", [1, 2, 4, 0])
+def test_neighbor_order_and_cells(slabs):
+    # order of a list: cells x-major, then the z-slab inside the cell
+    # (floor(S * (z / h - cz)), S = 4 by default), then id; the sets do not
+    # depend on S
     rng = np.random.default_rng(3)
     r, h = 0.025, 0.1
     x = rng.uniform(0.0, 1.0, (3000, 3)).astype(np.float32)
-    o = make(x, r=r, h=h)
+    o = make(x, r=r, h=h, cell_slabs=slabs)
     o.search(1e-9)
     xs = o.get(oracle.PREDICTED)[:, :3]
     D = 10
-    c = np.clip(np.floor(xs * np.float32(1 / h)).astype(np.int64), 0, D - 1)
+    t = xs * np.float32(1 / h)
+    c = np.clip(np.floor(t).astype(np.int64), 0, D - 1)
     key = (c[:, 0] * D + c[:, 1]) * D + c[:, 2]
+    S = slabs or 4
+    slab = np.clip(np.floor((t[:, 2] - c[:, 2].astype(np.float32)) * np.float32(S)).astype(np.int64), 0, S - 1)
     counts, offsets = o.cells()
     assert np.array_equal(counts, np.bincount(key, minlength=D ** 3).astype(np.uint32))
     assert np.array_equal(offsets, np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32))
     off, ids = o.neighbors()
+    ref = make(x, r=r, h=h, cell_slabs=1)
+    ref.search(1e-9)
+    roff, rids = ref.neighbors()
+    assert np.array_equal(off, roff)
     for i in range(0, 3000, 37):
         lst = ids[off[i]:off[i + 1]]
-        order = sorted(lst.tolist(), key=lambda j: (key[j], j))
+        order = sorted(lst.tolist(), key=lambda j: (key[j], slab[j], j))
         assert lst.tolist() == order
+        assert set(lst.tolist()) == set(rids[roff[i]:roff[i + 1]].tolist())
+    if S > 1:  # the slab order is not the id order
+        assert not np.array_equal(ids, rids)
 
 
 # --------------------------------------------------------------------------

@@ -26,8 +26,20 @@ struct Phys {
   uint32_t maxn; // neighbour capacity per particle
   uint32_t *nbr16;  // 16-bit neighbour table (offset pairs), slot-major
   uint32_t *wfmt;   // per wavefront: 1 = 16-bit table, 0 = 32-bit table
+  const uint32_t *cellsub;  // per cell: its particles below z-slabs 1, 2, 3 and its count (8 bits each)
+  uint32_t slab_log2;       // log2 of the z-slabs per cell (0: one slab)
   double Kp64, h2_64, V64;
 };
+
+// Sort keys inside a cell: the particle id in bits 0..26 (capacities stay
+// below 2^27) and its z-slab above it; the binning rank carries the slab in
+// its top two bits until the scatter forms the key.
+constexpr uint32_t kIdBits = 27;
+constexpr uint32_t kIdMask = (1u << kIdBits) - 1u;
+constexpr uint32_t kRankSlabShift = 30;
+constexpr uint32_t kRankMask = (1u << kRankSlabShift) - 1u;
+// cellsub word of a cell holding more particles than 8 bits count: no trimming
+constexpr uint32_t kSubSaturated = 0xff000000u;
 
 // One scene: clamp box, grid, particle range and boundary field.
 struct Scene {
@@ -164,6 +176,15 @@ __device__ __forceinline__ int cell_axis(const Scene &sc, float x, int a) {
   v = v < 0 ? 0 : v;
   v = v > sc.dims[a] - 1 ? sc.dims[a] - 1 : v;
   return v;
+}
+// z-slab of a position inside its (clamped) cell: floor(S * fz) with
+// fz = (z - lo) / h - cz in float, clamped to [0, S), S = 2^slab_log2.
+__device__ __forceinline__ uint32_t z_slab(const Scene &sc, uint32_t slab_log2, float z, int cz) {
+  const float fz = (z - sc.lo[2]) * sc.inv_cell - (float)cz;
+  const int top = (1 << slab_log2) - 1;
+  int k = (int)floorf(fz * (float)(1u << slab_log2));
+  k = k < 0 ? 0 : k;
+  return (uint32_t)(k > top ? top : k);
 }
 __device__ __forceinline__ uint32_t cell_local(const Scene &sc, int cx, int cy, int cz) {
   // hashed grid: the row-major index modulo the table size (2^k), so nearby

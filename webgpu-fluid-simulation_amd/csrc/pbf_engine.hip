@@ -91,6 +91,8 @@ struct pbf_ctx {
   uint32_t *nbr16 = nullptr, *wfmt = nullptr;  // 16-bit table and per-wavefront format
   float *lam = nullptr, *lam_sorted = nullptr;
   uint32_t *counts = nullptr, *offsets = nullptr;
+  uint32_t *cellsub = nullptr;  // per cell: z-slab boundaries and count (null with one slab per cell)
+  uint32_t slab_log2 = 2;
   std::vector<uint32_t *> scan_sums, scan_prefix;
   uint32_t *flags = nullptr;  // [0] overflow, [1] max neighbour count
   // scenes on device
@@ -207,6 +209,8 @@ Phys make_phys(const pbf_ctx *c, float dt) {
   P.maxn = c->maxn;
   P.nbr16 = c->nbr16;
   P.wfmt = c->wfmt;
+  P.cellsub = c->cellsub;
+  P.slab_log2 = c->cellsub ? c->slab_log2 : 0u;
   P.Kp64 = 315.0 / (64.0 * PI * std::pow(h, 9));
   P.h2_64 = h * h;
   P.V64 = V64;
@@ -322,7 +326,7 @@ void enqueue_search(pbf_ctx *c, const LaunchCtx &L, PassTimer &T, bool dbg, uint
   if (dbg) (void)hipMemcpyAsync(c->dbg_counts, c->counts, (size_t)c->ncells * 4, hipMemcpyDeviceToDevice, c->stream);
   t = T.begin(P_SCAN, L);
   launch_scan(c->stream, c->counts, c->ncells, c->offsets, 1, c->scan_sums.data(), c->scan_prefix.data(), 0, t.ev0,
-              t.ev1);
+              t.ev1, c->cellsub);
   T.end();
   t = T.begin(P_SCATTER, L);
   launch_scatter(t, c->cellkey, c->rank, c->offsets, c->ids, c->tmp_key, c->tmp_src);
@@ -672,6 +676,10 @@ int create_ctx_body(const pbf_params *params, uint32_t nscenes, const pbf_scene_
     return reject(PBF_E_INVALID, "hash_log2 must be in [4, 28]");
   if (p.capacity > kMaxCapacity)  // 32-bit gather offsets (slot * 16 bytes)
     return reject(PBF_E_CAPACITY, "capacity above 2^27 - 256 particles");
+  if (p.cell_slabs == 0) p.cell_slabs = 4;
+  if (p.cell_slabs != 1 && p.cell_slabs != 2 && p.cell_slabs != 4)
+    return reject(PBF_E_INVALID, "cell_slabs must be 1, 2 or 4");
+  c->slab_log2 = p.cell_slabs == 4 ? 2u : p.cell_slabs == 2 ? 1u : 0u;
   c->cfg = {5u, 0.01f, 0.1f, 0.5f, 9.8f};  // src/common/config.ts:22-28
   c->batch = batch;
   uint32_t cell_base = 0, pbase = 0;
@@ -724,6 +732,7 @@ int create_ctx_body(const pbf_params *params, uint32_t nscenes, const pbf_scene_
   step(dalloc(&c->lam, N), "alloc"); step(dalloc(&c->lam_sorted, N), "alloc");
   step(dalloc(&c->counts, (size_t)c->ncells), "alloc");
   step(dalloc(&c->offsets, (size_t)c->ncells + 1), "alloc");
+  if (c->slab_log2) step(dalloc(&c->cellsub, (size_t)c->ncells), "alloc");
   step(dalloc(&c->dbg_counts, (size_t)c->ncells), "alloc");
   step(dalloc(&c->dbg_ids, N), "alloc"); step(dalloc(&c->ids_save, N), "alloc");
   step(dalloc(&c->flags, 2), "alloc");
@@ -891,7 +900,7 @@ int pbf_destroy(pbf_ctx *c) {
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
   void *bufs[] = {c->pos, c->vel, c->acc, c->om, c->nrm, c->XA, c->XB, c->Xold, c->B, c->xs, c->stage4, c->ids,
                   c->cellkey, c->rank, c->tmp_key, c->tmp_src, c->ncount, c->nbr, c->nbr16, c->wfmt, c->lam, c->lam_sorted, c->counts,
-                  c->offsets, c->flags, c->d_scenes, c->sid, c->ranges, c->pool, c->ci, c->per_scene, c->hist,
+                  c->offsets, c->cellsub, c->flags, c->d_scenes, c->sid, c->ranges, c->pool, c->ci, c->per_scene, c->hist,
                   c->hist_ctr, c->dbg_counts, c->dbg_ids, c->ids_save, c->lam_acc, c->acc_sorted, c->feats,
                   c->vel_sorted, c->snap_pos, c->snap_vel, c->snap_acc, c->snap_om, c->snap_nrm, c->snap_ids,
                   c->snap_lam, c->snap_lacc, c->per_scene2, c->fb_count, c->fb_thr, c->lam_tmp,

@@ -48,8 +48,10 @@ constexpr uint32_t kListVariants = 4;
 
 void launch_predict_bin(const LaunchCtx &L, const float4 *pos, const float4 *vel, const float4 *acc, float4 *xs,
                         uint32_t *cellkey, uint32_t *rank, uint32_t *counts);
+// zero_in: clear `in` (and `zero2`, when not null, over the same n words)
 void launch_scan(hipStream_t st, uint32_t *in, uint32_t n, uint32_t *out, int zero_in, uint32_t *const *sums,
-                 uint32_t *const *prefix, int level, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                 uint32_t *const *prefix, int level, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
+                 uint32_t *zero2 = nullptr);
 void launch_scatter(const LaunchCtx &L, const uint32_t *cellkey, const uint32_t *rank, const uint32_t *offsets,
                     const uint32_t *ids, uint32_t *tmp_key, uint32_t *tmp_src);
 // dest[s]: the final slot of sorted slot s (read by launch_settle_extra)

@@ -338,7 +338,7 @@ __global__ void __launch_bounds__(BS) k_predict_bin(uint32_t n, const float4 *__
                                                     const uint32_t *__restrict__ sid, Phys P) {
   const uint32_t i = slot_index();
   const bool active = i < n;
-  uint32_t key = 0;
+  uint32_t key = 0, slab = 0;
   if (active) {
     const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
     float4 x = pos[i], v = vel[i], a = acc[i];
@@ -348,11 +348,13 @@ __global__ void __launch_bounds__(BS) k_predict_bin(uint32_t n, const float4 *__
     float px = fmaf(P.dt, vx, x.x), py = fmaf(P.dt, vy, x.y), pz = fmaf(P.dt, vz, x.z);
     clamp_rand(sc, px, py, pz);
     xs[i] = make_float4(px, py, pz, 0.0f);
-    key = sc.cell_base + cell_local(sc, cell_axis(sc, px, 0), cell_axis(sc, py, 1), cell_axis(sc, pz, 2));
+    const int cz = cell_axis(sc, pz, 2);
+    key = sc.cell_base + cell_local(sc, cell_axis(sc, px, 0), cell_axis(sc, py, 1), cz);
     cellkey[i] = key;
+    slab = P.slab_log2 ? z_slab(sc, P.slab_log2, pz, cz) : 0u;
   }
   uint32_t r = wave_atomic_rank(counts, key, active);
-  if (active) rank[i] = r;
+  if (active) rank[i] = r | (slab << kRankSlabShift);
 }
 
 // ---------------------------------------------------------------------------
@@ -386,9 +388,12 @@ __global__ void __launch_bounds__(BS) k_scan_reduce(const uint32_t *__restrict__
 }
 
 // out[0..n] = exclusive scan of in[0..n) (+ prefix[block]); out[n] = total.
+// zero_in: clear `in` (the cell counts, for the next binning) and, when not
+// null, `zero2` (the per-cell slab words, rewritten by the reorder for the
+// cells that hold particles) over the same range.
 __global__ void __launch_bounds__(BS) k_scan_down(uint32_t *__restrict__ in, uint32_t n,
                                                   const uint32_t *__restrict__ prefix, uint32_t *__restrict__ out,
-                                                  int zero_in) {
+                                                  int zero_in, uint32_t *__restrict__ zero2) {
   __shared__ uint32_t sh[BS + BS / 32];
   const int t = threadIdx.x;
   const uint32_t base = blockIdx.x * SCAN_E + (uint32_t)t * SCAN_V;
@@ -398,11 +403,17 @@ __global__ void __launch_bounds__(BS) k_scan_down(uint32_t *__restrict__ in, uin
     uint4 a = p[0], b = p[1];
     v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
     if (zero_in) { p[0] = make_uint4(0, 0, 0, 0); p[1] = make_uint4(0, 0, 0, 0); }
+    if (zero2) {
+      uint4 *z = reinterpret_cast<uint4 *>(zero2 + base);
+      z[0] = make_uint4(0, 0, 0, 0);
+      z[1] = make_uint4(0, 0, 0, 0);
+    }
   } else {
 #pragma unroll
     for (int k = 0; k < SCAN_V; ++k) {
       v[k] = (base + k < n) ? in[base + k] : 0u;
       if (zero_in && base + k < n) in[base + k] = 0u;
+      if (zero2 && base + k < n) zero2[base + k] = 0u;
     }
   }
   uint32_t e[SCAN_V];
@@ -455,15 +466,17 @@ __global__ void __launch_bounds__(BS) k_scan_down(uint32_t *__restrict__ in, uin
 }
 
 // ---------------------------------------------------------------------------
-// K4: scatter ids into cell order (rank inside a cell is still arbitrary).
+// K4: scatter ids into cell order (rank inside a cell is still arbitrary);
+// the sort key inside a cell is (z-slab, id).
 __global__ void __launch_bounds__(BS) k_scatter(uint32_t n, const uint32_t *__restrict__ cellkey,
                                                 const uint32_t *__restrict__ rank, const uint32_t *__restrict__ offsets,
                                                 const uint32_t *__restrict__ ids, uint32_t *__restrict__ tmp_key,
                                                 uint32_t *__restrict__ tmp_src) {
   const uint32_t i = slot_index();
   if (i >= n) return;
-  uint32_t s = offsets[cellkey[i]] + rank[i];
-  tmp_key[s] = ids[i];
+  const uint32_t rk = rank[i];
+  uint32_t s = offsets[cellkey[i]] + (rk & kRankMask);
+  tmp_key[s] = ids[i] | ((rk >> kRankSlabShift) << kIdBits);  // (z-slab, id)
   tmp_src[s] = i;
 }
 
@@ -484,9 +497,38 @@ __device__ __forceinline__ uint32_t rank_below(const uint32_t *__restrict__ keys
   return r;
 }
 
-// Stable order inside each cell (ascending particle id), then move the
+// The same count, plus the keys below z-slabs 1, 2 and 3 (the cell's slab
+// boundaries in its sorted order).
+__device__ __forceinline__ uint32_t rank_below_slabs(const uint32_t *__restrict__ keys, uint32_t o, uint32_t e,
+                                                     uint32_t key, uint32_t &b1, uint32_t &b2, uint32_t &b3) {
+  constexpr uint32_t s1 = 1u << kIdBits, s2 = 2u << kIdBits, s3 = 3u << kIdBits;
+  uint32_t r = 0, c1 = 0, c2 = 0, c3 = 0, t = o;
+  auto one = [&](uint32_t w) {
+    r += w < key ? 1u : 0u;
+    c1 += w < s1 ? 1u : 0u;
+    c2 += w < s2 ? 1u : 0u;
+    c3 += w < s3 ? 1u : 0u;
+  };
+  for (; t < e && (t & 3u); ++t) one(keys[t]);
+  const uint4 *q = reinterpret_cast<const uint4 *>(keys + t);
+  const uint32_t nv = (e - t) >> 2;
+  for (uint32_t v = 0; v < nv; ++v) {
+    const uint4 w = q[v];
+    one(w.x); one(w.y); one(w.z); one(w.w);
+  }
+  for (t += 4u * nv; t < e; ++t) one(keys[t]);
+  b1 = c1;
+  b2 = c2;
+  b3 = c3;
+  return r;
+}
+
+// Stable order inside each cell (ascending (z-slab, id) key), then move the
 // particle data into its final slot; dest[s] = that slot, for k_settle_extra
-// (lam_out and dest may be null: the plain solve needs neither).
+// (lam_out and dest may be null: the plain solve needs neither).  With
+// z-slabs (cellsub not null) the cell's first particle also writes the
+// cell's slab word: its particles below slabs 1, 2, 3 and its count, 8 bits
+// each, or kSubSaturated for a cell of 255 or more.
 __global__ void __launch_bounds__(BS) k_settle(uint32_t n, const uint32_t *__restrict__ tmp_key,
                                                const uint32_t *__restrict__ tmp_src,
                                                const uint32_t *__restrict__ cellkey,
@@ -494,15 +536,27 @@ __global__ void __launch_bounds__(BS) k_settle(uint32_t n, const uint32_t *__res
                                                const float4 *__restrict__ pos, const float *__restrict__ lam_in,
                                                float4 *__restrict__ X0, float4 *__restrict__ Xold,
                                                uint32_t *__restrict__ ids, float *__restrict__ lam_out,
-                                               uint32_t *__restrict__ dest) {
+                                               uint32_t *__restrict__ dest, uint32_t *__restrict__ cellsub) {
   const uint32_t s = slot_index();
   if (s >= n) return;
   const uint32_t p = tmp_src[s], key = tmp_key[s];
   const uint32_t c = cellkey[p];
-  const uint32_t f = offsets[c] + rank_below(tmp_key, offsets[c], offsets[c + 1], key);
+  const uint32_t o = offsets[c], e = offsets[c + 1];
+  uint32_t r;
+  if (cellsub) {
+    uint32_t b1, b2, b3;
+    r = rank_below_slabs(tmp_key, o, e, key, b1, b2, b3);
+    if (r == 0) {
+      const uint32_t cnt = e - o;
+      cellsub[c] = cnt >= 255u ? kSubSaturated : (b1 | (b2 << 8) | (b3 << 16) | (cnt << 24));
+    }
+  } else {
+    r = rank_below(tmp_key, o, e, key);
+  }
+  const uint32_t f = o + r;
   X0[f] = xs[p];
   Xold[f] = pos[p];
-  ids[f] = key;
+  ids[f] = key & kIdMask;
   if (lam_out) lam_out[f] = lam_in[p];
   if (dest) dest[s] = f;
 }
@@ -619,10 +673,10 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
     // Entries past R are rare; a batch that produced any is tested again
     // (uniform branch) and those entries go straight to the table.
     const Gather GX(X0, n);
-    auto test = [&](const float3 &xj, uint32_t j) {
+    auto test = [&](const float3 &xj, uint32_t j, bool live) {
       const float d2 = dist2(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
       tile[(cnt < (uint32_t)LIST_R ? cnt : (uint32_t)LIST_R) * LIST_BS + tid] = j;
-      cnt += d2 <= P.h2 ? 1u : 0u;
+      cnt += (live && d2 <= P.h2) ? 1u : 0u;
     };
     // spilled entries go to both tables (the wavefront's format is decided
     // at the end); `last` keeps the latest one, smin / smax their range
@@ -640,27 +694,34 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
         k += in ? 1u : 0u;
       }
     };
+    // Batches of eight candidate loads in flight.  A range's last 1..7
+    // candidates are one masked batch instead of smaller batches: the eight
+    // records loaded end at the range's end (they start before the range
+    // when it is shorter; record 0 on when it ends below 8, always inside
+    // the array's capacity of at least 256 slots) and only those inside the
+    // range are accepted, in order.  So a range costs the wavefront
+    // ceil(longest / 8) rounds of load latency.
     auto visit = [&](uint32_t t0, uint32_t t1) {
       uint32_t j = t0;
-      for (; j + 8 <= t1; j += 8) {  // eight candidate loads in flight
+      for (; j + 8 <= t1; j += 8) {
         float3 c[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) c[q] = GX.xyz(j, q);
         const uint32_t k0 = cnt;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) test(c[q], j + q);
+        for (int q = 0; q < 8; ++q) test(c[q], j + q, true);
         if (__ballot(cnt > (uint32_t)LIST_R)) spill(j, 8, k0);
       }
-      for (; j + 4 <= t1; j += 4) {
-        const float3 a = GX.xyz(j), b = GX.xyz(j, 1), c = GX.xyz(j, 2), d = GX.xyz(j, 3);
+      if (j < t1) {
+        const uint32_t b = t1 >= 8u ? t1 - 8u : 0u;  // b <= j
+        const uint32_t qlo = j - b, qhi = t1 - b;
+        float3 c[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) c[q] = GX.xyz(b, q);
         const uint32_t k0 = cnt;
-        test(a, j); test(b, j + 1); test(c, j + 2); test(d, j + 3);
-        if (__ballot(cnt > (uint32_t)LIST_R)) spill(j, 4, k0);
-      }
-      for (; j < t1; ++j) {
-        const uint32_t k0 = cnt;
-        test(GX.xyz(j), j);
-        if (__ballot(cnt > (uint32_t)LIST_R)) spill(j, 1, k0);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) test(c[q], b + q, (uint32_t)q >= qlo && (uint32_t)q < qhi);
+        if (__ballot(cnt > (uint32_t)LIST_R)) spill(j, t1 - j, k0);
       }
     };
     if (!HASH || sc.distinct27) {
@@ -668,22 +729,34 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
       // cells z-1..z+1 -- contiguous cells, and in a hashed grid contiguous
       // buckets unless the row-major index wraps the table (then cell by
       // cell).  With every bucket of the 27 cells distinct (always on a dense
-      // grid), skipping out-of-reach cells leaves the list unchanged.  The 18
-      // range bounds are loaded up front so their latencies overlap.
-      const float gz_lo = gap(-1, fz), gz_hi = gap(1, fz);
-      uint32_t t0[9], t1[9];
-      int zl[9], zh[9];
+      // grid), skipping out-of-reach candidates leaves the list unchanged.
+      // The range is cut to the z-slabs (height h / S) that the particle can
+      // reach: [fz - w, fz + w] in cell units with w^2 = 1.001 - gxy; a
+      // cell's particles are sorted by slab, so the reachable ones of the
+      // first and last cell are a prefix / suffix of the cell's slab words.
+      // The bounds are loaded up front so their latencies overlap.
+      const uint32_t sl2 = P.slab_log2;
+      const int S = 1 << sl2;
+      const float Sf = (float)S;
+      uint32_t t0[9], t1[9], u0[9], u1[9];
+      int zl[9], zh[9], sl[9], sh[9];
       uint32_t wrap = 0;
 #pragma unroll
       for (int r = 0; r < 9; ++r) {
         const int dx = r / 3 - 1, dy = r % 3 - 1;
         const int nx = cx + dx, ny = cy + dy;
         const float gxy = gap(dx, fx) + gap(dy, fy);
-        t0[r] = t1[r] = 0;
-        zl[r] = zh[r] = 0;
+        t0[r] = t1[r] = u0[r] = u1[r] = 0;
+        zl[r] = zh[r] = sl[r] = sh[r] = 0;
         if (nx < 0 || nx >= D0 || ny < 0 || ny >= D1 || gxy > kReach) continue;
-        const int zlo = (cz > 0 && gxy + gz_lo <= kReach) ? cz - 1 : cz;
-        const int zhi = (cz + 1 < D2 && gxy + gz_hi <= kReach) ? cz + 1 : cz;
+        const float w = __builtin_amdgcn_sqrtf(kReach - gxy);
+        int klo = (int)floorf((fz - w) * Sf), khi = (int)floorf((fz + w) * Sf);
+        const int kmin = cz > 0 ? -S : 0, kmax = cz + 1 < D2 ? 2 * S - 1 : S - 1;
+        klo = klo < kmin ? kmin : klo > S - 1 ? S - 1 : klo;
+        khi = khi > kmax ? kmax : khi < 0 ? 0 : khi;
+        const int zlo = klo < 0 ? cz - 1 : cz, zhi = khi >= S ? cz + 1 : cz;
+        sl[r] = klo & (S - 1);
+        sh[r] = khi & (S - 1);
         const uint32_t blo = cell_local(sc, nx, ny, zlo), bhi = cell_local(sc, nx, ny, zhi);
         if constexpr (HASH) {
           if (bhi - blo != (uint32_t)(zhi - zlo)) wrap |= 1u << r;
@@ -692,6 +765,26 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
         zh[r] = zhi;
         t0[r] = offsets[cbase + blo];
         t1[r] = offsets[cbase + bhi + 1];
+        if (sl2) {
+          u0[r] = P.cellsub[cbase + blo];
+          u1[r] = P.cellsub[cbase + bhi];
+        }
+      }
+      if (sl2) {
+        // first cell: skip its particles below slab sl; last cell: those
+        // above slab sh (none to skip in a cell too full for 8-bit counts)
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+          const uint32_t a = u0[r], b = u1[r];
+          const uint32_t below = (sl[r] == 0 || a >= kSubSaturated) ? 0u : (a >> (8 * (sl[r] - 1))) & 0xffu;
+          uint32_t above = (sh[r] == S - 1 || b >= kSubSaturated) ? 0u : (b >> 24) - ((b >> (8 * sh[r])) & 0xffu);
+          // a cut never leaves the column's range (the words of a sorted
+          // cell always satisfy this; the clamps keep it unconditional)
+          const uint32_t len = t1[r] - t0[r];
+          above = above < len ? above : len;
+          t0[r] += below < len - above ? below : len - above;
+          t1[r] -= above;
+        }
       }
 #pragma unroll
       for (int r = 0; r < 9; ++r) {
@@ -1291,16 +1384,16 @@ void launch_predict_bin(const LaunchCtx &L, const float4 *pos, const float4 *vel
 }
 
 void launch_scan(hipStream_t st, uint32_t *in, uint32_t n, uint32_t *out, int zero_in, uint32_t *const *sums,
-                 uint32_t *const *prefix, int level, hipEvent_t e0, hipEvent_t e1) {
+                 uint32_t *const *prefix, int level, hipEvent_t e0, hipEvent_t e1, uint32_t *zero2) {
   const dim3 b(BS);
   if (n <= SCAN_E) {
-    go(st, e0, e1, k_scan_down, dim3(1), b, in, n, (const uint32_t *)nullptr, out, zero_in);
+    go(st, e0, e1, k_scan_down, dim3(1), b, in, n, (const uint32_t *)nullptr, out, zero_in, zero2);
     return;
   }
   uint32_t nb = (n + SCAN_E - 1) / SCAN_E;
   go(st, e0, nullptr, k_scan_reduce, dim3(nb), b, (const uint32_t *)in, n, sums[level]);
-  launch_scan(st, sums[level], nb, prefix[level], 0, sums, prefix, level + 1, nullptr, nullptr);
-  go(st, nullptr, e1, k_scan_down, dim3(nb), b, in, n, (const uint32_t *)prefix[level], out, zero_in);
+  launch_scan(st, sums[level], nb, prefix[level], 0, sums, prefix, level + 1, nullptr, nullptr, nullptr);
+  go(st, nullptr, e1, k_scan_down, dim3(nb), b, in, n, (const uint32_t *)prefix[level], out, zero_in, zero2);
 }
 
 void launch_scatter(const LaunchCtx &L, const uint32_t *cellkey, const uint32_t *rank, const uint32_t *offsets,
@@ -1315,7 +1408,7 @@ void launch_settle(const LaunchCtx &L, const uint32_t *tmp_key, const uint32_t *
                    float4 *Xold, uint32_t *ids, float *lam_out, uint32_t *dest) {
   if (!L.n) return;
   go(L.stream, L.ev0, L.ev1, k_settle, dim3(nblocks(L.n, BS)), dim3(BS), L.n, tmp_key, tmp_src, cellkey, offsets, xs,
-     pos, lam_in, X0, Xold, ids, lam_out, dest);
+     pos, lam_in, X0, Xold, ids, lam_out, dest, L.P.slab_log2 ? const_cast<uint32_t *>(L.P.cellsub) : nullptr);
 }
 
 template <int R, int B>

@@ -13,7 +13,7 @@ export interface SimulationOptions {
 export interface PBFParams {
   capacity?: number; maxNeighbors?: number; particleRadius?: number; kernelRadius?: number;
   restDensity?: number; timeStep?: number; domainLo?: Vec3 | number[]; domainHi?: Vec3 | number[];
-  gridMode?: number; hashLog2?: number; flags?: number;
+  gridMode?: number; hashLog2?: number; flags?: number; cellSlabs?: number;
 }
 
 export const FIELD = { position: 0, velocity: 1, acceleration: 2, lambda: 3, omega: 4, normal: 5 } as const;

@@ -138,6 +138,7 @@ void read_params(napi_env env, napi_value o, pbf_params &p) {
   p.grid_mode = as_u32(prop(env, o, "gridMode", PBF_GRID_DENSE));
   p.hash_log2 = as_u32(prop(env, o, "hashLog2", 22));
   p.flags = as_u32(prop(env, o, "flags", 0));
+  p.cell_slabs = as_u32(prop(env, o, "cellSlabs", 0));
 }
 
 void finalize_ctx(napi_env, void *data, void *) { pbf_destroy(static_cast<pbf_ctx *>(data)); }

@@ -43,7 +43,7 @@ class Params(C.Structure):
                 ("rest_density", C.c_double), ("time_step", C.c_double),
                 ("domain_lo", C.c_double * 3), ("domain_hi", C.c_double * 3),
                 ("grid_mode", C.c_uint32), ("hash_log2", C.c_uint32), ("flags", C.c_uint32),
-                ("reserved", C.c_uint32 * 5)]
+                ("cell_slabs", C.c_uint32), ("reserved", C.c_uint32 * 4)]
 
 
 class Config(C.Structure):
@@ -331,7 +331,7 @@ class PBF:
     def __init__(self, particle_radius: float = 0.006, kernel_radius: float = 0.025, capacity: int = 1 << 17,
                  rest_density: float = 1000.0, time_step: float = 1.0 / 300.0, domain_lo=(0.0, 0.0, 0.0),
                  domain_hi=(1.0, 1.0, 1.0), max_neighbors: int = 96, grid_mode: int = PBF_GRID_DENSE,
-                 hash_log2: int = 22, flags: int = 0, scenes: Optional[Sequence[dict]] = None):
+                 hash_log2: int = 22, flags: int = 0, scenes: Optional[Sequence[dict]] = None, cell_slabs: int = 0):
         self.lib = load_library()
         p = Params()
         p.capacity = capacity
@@ -345,6 +345,7 @@ class PBF:
         p.grid_mode = grid_mode
         p.hash_log2 = hash_log2
         p.flags = flags
+        p.cell_slabs = cell_slabs  # z-slabs ordering a cell's particles: 1, 2 or 4 (0: 4)
         self.params = p
         self.ctx = C.c_void_p()
         if scenes:

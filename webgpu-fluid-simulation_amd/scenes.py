@@ -263,11 +263,12 @@ def cdm_text(fld: Field) -> str:
 
 
 # --- running a scene ----------------------------------------------------------
-def make_engine(scene: Scene, capacity: Optional[int] = None, flags: int = 0, max_neighbors: int = 96) -> PBF:
+def make_engine(scene: Scene, capacity: Optional[int] = None, flags: int = 0, max_neighbors: int = 96,
+                cell_slabs: int = 0) -> PBF:
     sim = PBF(particle_radius=scene.particle_radius, kernel_radius=scene.kernel_radius,
               capacity=capacity or max(scene.n, 1), domain_lo=scene.domain_lo, domain_hi=scene.domain_hi,
               grid_mode=scene.grid_mode, hash_log2=scene.hash_log2, flags=flags, max_neighbors=max_neighbors,
-              time_step=scene.substep_dt())
+              time_step=scene.substep_dt(), cell_slabs=cell_slabs)
     sim.setConfig(scene.config)
     load_field(sim, scene.field)
     sim.setParticlePosition(scene.positions)
@@ -283,12 +284,12 @@ def load_field(sim: PBF, fld: Optional[Field], scene_index: int = 0):
         sim.setCubicField(fld.res, fld.lo, fld.hi, fld.sdf, fld.vol, scene_index)
 
 
-def make_batch_engine(scenes: List[Scene], flags: int = 0, max_neighbors: int = 96) -> PBF:
+def make_batch_engine(scenes: List[Scene], flags: int = 0, max_neighbors: int = 96, cell_slabs: int = 0) -> PBF:
     s0 = scenes[0]
     total = sum(s.n for s in scenes)
     sim = PBF(particle_radius=s0.particle_radius, kernel_radius=s0.kernel_radius, capacity=total, flags=flags,
               max_neighbors=max_neighbors, time_step=s0.substep_dt(), grid_mode=s0.grid_mode,
-              hash_log2=s0.hash_log2,
+              hash_log2=s0.hash_log2, cell_slabs=cell_slabs,
               scenes=[{"domain_lo": s.domain_lo, "domain_hi": s.domain_hi, "count": s.n} for s in scenes])
     sim.setConfig(s0.config)
     for k, s in enumerate(scenes):
