@@ -399,6 +399,56 @@ int pbf_set_tuning(pbf_ctx *ctx, const pbf_tuning *tuning);
  * state as it found it.  out (may be NULL) receives the chosen setting. */
 int pbf_autotune(pbf_ctx *ctx, float dt, uint32_t iterations, uint32_t repeats, pbf_tuning *out);
 
+/* ------------------------------------------------------------------------
+ * MLS-MPM.  Reference: the MPM simulator class (src/simulator/MPM/MPM.ts,
+ * commented out in the reference and imported by nothing) and its shaders
+ * (src/simulator/MPM/MPMShader.ts: P2GComputeShader :3-148,
+ * GridComputeShader :151-193, G2PComputeShader :196-282).  Particles live in
+ * the unit box on an n_grid^3 node grid; one step = P2G, grid pass, G2P.
+ * The reference's P2G adds with float atomics (arrival order); here a node
+ * sums its particles in ascending (base cell, particle id) order, so every
+ * step is deterministic.
+ * ------------------------------------------------------------------------ */
+typedef struct pbf_mpm pbf_mpm;
+
+typedef struct {
+  uint32_t capacity;  /* max particles (the reference's 9000, MPM.ts:64); 0 -> 9000, <= 2^27 */
+  uint32_t n_grid;    /* nodes per axis (MPM.ts:62, 32); 0 -> 32, in [4, 1024] */
+  uint32_t bound;     /* boundary layer in nodes (MPM.ts:69, 3); 0 -> 3, <= n_grid / 2 */
+  uint32_t flags;     /* PBF_FLAG_NO_GRAPH: launch the passes directly */
+  double dt;          /* MPM.ts:66, 8e-4; 0 -> 8e-4 */
+  double rho;         /* particle density (MPM.ts:71, 1); 0 -> 1 */
+  double E;           /* Young's modulus (MPM.ts:76, 400); 0 -> 400 */
+  double gravity;     /* applied as (0, -gravity, 0); the reference never writes its gravity buffer */
+  uint32_t reserved[4];
+} pbf_mpm_params;
+
+enum {
+  PBF_MPM_POSITION = 0, /* n float4: x, y, z, 0 */
+  PBF_MPM_VELOCITY = 1, /* n float4: v, J (the Particle struct's v: vec4, MPMShader.ts:72-75) */
+  PBF_MPM_AFFINE = 2,   /* n x 9 floats: C column-major (WGSL mat3x3: column c at 3c) */
+  PBF_MPM_GRID = 3      /* n_grid^3 float4, x-major: grid velocity after the last grid pass */
+};
+
+/* Reference: new MPM() + MPM.initResource's buffers (MPM.ts:60-83,94-103). */
+int pbf_mpm_create(const pbf_mpm_params *params, pbf_mpm **out);
+int pbf_mpm_destroy(pbf_mpm *m);
+/* Reference: MPM.initResource's position write (MPM.ts:104-121): positions
+ * (float4 rows, w ignored), v = 0, C = 0, J = 1.  Errors: 'To Many Particles!'. */
+int pbf_mpm_set_positions(pbf_mpm *m, const float *xyzw, uint32_t count);
+/* Reference: MPM.run (MPM.ts:204-238), `steps` times; asynchronous. */
+int pbf_mpm_step(pbf_mpm *m, uint32_t steps);
+int pbf_mpm_sync(pbf_mpm *m);
+/* Readback / upload of a PBF_MPM_* field (synchronous); count = particles,
+ * or n_grid^3 for the grid. */
+int pbf_mpm_read(pbf_mpm *m, int field, float *out, size_t count);
+int pbf_mpm_write(pbf_mpm *m, int field, const float *in, size_t count);
+/* G2P alone (MPMShader.ts:240-280) on the current grid velocity. */
+int pbf_mpm_g2p(pbf_mpm *m);
+uint32_t pbf_mpm_particle_count(const pbf_mpm *m);
+/* Message of the last failed call on m (m = NULL: of a failed create). */
+const char *pbf_mpm_last_error(const pbf_mpm *m);
+
 #ifdef __cplusplus
 }
 #endif

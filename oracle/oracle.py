@@ -182,3 +182,74 @@ class Oracle:
 
 def sinf(x):
     return float(lib().orc_sinf(float(x)))
+
+
+# --- MLS-MPM (mpm_oracle.c) --------------------------------------------------
+MPM_POSITION, MPM_VELOCITY, MPM_AFFINE, MPM_GRID = range(4)
+
+
+class MpmParams(C.Structure):
+    _fields_ = [("n_grid", C.c_int), ("bound", C.c_uint), ("dt", C.c_double), ("rho", C.c_double),
+                ("E", C.c_double), ("gravity", C.c_double)]
+
+
+def _mpm_lib():
+    L = lib()
+    if not getattr(L, "_mpm_sigs", False):
+        P, I = C.c_void_p, C.c_int
+        pf = C.POINTER(C.c_float)
+        for name, (res, args) in {
+            "orc_mpm_create": (P, [C.POINTER(MpmParams), I, pf]), "orc_mpm_destroy": (None, [P]),
+            "orc_mpm_step": (None, [P]), "orc_mpm_get": (None, [P, I, pf]), "orc_mpm_set": (None, [P, I, pf]),
+            "orc_mpm_p2g": (None, [P, pf]), "orc_mpm_g2p_only": (None, [P]),
+        }.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        L._mpm_sigs = True
+    return L
+
+
+class MpmOracle:
+    """CPU restatement of the MLS-MPM step (MPMShader.ts) on n particles."""
+
+    def __init__(self, xyzw, n_grid=32, bound=3, dt=8e-4, rho=1.0, E=400.0, gravity=9.8):
+        self.L = _mpm_lib()
+        p = MpmParams(int(n_grid), int(bound), float(dt), float(rho), float(E), float(gravity))
+        a = np.ascontiguousarray(np.asarray(xyzw, np.float32).reshape(-1, 4))
+        self.n, self.ng = a.shape[0], int(n_grid)
+        self.m = self.L.orc_mpm_create(C.byref(p), self.n, _f(a))
+
+    def __del__(self):
+        try:
+            if self.m:
+                self.L.orc_mpm_destroy(self.m)
+                self.m = None
+        except Exception:
+            pass
+
+    def _shape(self, field):
+        return {MPM_POSITION: (self.n, 4), MPM_VELOCITY: (self.n, 4), MPM_AFFINE: (self.n, 9),
+                MPM_GRID: (self.ng ** 3, 4)}[field]
+
+    def step(self, steps=1):
+        for _ in range(steps):
+            self.L.orc_mpm_step(self.m)
+
+    def get(self, field):
+        out = np.zeros(self._shape(field), np.float32)
+        self.L.orc_mpm_get(self.m, field, _f(out))
+        return out
+
+    def set(self, field, arr):
+        a = np.ascontiguousarray(np.asarray(arr, np.float32).reshape(self._shape(field)))
+        self.L.orc_mpm_set(self.m, field, _f(a))
+
+    def p2g(self):
+        """P2G + grid pass; returns the node sums (momentum xyz, mass) before the grid pass."""
+        raw = np.zeros((self.ng ** 3, 4), np.float32)
+        self.L.orc_mpm_p2g(self.m, _f(raw))
+        return raw
+
+    def g2p(self):
+        self.L.orc_mpm_g2p_only(self.m)

@@ -105,6 +105,30 @@ void orc_kernels(const orc_sim *s, int n, const float *d2, float *poly6, float *
 float orc_sinf(float x);
 void orc_clamp(const orc_sim *s, int n, const float *x3, float *out3);
 
+/* ------------------------------------------------------------------ */
+/* MLS-MPM (mpm_oracle.c; src/simulator/MPM/MPMShader.ts, MPM.ts).      */
+typedef struct orc_mpm orc_mpm;
+typedef struct {
+  int n_grid;       /* grid nodes per axis over the unit box (reference 32) */
+  unsigned bound;   /* boundary layer in nodes (reference 3) */
+  double dt;        /* reference 8e-4 */
+  double rho;       /* particle density (reference 1) */
+  double E;         /* Young's modulus (reference 400) */
+  double gravity;   /* magnitude, applied as (0, -gravity, 0) */
+} orc_mpm_params;
+enum { ORC_MPM_POSITION = 0, ORC_MPM_VELOCITY = 1 /* v xyz, J in w */, ORC_MPM_AFFINE = 2 /* 9 per particle */,
+       ORC_MPM_GRID = 3 /* n_grid^3 x 4 grid velocity */ };
+orc_mpm *orc_mpm_create(const orc_mpm_params *p, int n, const float *xyzw);
+void orc_mpm_destroy(orc_mpm *m);
+void orc_mpm_step(orc_mpm *m);
+void orc_mpm_get(const orc_mpm *m, int field, float *out);
+void orc_mpm_set(orc_mpm *m, int field, const float *in);
+/* P2G and the grid pass only (J advanced, grid velocity written); raw gets
+ * the node sums (momentum xyz, mass) before the grid pass. */
+void orc_mpm_p2g(orc_mpm *m, float *raw);
+/* G2P only, on the grid velocity set with orc_mpm_set(ORC_MPM_GRID). */
+void orc_mpm_g2p_only(orc_mpm *m);
+
 #ifdef __cplusplus
 }
 #endif

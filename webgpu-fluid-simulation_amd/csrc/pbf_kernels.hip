@@ -299,34 +299,6 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
   return v;
 }
 
-// Rank of this lane's key among equal keys, one global atomic per distinct
-// key per wavefront (keys of cell-sorted particles repeat within a wave).
-// The groups are found first with ballots only; then every group's leader
-// issues its atomic in one instruction, so the wavefront waits for one
-// atomic round trip instead of one per group.
-__device__ __forceinline__ uint32_t wave_atomic_rank(uint32_t *counts, uint32_t key, bool active) {
-  const uint32_t lane = __lane_id();
-  unsigned long long pending = __ballot(active);
-  const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  unsigned long long my_same = 0;
-  int my_leader = 0;
-  while (pending) {
-    const int leader = __ffsll((long long)pending) - 1;
-    const uint32_t lkey = (uint32_t)__shfl((int)key, leader, 64);
-    const bool mine = active && key == lkey;
-    const unsigned long long same = __ballot(mine) & pending;
-    if (mine) {
-      my_same = same;
-      my_leader = leader;
-    }
-    pending &= ~same;
-  }
-  uint32_t base = 0;
-  if (active && (int)lane == my_leader) base = atomicAdd(&counts[key], (uint32_t)__popcll(my_same));
-  base = (uint32_t)__shfl((int)base, my_leader, 64);
-  return base + (uint32_t)__popcll(my_same & lt);
-}
-
 // ---------------------------------------------------------------------------
 // K1 + K2: predict and bin.
 template <bool MULTI>

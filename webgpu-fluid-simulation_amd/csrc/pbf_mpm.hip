@@ -1,0 +1,544 @@
+// pbf_mpm.hip -- MLS-MPM step behind the pbf_mpm_* entry points of
+// include/pbf.h (reference: src/simulator/MPM/MPMShader.ts, the P2G / grid /
+// G2P shaders, and MPM.ts, their commented-out driver).
+//
+// The reference scatters every particle into its 27 grid nodes with float
+// compare-and-swap loops (MPMShader.ts:8-56), so a node's sum depends on the
+// order the atomics land in.  Here P2G is a gather: particles are binned by
+// their base cell and stably sorted by id (the binning and scan of the PBF
+// neighbour search), then one thread per grid node sums the particles of
+// the 27 base cells that reach it, in ascending (cell, id) order, and
+// applies the grid pass (MPMShader.ts:163-191) to its own sum in registers.
+// No atomics touch the grid, every step is deterministic, and the result is
+// bit-identical to the CPU restatement that sums in the same order.
+//
+// Per step: prepare (J update, stress, affine matrix, base cell and binning
+// rank; the per-particle P2G record written once), scan, scatter, settle
+// (records moved into cell order), node gather + grid pass, G2P.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/pbf.h"
+#include "pbf_device.h"
+#include "pbf_internal.h"
+
+using namespace pbf;
+
+namespace {
+
+constexpr uint32_t MBS = 256;
+
+struct MpmK {
+  float dx, dt, p_vol, p_mass, E, inv4_dx2;
+  float g[4];
+  uint32_t ng, bound, G;
+};
+
+__device__ __forceinline__ uint32_t f2u_sat(float f) { return f > 0.0f ? (uint32_t)f : 0u; }
+
+// quadratic B-spline weights of the shader (MPMShader.ts:106-110)
+__device__ __forceinline__ void bspline(float f, float &w0, float &w1, float &w2) {
+  const float t0 = 1.5f - f, t1 = f - 1.0f, t2 = f - 0.5f;
+  w0 = (0.5f * t0) * t0;
+  w1 = 0.75f - t1 * t1;
+  w2 = (0.5f * t2) * t2;
+}
+
+__device__ __forceinline__ float pick3(int k, float a, float b, float c) { return k == 0 ? a : k == 1 ? b : c; }
+
+// Prepare: J (MPMShader.ts:101), base cell and fractional offset
+// (:103-105), stress and affine matrix (:112-113), p_mass v; the record
+// (fx, p_mass v, affine columns) in particle order and the binning rank.
+__global__ void __launch_bounds__(MBS) k_mpm_prepare(uint32_t n, const float4 *__restrict__ x, float4 *v,
+                                                     const float4 *__restrict__ C0, const float4 *__restrict__ C1,
+                                                     const float4 *__restrict__ C2, float4 *__restrict__ rec,
+                                                     uint32_t *__restrict__ cellkey, uint32_t *__restrict__ rank,
+                                                     uint32_t *__restrict__ counts, MpmK K) {
+  const uint32_t p = blockIdx.x * MBS + threadIdx.x;
+  const bool active = p < n;
+  uint32_t key = 0;
+  if (active) {
+    const float4 xp = x[p], vp = v[p], c0 = C0[p], c1 = C1[p], c2 = C2[p];
+    const float J = vp.w * (1.0f + K.dt * ((c0.x + c1.y) + c2.z));
+    v[p] = make_float4(vp.x, vp.y, vp.z, J);
+    const float X0 = xp.x / K.dx, X1 = xp.y / K.dx, X2 = xp.z / K.dx;
+    const uint32_t b0 = f2u_sat(X0 - 0.5f), b1 = f2u_sat(X1 - 0.5f), b2 = f2u_sat(X2 - 0.5f);
+    const float f0 = X0 - (float)b0, f1 = X1 - (float)b1, f2 = X2 - (float)b2;
+    const float stress = (((((-K.dt) * 4.0f) * K.E) * K.p_vol) * (J - 1.0f)) / (K.dx * K.dx);
+    const float z = 0.0f * stress;  // off-diagonal of stress * identity
+    const float m = K.p_mass;
+    const size_t s = (size_t)p * 5;
+    rec[s] = make_float4(f0, f1, f2, 0.0f);
+    rec[s + 1] = make_float4(m * vp.x, m * vp.y, m * vp.z, 0.0f);
+    rec[s + 2] = make_float4(stress + m * c0.x, z + m * c0.y, z + m * c0.z, 0.0f);
+    rec[s + 3] = make_float4(z + m * c1.x, stress + m * c1.y, z + m * c1.z, 0.0f);
+    rec[s + 4] = make_float4(z + m * c2.x, z + m * c2.y, stress + m * c2.z, 0.0f);
+    key = (b0 < K.ng && b1 < K.ng && b2 < K.ng) ? (b0 * K.ng + b1) * K.ng + b2 : K.G;  // K.G: outside
+    cellkey[p] = key;
+  }
+  const uint32_t r = wave_atomic_rank(counts, key, active);
+  if (active) rank[p] = r;
+}
+
+__global__ void __launch_bounds__(MBS) k_mpm_scatter(uint32_t n, const uint32_t *__restrict__ cellkey,
+                                                     const uint32_t *__restrict__ rank,
+                                                     const uint32_t *__restrict__ offsets,
+                                                     uint32_t *__restrict__ tmp_id) {
+  const uint32_t p = blockIdx.x * MBS + threadIdx.x;
+  if (p >= n) return;
+  tmp_id[offsets[cellkey[p]] + rank[p]] = p;
+}
+
+// Records into cell order, ascending id inside a cell (the bin of
+// particles outside the grid is never read: left in scatter order).
+__global__ void __launch_bounds__(MBS) k_mpm_settle(uint32_t n, const uint32_t *__restrict__ tmp_id,
+                                                    const uint32_t *__restrict__ cellkey,
+                                                    const uint32_t *__restrict__ offsets,
+                                                    const float4 *__restrict__ rec, float4 *__restrict__ srec,
+                                                    uint32_t outside) {
+  const uint32_t s = blockIdx.x * MBS + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t p = tmp_id[s], c = cellkey[p];
+  uint32_t slot = s;
+  if (c != outside) {
+    const uint32_t o = offsets[c], e = offsets[c + 1];
+    uint32_t r = 0;
+    for (uint32_t t = o; t < e; ++t) r += tmp_id[t] < p ? 1u : 0u;
+    slot = o + r;
+  }
+  const size_t f = (size_t)slot * 5, q = (size_t)p * 5;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) srec[f + k] = rec[q + k];
+}
+
+// P2G as a gather (MPMShader.ts:4-62 summed per node) and the grid pass
+// (MPMShader.ts:163-191): one thread per node, x-major node index.
+__global__ void __launch_bounds__(MBS) k_mpm_p2g_grid(const uint32_t *__restrict__ offsets,
+                                                      const float4 *__restrict__ srec, float4 *__restrict__ grid,
+                                                      MpmK K) {
+  const uint32_t node = blockIdx.x * MBS + threadIdx.x;
+  if (node >= K.G) return;
+  const int I = (int)(node / (K.ng * K.ng)), J = (int)((node / K.ng) % K.ng), Kz = (int)(node % K.ng);
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, am = 0.0f;
+  for (int bx = I - 2; bx <= I; ++bx) {
+    if (bx < 0) continue;
+    const int oi = I - bx;
+    for (int by = J - 2; by <= J; ++by) {
+      if (by < 0) continue;
+      const int oj = J - by;
+      // the three cells bz = K-2..K of this (bx, by) are one slot range
+      const int bz0 = Kz - 2 < 0 ? 0 : Kz - 2;
+      const uint32_t c0 = ((uint32_t)bx * K.ng + (uint32_t)by) * K.ng;
+      const uint32_t t0 = offsets[c0 + (uint32_t)bz0], t1 = offsets[c0 + (uint32_t)Kz + 1];
+      uint32_t cell_end = offsets[c0 + (uint32_t)bz0 + 1];
+      int bz = bz0;
+      for (uint32_t t = t0; t < t1; ++t) {
+        while (t >= cell_end) {  // next non-empty cell of the range
+          ++bz;
+          cell_end = offsets[c0 + (uint32_t)bz + 1];
+        }
+        const int ok = Kz - bz;
+        const size_t q = (size_t)t * 5;
+        const float4 f = srec[q], pm = srec[q + 1], A0 = srec[q + 2], A1 = srec[q + 3], A2 = srec[q + 4];
+        float wx0, wx1, wx2, wy0, wy1, wy2, wz0, wz1, wz2;
+        bspline(f.x, wx0, wx1, wx2);
+        bspline(f.y, wy0, wy1, wy2);
+        bspline(f.z, wz0, wz1, wz2);
+        const float dpx = ((float)oi - f.x) * K.dx, dpy = ((float)oj - f.y) * K.dx, dpz = ((float)ok - f.z) * K.dx;
+        const float weight = (pick3(oi, wx0, wx1, wx2) * pick3(oj, wy0, wy1, wy2)) * pick3(ok, wz0, wz1, wz2);
+        const float ad0 = (A0.x * dpx + A1.x * dpy) + A2.x * dpz;
+        const float ad1 = (A0.y * dpx + A1.y * dpy) + A2.y * dpz;
+        const float ad2 = (A0.z * dpx + A1.z * dpy) + A2.z * dpz;
+        a0 += weight * (pm.x + ad0);
+        a1 += weight * (pm.y + ad1);
+        a2 += weight * (pm.z + ad2);
+        am += weight * K.p_mass;
+      }
+    }
+  }
+  float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f, v3 = 0.0f;
+  if (am > 0.0f) {
+    v0 = a0 / am;
+    v1 = a1 / am;
+    v2 = a2 / am;
+    v3 = am / am;
+  }
+  v0 = v0 + K.dt * K.g[0];
+  v1 = v1 + K.dt * K.g[1];
+  v2 = v2 + K.dt * K.g[2];
+  v3 = v3 + K.dt * K.g[3];
+  const uint32_t hi = K.ng - K.bound;
+  if (((uint32_t)I < K.bound && v0 < 0.0f) || ((uint32_t)I >= hi && v0 > 0.0f)) v0 = 0.0f;
+  if (((uint32_t)J < K.bound && v1 < 0.0f) || ((uint32_t)J >= hi && v1 > 0.0f)) v1 = 0.0f;
+  if (((uint32_t)Kz < K.bound && v2 < 0.0f) || ((uint32_t)Kz >= hi && v2 > 0.0f)) v2 = 0.0f;
+  grid[node] = make_float4(v0, v1, v2, v3);
+}
+
+// G2P (MPMShader.ts:240-280): 27 nodes in the shader's order (i outer, k
+// inner); nodes outside the grid read as zero.
+__global__ void __launch_bounds__(MBS) k_mpm_g2p(uint32_t n, float4 *x, float4 *v, float4 *__restrict__ C0,
+                                                 float4 *__restrict__ C1, float4 *__restrict__ C2,
+                                                 const float4 *__restrict__ grid, MpmK K) {
+  const uint32_t p = blockIdx.x * MBS + threadIdx.x;
+  if (p >= n) return;
+  const float4 xp = x[p];
+  const float X[3] = {xp.x / K.dx, xp.y / K.dx, xp.z / K.dx};
+  uint32_t b[3];
+  float f[3], w[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    b[a] = f2u_sat(X[a] - 0.5f);
+    f[a] = X[a] - (float)b[a];
+    bspline(f[a], w[0][a], w[1][a], w[2][a]);
+  }
+  float nv[3] = {0.0f, 0.0f, 0.0f}, nC[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float dpos[3] = {((float)i - f[0]) * K.dx, ((float)j - f[1]) * K.dx, ((float)k - f[2]) * K.dx};
+        const float weight = (w[i][0] * w[j][1]) * w[k][2];
+        const uint32_t gi = b[0] + i, gj = b[1] + j, gk = b[2] + k;
+        float vel[3] = {0.0f, 0.0f, 0.0f};
+        if (gi < K.ng && gj < K.ng && gk < K.ng) {
+          const float4 g = grid[(gi * K.ng + gj) * K.ng + gk];
+          vel[0] = g.x;
+          vel[1] = g.y;
+          vel[2] = g.z;
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) nv[a] = nv[a] + weight * vel[a];
+        const float s = K.inv4_dx2 * weight;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) nC[3 * c + r] = nC[3 * c + r] + s * (vel[c] * dpos[r]);
+      }
+  const float J = v[p].w;
+  v[p] = make_float4(nv[0], nv[1], nv[2], J);
+  C0[p] = make_float4(nC[0], nC[1], nC[2], 0.0f);
+  C1[p] = make_float4(nC[3], nC[4], nC[5], 0.0f);
+  C2[p] = make_float4(nC[6], nC[7], nC[8], 0.0f);
+  x[p] = make_float4(xp.x + K.dt * nv[0], xp.y + K.dt * nv[1], xp.z + K.dt * nv[2], xp.w);
+}
+
+__global__ void __launch_bounds__(MBS) k_mpm_init(uint32_t n, const float4 *__restrict__ in, float4 *__restrict__ x,
+                                                  float4 *__restrict__ v, float4 *__restrict__ C0,
+                                                  float4 *__restrict__ C1, float4 *__restrict__ C2) {
+  const uint32_t p = blockIdx.x * MBS + threadIdx.x;
+  if (p >= n) return;
+  const float4 a = in[p];
+  x[p] = make_float4(a.x, a.y, a.z, 0.0f);
+  v[p] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);  // J = 1 (MPM.ts:115-119)
+  C0[p] = C1[p] = C2[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+inline uint32_t nblk(uint64_t n) { return (uint32_t)((n + MBS - 1) / MBS); }
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count) {
+  return hipMalloc(reinterpret_cast<void **>(p), (count ? count : 1) * sizeof(T));
+}
+
+}  // namespace
+
+struct pbf_mpm {
+  pbf_mpm_params params;
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t n = 0, cap = 0, ng = 0, G = 0;
+  MpmK K;
+  float4 *x = nullptr, *v = nullptr, *C0 = nullptr, *C1 = nullptr, *C2 = nullptr, *grid = nullptr;
+  float4 *rec = nullptr, *srec = nullptr, *stage = nullptr;
+  uint32_t *cellkey = nullptr, *rank = nullptr, *counts = nullptr, *offsets = nullptr, *tmp_id = nullptr;
+  std::vector<uint32_t *> scan_sums, scan_prefix;
+  hipGraphExec_t graph = nullptr;
+  uint32_t graph_n = 0;
+};
+
+namespace {
+
+int mfail(pbf_mpm *m, int code, const std::string &msg) {
+  if (m) m->err = msg;
+  set_thread_error(msg);
+  return code;
+}
+
+#define MCHK(expr)                                                                     \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) return mfail(m, PBF_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+void enqueue_step(pbf_mpm *m) {
+  const uint32_t n = m->n;
+  hipStream_t st = m->stream;
+  hipLaunchKernelGGL(k_mpm_prepare, dim3(nblk(n)), dim3(MBS), 0, st, n, m->x, m->v, m->C0, m->C1, m->C2, m->rec,
+                     m->cellkey, m->rank, m->counts, m->K);
+  // counts over G + 1 bins (the last: particles whose base cell is outside
+  // the grid, which the reference's bounds test skips); zeroed by the scan
+  launch_scan(st, m->counts, m->G + 1, m->offsets, 1, m->scan_sums.data(), m->scan_prefix.data(), 0);
+  hipLaunchKernelGGL(k_mpm_scatter, dim3(nblk(n)), dim3(MBS), 0, st, n, m->cellkey, m->rank, m->offsets, m->tmp_id);
+  hipLaunchKernelGGL(k_mpm_settle, dim3(nblk(n)), dim3(MBS), 0, st, n, m->tmp_id, m->cellkey, m->offsets, m->rec,
+                     m->srec, m->G);
+  hipLaunchKernelGGL(k_mpm_p2g_grid, dim3(nblk(m->G)), dim3(MBS), 0, st, m->offsets, m->srec, m->grid, m->K);
+  hipLaunchKernelGGL(k_mpm_g2p, dim3(nblk(n)), dim3(MBS), 0, st, n, m->x, m->v, m->C0, m->C1, m->C2, m->grid, m->K);
+}
+
+void drop_graph(pbf_mpm *m) {
+  if (m->graph) {
+    (void)hipStreamSynchronize(m->stream);
+    (void)hipGraphExecDestroy(m->graph);
+    m->graph = nullptr;
+  }
+}
+
+}  // namespace
+
+#define MPM_CATCH(m)                                                         \
+  catch (const std::bad_alloc &) {                                           \
+    return mfail(m, PBF_E_CAPACITY, "out of host memory");                   \
+  }                                                                          \
+  catch (const std::exception &ex) {                                         \
+    return mfail(m, PBF_E_INVALID, ex.what());                               \
+  }
+
+extern "C" {
+
+int pbf_mpm_create(const pbf_mpm_params *params, pbf_mpm **out) try {
+  if (!params || !out) return mfail(nullptr, PBF_E_INVALID, "params and out are required");
+  *out = nullptr;
+  pbf_mpm_params p = *params;
+  if (p.n_grid == 0) p.n_grid = 32;
+  if (p.bound == 0) p.bound = 3;
+  if (!(p.dt > 0.0)) p.dt = 8e-4;
+  if (!(p.rho > 0.0)) p.rho = 1.0;
+  if (!(p.E > 0.0)) p.E = 400.0;
+  if (p.capacity == 0) p.capacity = 9000;
+  if (!std::isfinite(p.dt) || !std::isfinite(p.rho) || !std::isfinite(p.E) || !std::isfinite(p.gravity))
+    return mfail(nullptr, PBF_E_INVALID, "dt, rho, E and gravity must be finite");
+  if (p.n_grid < 4 || p.n_grid > 1024) return mfail(nullptr, PBF_E_INVALID, "n_grid must be in [4, 1024]");
+  if (2 * p.bound > p.n_grid) return mfail(nullptr, PBF_E_INVALID, "bound must be at most n_grid / 2");
+  if (p.capacity > (1u << 27)) return mfail(nullptr, PBF_E_CAPACITY, "capacity above 2^27 particles");
+  pbf_mpm *m = new pbf_mpm();
+  m->params = p;
+  m->cap = p.capacity;
+  m->ng = p.n_grid;
+  m->G = p.n_grid * p.n_grid * p.n_grid;
+  const double dx = 1.0 / (double)p.n_grid;
+  MpmK &K = m->K;
+  K.dx = (float)dx;
+  K.dt = (float)p.dt;
+  K.p_vol = (float)((0.5 * dx) * (0.5 * dx));  // MPM.ts:77-78
+  K.p_mass = (float)(p.rho * ((0.5 * dx) * (0.5 * dx)));
+  K.E = (float)p.E;
+  K.inv4_dx2 = 4.0f / (K.dx * K.dx);
+  K.g[0] = 0.0f;
+  K.g[1] = -(float)p.gravity;
+  K.g[2] = 0.0f;
+  K.g[3] = 0.0f;
+  K.ng = p.n_grid;
+  K.bound = p.bound;
+  K.G = m->G;
+  int rc = PBF_OK;
+  auto step = [&](hipError_t e, const char *what) {
+    if (rc == PBF_OK && e != hipSuccess) rc = mfail(m, PBF_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  step(hipGetDevice(&m->device), "hipGetDevice");
+  step(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "hipStreamCreate");
+  const size_t N = m->cap;
+  step(dalloc(&m->x, N), "alloc"); step(dalloc(&m->v, N), "alloc");
+  step(dalloc(&m->C0, N), "alloc"); step(dalloc(&m->C1, N), "alloc"); step(dalloc(&m->C2, N), "alloc");
+  step(dalloc(&m->rec, 5 * N), "alloc"); step(dalloc(&m->srec, 5 * N), "alloc");
+  step(dalloc(&m->stage, N), "alloc");
+  step(dalloc(&m->cellkey, N), "alloc"); step(dalloc(&m->rank, N), "alloc"); step(dalloc(&m->tmp_id, N), "alloc");
+  step(dalloc(&m->grid, (size_t)m->G), "alloc");
+  step(dalloc(&m->counts, (size_t)m->G + 1), "alloc");
+  step(dalloc(&m->offsets, (size_t)m->G + 2), "alloc");
+  {
+    uint32_t len = m->G + 1;
+    while (rc == PBF_OK && len > 2048u) {
+      const uint32_t nb = (len + 2047u) / 2048u;
+      uint32_t *s = nullptr, *q = nullptr;
+      step(dalloc(&s, nb), "alloc");
+      step(dalloc(&q, (size_t)nb + 1), "alloc");
+      m->scan_sums.push_back(s);
+      m->scan_prefix.push_back(q);
+      len = nb;
+    }
+    m->scan_sums.push_back(nullptr);
+    m->scan_prefix.push_back(nullptr);
+  }
+  if (rc == PBF_OK) step(hipMemset(m->counts, 0, ((size_t)m->G + 1) * 4), "hipMemset");
+  if (rc == PBF_OK) step(hipMemset(m->grid, 0, (size_t)m->G * 16), "hipMemset");
+  if (rc != PBF_OK) {
+    const std::string msg = m->err;
+    pbf_mpm_destroy(m);
+    set_thread_error(msg);
+    return rc;
+  }
+  *out = m;
+  return PBF_OK;
+}
+MPM_CATCH(nullptr)
+
+int pbf_mpm_destroy(pbf_mpm *m) {
+  if (!m) return PBF_OK;
+  (void)hipSetDevice(m->device);
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  drop_graph(m);
+  void *bufs[] = {m->x, m->v, m->C0, m->C1, m->C2, m->grid, m->rec, m->srec, m->stage, m->cellkey, m->rank,
+                  m->counts, m->offsets, m->tmp_id};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  for (auto q : m->scan_sums)
+    if (q) (void)hipFree(q);
+  for (auto q : m->scan_prefix)
+    if (q) (void)hipFree(q);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+  return PBF_OK;
+}
+
+int pbf_mpm_set_positions(pbf_mpm *m, const float *xyzw, uint32_t count) try {
+  if (!m || (!xyzw && count)) return mfail(m, PBF_E_INVALID, "context and positions are required");
+  if (count > m->cap) return mfail(m, PBF_E_CAPACITY, "To Many Particles!");
+  MCHK(hipSetDevice(m->device));
+  MCHK(hipStreamSynchronize(m->stream));
+  drop_graph(m);
+  m->n = count;
+  if (count) {
+    MCHK(hipMemcpy(m->stage, xyzw, (size_t)count * 16, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mpm_init, dim3(nblk(count)), dim3(MBS), 0, m->stream, count, m->stage, m->x, m->v, m->C0,
+                       m->C1, m->C2);
+    MCHK(hipGetLastError());
+    MCHK(hipStreamSynchronize(m->stream));
+  }
+  return PBF_OK;
+}
+MPM_CATCH(m)
+
+int pbf_mpm_step(pbf_mpm *m, uint32_t steps) try {
+  if (!m) return mfail(m, PBF_E_INVALID, "null context");
+  if (m->n == 0 || steps == 0) return PBF_OK;
+  MCHK(hipSetDevice(m->device));
+  if (m->params.flags & PBF_FLAG_NO_GRAPH) {
+    for (uint32_t s = 0; s < steps; ++s) enqueue_step(m);
+    MCHK(hipGetLastError());
+    return PBF_OK;
+  }
+  if (!m->graph || m->graph_n != m->n) {
+    drop_graph(m);
+    hipGraph_t g = nullptr;
+    MCHK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(m);
+    const hipError_t ec = hipStreamEndCapture(m->stream, &g);
+    if (ec != hipSuccess) return mfail(m, PBF_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&m->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      m->graph = nullptr;
+      return mfail(m, PBF_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+    m->graph_n = m->n;
+  }
+  for (uint32_t s = 0; s < steps; ++s) MCHK(hipGraphLaunch(m->graph, m->stream));
+  return PBF_OK;
+}
+MPM_CATCH(m)
+
+int pbf_mpm_sync(pbf_mpm *m) try {
+  if (!m) return mfail(m, PBF_E_INVALID, "null context");
+  MCHK(hipSetDevice(m->device));
+  MCHK(hipStreamSynchronize(m->stream));
+  return PBF_OK;
+}
+MPM_CATCH(m)
+
+int pbf_mpm_read(pbf_mpm *m, int field, float *out, size_t count) try {
+  if (!m || !out) return mfail(m, PBF_E_INVALID, "context and output are required");
+  MCHK(hipSetDevice(m->device));
+  MCHK(hipStreamSynchronize(m->stream));
+  const size_t n = m->n;
+  if (field == PBF_MPM_GRID) {
+    if (count != m->G) return mfail(m, PBF_E_LENGTH, "grid readback count must be n_grid^3");
+    MCHK(hipMemcpy(out, m->grid, (size_t)m->G * 16, hipMemcpyDeviceToHost));
+    return PBF_OK;
+  }
+  if (count != n) return mfail(m, PBF_E_LENGTH, "readback count must equal the particle count");
+  if (!n) return PBF_OK;
+  if (field == PBF_MPM_POSITION) {
+    MCHK(hipMemcpy(out, m->x, n * 16, hipMemcpyDeviceToHost));
+  } else if (field == PBF_MPM_VELOCITY) {
+    MCHK(hipMemcpy(out, m->v, n * 16, hipMemcpyDeviceToHost));
+  } else if (field == PBF_MPM_AFFINE) {
+    std::vector<float> c(n * 4);
+    float4 *cols[3] = {m->C0, m->C1, m->C2};
+    for (int k = 0; k < 3; ++k) {
+      MCHK(hipMemcpy(c.data(), cols[k], n * 16, hipMemcpyDeviceToHost));
+      for (size_t p = 0; p < n; ++p)
+        for (int r = 0; r < 3; ++r) out[9 * p + 3 * k + r] = c[4 * p + r];
+    }
+  } else {
+    return mfail(m, PBF_E_INVALID, "unknown MPM field");
+  }
+  return PBF_OK;
+}
+MPM_CATCH(m)
+
+int pbf_mpm_write(pbf_mpm *m, int field, const float *in, size_t count) try {
+  if (!m || !in) return mfail(m, PBF_E_INVALID, "context and input are required");
+  MCHK(hipSetDevice(m->device));
+  MCHK(hipStreamSynchronize(m->stream));
+  const size_t n = m->n;
+  if (field == PBF_MPM_GRID) {
+    if (count != m->G) return mfail(m, PBF_E_LENGTH, "grid count must be n_grid^3");
+    MCHK(hipMemcpy(m->grid, in, (size_t)m->G * 16, hipMemcpyHostToDevice));
+    return PBF_OK;
+  }
+  if (count != n) return mfail(m, PBF_E_LENGTH, "count must equal the particle count");
+  if (!n) return PBF_OK;
+  if (field == PBF_MPM_POSITION) {
+    MCHK(hipMemcpy(m->x, in, n * 16, hipMemcpyHostToDevice));
+  } else if (field == PBF_MPM_VELOCITY) {
+    MCHK(hipMemcpy(m->v, in, n * 16, hipMemcpyHostToDevice));
+  } else if (field == PBF_MPM_AFFINE) {
+    std::vector<float> c(n * 4, 0.0f);
+    float4 *cols[3] = {m->C0, m->C1, m->C2};
+    for (int k = 0; k < 3; ++k) {
+      for (size_t p = 0; p < n; ++p)
+        for (int r = 0; r < 3; ++r) c[4 * p + r] = in[9 * p + 3 * k + r];
+      MCHK(hipMemcpy(cols[k], c.data(), n * 16, hipMemcpyHostToDevice));
+    }
+  } else {
+    return mfail(m, PBF_E_INVALID, "unknown MPM field");
+  }
+  return PBF_OK;
+}
+MPM_CATCH(m)
+
+int pbf_mpm_g2p(pbf_mpm *m) try {
+  if (!m) return mfail(m, PBF_E_INVALID, "null context");
+  if (!m->n) return PBF_OK;
+  MCHK(hipSetDevice(m->device));
+  hipLaunchKernelGGL(k_mpm_g2p, dim3(nblk(m->n)), dim3(MBS), 0, m->stream, m->n, m->x, m->v, m->C0, m->C1, m->C2,
+                     m->grid, m->K);
+  MCHK(hipGetLastError());
+  return PBF_OK;
+}
+MPM_CATCH(m)
+
+uint32_t pbf_mpm_particle_count(const pbf_mpm *m) { return m ? m->n : 0u; }
+
+const char *pbf_mpm_last_error(const pbf_mpm *m) { return m ? m->err.c_str() : pbf_last_error(nullptr); }
+
+}  // extern "C"

@@ -46,6 +46,15 @@ class Params(C.Structure):
                 ("cell_slabs", C.c_uint32), ("reserved", C.c_uint32 * 4)]
 
 
+class MpmParams(C.Structure):
+    _fields_ = [("capacity", C.c_uint32), ("n_grid", C.c_uint32), ("bound", C.c_uint32), ("flags", C.c_uint32),
+                ("dt", C.c_double), ("rho", C.c_double), ("E", C.c_double), ("gravity", C.c_double),
+                ("reserved", C.c_uint32 * 4)]
+
+
+MPM_POSITION, MPM_VELOCITY, MPM_AFFINE, MPM_GRID = range(4)
+
+
 class Config(C.Structure):
     _fields_ = [("iteration", C.c_uint32), ("xsph", C.c_float), ("vorticity", C.c_float),
                 ("surface_tension", C.c_float), ("gravity", C.c_float)]
@@ -126,6 +135,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "pbf_get_tuning": (I, [P, C.POINTER(Tuning)]),
         "pbf_set_tuning": (I, [P, C.POINTER(Tuning)]),
         "pbf_autotune": (I, [P, F, U32, U32, C.POINTER(Tuning)]),
+        "pbf_mpm_create": (I, [C.POINTER(MpmParams), C.POINTER(P)]), "pbf_mpm_destroy": (I, [P]),
+        "pbf_mpm_set_positions": (I, [P, pf, U32]), "pbf_mpm_step": (I, [P, U32]), "pbf_mpm_sync": (I, [P]),
+        "pbf_mpm_read": (I, [P, I, pf, S]), "pbf_mpm_write": (I, [P, I, pf, S]), "pbf_mpm_g2p": (I, [P]),
+        "pbf_mpm_particle_count": (U32, [P]), "pbf_mpm_last_error": (C.c_char_p, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -603,3 +616,78 @@ class PBF:
         t = Tuning()
         self._check(self.lib.pbf_autotune(self.ctx, float(dt), int(iterations), int(repeats), C.byref(t)))
         return {"walk_block": dict(zip(self.WALKS, list(t.walk_block)[:6])), "list_variant": int(t.list_variant)}
+
+
+class MPM:
+    """Mirror of the reference's MLS-MPM simulator (src/simulator/MPM/MPM.ts,
+    commented out in the reference) on libpbf: the constructor's parameters
+    (MPM.ts:60-83), initResource (MPM.ts:94-121: particles uniform in
+    [0.15, 0.55]^3, v = 0, C = 0, J = 1) and run (MPM.ts:204-238: one P2G,
+    grid and G2P pass)."""
+
+    def __init__(self, n_particle: int = 9000, n_grid: int = 32, dt: float = 8e-4, bound: int = 3,
+                 rho: float = 1.0, E: float = 400.0, gravity: float = 9.8, flags: int = 0,
+                 capacity: Optional[int] = None):
+        self.lib = load_library()
+        p = MpmParams()
+        p.capacity = capacity or n_particle
+        p.n_grid, p.bound, p.flags = n_grid, bound, flags
+        p.dt, p.rho, p.E, p.gravity = dt, rho, E, gravity
+        self.params = p
+        self.particleCount = n_particle
+        self.n_grid = n_grid
+        self.m = C.c_void_p()
+        rc = self.lib.pbf_mpm_create(C.byref(p), C.byref(self.m))
+        if rc != PBF_OK:
+            raise PBFError(rc, self.lib.pbf_mpm_last_error(None).decode())
+
+    def _check(self, rc):
+        if rc != PBF_OK:
+            raise PBFError(rc, self.lib.pbf_mpm_last_error(self.m).decode())
+
+    def initResource(self, seed: int = 0):
+        """Random particle positions in [0.15, 0.55]^3 (MPM.ts:104-112, seeded)."""
+        rng = np.random.default_rng(seed)
+        x = np.zeros((self.particleCount, 4), np.float32)
+        x[:, :3] = (rng.random((self.particleCount, 3)) * 0.4 + 0.15).astype(np.float32)
+        self.setParticlePosition(x)
+        return x
+
+    def setParticlePosition(self, xyzw):
+        a = np.ascontiguousarray(np.asarray(xyzw, np.float32).reshape(-1, 4))
+        self._check(self.lib.pbf_mpm_set_positions(self.m, _fptr(a), a.shape[0]))
+        self.particleCount = a.shape[0]
+
+    def run(self, steps: int = 1):
+        self._check(self.lib.pbf_mpm_step(self.m, int(steps)))
+
+    def sync(self):
+        self._check(self.lib.pbf_mpm_sync(self.m))
+
+    def g2p(self):
+        self._check(self.lib.pbf_mpm_g2p(self.m))
+
+    def _shape(self, field):
+        n = self.particleCount
+        return {MPM_POSITION: (n, 4), MPM_VELOCITY: (n, 4), MPM_AFFINE: (n, 9),
+                MPM_GRID: (self.n_grid ** 3, 4)}[field]
+
+    def read(self, field: int) -> np.ndarray:
+        out = np.zeros(self._shape(field), np.float32)
+        self._check(self.lib.pbf_mpm_read(self.m, field, _fptr(out), out.shape[0]))
+        return out
+
+    def write(self, field: int, arr):
+        a = np.ascontiguousarray(np.asarray(arr, np.float32).reshape(self._shape(field)))
+        self._check(self.lib.pbf_mpm_write(self.m, field, _fptr(a), a.shape[0]))
+
+    def close(self):
+        if self.m:
+            self.lib.pbf_mpm_destroy(self.m)
+            self.m = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
