@@ -350,6 +350,60 @@ def warm_start_profile():
             "iterations": 20, "training_pairs": n, "epochs": 2, "final_loss": float(res.losses[-1])}
 
 
+# MLS-MPM algorithmic bytes: per particle and step (prepare reads x, v, C
+# and writes v, the 80-byte P2G record, key and rank; scatter 16; settle
+# moves the record; the node gather reads it once; G2P reads x, v and writes
+# x, v, C) and per grid node (scan 12, the gather's cell offsets 8, grid write
+# 16, G2P's node read 16).
+MPM_PASS_BYTES = {"prepare": 184, "scatter": 16, "settle": 176, "p2g_grid": 80, "g2p": 112}
+MPM_NODE_BYTES = {"scan": 12, "p2g_grid": 24, "g2p": 16}
+
+
+def mpm_measure(steps: int, cpu_seconds: float):
+    """MLS-MPM: the reference configuration (MPM.ts: 9,000 particles in
+    [0.15, 0.55]^3, 32^3 grid, frames of 15 steps) and a 1,048,576-particle
+    block on a 128^3 grid with dt scaled by the grid spacing (2e-4: the
+    explicit step's elastic-wave limit dx / sqrt(E / rho) is 3.9e-4 there);
+    particle-steps/s with graph replay, and the oracle on a bounded sample of
+    the 1M case."""
+    from pbf_host import MPM
+    out = {}
+    for name, n, ng, k, dt in (("reference_9k", 9000, 32, 15, 8e-4), ("block_1m", 1 << 20, 128, 1, 2e-4)):
+        sim = MPM(n_particle=n, n_grid=ng, dt=dt)
+        x = sim.initResource(seed=3)
+        sim.run(5 * k)
+        sim.sync()
+        reps = max(1, steps // k)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            sim.run(k)
+        sim.sync()
+        el = time.perf_counter() - t0
+        v = sim.read(1)
+        assert np.isfinite(v).all(), "non-finite MPM velocities"
+        nsteps = reps * k
+        alg = (sum(MPM_PASS_BYTES.values()) * n + sum(MPM_NODE_BYTES.values()) * ng ** 3) * nsteps / el / 1e9
+        out[name] = {"particles": n, "n_grid": ng, "dt": dt, "steps": nsteps, "particle_steps_per_s": n * nsteps / el,
+                     "us_per_step": 1e6 * el / nsteps, "algorithmic_GBs": round(alg, 1)}
+        log(f"mpm {name}: {n * nsteps / el / 1e6:.1f} M particle-steps/s, {1e6 * el / nsteps:.1f} us/step, "
+            f"{alg:.0f} GB/s algorithmic")
+        sim.close()
+        if name == "block_1m" and cpu_seconds > 0:
+            import oracle
+            o = oracle.MpmOracle(x, n_grid=ng, dt=dt)
+            o.step(1)
+            k2, t1 = 0, time.perf_counter()
+            while time.perf_counter() - t1 < cpu_seconds:
+                o.step(1)
+                k2 += 1
+            el2 = time.perf_counter() - t1
+            threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+            out["cpu_baseline"] = {"value": n * k2 / el2, "unit": "particle-steps/s", "cores": threads,
+                                   "kind": "port", "sample": f"oracle/mpm_oracle.c on the 1M block: {k2} steps "
+                                                             f"after 1 warm-up, {el2:.1f} s"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -367,6 +421,10 @@ def main():
     ap.add_argument("--direct", action="store_true",
                     help="launch the passes directly instead of replaying a hipGraph per substep (for counter "
                          "passes under rocprofv3 --pmc)")
+    ap.add_argument("--no-mpm", action="store_true", help="skip the MLS-MPM measurement")
+    ap.add_argument("--mpm-only", action="store_true",
+                    help="only run the MLS-MPM measurement (reference configuration and the 1M block); prints "
+                         "one JSON line")
     ap.add_argument("--warm-start-profile", action="store_true",
                     help="only run the warm-start network's kernels for a kernel trace: 20 substeps of the 1M dam "
                          "break with the deep ratio network (20 iterations, fallback on), then 2 epochs of "
@@ -374,6 +432,9 @@ def main():
     args = ap.parse_args()
     if args.warm_start_profile:
         print(json.dumps(warm_start_profile()))
+        return
+    if args.mpm_only:
+        print(json.dumps({"mode": "mpm", "mpm": mpm_measure(args.steps, 0.0 if args.no_cpu else 3.0)}))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine runs on a single GPU and has no multi-GPU path.",
@@ -514,6 +575,10 @@ def main():
     if not args.no_batch:
         batched = batched_measure(200, 3, not args.no_tune)
 
+    mpm = None
+    if headline and not args.no_mpm:
+        mpm = mpm_measure(200, 0.0 if args.no_cpu else 3.0)
+
     out = {
         "metric": "particle-substeps/s at 1M; iters to \u22640.5% density error, plain vs learned",
         "value": round(value, 1),
@@ -538,6 +603,7 @@ def main():
         "density_error_after_run": {"mean": mean_err, "max": max_err},
         "learned_warm_start": warm,
         "batched": batched,
+        "mpm": mpm,
         "tuning": tuning,
         "passes": per_pass,
     }

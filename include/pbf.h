@@ -156,7 +156,7 @@ int pbf_clear_boundary(pbf_ctx *ctx);
 
 /* OR'ed into the `kind` of the map builders below: store the reference's
  * own boundary volume instead of this library's definition -- the one its
- * public/model/*.cdm files hold (read by volumeMap.ts:23-39, sampled by
+ * .cdm files under public/model/ hold (read by volumeMap.ts:23-39, sampled by
  * boundaryVolume.ts:40-51): V(x) = 0.8 sum over the 16^3 Gauss-Legendre
  * points xi of [-h, h]^3 with |xi| <= h of w(xi) gamma(sdf(x + xi)), gamma = 1
  * in the solid, the cubic-spline ratio W(d) / W(0) of support h for
@@ -413,7 +413,7 @@ typedef struct pbf_mpm pbf_mpm;
 
 typedef struct {
   uint32_t capacity;  /* max particles (the reference's 9000, MPM.ts:64); 0 -> 9000, <= 2^27 */
-  uint32_t n_grid;    /* nodes per axis (MPM.ts:62, 32); 0 -> 32, in [4, 1024] */
+  uint32_t n_grid;    /* nodes per axis (MPM.ts:62, 32); 0 -> 32, a multiple of 4 in [4, 1024] */
   uint32_t bound;     /* boundary layer in nodes (MPM.ts:69, 3); 0 -> 3, <= n_grid / 2 */
   uint32_t flags;     /* PBF_FLAG_NO_GRAPH: launch the passes directly */
   double dt;          /* MPM.ts:66, 8e-4; 0 -> 8e-4 */

@@ -16,9 +16,11 @@
  *   - mat3x3 values are column-major like WGSL: C[c][r] is column c, row r;
  *     affine * dpos = (col0 * dpos.x + col1 * dpos.y) + col2 * dpos.z;
  *   - P2G sums: the reference adds particles into a node with float atomics
- *     in whatever order they arrive.  Here a node sums its particles in a
- *     fixed order: base cells in ascending linear index (x-major, the same
- *     layout as the grid), and inside a base cell ascending particle id.
+ *     in whatever order they arrive.  Here the order is fixed: for each of
+ *     the node's nine base-cell columns (bx, by) in ascending order, a
+ *     partial sum over its cells bz = K-2..K, each cell's particles by
+ *     ascending id; the node's sum then adds the nine partials in column
+ *     order (s0 + s1) + ... + s8.
  *   - G2P sums the 27 nodes in the shader's order (i outer, k inner).
  */
 #include "pbf_oracle.h"
@@ -183,9 +185,14 @@ static void p2g_grid(orc_mpm *m, float *raw) {
 #pragma omp parallel for schedule(static)
   for (int node = 0; node < (int)G; ++node) {
     const int I = node / (ng * ng), Jn = (node / ng) % ng, K = node % ng;
+    /* one partial sum per base-cell column (bx, by), over its cells
+       bz = K-2..K and their particles in order; the node's sum adds the nine
+       partials in column order (a column outside the grid adds +0) */
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    int col = 0;
     for (int bx = I - 2; bx <= I; ++bx)
-      for (int by = Jn - 2; by <= Jn; ++by)
+      for (int by = Jn - 2; by <= Jn; ++by, ++col) {
+        float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         for (int bz = K - 2; bz <= K; ++bz) {
           if (bx < 0 || by < 0 || bz < 0) continue;
           const size_t cell = ((size_t)bx * ng + by) * ng + bz;
@@ -200,11 +207,13 @@ static void p2g_grid(orc_mpm *m, float *raw) {
             const float *A = m->aff + 9 * p;
             for (int r = 0; r < 3; ++r) {
               float ad = (A[r] * dpos[0] + A[3 + r] * dpos[1]) + A[6 + r] * dpos[2];
-              acc[r] += weight * (m->pmv[3 * p + r] + ad);
+              part[r] += weight * (m->pmv[3 * p + r] + ad);
             }
-            acc[3] += weight * m->p_mass;
+            part[3] += weight * m->p_mass;
           }
         }
+        for (int a = 0; a < 4; ++a) acc[a] = col == 0 ? part[a] : acc[a] + part[a];
+      }
     if (raw)
       for (int a = 0; a < 4; ++a) raw[4 * (size_t)node + a] = acc[a];
     /* grid pass */

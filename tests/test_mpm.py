@@ -94,9 +94,11 @@ def test_one_million_particles_two_steps_bit_exact():
     x = np.zeros((n, 4), np.float32)
     x[:, :3] = (rng.random((n, 3)) * 0.4 + 0.15).astype(np.float32)
     print(f"[mpm] n={n} grid {ng}^3", flush=True)
-    sim = MPM(n_particle=n, n_grid=ng)
+    # dt scaled with the spacing (the explicit step's elastic-wave limit
+    # dx / sqrt(E / rho) is 3.9e-4 on this grid)
+    sim = MPM(n_particle=n, n_grid=ng, dt=2e-4)
     sim.setParticlePosition(x)
-    o = oracle.MpmOracle(x, n_grid=ng)
+    o = oracle.MpmOracle(x, n_grid=ng, dt=2e-4)
     for _ in range(2):
         sim.run(1)
         o.step(1)

@@ -7,8 +7,9 @@
 // order the atomics land in.  Here P2G is a gather: particles are binned by
 // their base cell and stably sorted by id (the binning and scan of the PBF
 // neighbour search), then one thread per grid node sums the particles of
-// the 27 base cells that reach it, in ascending (cell, id) order, and
-// applies the grid pass (MPMShader.ts:163-191) to its own sum in registers.
+// the 27 base cells that reach it in a fixed order (nine column partials,
+// each over its cells and their particles in ascending (cell, id) order,
+// added in column order), and applies the grid pass (MPMShader.ts:163-191).
 // No atomics touch the grid, every step is deterministic, and the result is
 // bit-identical to the CPU restatement that sums in the same order.
 //
@@ -33,6 +34,7 @@ using namespace pbf;
 namespace {
 
 constexpr uint32_t MBS = 256;
+constexpr uint32_t kRec = 4;  // float4s per P2G record
 
 struct MpmK {
   float dx, dt, p_vol, p_mass, E, inv4_dx2;
@@ -73,12 +75,15 @@ __global__ void __launch_bounds__(MBS) k_mpm_prepare(uint32_t n, const float4 *_
     const float stress = (((((-K.dt) * 4.0f) * K.E) * K.p_vol) * (J - 1.0f)) / (K.dx * K.dx);
     const float z = 0.0f * stress;  // off-diagonal of stress * identity
     const float m = K.p_mass;
-    const size_t s = (size_t)p * 5;
-    rec[s] = make_float4(f0, f1, f2, 0.0f);
-    rec[s + 1] = make_float4(m * vp.x, m * vp.y, m * vp.z, 0.0f);
-    rec[s + 2] = make_float4(stress + m * c0.x, z + m * c0.y, z + m * c0.z, 0.0f);
-    rec[s + 3] = make_float4(z + m * c1.x, stress + m * c1.y, z + m * c1.z, 0.0f);
-    rec[s + 4] = make_float4(z + m * c2.x, z + m * c2.y, stress + m * c2.z, 0.0f);
+    // record (64 bytes): fx, p_mass v, the affine matrix's three columns
+    const size_t s = (size_t)p * kRec;
+    const float a00 = stress + m * c0.x, a01 = z + m * c0.y, a02 = z + m * c0.z;
+    const float a10 = z + m * c1.x, a11 = stress + m * c1.y, a12 = z + m * c1.z;
+    const float a20 = z + m * c2.x, a21 = z + m * c2.y, a22 = stress + m * c2.z;
+    rec[s] = make_float4(f0, f1, f2, m * vp.x);
+    rec[s + 1] = make_float4(m * vp.y, m * vp.z, a00, a01);
+    rec[s + 2] = make_float4(a02, a10, a11, a12);
+    rec[s + 3] = make_float4(a20, a21, a22, 0.0f);
     key = (b0 < K.ng && b1 < K.ng && b2 < K.ng) ? (b0 * K.ng + b1) * K.ng + b2 : K.G;  // K.G: outside
     cellkey[p] = key;
   }
@@ -112,62 +117,119 @@ __global__ void __launch_bounds__(MBS) k_mpm_settle(uint32_t n, const uint32_t *
     for (uint32_t t = o; t < e; ++t) r += tmp_id[t] < p ? 1u : 0u;
     slot = o + r;
   }
-  const size_t f = (size_t)slot * 5, q = (size_t)p * 5;
+  const size_t f = (size_t)slot * kRec, q = (size_t)p * kRec;
 #pragma unroll
-  for (int k = 0; k < 5; ++k) srec[f + k] = rec[q + k];
+  for (int k = 0; k < (int)kRec; ++k) srec[f + k] = rec[q + k];
 }
 
 // P2G as a gather (MPMShader.ts:4-62 summed per node) and the grid pass
-// (MPMShader.ts:163-191): one thread per node, x-major node index.
-__global__ void __launch_bounds__(MBS) k_mpm_p2g_grid(const uint32_t *__restrict__ offsets,
-                                                      const float4 *__restrict__ srec, float4 *__restrict__ grid,
-                                                      MpmK K) {
-  const uint32_t node = blockIdx.x * MBS + threadIdx.x;
-  if (node >= K.G) return;
-  const int I = (int)(node / (K.ng * K.ng)), J = (int)((node / K.ng) % K.ng), Kz = (int)(node % K.ng);
-  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, am = 0.0f;
-  for (int bx = I - 2; bx <= I; ++bx) {
-    if (bx < 0) continue;
-    const int oi = I - bx;
-    for (int by = J - 2; by <= J; ++by) {
-      if (by < 0) continue;
-      const int oj = J - by;
-      // the three cells bz = K-2..K of this (bx, by) are one slot range
-      const int bz0 = Kz - 2 < 0 ? 0 : Kz - 2;
-      const uint32_t c0 = ((uint32_t)bx * K.ng + (uint32_t)by) * K.ng;
-      const uint32_t t0 = offsets[c0 + (uint32_t)bz0], t1 = offsets[c0 + (uint32_t)Kz + 1];
-      uint32_t cell_end = offsets[c0 + (uint32_t)bz0 + 1];
-      int bz = bz0;
-      for (uint32_t t = t0; t < t1; ++t) {
-        while (t >= cell_end) {  // next non-empty cell of the range
-          ++bz;
-          cell_end = offsets[c0 + (uint32_t)bz + 1];
-        }
-        const int ok = Kz - bz;
-        const size_t q = (size_t)t * 5;
-        const float4 f = srec[q], pm = srec[q + 1], A0 = srec[q + 2], A1 = srec[q + 3], A2 = srec[q + 4];
-        float wx0, wx1, wx2, wy0, wy1, wy2, wz0, wz1, wz2;
-        bspline(f.x, wx0, wx1, wx2);
-        bspline(f.y, wy0, wy1, wy2);
-        bspline(f.z, wz0, wz1, wz2);
-        const float dpx = ((float)oi - f.x) * K.dx, dpy = ((float)oj - f.y) * K.dx, dpz = ((float)ok - f.z) * K.dx;
-        const float weight = (pick3(oi, wx0, wx1, wx2) * pick3(oj, wy0, wy1, wy2)) * pick3(ok, wz0, wz1, wz2);
-        const float ad0 = (A0.x * dpx + A1.x * dpy) + A2.x * dpz;
-        const float ad1 = (A0.y * dpx + A1.y * dpy) + A2.y * dpz;
-        const float ad2 = (A0.z * dpx + A1.z * dpy) + A2.z * dpz;
-        a0 += weight * (pm.x + ad0);
-        a1 += weight * (pm.y + ad1);
-        a2 += weight * (pm.z + ad2);
-        am += weight * K.p_mass;
+// (MPMShader.ts:163-191).  A work-group of nine wavefronts takes 256
+// consecutive nodes (z fastest); wavefront c sums base-cell column
+// c = (bx, by) for all of them, lane l for the run of four nodes
+// K0 = 4 (l mod ...) .. K0+3 in one node column.  The lane walks the cells
+// bz = K0-2 .. K0+3 of its column once, each cell's records in slot order
+// (four loads in flight), and adds every record into the partials of the
+// nodes it reaches (bz .. bz+2 inside the run) -- a record is loaded once
+// for three nodes and a node's partial still runs over its cells K-2..K in
+// slot order.  Then each node's nine partials are added in column order and
+// the grid pass applied.
+constexpr uint32_t kRun = 4;                // nodes per lane (consecutive z)
+constexpr uint32_t kP2GNodes = 64 * kRun;   // nodes per work-group
+constexpr uint32_t kP2GBS = 9 * 64;         // one wavefront per column
+
+__device__ __forceinline__ void p2g_run(const uint32_t *__restrict__ offsets, const float4 *__restrict__ srec,
+                                        const MpmK &K, uint32_t c0, int oi, int oj, int K0, float4 acc[kRun]) {
+#pragma unroll
+  for (int c = 0; c < (int)kRun + 2; ++c) {  // cell bz = K0 - 2 + c
+    const int bz = K0 - 2 + c;
+    if (bz < 0 || bz >= (int)K.ng) continue;
+    const uint32_t t0 = offsets[c0 + (uint32_t)bz], t1 = offsets[c0 + (uint32_t)bz + 1];
+    auto add = [&](const float4 &r0, const float4 &r1, const float4 &r2, const float4 &r3) {
+      float wx0, wx1, wx2, wy0, wy1, wy2, wz[3];
+      bspline(r0.x, wx0, wx1, wx2);
+      bspline(r0.y, wy0, wy1, wy2);
+      bspline(r0.z, wz[0], wz[1], wz[2]);
+      const float wxy = pick3(oi, wx0, wx1, wx2) * pick3(oj, wy0, wy1, wy2);
+      const float dpx = ((float)oi - r0.x) * K.dx, dpy = ((float)oj - r0.y) * K.dx;
+#pragma unroll
+      for (int ok = 0; ok < 3; ++ok) {
+        const int node = c - 2 + ok;  // index of the target node in the run
+        if (node < 0 || node >= (int)kRun) continue;
+        const float dpz = ((float)ok - r0.z) * K.dx;
+        const float weight = wxy * wz[ok];
+        // columns A0 = (r1.z, r1.w, r2.x), A1 = (r2.y, r2.z, r2.w), A2 = (r3.x, r3.y, r3.z)
+        const float ad0 = (r1.z * dpx + r2.y * dpy) + r3.x * dpz;
+        const float ad1 = (r1.w * dpx + r2.z * dpy) + r3.y * dpz;
+        const float ad2 = (r2.x * dpx + r2.w * dpy) + r3.z * dpz;
+        acc[node].x += weight * (r0.w + ad0);
+        acc[node].y += weight * (r1.x + ad1);
+        acc[node].z += weight * (r1.y + ad2);
+        acc[node].w += weight * K.p_mass;
       }
+    };
+    uint32_t t = t0;
+    for (; t + 4 <= t1; t += 4) {
+      float4 r[4][kRec];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int k = 0; k < (int)kRec; ++k) r[q][k] = srec[(size_t)(t + q) * kRec + k];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) add(r[q][0], r[q][1], r[q][2], r[q][3]);
+    }
+    for (; t < t1; ++t) {
+      const size_t b = (size_t)t * kRec;
+      add(srec[b], srec[b + 1], srec[b + 2], srec[b + 3]);
     }
   }
+}
+
+__global__ void __launch_bounds__(kP2GBS) k_mpm_p2g_grid(const uint32_t *__restrict__ offsets,
+                                                         const float4 *__restrict__ srec, float4 *__restrict__ grid,
+                                                         MpmK K) {
+  __shared__ float4 part[9 * kP2GNodes];
+  const uint32_t tid = threadIdx.x, lane = tid % 64u, col = tid / 64u;
+  // plain order: the work-groups in flight cover one contiguous stretch of
+  // nodes (an XCD-contiguous mapping, whole or in runs of 16 work-groups,
+  // measured 1.7-1.8x slower: the particles fill only part of the grid)
+  const uint32_t base = blockIdx.x * kP2GNodes;
+  {
+    // this lane's run: nodes base + kRun * lane .. + kRun - 1 (one node
+    // column: ng is a multiple of kRun)
+    const uint32_t n0 = base + kRun * lane;
+    float4 acc[kRun];
+#pragma unroll
+    for (int k = 0; k < (int)kRun; ++k) acc[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (n0 < K.G) {
+      const int I = (int)(n0 / (K.ng * K.ng)), J = (int)((n0 / K.ng) % K.ng), K0 = (int)(n0 % K.ng);
+      // column col: bx = I - 2 + col / 3, by = J - 2 + col % 3 (ascending linear index)
+      const int oi = 2 - (int)(col / 3u), oj = 2 - (int)(col % 3u);
+      const int bx = I - oi, by = J - oj;
+      if (bx >= 0 && by >= 0) p2g_run(offsets, srec, K, ((uint32_t)bx * K.ng + (uint32_t)by) * K.ng, oi, oj, K0, acc);
+    }
+#pragma unroll
+    for (int k = 0; k < (int)kRun; ++k) part[col * kP2GNodes + kRun * lane + k] = acc[k];
+  }
+  __syncthreads();
+  if (tid >= kP2GNodes) return;
+  const uint32_t node = base + tid;
+  if (node >= K.G) return;
+  const int I = (int)(node / (K.ng * K.ng)), J = (int)((node / K.ng) % K.ng), Kz = (int)(node % K.ng);
+  float4 a = part[tid];
+#pragma unroll
+  for (int c = 1; c < 9; ++c) {
+    const float4 q = part[c * kP2GNodes + tid];
+    a.x = a.x + q.x;
+    a.y = a.y + q.y;
+    a.z = a.z + q.z;
+    a.w = a.w + q.w;
+  }
   float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f, v3 = 0.0f;
-  if (am > 0.0f) {
-    v0 = a0 / am;
-    v1 = a1 / am;
-    v2 = a2 / am;
-    v3 = am / am;
+  if (a.w > 0.0f) {
+    v0 = a.x / a.w;
+    v1 = a.y / a.w;
+    v2 = a.z / a.w;
+    v3 = a.w / a.w;
   }
   v0 = v0 + K.dt * K.g[0];
   v1 = v1 + K.dt * K.g[1];
@@ -290,7 +352,8 @@ void enqueue_step(pbf_mpm *m) {
   hipLaunchKernelGGL(k_mpm_scatter, dim3(nblk(n)), dim3(MBS), 0, st, n, m->cellkey, m->rank, m->offsets, m->tmp_id);
   hipLaunchKernelGGL(k_mpm_settle, dim3(nblk(n)), dim3(MBS), 0, st, n, m->tmp_id, m->cellkey, m->offsets, m->rec,
                      m->srec, m->G);
-  hipLaunchKernelGGL(k_mpm_p2g_grid, dim3(nblk(m->G)), dim3(MBS), 0, st, m->offsets, m->srec, m->grid, m->K);
+  hipLaunchKernelGGL(k_mpm_p2g_grid, dim3((m->G + kP2GNodes - 1) / kP2GNodes), dim3(kP2GBS), 0, st, m->offsets,
+                     m->srec, m->grid, m->K);
   hipLaunchKernelGGL(k_mpm_g2p, dim3(nblk(n)), dim3(MBS), 0, st, n, m->x, m->v, m->C0, m->C1, m->C2, m->grid, m->K);
 }
 
@@ -326,7 +389,8 @@ int pbf_mpm_create(const pbf_mpm_params *params, pbf_mpm **out) try {
   if (p.capacity == 0) p.capacity = 9000;
   if (!std::isfinite(p.dt) || !std::isfinite(p.rho) || !std::isfinite(p.E) || !std::isfinite(p.gravity))
     return mfail(nullptr, PBF_E_INVALID, "dt, rho, E and gravity must be finite");
-  if (p.n_grid < 4 || p.n_grid > 1024) return mfail(nullptr, PBF_E_INVALID, "n_grid must be in [4, 1024]");
+  if (p.n_grid < 4 || p.n_grid > 1024 || p.n_grid % 4 != 0)
+    return mfail(nullptr, PBF_E_INVALID, "n_grid must be a multiple of 4 in [4, 1024]");
   if (2 * p.bound > p.n_grid) return mfail(nullptr, PBF_E_INVALID, "bound must be at most n_grid / 2");
   if (p.capacity > (1u << 27)) return mfail(nullptr, PBF_E_CAPACITY, "capacity above 2^27 particles");
   pbf_mpm *m = new pbf_mpm();
@@ -358,7 +422,7 @@ int pbf_mpm_create(const pbf_mpm_params *params, pbf_mpm **out) try {
   const size_t N = m->cap;
   step(dalloc(&m->x, N), "alloc"); step(dalloc(&m->v, N), "alloc");
   step(dalloc(&m->C0, N), "alloc"); step(dalloc(&m->C1, N), "alloc"); step(dalloc(&m->C2, N), "alloc");
-  step(dalloc(&m->rec, 5 * N), "alloc"); step(dalloc(&m->srec, 5 * N), "alloc");
+  step(dalloc(&m->rec, kRec * N), "alloc"); step(dalloc(&m->srec, kRec * N), "alloc");
   step(dalloc(&m->stage, N), "alloc");
   step(dalloc(&m->cellkey, N), "alloc"); step(dalloc(&m->rank, N), "alloc"); step(dalloc(&m->tmp_id, N), "alloc");
   step(dalloc(&m->grid, (size_t)m->G), "alloc");
