@@ -63,9 +63,7 @@ __global__ void __launch_bounds__(MBS) k_mpm_prepare(uint32_t n, const float4 *_
                                                      uint32_t *__restrict__ cellkey, uint32_t *__restrict__ rank,
                                                      uint32_t *__restrict__ counts, MpmK K) {
   const uint32_t p = blockIdx.x * MBS + threadIdx.x;
-  const bool active = p < n;
-  uint32_t key = 0;
-  if (active) {
+  if (p < n) {
     const float4 xp = x[p], vp = v[p], c0 = C0[p], c1 = C1[p], c2 = C2[p];
     const float J = vp.w * (1.0f + K.dt * ((c0.x + c1.y) + c2.z));
     v[p] = make_float4(vp.x, vp.y, vp.z, J);
@@ -84,11 +82,12 @@ __global__ void __launch_bounds__(MBS) k_mpm_prepare(uint32_t n, const float4 *_
     rec[s + 1] = make_float4(m * vp.y, m * vp.z, a00, a01);
     rec[s + 2] = make_float4(a02, a10, a11, a12);
     rec[s + 3] = make_float4(a20, a21, a22, 0.0f);
-    key = (b0 < K.ng && b1 < K.ng && b2 < K.ng) ? (b0 * K.ng + b1) * K.ng + b2 : K.G;  // K.G: outside
+    const uint32_t key = (b0 < K.ng && b1 < K.ng && b2 < K.ng) ? (b0 * K.ng + b1) * K.ng + b2 : K.G;  // K.G: outside
     cellkey[p] = key;
+    // one atomic per lane: the particles stay in the caller's order, so a
+    // wavefront's keys rarely repeat (the settle orders each cell by id)
+    rank[p] = atomicAdd(&counts[key], 1u);
   }
-  const uint32_t r = wave_atomic_rank(counts, key, active);
-  if (active) rank[p] = r;
 }
 
 __global__ void __launch_bounds__(MBS) k_mpm_scatter(uint32_t n, const uint32_t *__restrict__ cellkey,
@@ -123,24 +122,27 @@ __global__ void __launch_bounds__(MBS) k_mpm_settle(uint32_t n, const uint32_t *
 }
 
 // P2G as a gather (MPMShader.ts:4-62 summed per node) and the grid pass
-// (MPMShader.ts:163-191).  A work-group of nine wavefronts takes 256
+// (MPMShader.ts:163-191).  A work-group of nine wavefronts takes 64 RUN
 // consecutive nodes (z fastest); wavefront c sums base-cell column
-// c = (bx, by) for all of them, lane l for the run of four nodes
-// K0 = 4 (l mod ...) .. K0+3 in one node column.  The lane walks the cells
-// bz = K0-2 .. K0+3 of its column once, each cell's records in slot order
-// (four loads in flight), and adds every record into the partials of the
-// nodes it reaches (bz .. bz+2 inside the run) -- a record is loaded once
-// for three nodes and a node's partial still runs over its cells K-2..K in
-// slot order.  Then each node's nine partials are added in column order and
-// the grid pass applied.
-constexpr uint32_t kRun = 4;                // nodes per lane (consecutive z)
-constexpr uint32_t kP2GNodes = 64 * kRun;   // nodes per work-group
-constexpr uint32_t kP2GBS = 9 * 64;         // one wavefront per column
+// c = (bx, by) for all of them, lane l for the run of RUN nodes from
+// K0 = RUN l (mod n_grid) in one node column.  The lane walks the cells
+// bz = K0-2 .. K0+RUN-1 of its column once, each cell's records in slot
+// order (four loads in flight), and adds every record into the partials of
+// the nodes it reaches (bz .. bz+2 inside the run) -- a record is loaded
+// once for up to three nodes and a node's partial still runs over its cells
+// K-2..K in slot order.  Then each node's nine partials are added in column
+// order and the grid pass applied.
+// RUN nodes per lane (consecutive z): 4 on large grids (a record loaded once
+// for three nodes), 1 on small ones (more lanes for few particles: 13 vs 28
+// us per step at the reference's 9,000 particles on 32^3 nodes).
+constexpr uint32_t kP2GBS = 9 * 64;  // one wavefront per column
+constexpr uint32_t kSmallGrid = 1u << 16;
 
+template <int RUN>
 __device__ __forceinline__ void p2g_run(const uint32_t *__restrict__ offsets, const float4 *__restrict__ srec,
-                                        const MpmK &K, uint32_t c0, int oi, int oj, int K0, float4 acc[kRun]) {
+                                        const MpmK &K, uint32_t c0, int oi, int oj, int K0, float4 acc[RUN]) {
 #pragma unroll
-  for (int c = 0; c < (int)kRun + 2; ++c) {  // cell bz = K0 - 2 + c
+  for (int c = 0; c < RUN + 2; ++c) {  // cell bz = K0 - 2 + c
     const int bz = K0 - 2 + c;
     if (bz < 0 || bz >= (int)K.ng) continue;
     const uint32_t t0 = offsets[c0 + (uint32_t)bz], t1 = offsets[c0 + (uint32_t)bz + 1];
@@ -154,7 +156,7 @@ __device__ __forceinline__ void p2g_run(const uint32_t *__restrict__ offsets, co
 #pragma unroll
       for (int ok = 0; ok < 3; ++ok) {
         const int node = c - 2 + ok;  // index of the target node in the run
-        if (node < 0 || node >= (int)kRun) continue;
+        if (node < 0 || node >= RUN) continue;
         const float dpz = ((float)ok - r0.z) * K.dx;
         const float weight = wxy * wz[ok];
         // columns A0 = (r1.z, r1.w, r2.x), A1 = (r2.y, r2.z, r2.w), A2 = (r3.x, r3.y, r3.z)
@@ -184,9 +186,11 @@ __device__ __forceinline__ void p2g_run(const uint32_t *__restrict__ offsets, co
   }
 }
 
+template <int RUN>
 __global__ void __launch_bounds__(kP2GBS) k_mpm_p2g_grid(const uint32_t *__restrict__ offsets,
                                                          const float4 *__restrict__ srec, float4 *__restrict__ grid,
                                                          MpmK K) {
+  constexpr uint32_t kP2GNodes = 64u * RUN;  // nodes per work-group
   __shared__ float4 part[9 * kP2GNodes];
   const uint32_t tid = threadIdx.x, lane = tid % 64u, col = tid / 64u;
   // plain order: the work-groups in flight cover one contiguous stretch of
@@ -194,21 +198,22 @@ __global__ void __launch_bounds__(kP2GBS) k_mpm_p2g_grid(const uint32_t *__restr
   // measured 1.7-1.8x slower: the particles fill only part of the grid)
   const uint32_t base = blockIdx.x * kP2GNodes;
   {
-    // this lane's run: nodes base + kRun * lane .. + kRun - 1 (one node
-    // column: ng is a multiple of kRun)
-    const uint32_t n0 = base + kRun * lane;
-    float4 acc[kRun];
+    // this lane's run: nodes base + RUN * lane .. + RUN - 1 (one node
+    // column: ng is a multiple of RUN)
+    const uint32_t n0 = base + RUN * lane;
+    float4 acc[RUN];
 #pragma unroll
-    for (int k = 0; k < (int)kRun; ++k) acc[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int k = 0; k < RUN; ++k) acc[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (n0 < K.G) {
       const int I = (int)(n0 / (K.ng * K.ng)), J = (int)((n0 / K.ng) % K.ng), K0 = (int)(n0 % K.ng);
       // column col: bx = I - 2 + col / 3, by = J - 2 + col % 3 (ascending linear index)
       const int oi = 2 - (int)(col / 3u), oj = 2 - (int)(col % 3u);
       const int bx = I - oi, by = J - oj;
-      if (bx >= 0 && by >= 0) p2g_run(offsets, srec, K, ((uint32_t)bx * K.ng + (uint32_t)by) * K.ng, oi, oj, K0, acc);
+      if (bx >= 0 && by >= 0)
+        p2g_run<RUN>(offsets, srec, K, ((uint32_t)bx * K.ng + (uint32_t)by) * K.ng, oi, oj, K0, acc);
     }
 #pragma unroll
-    for (int k = 0; k < (int)kRun; ++k) part[col * kP2GNodes + kRun * lane + k] = acc[k];
+    for (int k = 0; k < RUN; ++k) part[col * kP2GNodes + RUN * lane + k] = acc[k];
   }
   __syncthreads();
   if (tid >= kP2GNodes) return;
@@ -352,8 +357,12 @@ void enqueue_step(pbf_mpm *m) {
   hipLaunchKernelGGL(k_mpm_scatter, dim3(nblk(n)), dim3(MBS), 0, st, n, m->cellkey, m->rank, m->offsets, m->tmp_id);
   hipLaunchKernelGGL(k_mpm_settle, dim3(nblk(n)), dim3(MBS), 0, st, n, m->tmp_id, m->cellkey, m->offsets, m->rec,
                      m->srec, m->G);
-  hipLaunchKernelGGL(k_mpm_p2g_grid, dim3((m->G + kP2GNodes - 1) / kP2GNodes), dim3(kP2GBS), 0, st, m->offsets,
-                     m->srec, m->grid, m->K);
+  if (m->G <= kSmallGrid)
+    hipLaunchKernelGGL(k_mpm_p2g_grid<1>, dim3((m->G + 63u) / 64u), dim3(kP2GBS), 0, st, m->offsets, m->srec,
+                       m->grid, m->K);
+  else
+    hipLaunchKernelGGL(k_mpm_p2g_grid<4>, dim3((m->G + 255u) / 256u), dim3(kP2GBS), 0, st, m->offsets, m->srec,
+                       m->grid, m->K);
   hipLaunchKernelGGL(k_mpm_g2p, dim3(nblk(n)), dim3(MBS), 0, st, n, m->x, m->v, m->C0, m->C1, m->C2, m->grid, m->K);
 }
 
