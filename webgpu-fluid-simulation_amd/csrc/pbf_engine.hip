@@ -520,7 +520,8 @@ int run_learned_substep(pbf_ctx *c, float dt, uint32_t iters) {
   });
   if (rc != PBF_OK) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (*static_cast<volatile uint32_t *>(c->decision)) {
+  const bool rejected = *static_cast<volatile uint32_t *>(c->decision) != 0u;
+  if (rejected) {
     rc = run_enqueued(c, dt, fb_iters, false, M_COLLECT | M_RESOLVE, 0u, [&](const LaunchCtx &L, PassTimer &T) {
       copy_state(c, false, false);
       enqueue_substep(c, L, fb_iters, false, T, M_COLLECT);
@@ -529,7 +530,13 @@ int run_learned_substep(pbf_ctx *c, float dt, uint32_t iters) {
   }
   if (c->record_diag) {
     LaunchCtx L = make_launch(c, dt);
-    launch_diag(L, c->pos, c->nbr, c->ncount, c->ci, c->ranges, ns, c->per_scene, c->hist, c->hist_cap, c->hist_ctr);
+    // an accepted substep's density error is the check's own (per_scene,
+    // left by its diagnostic walk); a re-solved one is measured again
+    if (rejected)
+      launch_diag(L, c->pos, c->nbr, c->ncount, c->ci, c->ranges, ns, c->per_scene, c->hist, c->hist_cap,
+                  c->hist_ctr);
+    else
+      launch_diag_history(L, c->per_scene, ns, c->hist, c->hist_cap, c->hist_ctr);
     HIPCHK(hipGetLastError());
   }
   return PBF_OK;

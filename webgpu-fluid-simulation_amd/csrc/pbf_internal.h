@@ -94,6 +94,10 @@ void launch_xsph(const LaunchCtx &L, const float4 *pos, const float4 *om, const 
 void launch_diag(const LaunchCtx &L, const float4 *pos, const uint32_t *nbr, const uint32_t *ncount, double *ci,
                  const uint32_t *ranges, uint32_t nscenes, double *per_scene, double *hist, uint32_t hist_cap,
                  uint32_t *hist_ctr);
+// Appends the mean / max over the per-scene totals of an earlier launch_diag
+// to the history (the same entry launch_diag with a history writes).
+void launch_diag_history(const LaunchCtx &L, const double *per_scene, uint32_t nscenes, double *hist,
+                         uint32_t hist_cap, uint32_t *hist_ctr);
 void launch_unsort4(hipStream_t st, uint32_t n, const float4 *src, const uint32_t *ids, float4 *dst, int clear_w);
 void launch_unsort1(hipStream_t st, uint32_t n, const float *src, const uint32_t *ids, float4 *dst);
 void launch_sort4(hipStream_t st, uint32_t n, const float4 *src, const uint32_t *ids, float4 *dst, int clear_w);

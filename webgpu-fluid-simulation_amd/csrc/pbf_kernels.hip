@@ -1539,6 +1539,11 @@ void launch_diag(const LaunchCtx &L, const float4 *pos, const uint32_t *nbr, con
        hist_cap, hist_ctr);
 }
 
+void launch_diag_history(const LaunchCtx &L, const double *per_scene, uint32_t nscenes, double *hist,
+                         uint32_t hist_cap, uint32_t *hist_ctr) {
+  go(L.stream, L.ev0, L.ev1, k_diag_history, dim3(1), dim3(64), per_scene, nscenes, L.n, hist, hist_cap, hist_ctr);
+}
+
 void launch_speed(const LaunchCtx &L, const float4 *vel, double *ci, const uint32_t *ranges, uint32_t nscenes,
                   double *per_scene) {
   if (!L.n) return;
