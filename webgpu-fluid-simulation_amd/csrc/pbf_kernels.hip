@@ -573,11 +573,13 @@ __global__ void __launch_bounds__(BS) k_smooth_warm(uint32_t n, const float4 *__
   if (i >= n) return;
   const float4 xi = X[i];
   float sw = 0.0f, sl = 0.0f;
-  for_each_neighbor(nbr, i, P, ncount[i], [&](uint32_t j) {
-    const float4 xj = X[j];
+  // wavefront-uniform walk: padding entries (valid false) leave both sums
+  // as they are, so they equal the per-lane sums over the list
+  const Gather GX(X, n);
+  for_each_neighbor_w<8>(nbr, i, P, ncount[i], [&](uint32_t j) { return GX[j]; }, [&](const float4 &xj, bool valid) {
     const float w = poly6(P, dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z));
-    sw += w;
-    sl = fmaf(w, xj.w, sl);
+    sw = valid ? sw + w : sw;
+    sl = valid ? fmaf(w, xj.w, sl) : sl;
   });
   tmp[i] = sw > 0.0f ? sl / sw : xi.w;
 }
@@ -1250,10 +1252,10 @@ __global__ void k_fallback_gate(const double *__restrict__ err, const double *__
                                 const double *__restrict__ e0, const double *__restrict__ e1, uint32_t nscenes,
                                 uint32_t n, const double *__restrict__ threshold, uint32_t *__restrict__ decision,
                                 uint32_t *__restrict__ count) {
-  if (threadIdx.x != 0) return;
+  // one wavefront: lane l takes scenes l, l + 64, ..., then a fixed butterfly
   double t = 0.0, v = 0.0, k0 = 0.0, E0 = 0.0, E1 = 0.0;
   bool finite = true;
-  for (uint32_t s = 0; s < nscenes; ++s) {
+  for (uint32_t s = threadIdx.x; s < nscenes; s += 64u) {
     t += err[2 * s];
     // a NaN speed drops out of the max but not out of the sum
     finite = finite && speed[2 * s] == speed[2 * s];
@@ -1263,6 +1265,17 @@ __global__ void k_fallback_gate(const double *__restrict__ err, const double *__
     E0 += e0[2 * s] + e0[2 * s + 1];
     E1 += e1[2 * s] + e1[2 * s + 1];
   }
+  finite = !__any(!finite);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    t += __shfl_xor(t, o, 64);
+    const double w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+    k0 += __shfl_xor(k0, o, 64);
+    E0 += __shfl_xor(E0, o, 64);
+    E1 += __shfl_xor(E1, o, 64);
+  }
+  if (threadIdx.x != 0) return;
   const double mean = n ? t / (double)n : 0.0;
   finite = finite && E1 - E1 == 0.0;
   const bool bad = !finite || !(mean <= *threshold) || !(v <= 2.0) || E1 - E0 > 1e-3 * k0;
@@ -1271,14 +1284,22 @@ __global__ void k_fallback_gate(const double *__restrict__ err, const double *__
 }
 
 // Sum the per-scene totals into history[ctr % cap] = (mean, max) over all.
-__global__ void k_diag_history(const double *__restrict__ per_scene, uint32_t nscenes, uint32_t n,
-                               double *__restrict__ hist, uint32_t cap, uint32_t *__restrict__ ctr) {
-  if (threadIdx.x != 0) return;
+// One wavefront: lane l sums scenes l, l + 64, ..., then a fixed butterfly.
+__global__ void __launch_bounds__(64) k_diag_history(const double *__restrict__ per_scene, uint32_t nscenes,
+                                                     uint32_t n, double *__restrict__ hist, uint32_t cap,
+                                                     uint32_t *__restrict__ ctr) {
   double t = 0.0, m = 0.0;
-  for (uint32_t s = 0; s < nscenes; ++s) {
+  for (uint32_t s = threadIdx.x; s < nscenes; s += 64u) {
     t += per_scene[2 * s];
     m = per_scene[2 * s + 1] > m ? per_scene[2 * s + 1] : m;
   }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    t += __shfl_xor(t, o, 64);
+    const double w = __shfl_xor(m, o, 64);
+    m = w > m ? w : m;
+  }
+  if (threadIdx.x != 0) return;
   uint32_t c = *ctr;
   hist[2 * (c % cap)] = n ? t / (double)n : 0.0;
   hist[2 * (c % cap) + 1] = m;
