@@ -1,7 +1,8 @@
 """Edge cases of the GPU path against the oracle: empty and single-particle
 systems, ragged counts around wavefront / work-group / scan-block sizes,
-particles on cell faces and domain walls, crowded cells, a three-level scan
-(16.7 M cells), and the capacity limit.  All comparisons are bit-exact."""
+particles on cell faces and domain walls, crowded cells and cells too full
+for the 8-bit z-slab words, a three-level scan (16.7 M cells), and the
+capacity limit.  All comparisons are bit-exact."""
 import numpy as np
 import pytest
 
@@ -214,4 +215,43 @@ def test_boundary_field_limits():
     sim.setVolumeMap((1, 1, 1), (0, 0, 0), (1, 1, 1), one)
     sim.step(1.0 / 60.0, 1, 2)
     sim.sync()
+    sim.close()
+
+
+@pytest.mark.parametrize("slabs", [4, 2])
+def test_cells_past_the_slab_word_range(slabs):
+    """Cells of 254, 255, 256 and 400 particles: the z-slab word counts to
+    254 and marks fuller cells as untrimmed; their neighbours' lists (order
+    included) and the counts still equal the oracle's, with crowded cells as
+    the first, middle and last cell of a column range."""
+    rng = np.random.default_rng(21)
+    parts = []
+
+    def cell(c, k):  # k particles strictly inside cell c (h = 0.1), clear of its faces
+        parts.append(np.asarray(c, np.float32) * 0.1 + rng.uniform(0.003, 0.097, (k, 3)).astype(np.float32))
+
+    cell((5, 5, 5), 400)
+    cell((5, 5, 6), 256)   # the next cell of the same column
+    cell((5, 5, 4), 60)
+    cell((8, 5, 5), 255)
+    cell((5, 8, 5), 254)
+    cell((8, 8, 8), 256)
+    cell((8, 8, 9), 40)
+    parts.append(rng.uniform(0.1, 1.5, (3000, 3)).astype(np.float32))
+    pts = np.concatenate(parts)
+    pos = np.zeros((len(pts), 4), np.float32)
+    pos[:, :3] = pts
+    sc = _scene(pos, name="saturated", field=False)
+    print(f"[saturated cells] n={sc.n} slabs={slabs}", flush=True)
+    sim = scenes.make_engine(sc, max_neighbors=2000, cell_slabs=slabs)
+    o = make_oracle(sc, cell_slabs=slabs)
+    dt = sc.substep_dt()
+    sim.search(dt)
+    o.search(dt)
+    gc, go = sim.read_cells()
+    oc, oo = o.cells()
+    assert gc.max() >= 400 and np.array_equal(gc, oc) and np.array_equal(go, oo)
+    goff, gids = sim.read_neighbors()
+    ooff, oids = o.neighbors()
+    assert np.array_equal(goff, ooff) and np.array_equal(gids, oids)
     sim.close()
