@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(MBS) k_mpm_settle(uint32_t n, const uint32_t *
                                                     const uint32_t *__restrict__ cellkey,
                                                     const uint32_t *__restrict__ offsets,
                                                     const float4 *__restrict__ rec, float4 *__restrict__ srec,
-                                                    uint32_t outside) {
+                                                    uint32_t *__restrict__ order, uint32_t outside) {
   const uint32_t s = blockIdx.x * MBS + threadIdx.x;
   if (s >= n) return;
   const uint32_t p = tmp_id[s], c = cellkey[p];
@@ -119,6 +119,7 @@ __global__ void __launch_bounds__(MBS) k_mpm_settle(uint32_t n, const uint32_t *
   const size_t f = (size_t)slot * kRec, q = (size_t)p * kRec;
 #pragma unroll
   for (int k = 0; k < (int)kRec; ++k) srec[f + k] = rec[q + k];
+  order[slot] = p;
 }
 
 // P2G as a gather (MPMShader.ts:4-62 summed per node) and the grid pass
@@ -248,12 +249,16 @@ __global__ void __launch_bounds__(kP2GBS) k_mpm_p2g_grid(const uint32_t *__restr
 }
 
 // G2P (MPMShader.ts:240-280): 27 nodes in the shader's order (i outer, k
-// inner); nodes outside the grid read as zero.
+// inner); nodes outside the grid read as zero.  Thread s takes particle
+// order[s] (the cell order of the step's sort, so a wavefront's node
+// gathers fall on neighbouring nodes), or s when order is null.
 __global__ void __launch_bounds__(MBS) k_mpm_g2p(uint32_t n, float4 *x, float4 *v, float4 *__restrict__ C0,
                                                  float4 *__restrict__ C1, float4 *__restrict__ C2,
-                                                 const float4 *__restrict__ grid, MpmK K) {
-  const uint32_t p = blockIdx.x * MBS + threadIdx.x;
-  if (p >= n) return;
+                                                 const float4 *__restrict__ grid, const uint32_t *__restrict__ order,
+                                                 MpmK K) {
+  const uint32_t s = blockIdx.x * MBS + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t p = order ? order[s] : s;
   const float4 xp = x[p];
   const float X[3] = {xp.x / K.dx, xp.y / K.dx, xp.z / K.dx};
   uint32_t b[3];
@@ -327,6 +332,7 @@ struct pbf_mpm {
   float4 *x = nullptr, *v = nullptr, *C0 = nullptr, *C1 = nullptr, *C2 = nullptr, *grid = nullptr;
   float4 *rec = nullptr, *srec = nullptr, *stage = nullptr;
   uint32_t *cellkey = nullptr, *rank = nullptr, *counts = nullptr, *offsets = nullptr, *tmp_id = nullptr;
+  uint32_t *order = nullptr;  // slot (cell order of the step) -> particle
   std::vector<uint32_t *> scan_sums, scan_prefix;
   hipGraphExec_t graph = nullptr;
   uint32_t graph_n = 0;
@@ -356,14 +362,15 @@ void enqueue_step(pbf_mpm *m) {
   launch_scan(st, m->counts, m->G + 1, m->offsets, 1, m->scan_sums.data(), m->scan_prefix.data(), 0);
   hipLaunchKernelGGL(k_mpm_scatter, dim3(nblk(n)), dim3(MBS), 0, st, n, m->cellkey, m->rank, m->offsets, m->tmp_id);
   hipLaunchKernelGGL(k_mpm_settle, dim3(nblk(n)), dim3(MBS), 0, st, n, m->tmp_id, m->cellkey, m->offsets, m->rec,
-                     m->srec, m->G);
+                     m->srec, m->order, m->G);
   if (m->G <= kSmallGrid)
     hipLaunchKernelGGL(k_mpm_p2g_grid<1>, dim3((m->G + 63u) / 64u), dim3(kP2GBS), 0, st, m->offsets, m->srec,
                        m->grid, m->K);
   else
     hipLaunchKernelGGL(k_mpm_p2g_grid<4>, dim3((m->G + 255u) / 256u), dim3(kP2GBS), 0, st, m->offsets, m->srec,
                        m->grid, m->K);
-  hipLaunchKernelGGL(k_mpm_g2p, dim3(nblk(n)), dim3(MBS), 0, st, n, m->x, m->v, m->C0, m->C1, m->C2, m->grid, m->K);
+  hipLaunchKernelGGL(k_mpm_g2p, dim3(nblk(n)), dim3(MBS), 0, st, n, m->x, m->v, m->C0, m->C1, m->C2, m->grid,
+                     (const uint32_t *)m->order, m->K);
 }
 
 void drop_graph(pbf_mpm *m) {
@@ -434,6 +441,7 @@ int pbf_mpm_create(const pbf_mpm_params *params, pbf_mpm **out) try {
   step(dalloc(&m->rec, kRec * N), "alloc"); step(dalloc(&m->srec, kRec * N), "alloc");
   step(dalloc(&m->stage, N), "alloc");
   step(dalloc(&m->cellkey, N), "alloc"); step(dalloc(&m->rank, N), "alloc"); step(dalloc(&m->tmp_id, N), "alloc");
+  step(dalloc(&m->order, N), "alloc");
   step(dalloc(&m->grid, (size_t)m->G), "alloc");
   step(dalloc(&m->counts, (size_t)m->G + 1), "alloc");
   step(dalloc(&m->offsets, (size_t)m->G + 2), "alloc");
@@ -470,7 +478,7 @@ int pbf_mpm_destroy(pbf_mpm *m) {
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   drop_graph(m);
   void *bufs[] = {m->x, m->v, m->C0, m->C1, m->C2, m->grid, m->rec, m->srec, m->stage, m->cellkey, m->rank,
-                  m->counts, m->offsets, m->tmp_id};
+                  m->counts, m->offsets, m->tmp_id, m->order};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto q : m->scan_sums)
@@ -604,7 +612,7 @@ int pbf_mpm_g2p(pbf_mpm *m) try {
   if (!m->n) return PBF_OK;
   MCHK(hipSetDevice(m->device));
   hipLaunchKernelGGL(k_mpm_g2p, dim3(nblk(m->n)), dim3(MBS), 0, m->stream, m->n, m->x, m->v, m->C0, m->C1, m->C2,
-                     m->grid, m->K);
+                     m->grid, (const uint32_t *)nullptr, m->K);
   MCHK(hipGetLastError());
   return PBF_OK;
 }
