@@ -211,3 +211,36 @@ def test_ratio_network_gain_on_the_256_scene_batch():
     assert e12 <= 0.005 < e4
     assert enet <= 0.005 and fb <= 5
     assert dnet <= 2 * d12 + 0.005
+
+
+@pytest.mark.timeout(600)
+def test_200_step_density_statistics_dam_break_1m():
+    """The 1M dam break (128 x 128 x 64, vorticity, tension, 128^3 map), 200
+    steps of 2 substeps: the mean (over substeps) of the mean density error
+    and the largest max error agree with the oracle's within 2 % -- the
+    north-star bar for a 200-step run; the FP64 statistics of every substep
+    are compared too (the engine follows the oracle bit for bit, so they
+    agree to the diagnostics' own summation order)."""
+    sc = scenes.dam_break_1m()
+    steps = 200
+    print(f"[200 steps 1M] n={sc.n}", flush=True)
+    sim = scenes.make_engine(sc)
+    sim.record_density_error(True)
+    for _ in range(steps):
+        sim.step(sc.dt, sc.substeps, sc.iterations)
+    g = sim.density_history(cap=4096)
+    sim.close()
+    o = make_oracle(sc)
+    oh = []
+    for s in range(steps * sc.substeps):
+        o.substep(sc.substep_dt(), sc.iterations)
+        oh.append(o.density_error())
+        if s % 100 == 99:
+            print(f"[200 steps 1M] oracle substep {s + 1}", flush=True)
+    oh = np.array(oh)
+    assert g.shape == oh.shape
+    gm, om = g[:, 0].mean(), oh[:, 0].mean()
+    gx, ox = g[:, 1].max(), oh[:, 1].max()
+    print(f"[200 steps 1M] mean error gpu {gm:.6g} cpu {om:.6g}; max gpu {gx:.6g} cpu {ox:.6g}", flush=True)
+    assert gm == pytest.approx(om, rel=0.02) and gx == pytest.approx(ox, rel=0.02)
+    assert np.allclose(g, oh, rtol=1e-9, atol=1e-15)
