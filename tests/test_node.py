@@ -1,6 +1,6 @@
-"""The Node-API layer: the addon loads and exposes the reference-shaped API
-(CPU); the Cube Drop flow run from JavaScript equals the same flow run from
-Python through the same C ABI (GPU)."""
+"""The Node-API layer: the addon loads and exposes the reference-shaped API,
+PBF and MPM (CPU); the Cube Drop flow and an MPM run from JavaScript equal the
+same runs from Python through the same C ABI (GPU)."""
 import json
 import os
 import shutil
@@ -67,3 +67,47 @@ def test_node_cube_drop_equals_python(tmp_path):
     for _ in range(3):
         sim.run()
     assert np.array_equal(pj, sim.read_positions())
+
+
+@needs_node
+def test_addon_exports_the_mpm_class():
+    js = ("const a=require(%r); const {MPM}=require(%r);"
+          "let msg=''; try { new MPM({nGrid: 6}); } catch (e) { msg = e.message; }"
+          "console.log(JSON.stringify({fns:Object.keys(a), methods:Object.getOwnPropertyNames(MPM.prototype),"
+          "msg}))") % (ADDON, os.path.join(NODE_DIR, "MPM.js"))
+    out = json.loads(subprocess.run([NODE, "-e", js], capture_output=True, text=True, check=True).stdout)
+    for f in ("mpmCreate", "mpmSetPositions", "mpmStep", "mpmRead", "mpmSync"):
+        assert f in out["fns"], f
+    for m in ("initResource", "setParticlePosition", "run", "readPositions", "readVelocities", "readAffine",
+              "readGrid"):
+        assert m in out["methods"], m
+    assert "n_grid must be a multiple of 4" in out["msg"]
+
+
+@needs_node
+@pytest.mark.gpu
+def test_node_mpm_equals_python(tmp_path):
+    """MPM.js (Node-API) and pbf_host.MPM (ctypes) on the same 9,000
+    positions, 15 steps: identical positions, velocities and C."""
+    from pbf_host import MPM, MPM_AFFINE, MPM_POSITION, MPM_VELOCITY
+    rng = np.random.default_rng(2)
+    x = np.zeros((9000, 4), np.float32)
+    x[:, :3] = (rng.random((9000, 3)) * 0.4 + 0.15).astype(np.float32)
+    fin, fout = tmp_path / "x.bin", tmp_path / "out.bin"
+    x.tofile(fin)
+    js = ("const fs=require('fs'); const {MPM}=require(%r);"
+          "const b=fs.readFileSync(%r); const x=new Float32Array(b.buffer, b.byteOffset, b.length/4);"
+          "const m=new MPM(); m.setParticlePosition(new Float32Array(x)); m.run(15);"
+          "const p=m.readPositions(), v=m.readVelocities(), c=m.readAffine();"
+          "const o=new Float32Array(p.length+v.length+c.length); o.set(p,0); o.set(v,p.length);"
+          "o.set(c,p.length+v.length); fs.writeFileSync(%r, Buffer.from(o.buffer));"
+          "console.log('ok')") % (os.path.join(NODE_DIR, "MPM.js"), str(fin), str(fout))
+    res = subprocess.run([NODE, "-e", js], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    o = np.fromfile(fout, np.float32)
+    sim = MPM()
+    sim.setParticlePosition(x)
+    sim.run(15)
+    want = np.concatenate([sim.read(MPM_POSITION).ravel(), sim.read(MPM_VELOCITY).ravel(), sim.read(MPM_AFFINE).ravel()])
+    sim.close()
+    assert np.array_equal(o, want)

@@ -475,6 +475,104 @@ napi_value Version(napi_env env, napi_callback_info) {
   return r;
 }
 
+// --- MLS-MPM (pbf_mpm_*): handles are externals of pbf_mpm*.
+void finalize_mpm(napi_env, void *data, void *) { pbf_mpm_destroy(static_cast<pbf_mpm *>(data)); }
+
+pbf_mpm *mpm_of(napi_env env, napi_value v) {
+  void *p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, nullptr, "expected an MPM context handle");
+    return nullptr;
+  }
+  return static_cast<pbf_mpm *>(p);
+}
+
+napi_value throw_mpm(napi_env env, pbf_mpm *m, int rc) {
+  std::string msg = pbf_mpm_last_error(m);
+  if (msg.empty()) msg = "pbf_mpm error " + std::to_string(rc);
+  napi_throw_error(env, nullptr, msg.c_str());
+  return nullptr;
+}
+
+#define MPM_OR_RETURN(idx)                     \
+  Args a;                                      \
+  if (!get_args(env, info, a)) return nullptr; \
+  pbf_mpm *m = mpm_of(env, a.v[idx]);          \
+  if (!m) return nullptr;
+
+#define MPM_CHECK(rc)                                  \
+  do {                                                 \
+    int r_ = (rc);                                     \
+    if (r_ != PBF_OK) return throw_mpm(env, m, r_);    \
+  } while (0)
+
+// mpmCreate({capacity, nGrid, bound, dt, rho, E, gravity, flags}) -> handle
+// (the reference's MPM constructor, MPM.ts:60-83)
+napi_value MpmCreate(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a) || a.n < 1) return throw_mpm(env, nullptr, PBF_E_INVALID);
+  pbf_mpm_params p;
+  std::memset(&p, 0, sizeof(p));
+  p.capacity = as_u32(prop(env, a.v[0], "capacity", 9000));
+  p.n_grid = as_u32(prop(env, a.v[0], "nGrid", 32));
+  p.bound = as_u32(prop(env, a.v[0], "bound", 3));
+  p.flags = as_u32(prop(env, a.v[0], "flags", 0));
+  p.dt = prop(env, a.v[0], "dt", 8e-4);
+  p.rho = prop(env, a.v[0], "rho", 1.0);
+  p.E = prop(env, a.v[0], "E", 400.0);
+  p.gravity = prop(env, a.v[0], "gravity", 9.8);
+  pbf_mpm *m = nullptr;
+  const int rc = pbf_mpm_create(&p, &m);
+  if (rc != PBF_OK) return throw_mpm(env, nullptr, rc);
+  napi_value ext;
+  NAPI_CALL(env, napi_create_external(env, m, finalize_mpm, nullptr, &ext));
+  return ext;
+}
+
+// mpmSetPositions(handle, Float32Array xyzw) (MPM.initResource, MPM.ts:104-121)
+napi_value MpmSetPositions(napi_env env, napi_callback_info info) {
+  MPM_OR_RETURN(0);
+  float *d = nullptr;
+  size_t len = 0;
+  if (a.n < 2 || !float_array(env, a.v[1], &d, &len)) return nullptr;
+  if (len % 4 != 0 || len / 4 > 0xffffffffu) return throw_mpm(env, m, PBF_E_LENGTH);
+  MPM_CHECK(pbf_mpm_set_positions(m, d, (uint32_t)(len / 4)));
+  return undefined(env);
+}
+
+// mpmStep(handle, steps) (MPM.run, MPM.ts:204-238, `steps` times)
+napi_value MpmStep(napi_env env, napi_callback_info info) {
+  MPM_OR_RETURN(0);
+  MPM_CHECK(pbf_mpm_step(m, a.n > 1 ? as_u32(num(env, a.v[1], 1)) : 1u));
+  return undefined(env);
+}
+
+// mpmRead(handle, field[, nGrid]) -> Float32Array (PBF_MPM_POSITION /
+// VELOCITY: 4 per particle, AFFINE: 9 per particle, GRID: 4 per node, of
+// nGrid^3 nodes)
+napi_value MpmRead(napi_env env, napi_callback_info info) {
+  MPM_OR_RETURN(0);
+  const int field = (int)as_u32(num(env, a.v[1]));
+  size_t count = pbf_mpm_particle_count(m), per = 4;
+  if (field == PBF_MPM_AFFINE) per = 9;
+  if (field == PBF_MPM_GRID) {
+    const uint32_t g = a.n > 2 ? as_u32(num(env, a.v[2])) : 0u;
+    if (g == 0 || g > 1024) return throw_mpm(env, m, PBF_E_INVALID);
+    count = (size_t)g * g * g;
+  }
+  float *d = nullptr;
+  napi_value arr = new_float_array(env, per * count, &d);
+  if (!arr) return nullptr;
+  MPM_CHECK(pbf_mpm_read(m, field, d, count));
+  return arr;
+}
+
+napi_value MpmSync(napi_env env, napi_callback_info info) {
+  MPM_OR_RETURN(0);
+  MPM_CHECK(pbf_mpm_sync(m));
+  return undefined(env);
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   struct {
     const char *name;
@@ -498,7 +596,12 @@ napi_value Init(napi_env env, napi_value exports) {
              {"readField", ReadField},
              {"densityError", DensityError},
              {"particleCount", ParticleCount},
-             {"version", Version}};
+             {"version", Version},
+             {"mpmCreate", MpmCreate},
+             {"mpmSetPositions", MpmSetPositions},
+             {"mpmStep", MpmStep},
+             {"mpmRead", MpmRead},
+             {"mpmSync", MpmSync}};
   for (auto &f : fns) {
     napi_value v;
     napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.fn, nullptr, &v);
