@@ -583,6 +583,12 @@ def main():
     whole = substep_bytes(N, C, sc.iterations, B) * substeps / el / 1e9
     copy_gbs = copy_bandwidth(1 << 30, 20)
     log(f"streaming copy probe: {copy_gbs:.0f} GB/s (read + write, 1 GiB)")
+    if roof:
+        # the same fractions against the bandwidth a streaming copy reaches on
+        # this device (the measured denominator beside the 8 TB/s spec)
+        roof["frac_of_measured_copy"] = round(roof["achieved"] / copy_gbs, 4)
+        if roof.get("traffic_GBs"):
+            roof["traffic_frac_of_measured_copy"] = round(roof["traffic_GBs"] / copy_gbs, 4)
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: oracle for ~{args.cpu_seconds:.0f} s ...")
@@ -636,6 +642,7 @@ def main():
         "value_with_pass_events": round(N * substeps / el_events, 1) if el_events else None,
         "whole_substep_algorithmic_GBs": round(whole, 1),
         "measured_copy_GBs": round(copy_gbs, 1),
+        "whole_substep_frac_of_measured_copy": round(whole / copy_gbs, 4),
         "density_error_after_run": {"mean": mean_err, "max": max_err},
         "learned_warm_start": warm,
         "batched": batched,
