@@ -255,3 +255,38 @@ def test_cells_past_the_slab_word_range(slabs):
     ooff, oids = o.neighbors()
     assert np.array_equal(goff, ooff) and np.array_equal(gids, oids)
     sim.close()
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_random_scenes_bit_exact(seed):
+    """Seeded random scenes: particle count, box, clustering, z-slabs per
+    cell (1, 2, 4) and dense or hashed grid (any table size that can hold
+    the cells' rows), each stepped once on both sides, bit for bit."""
+    rng = np.random.default_rng(1000 + seed)
+    L = float(rng.choice([1.2, 1.6, 2.4]))
+    n = int(rng.integers(1, 5000))
+    pts = rng.uniform(0.02, L - 0.02, (n, 3))
+    k = int(rng.integers(0, 4))  # a few dense clumps
+    for c in range(k):
+        m = min(n, int(rng.integers(50, 400)))
+        centre = rng.uniform(0.2, L - 0.2, 3)
+        pts[rng.choice(n, m, replace=False)] = centre + rng.normal(0.0, 0.03, (m, 3))
+    pts = np.clip(pts, 0.001, L - 0.001)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = pts
+    sc = _scene(pos, L=L, name=f"random{seed}", iters=int(rng.integers(1, 5)), field=bool(rng.integers(0, 2)))
+    slabs = int(rng.choice([1, 2, 4]))
+    if rng.integers(0, 2):
+        sc.grid_mode, sc.hash_log2 = 1, int(rng.integers(6, 14))
+    print(f"[random {seed}] n={n} L={L} slabs={slabs} grid={sc.grid_mode}/{sc.hash_log2}", flush=True)
+    sim = scenes.make_engine(sc, max_neighbors=512, cell_slabs=slabs)
+    o = make_oracle(sc, cell_slabs=slabs)
+    sim.step(sc.dt, sc.substeps, sc.iterations)
+    o.step(sc.dt, sc.substeps, sc.iterations)
+    g, c = gpu_fields(sim), oracle_fields(o)
+    for key in g:
+        assert ulp_diff(g[key], c[key]) == 0, (key, ulp_diff(g[key], c[key]))
+    goff, gids = sim.read_neighbors()
+    ooff, oids = o.neighbors()
+    assert np.array_equal(goff, ooff) and np.array_equal(gids, oids)
+    sim.close()
