@@ -134,8 +134,9 @@ __global__ void __launch_bounds__(MBS) k_mpm_settle(uint32_t n, const uint32_t *
 // K-2..K in slot order.  Then each node's nine partials are added in column
 // order and the grid pass applied.
 // RUN nodes per lane (consecutive z): 4 on large grids (a record loaded once
-// for three nodes), 1 on small ones (more lanes for few particles: 13 vs 28
-// us per step at the reference's 9,000 particles on 32^3 nodes).
+// for up to three nodes; at 1M particles on 128^3 nodes runs of 2, 4 and 8
+// take 395, 311 and 513 us), 1 on small ones (more lanes for few particles:
+// 17 vs 29 us at the reference's 9,000 particles on 32^3 nodes).
 constexpr uint32_t kP2GBS = 9 * 64;  // one wavefront per column
 constexpr uint32_t kSmallGrid = 1u << 16;
 
