@@ -274,6 +274,15 @@ int pbf_pass_times(pbf_ctx *ctx, double *ms, uint32_t *launches, uint32_t capaci
  * kernel for scaling hardware byte counters. No reference counterpart. */
 int pbf_copy_bandwidth(size_t bytes, uint32_t iterations, double *gbps);
 
+/* Numerics self-check, no reference counterpart: the pair terms take the
+ * square root of a squared distance from the hardware reciprocal square
+ * root and one Newton step, which is the correctly rounded square root (as
+ * sqrtf) for every float in [2^-100, FLT_MAX] on gfx950.  This runs that
+ * sequence on the device over all 1,912,602,624 of those floats and returns
+ * how many differ from the correctly rounded value (0 on a device the
+ * kernels' bit-exact results hold on) and how many were checked. */
+int pbf_check_device_sqrt(uint64_t *failures, uint64_t *checked);
+
 /* ---- Learned warm start of the lambda solve ----------------------------
  * No reference counterpart: the reference solves every substep from
  * lambda = 0 with a fixed iteration count (PBF.ts:419-431).  A 16 -> 64 ->

@@ -12,9 +12,20 @@ import pytest
 import oracle
 import scenes
 from pairs import gpu_fields, make_oracle, oracle_fields, rel_err, ulp_diff
-from pbf_host import PBF_FLAG_NO_GRAPH, PBFError
+from pbf_host import PBF_FLAG_NO_GRAPH, PBFError, check_device_sqrt
 
 pytestmark = pytest.mark.gpu
+
+
+def test_device_pair_sqrt_is_correctly_rounded():
+    """The pair terms' square root (hardware reciprocal square root plus one
+    Newton step) equals the correctly rounded one -- the oracle's sqrtf --
+    on every float in [2^-100, FLT_MAX]; the kernels' bit-exact results
+    rest on it."""
+    print("[sqrt] exhaustive device check", flush=True)
+    bad, n = check_device_sqrt()
+    assert n == 0x7F7FFFFF - 0x0D800000 + 1
+    assert bad == 0
 
 
 def small_scenes():

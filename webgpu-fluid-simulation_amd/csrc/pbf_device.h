@@ -112,25 +112,31 @@ __device__ __forceinline__ void clamp_rand(const Scene &sc, float &px, float &py
 
 // ---------------------------------------------------------------------------
 // Correctly rounded square root and division for the operand ranges of the
-// pair terms.  The hardware v_sqrt_f32 / v_rcp_f32 are refined to the
-// correctly rounded result the usual way (square root: test the neighbours
-// s -/+ 1 ulp with an exact fma residual; division: one Newton step on the
-// reciprocal, then two fma remainder corrections of the quotient).  The
-// general forms add pre- and post-scaling for denormal, huge or special
-// operands; the pair terms never produce those (d2 is 0 or at least the
-// square of one position ulp, r lies in [1e-7, h], densities are ~rho0), so
-// the scaling is left out.  Results equal sqrtf() and '/' with
-// -fhip-fp32-correctly-rounded-divide-sqrt, which the parity tests check bit
-// for bit.
-__device__ __forceinline__ float sqrt_n(float x) {
-  float s = __builtin_amdgcn_sqrtf(x);
-  const int si = __float_as_int(s);
-  const float dn = __int_as_float(si - 1), up = __int_as_float(si + 1);
-  const float rd = fmaf(-dn, s, x), ru = fmaf(-up, s, x);
-  s = rd <= 0.0f ? dn : s;
-  s = ru > 0.0f ? up : s;
-  return s;
+// pair terms, equal to sqrtf() and '/' with
+// -fhip-fp32-correctly-rounded-divide-sqrt there (the parity tests check
+// them bit for bit against the CPU restatement).
+//
+// Square root of a pair distance d2: the hardware reciprocal square root y
+// and one Newton step on s = d2 y, s + (d2 - s^2) y / 2 with the residual
+// exact in an fma -- five instructions where testing the neighbours
+// s -/+ 1 ulp of v_sqrt_f32 takes nine (v_sqrt_f32 is off by one ulp for
+// about 15 % of inputs, in both directions).  The result is the correctly
+// rounded square root for every float in [2^-100, FLT_MAX]: checked
+// exhaustively on gfx950 (k_check_sqrt, pbf_check_device_sqrt).  d2 = 0
+// gives NaN and the callers select their term to 0 there; a d2 in
+// (0, 2^-100) needs two particles closer than 1e-15 in every coordinate,
+// which positions of any useful scale cannot be.
+__device__ __forceinline__ float sqrt_pair(float x) {
+  const float y = __builtin_amdgcn_rsqf(x);
+  const float s = x * y;
+  const float r = fmaf(-s, s, x);
+  return fmaf(r, 0.5f * y, s);
 }
+// Division: one Newton step on the hardware reciprocal (the correctly
+// rounded 1/b for every b in [2^-100, 2^126), checked exhaustively on gfx950),
+// then two fma remainder corrections of the quotient.  The general form adds
+// pre- and post-scaling for denormal, huge or special operands; the pair
+// terms never produce those (r lies in [2^-50, h], densities are ~rho0).
 __device__ __forceinline__ float div_n(float a, float b) {
   float y = __builtin_amdgcn_rcpf(b);
   const float e = fmaf(-b, y, 1.0f);
@@ -153,13 +159,13 @@ __device__ __forceinline__ float poly6(const Phys &P, float d2) {
   return d2 <= P.h2 ? w : 0.0f;
 }
 __device__ __forceinline__ float spiky_f(const Phys &P, float d2) {
-  float r = sqrt_n(d2);
+  float r = sqrt_pair(d2);
   float q = P.h - r;
   float f = div_n(P.Ks * (q * q), r);
   return (d2 > 0.0f && d2 <= P.h2) ? f : 0.0f;
 }
 __device__ __forceinline__ float cohesion_f(const Phys &P, float d2) {
-  float r = sqrt_n(d2);
+  float r = sqrt_pair(d2);
   float a = P.h - r;
   float t = ((a * a) * a) * ((r * r) * r);
   float res = (r > P.half_h) ? t : fmaf(2.0f, t, P.bias);

@@ -1580,4 +1580,29 @@ int pbf_copy_bandwidth(size_t bytes, uint32_t iterations, double *gbps) {
   return PBF_OK;
 }
 
+int pbf_check_device_sqrt(uint64_t *failures, uint64_t *checked) {
+  if (!failures) return PBF_E_INVALID;
+  const uint32_t lo = 0x0d800000u, hi = 0x7f7fffffu;  // 2^-100 .. FLT_MAX
+  unsigned long long *bad = nullptr;
+  hipStream_t st = nullptr;
+  int rc = PBF_OK;
+  auto ok = [&](hipError_t e) {
+    if (e != hipSuccess && rc == PBF_OK) rc = PBF_E_DEVICE;
+    return rc == PBF_OK;
+  };
+  if (ok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) && ok(hipMalloc(&bad, sizeof *bad)) &&
+      ok(hipMemsetAsync(bad, 0, sizeof *bad, st))) {
+    launch_check_sqrt(st, lo, hi, bad);
+    unsigned long long h = 0;
+    if (ok(hipGetLastError()) && ok(hipMemcpyAsync(&h, bad, sizeof h, hipMemcpyDeviceToHost, st)) &&
+        ok(hipStreamSynchronize(st))) {
+      *failures = h;
+      if (checked) *checked = (uint64_t)hi - lo + 1;
+    }
+  }
+  if (bad) (void)hipFree(bad);
+  if (st) (void)hipStreamDestroy(st);
+  return rc;
+}
+
 }  // extern "C"

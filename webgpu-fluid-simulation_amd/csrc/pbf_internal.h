@@ -105,6 +105,7 @@ void launch_sort1(hipStream_t st, uint32_t n, const float4 *src, const uint32_t 
 void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t n, uint32_t v);
 void launch_iota(hipStream_t st, uint32_t *p, uint32_t n);
 void launch_copy_probe(hipStream_t st, const void *src, void *dst, size_t bytes);
+void launch_check_sqrt(hipStream_t st, uint32_t lo, uint32_t hi, unsigned long long *bad);
 
 // Message of the last failed context-free call on this thread
 // (pbf_last_error(NULL)).

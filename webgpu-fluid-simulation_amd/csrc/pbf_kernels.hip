@@ -892,7 +892,7 @@ __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float d2 = dist2(dx, dy, dz);
     const bool in = d2 <= P.h2;
-    const float r = sqrt_n(d2);
+    const float r = sqrt_pair(d2);
     const float q = P.h - r;
     const float fs = div_n(P.Ks * (q * q), r);
     const float f = (in && d2 > 0.0f) ? fs : 0.0f;
@@ -1351,6 +1351,31 @@ __global__ void __launch_bounds__(BS) k_copy_probe(const uint4 *__restrict__ src
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) dst[i] = src[i];
 }
+// Exhaustive check of sqrt_pair over the float bit patterns [lo, hi]: counts
+// the x whose result is not the correctly rounded square root.  Exact test
+// in double: s is correctly rounded iff m_lo^2 < x < m_hi^2 for the
+// midpoints m to s's float neighbours (25-bit values, so the squares are
+// exact; x is a float, so it never equals one).
+__global__ void __launch_bounds__(BS) k_check_sqrt(uint32_t lo, uint32_t hi, uint32_t per,
+                                                   unsigned long long *bad) {
+  const uint64_t first = (uint64_t)lo + ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * per;
+  uint32_t nbad = 0;
+  for (uint32_t k = 0; k < per; ++k) {
+    const uint64_t bits = first + k;
+    if (bits > hi) break;
+    const float x = __uint_as_float((uint32_t)bits);
+    const float s = sqrt_pair(x);
+    const uint32_t sb = __float_as_uint(s);
+    bool ok = s > 0.0f && sb < 0x7f800000u;
+    if (ok) {
+      const double d = (double)s, m0 = (d + (double)__uint_as_float(sb - 1)) * 0.5,
+                   m1 = (d + (double)__uint_as_float(sb + 1)) * 0.5, X = (double)x;
+      ok = m0 * m0 < X && X < m1 * m1;
+    }
+    nbad += ok ? 0u : 1u;
+  }
+  if (nbad) atomicAdd(bad, (unsigned long long)nbad);
+}
 __global__ void __launch_bounds__(BS) k_iota(uint32_t *p, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = i;
@@ -1597,6 +1622,12 @@ void launch_copy_probe(hipStream_t st, const void *src, void *dst, size_t bytes)
   size_t n = bytes / 16;
   uint32_t grid = (uint32_t)std::min<size_t>((n + BS - 1) / BS, 256u * 64u);
   if (n) hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(BS), 0, st, (const uint4 *)src, (uint4 *)dst, n);
+}
+void launch_check_sqrt(hipStream_t st, uint32_t lo, uint32_t hi, unsigned long long *bad) {
+  if (hi < lo) return;
+  const uint32_t per = 256;
+  const uint64_t threads = ((uint64_t)hi - lo + per) / per;
+  hipLaunchKernelGGL(k_check_sqrt, dim3((uint32_t)((threads + BS - 1) / BS)), dim3(BS), 0, st, lo, hi, per, bad);
 }
 void launch_iota(hipStream_t st, uint32_t *p, uint32_t n) {
   if (n) hipLaunchKernelGGL(k_iota, dim3(nblocks(n, BS)), dim3(BS), 0, st, p, n);

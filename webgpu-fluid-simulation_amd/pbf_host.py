@@ -123,6 +123,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "pbf_last_error": (C.c_char_p, [P]), "pbf_version": (C.c_char_p, []),
         "pbf_pass_times": (I, [P, C.POINTER(D), pu, U32, pu, C.POINTER(C.c_char_p)]),
         "pbf_copy_bandwidth": (I, [S, U32, C.POINTER(D)]),
+        "pbf_check_device_sqrt": (I, [C.POINTER(U64), C.POINTER(U64)]),
         "pbf_mlp_blob_size": (U32, []),
         "pbf_mlp_init": (I, [U64, pf]),
         "pbf_mlp_forward_backward": (I, [pf, pf, pf, U32, pf, pf, pf]),
@@ -279,6 +280,17 @@ def copy_bandwidth(nbytes: int = 1 << 30, iterations: int = 20) -> float:
     if rc != PBF_OK:
         raise PBFError(rc, "pbf_copy_bandwidth failed")
     return g.value
+
+
+def check_device_sqrt() -> tuple:
+    """(failures, checked): the device square root of the pair terms against
+    the correctly rounded value over every float in [2^-100, FLT_MAX]."""
+    lib = load_library()
+    bad, n = C.c_uint64(), C.c_uint64()
+    rc = lib.pbf_check_device_sqrt(C.byref(bad), C.byref(n))
+    if rc != PBF_OK:
+        raise PBFError(rc, "pbf_check_device_sqrt failed")
+    return bad.value, n.value
 
 
 MLP_PARAMS = 5313
