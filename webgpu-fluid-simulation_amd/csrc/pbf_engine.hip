@@ -63,7 +63,6 @@ struct GraphEntry {
 };
 
 constexpr size_t kGraphCache = 16;  // executable graphs kept per context
-constexpr uint32_t kMaxReduceBlocks = 64;  // blocks per scene of a per-scene reduction
 
 struct PendingEvent {
   int pass;
@@ -348,7 +347,12 @@ void enqueue_search(pbf_ctx *c, const LaunchCtx &L, PassTimer &T, bool dbg, uint
 }
 
 // The launch sequence of one substep (PBF.ts:398-448).
-void enqueue_substep(pbf_ctx *c, const LaunchCtx &L, uint32_t iters, bool diag, PassTimer &T, uint32_t mode_bits) {
+// diag: record the substep's density error (history); ci_out: leave the
+// per-particle FP64 error in c->ci for a check -- from the density pass, or,
+// with cubic fields, from the diagnostic's own walk, which then also reduces
+// it into per_scene.
+void enqueue_substep(pbf_ctx *c, const LaunchCtx &L, uint32_t iters, bool diag, PassTimer &T, uint32_t mode_bits,
+                     bool ci_out = false) {
   const bool project = (mode_bits & M_PROJECT) != 0, smooth = (mode_bits & M_SMOOTH) != 0;
   const uint32_t mode = mode_bits & ~(M_PROJECT | M_SMOOTH);
   enqueue_search(c, L, T, false, mode);
@@ -375,7 +379,8 @@ void enqueue_substep(pbf_ctx *c, const LaunchCtx &L, uint32_t iters, bool diag, 
     std::swap(X, Xn);
   }
   LaunchCtx t = T.begin(P_DENSITY, L);
-  launch_density(t, X, c->Xold, c->nbr, c->ncount, c->pos, c->vel, c->lam);
+  const bool fused = (diag || ci_out) && L.fmode != 1;
+  launch_density(t, X, c->Xold, c->nbr, c->ncount, c->pos, c->vel, c->lam, fused ? c->ci : nullptr);
   T.end();
   t = T.begin(P_VORT, L);
   // the XSPH velocity sum goes to the free Jacobi buffer
@@ -386,9 +391,16 @@ void enqueue_substep(pbf_ctx *c, const LaunchCtx &L, uint32_t iters, bool diag, 
   T.end();
   if (diag) {
     t = T.begin(P_DIAG, L);
-    launch_diag(t, c->pos, c->nbr, c->ncount, c->ci, c->ranges, (uint32_t)c->scenes.size(), c->per_scene, c->hist,
-                c->hist_cap, c->hist_ctr);
+    if (fused)
+      launch_diag_reduce(t, c->ci, c->ranges, (uint32_t)c->scenes.size(), c->per_scene, c->hist, c->hist_cap,
+                         c->hist_ctr);
+    else
+      launch_diag(t, c->pos, c->nbr, c->ncount, c->ci, c->ranges, (uint32_t)c->scenes.size(), c->per_scene, c->hist,
+                  c->hist_cap, c->hist_ctr);
     T.end();
+  } else if (ci_out && !fused) {
+    launch_diag(L, c->pos, c->nbr, c->ncount, c->ci, c->ranges, (uint32_t)c->scenes.size(), c->per_scene, nullptr,
+                0, nullptr);
   }
 }
 
@@ -511,10 +523,8 @@ int run_learned_substep(pbf_ctx *c, float dt, uint32_t iters) {
   rc = run_enqueued(c, dt, iters, false, mode | M_FALLBACK, 0u, [&](const LaunchCtx &L, PassTimer &T) {
     launch_energy(L, c->pos, c->vel, c->ranges, ns, c->energy0);
     copy_state(c, true, false);
-    enqueue_substep(c, L, iters, false, T, mode);
-    launch_diag(L, c->pos, c->nbr, c->ncount, c->ci, c->ranges, ns, c->per_scene, nullptr, 0, nullptr);
-    launch_speed(L, c->vel, c->ci, c->ranges, ns, c->per_scene2);
-    launch_energy(L, c->pos, c->vel, c->ranges, ns, c->energy1);
+    enqueue_substep(c, L, iters, false, T, mode, true);
+    launch_check_reduce(L, c->ci, c->pos, c->vel, c->ranges, ns, c->per_scene, c->per_scene2, c->energy1);
     launch_fallback_gate(c->stream, c->per_scene, c->per_scene2, c->energy0, c->energy1, ns, c->n, c->fb_thr,
                          c->decision_dev, c->fb_count);
   });
@@ -750,7 +760,7 @@ int create_ctx_body(const pbf_params *params, uint32_t nscenes, const pbf_scene_
   step(dalloc(&c->per_scene, 2 * (size_t)nscenes), "alloc");
   step(dalloc(&c->per_scene2, 2 * (size_t)nscenes), "alloc");
   step(dalloc(&c->energy0, 2 * (size_t)nscenes), "alloc");
-  step(dalloc(&c->partial, 2 * (size_t)nscenes * kMaxReduceBlocks), "alloc");
+  step(dalloc(&c->partial, 6 * (size_t)nscenes * kMaxReduceBlocks), "alloc");
   step(hipHostMalloc(&c->decision, 4, hipHostMallocMapped), "hipHostMalloc");
   if (rc == PBF_OK) {
     *static_cast<uint32_t *>(c->decision) = 0u;

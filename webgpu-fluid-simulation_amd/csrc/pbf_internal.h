@@ -41,10 +41,11 @@ struct LaunchCtx {
   uint32_t bs = 256;   // work-group size of the neighbour walks (64..256, a multiple of 64)
   uint32_t lvar = 0;   // neighbour-list kernel variant (LDS rows x work-group size)
   uint32_t red_k = 1;  // blocks per scene of the per-scene reductions
-  double *partial = nullptr;  // their stage-1 partials: 2 x scenes x red_k doubles
+  double *partial = nullptr;  // their stage-1 partials: up to 6 x scenes x red_k doubles
 };
 
 constexpr uint32_t kListVariants = 4;
+constexpr uint32_t kMaxReduceBlocks = 64;  // blocks per scene of a per-scene reduction (at most one wavefront)
 
 void launch_predict_bin(const LaunchCtx &L, const float4 *pos, const float4 *vel, const float4 *acc, float4 *xs,
                         uint32_t *cellkey, uint32_t *rank, uint32_t *counts);
@@ -80,12 +81,18 @@ void launch_fallback_gate(hipStream_t st, const double *err, const double *speed
 void launch_smooth_warm(const LaunchCtx &L, float4 *X, const uint32_t *nbr, const uint32_t *ncount, float *tmp,
                         float *lam_acc);
 void launch_warm_previous(const LaunchCtx &L, const float *acc_prev, float4 *X, float *lam_acc, float relax);
-void launch_speed(const LaunchCtx &L, const float4 *vel, double *ci, const uint32_t *ranges, uint32_t nscenes,
-                  double *per_scene);
+// A learned substep's check figures in one pass per scene: (sum, max) of ci
+// into err, (sum, max) of |v| dt / h into speed, the energy sums into energy.
+void launch_check_reduce(const LaunchCtx &L, const double *ci, const float4 *pos, const float4 *vel,
+                         const uint32_t *ranges, uint32_t nscenes, double *err, double *speed, double *energy);
 void launch_dp(const LaunchCtx &L, const float4 *X, const float4 *B, const uint32_t *nbr, const uint32_t *ncount,
                float4 *Xn);
+// ci: when not null, also the FP64 density-constraint error of every particle
+// (the diagnostic's per-particle figure; launch_diag_reduce then reduces it).
+// Not with cubic fields (fmode 1): the 32-node element's registers would
+// cost the walk its occupancy, so those keep launch_diag's own walk.
 void launch_density(const LaunchCtx &L, const float4 *X, const float4 *Xold, const uint32_t *nbr,
-                    const uint32_t *ncount, float4 *pos, float4 *vel, float *lam);
+                    const uint32_t *ncount, float4 *pos, float4 *vel, float *lam, double *ci = nullptr);
 // visc: the XSPH velocity sum, written by the vorticity pass, read by XSPH
 void launch_vorticity(const LaunchCtx &L, const float4 *pos, const float4 *vel, const uint32_t *nbr,
                       const uint32_t *ncount, float4 *om, float4 *nrm, float4 *visc);
@@ -94,6 +101,10 @@ void launch_xsph(const LaunchCtx &L, const float4 *pos, const float4 *om, const 
 void launch_diag(const LaunchCtx &L, const float4 *pos, const uint32_t *nbr, const uint32_t *ncount, double *ci,
                  const uint32_t *ranges, uint32_t nscenes, double *per_scene, double *hist, uint32_t hist_cap,
                  uint32_t *hist_ctr);
+// The reduction half of launch_diag, over a ci the density pass already
+// wrote (launch_density with ci), with the same history entry.
+void launch_diag_reduce(const LaunchCtx &L, const double *ci, const uint32_t *ranges, uint32_t nscenes,
+                        double *per_scene, double *hist, uint32_t hist_cap, uint32_t *hist_ctr);
 // Appends the mean / max over the per-scene totals of an earlier launch_diag
 // to the history (the same entry launch_diag with a history writes).
 void launch_diag_history(const LaunchCtx &L, const double *per_scene, uint32_t nscenes, double *hist,

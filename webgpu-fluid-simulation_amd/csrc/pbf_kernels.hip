@@ -1036,19 +1036,53 @@ __global__ void __launch_bounds__(BS) k_dp(uint32_t n, const float4 *__restrict_
 
 // ---------------------------------------------------------------------------
 // K12: density and velocity; the state arrays now hold the new order.
+// FP64 density-constraint error of the diagnostic: the pair term and the
+// boundary term with the clamp to C >= 0 (shared by k_diag and the density
+// pass's fused form, so both give the same bits).
+__device__ __forceinline__ double diag_pair(const Phys &P, float ax, float ay, float az, float bx, float by,
+                                            float bz) {
+  const double dx = (double)ax - (double)bx, dy = (double)ay - (double)by, dz = (double)az - (double)bz;
+  const double d2 = (dx * dx + dy * dy) + dz * dz;
+  if (d2 > P.h2_64) return 0.0;
+  const double t = P.h2_64 - d2;
+  return P.Kp64 * ((t * t) * t);
+}
+template <int FK>
+__device__ __forceinline__ double diag_finish(const Phys &P, const Scene &sc, const float *__restrict__ pool,
+                                              const float4 &xi, double rho) {
+  rho *= P.V64;
+  if constexpr (FK != 0) {
+    const float4 b = boundary_sample<FK == 2 ? 2 : 0>(P, sc, pool, xi.x, xi.y, xi.z);
+    if (b.w > 0.0f) rho += (double)b.w * diag_pair(P, xi.x, xi.y, xi.z, b.x, b.y, b.z);
+  }
+  const double c = rho - 1.0;
+  return c > 0.0 ? c : 0.0;
+}
+
+// K12: density and velocity; the state arrays now hold the new order.  DIAG:
+// also the FP64 density-constraint error of every particle into ci (the
+// diagnostic's walk, fused: the same neighbours at the same positions, in
+// the same order), for a substep whose error is recorded or checked.
+template <bool DIAG, bool MULTI, int FK>
 __global__ void __launch_bounds__(BS) k_density(uint32_t n, const float4 *__restrict__ X,
                                                 const float4 *__restrict__ Xold, const uint32_t *__restrict__ nbr,
                                                 const uint32_t *__restrict__ ncount, float4 *__restrict__ pos,
-                                                float4 *__restrict__ vel, float *__restrict__ lam, Phys P) {
+                                                float4 *__restrict__ vel, float *__restrict__ lam, Phys P,
+                                                const float *__restrict__ pool, Scene s0,
+                                                const Scene *__restrict__ scenes, const uint32_t *__restrict__ sid,
+                                                double *__restrict__ ci) {
   const uint32_t i = slot_index();
   if (i >= n) return;
   const float4 xi = X[i];
   float rho = 0.0f;
+  double rho64 = 0.0;
   const Gather GX(X, n);
   for_each_neighbor_w<8>(nbr, i, P, ncount[i], [&](uint32_t j) { return GX.xyz(j); },
                          [&](const float3 &xj, bool valid) {
     rho = fmaf(poly6(P, dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z)), valid ? 1.0f : 0.0f, rho);
+    if constexpr (DIAG) rho64 += valid ? diag_pair(P, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z) : 0.0;
   });
+  if constexpr (DIAG) ci[i] = diag_finish<FK>(P, scene_at<MULTI>(s0, scenes, sid, i), pool, xi, rho64);
   rho *= P.m;
   const float4 xo = Xold[i];
   // w carries 1/rho for the vorticity and XSPH passes
@@ -1142,28 +1176,9 @@ __global__ void __launch_bounds__(BS) k_diag(uint32_t n, const float4 *__restric
   double rho = 0.0;
   for_each_neighbor(nbr, i, P, ncount[i], [&](uint32_t j) {
     const float4 xj = pos[j];
-    double dx = (double)xi.x - (double)xj.x, dy = (double)xi.y - (double)xj.y, dz = (double)xi.z - (double)xj.z;
-    double d2 = (dx * dx + dy * dy) + dz * dz;
-    if (d2 <= P.h2_64) {
-      double t = P.h2_64 - d2;
-      rho += P.Kp64 * ((t * t) * t);
-    }
+    rho += diag_pair(P, xi.x, xi.y, xi.z, xj.x, xj.y, xj.z);
   });
-  rho *= P.V64;
-  if constexpr (FK != 0) {
-    const Scene &sc = scene_at<MULTI>(s0, scenes, sid, i);
-    float4 b = boundary_sample<FK == 2 ? 2 : 0>(P, sc, pool, xi.x, xi.y, xi.z);
-    if (b.w > 0.0f) {
-      double dx = (double)xi.x - (double)b.x, dy = (double)xi.y - (double)b.y, dz = (double)xi.z - (double)b.z;
-      double d2 = (dx * dx + dy * dy) + dz * dz;
-      if (d2 <= P.h2_64) {
-        double t = P.h2_64 - d2;
-        rho += (double)b.w * (P.Kp64 * ((t * t) * t));
-      }
-    }
-  }
-  double c = rho - 1.0;
-  ci[i] = c > 0.0 ? c : 0.0;
+  ci[i] = diag_finish<FK>(P, scene_at<MULTI>(s0, scenes, sid, i), pool, xi, rho);
 }
 
 // Per-scene reductions in two deterministic stages.  Stage 1: block (k, s)
@@ -1213,14 +1228,24 @@ __global__ void __launch_bounds__(1024) k_reduce_stage1(const double *__restrict
   }
 }
 
+// Stage 2 (64 threads, K <= kMaxReduceBlocks = 64): the K partials of a
+// scene are loaded at once, then combined in order by one thread.
 template <int KIND>
-__global__ void k_reduce_stage2(const double *__restrict__ partial, uint32_t K, double *__restrict__ out) {
-  if (threadIdx.x != 0) return;
-  const uint32_t s = blockIdx.x;
+__global__ void __launch_bounds__(64) k_reduce_stage2(const double *__restrict__ partial, uint32_t K,
+                                                      double *__restrict__ out) {
+  static_assert(kMaxReduceBlocks <= 64, "one partial per thread");
+  __shared__ double pa[64], pb[64];
+  const uint32_t s = blockIdx.x, k = threadIdx.x;
+  if (k < K) {
+    pa[k] = partial[2 * ((size_t)s * K + k)];
+    pb[k] = partial[2 * ((size_t)s * K + k) + 1];
+  }
+  __syncthreads();
+  if (k != 0) return;
   double t = 0.0, u = 0.0;
-  for (uint32_t k = 0; k < K; ++k) {
-    t += partial[2 * ((size_t)s * K + k)];
-    const double v = partial[2 * ((size_t)s * K + k) + 1];
+  for (uint32_t q = 0; q < K; ++q) {
+    t += pa[q];
+    const double v = pb[q];
     if constexpr (KIND == 0) u = v > u ? v : u;
     else u += v;
   }
@@ -1228,13 +1253,69 @@ __global__ void k_reduce_stage2(const double *__restrict__ partial, uint32_t K, 
   out[2 * s + 1] = u;
 }
 
-// |v| dt / h per particle (the learned solve's rejection test).
-__global__ void __launch_bounds__(BS) k_speed(uint32_t n, const float4 *__restrict__ vel, float scale,
-                                              double *__restrict__ ci) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float4 v = vel[i];
-  ci[i] = (double)(sqrtf(dist2(v.x, v.y, v.z)) * scale);
+// The figures of a learned substep's check in one pass, each reduced exactly
+// as its own two-stage reduction would (same slots per thread, same
+// butterfly, same combine order, so the same bits): (sum, max) of the
+// density error ci, (sum, max) of |v| dt / h (scale = dt / h), and the
+// (kinetic, g y) energy sums.  Partials: six doubles per block.
+__global__ void __launch_bounds__(1024) k_check_stage1(const double *__restrict__ ci, const float4 *__restrict__ pos,
+                                                       const float4 *__restrict__ vel, float g, float scale,
+                                                       const uint32_t *__restrict__ ranges,
+                                                       double *__restrict__ partial) {
+  __shared__ double sv[6][16];
+  const uint32_t k = blockIdx.x, K = gridDim.x, s = blockIdx.y;
+  const uint32_t b = ranges[2 * s], cnt = ranges[2 * s + 1];
+  double r[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (uint32_t q = k * 1024u + threadIdx.x; q < cnt; q += K * 1024u) {
+    const double e = ci[b + q];
+    r[0] += e;
+    r[1] = e > r[1] ? e : r[1];
+    const float4 v = vel[b + q];
+    const double sp = (double)(sqrtf(dist2(v.x, v.y, v.z)) * scale);
+    r[2] += sp;
+    r[3] = sp > r[3] ? sp : r[3];
+    r[4] += 0.5 * ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z);
+    r[5] += (double)g * (double)pos[b + q].y;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const double w = __shfl_xor(r[j], o, 64);
+      r[j] = (j == 1 || j == 3) ? (w > r[j] ? w : r[j]) : r[j] + w;
+    }
+  }
+  if (__lane_id() == 0)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) sv[j][threadIdx.x >> 6] = r[j];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int j = threadIdx.x;
+    double t = 0.0;
+    for (uint32_t w = 0; w < blockDim.x / 64; ++w) t = (j == 1 || j == 3) ? (sv[j][w] > t ? sv[j][w] : t) : t + sv[j][w];
+    partial[6 * ((size_t)s * K + k) + j] = t;
+  }
+}
+// Stage 2 (64 threads): the K partials of each scene loaded at once, then
+// combined in order, into the three per-scene (x, y) pairs the gate reads.
+__global__ void __launch_bounds__(64) k_check_stage2(const double *__restrict__ partial, uint32_t K,
+                                                     double *__restrict__ err, double *__restrict__ speed,
+                                                     double *__restrict__ energy) {
+  __shared__ double pv[6][64];
+  const uint32_t s = blockIdx.x, k = threadIdx.x;
+  if (k < K)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) pv[j][k] = partial[6 * ((size_t)s * K + k) + j];
+  __syncthreads();
+  const uint32_t j = threadIdx.x;
+  if (j >= 6) return;
+  double t = 0.0;
+  for (uint32_t q = 0; q < K; ++q) {
+    const double v = pv[j][q];
+    t = (j == 1 || j == 3) ? (v > t ? v : t) : t + v;
+  }
+  double *out = j < 2 ? err : j < 4 ? speed : energy;
+  out[2 * s + (j & 1u)] = t;
 }
 
 // The learned substep's check, on the device: *decision = 1 (redo the
@@ -1545,9 +1626,18 @@ void launch_dp(const LaunchCtx &L, const float4 *X, const float4 *B, const uint3
 }
 
 void launch_density(const LaunchCtx &L, const float4 *X, const float4 *Xold, const uint32_t *nbr,
-                    const uint32_t *ncount, float4 *pos, float4 *vel, float *lam) {
+                    const uint32_t *ncount, float4 *pos, float4 *vel, float *lam, double *ci) {
   if (!L.n) return;
-  go(L.stream, L.ev0, L.ev1, k_density, dim3(nblocks(L.n, L.bs)), dim3(L.bs), L.n, X, Xold, nbr, ncount, pos, vel, lam, L.P);
+  const dim3 g(nblocks(L.n, L.bs));
+  if (!ci)
+    go(L.stream, L.ev0, L.ev1, k_density<false, false, 0>, g, dim3(L.bs), L.n, X, Xold, nbr, ncount, pos, vel, lam,
+       L.P, L.pool, L.s0, L.scenes, L.sid, ci);
+  else if (L.multi)
+    go(L.stream, L.ev0, L.ev1, L.fmode ? k_density<true, true, 2> : k_density<true, true, 0>, g, dim3(L.bs), L.n, X,
+       Xold, nbr, ncount, pos, vel, lam, L.P, L.pool, L.s0, L.scenes, L.sid, ci);
+  else
+    go(L.stream, L.ev0, L.ev1, L.fmode ? k_density<true, false, 2> : k_density<true, false, 0>, g, dim3(L.bs), L.n, X,
+       Xold, nbr, ncount, pos, vel, lam, L.P, L.pool, L.s0, L.scenes, L.sid, ci);
 }
 
 void launch_vorticity(const LaunchCtx &L, const float4 *pos, const float4 *vel, const uint32_t *nbr,
@@ -1566,8 +1656,9 @@ void launch_xsph(const LaunchCtx &L, const float4 *pos, const float4 *om, const 
 
 template <int KIND>
 static void reduce_scenes(const LaunchCtx &L, const double *ci, const float4 *pos, const float4 *vel, float g,
-                          const uint32_t *ranges, uint32_t nscenes, double *out, hipEvent_t e1 = nullptr) {
-  go(L.stream, nullptr, nullptr, k_reduce_stage1<KIND>, dim3(L.red_k, nscenes), dim3(1024), ci, pos, vel, g, ranges,
+                          const uint32_t *ranges, uint32_t nscenes, double *out, hipEvent_t e1 = nullptr,
+                          hipEvent_t e0 = nullptr) {
+  go(L.stream, e0, nullptr, k_reduce_stage1<KIND>, dim3(L.red_k, nscenes), dim3(1024), ci, pos, vel, g, ranges,
      L.partial);
   go(L.stream, nullptr, e1, k_reduce_stage2<KIND>, dim3(nscenes), dim3(64), (const double *)L.partial, L.red_k, out);
 }
@@ -1579,7 +1670,15 @@ void launch_diag(const LaunchCtx &L, const float4 *pos, const uint32_t *nbr, con
   const dim3 g(nblocks(L.n, L.bs));
   if (L.multi) fk_dispatch<true, SelDiag>(L, L.ev0, nullptr, g, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
   else fk_dispatch<false, SelDiag>(L, L.ev0, nullptr, g, L.n, pos, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, ci);
-  reduce_scenes<0>(L, ci, nullptr, nullptr, 0.0f, ranges, nscenes, per_scene, hist ? nullptr : L.ev1);
+  LaunchCtx R = L;
+  R.ev0 = nullptr;
+  launch_diag_reduce(R, ci, ranges, nscenes, per_scene, hist, hist_cap, hist_ctr);
+}
+
+void launch_diag_reduce(const LaunchCtx &L, const double *ci, const uint32_t *ranges, uint32_t nscenes,
+                        double *per_scene, double *hist, uint32_t hist_cap, uint32_t *hist_ctr) {
+  if (!L.n) return;
+  reduce_scenes<0>(L, ci, nullptr, nullptr, 0.0f, ranges, nscenes, per_scene, hist ? nullptr : L.ev1, L.ev0);
   if (hist)
     go(L.stream, nullptr, L.ev1, k_diag_history, dim3(1), dim3(64), (const double *)per_scene, nscenes, L.n, hist,
        hist_cap, hist_ctr);
@@ -1590,11 +1689,13 @@ void launch_diag_history(const LaunchCtx &L, const double *per_scene, uint32_t n
   go(L.stream, L.ev0, L.ev1, k_diag_history, dim3(1), dim3(64), per_scene, nscenes, L.n, hist, hist_cap, hist_ctr);
 }
 
-void launch_speed(const LaunchCtx &L, const float4 *vel, double *ci, const uint32_t *ranges, uint32_t nscenes,
-                  double *per_scene) {
+void launch_check_reduce(const LaunchCtx &L, const double *ci, const float4 *pos, const float4 *vel,
+                         const uint32_t *ranges, uint32_t nscenes, double *err, double *speed, double *energy) {
   if (!L.n) return;
-  hipLaunchKernelGGL(k_speed, dim3(nblocks(L.n, BS)), dim3(BS), 0, L.stream, L.n, vel, L.P.dt / L.P.h, ci);
-  reduce_scenes<0>(L, ci, nullptr, nullptr, 0.0f, ranges, nscenes, per_scene);
+  go(L.stream, nullptr, nullptr, k_check_stage1, dim3(L.red_k, nscenes), dim3(1024), ci, pos, vel, -L.P.gy,
+     L.P.dt / L.P.h, ranges, L.partial);
+  go(L.stream, nullptr, nullptr, k_check_stage2, dim3(nscenes), dim3(64), (const double *)L.partial, L.red_k, err,
+     speed, energy);
 }
 
 void launch_energy(const LaunchCtx &L, const float4 *pos, const float4 *vel, const uint32_t *ranges,
