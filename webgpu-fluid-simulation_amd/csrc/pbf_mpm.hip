@@ -199,7 +199,24 @@ __global__ void __launch_bounds__(kP2GBS) k_mpm_p2g_grid(const uint32_t *__restr
   // nodes (an XCD-contiguous mapping, whole or in runs of 16 work-groups,
   // measured 1.7-1.8x slower: the particles fill only part of the grid)
   const uint32_t base = blockIdx.x * kP2GNodes;
+  // Work-group-uniform: does any particle lie in the cells its nodes gather
+  // from?  With all nodes in one x-layer I, those are the cells (bx, by, *)
+  // with bx in I-2..I and by in J_first-2..J_last, three contiguous cell
+  // ranges.  Where none holds a particle every partial is +0, so the walks
+  // and the LDS exchange are skipped and the nodes get the grid pass of a
+  // zero sum (most work-groups: the particles fill a small part of the grid).
+  bool any = true;
   {
+    const uint32_t first = base, last = (base + kP2GNodes < K.G ? base + kP2GNodes : K.G) - 1u;
+    const uint32_t ng2 = K.ng * K.ng, I0 = first / ng2;
+    if (I0 == last / ng2) {
+      const uint32_t J0 = (first / K.ng) % K.ng, J1 = (last / K.ng) % K.ng, jlo = J0 >= 2u ? J0 - 2u : 0u;
+      any = false;
+      for (uint32_t bx = I0 >= 2u ? I0 - 2u : 0u; bx <= I0; ++bx)
+        any = any || offsets[(bx * K.ng + J1) * K.ng + K.ng] != offsets[(bx * K.ng + jlo) * K.ng];
+    }
+  }
+  if (any) {
     // this lane's run: nodes base + RUN * lane .. + RUN - 1 (one node
     // column: ng is a multiple of RUN)
     const uint32_t n0 = base + RUN * lane;
@@ -216,20 +233,23 @@ __global__ void __launch_bounds__(kP2GBS) k_mpm_p2g_grid(const uint32_t *__restr
     }
 #pragma unroll
     for (int k = 0; k < RUN; ++k) part[col * kP2GNodes + RUN * lane + k] = acc[k];
+    __syncthreads();
   }
-  __syncthreads();
   if (tid >= kP2GNodes) return;
   const uint32_t node = base + tid;
   if (node >= K.G) return;
   const int I = (int)(node / (K.ng * K.ng)), J = (int)((node / K.ng) % K.ng), Kz = (int)(node % K.ng);
-  float4 a = part[tid];
+  float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (any) {
+    a = part[tid];
 #pragma unroll
-  for (int c = 1; c < 9; ++c) {
-    const float4 q = part[c * kP2GNodes + tid];
-    a.x = a.x + q.x;
-    a.y = a.y + q.y;
-    a.z = a.z + q.z;
-    a.w = a.w + q.w;
+    for (int c = 1; c < 9; ++c) {
+      const float4 q = part[c * kP2GNodes + tid];
+      a.x = a.x + q.x;
+      a.y = a.y + q.y;
+      a.z = a.z + q.z;
+      a.w = a.w + q.w;
+    }
   }
   float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f, v3 = 0.0f;
   if (a.w > 0.0f) {
