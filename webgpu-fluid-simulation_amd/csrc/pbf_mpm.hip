@@ -16,6 +16,12 @@
 // Per step: prepare (J update, stress, affine matrix, base cell and binning
 // rank; the per-particle P2G record written once), scan, scatter, settle
 // (records moved into cell order), node gather + grid pass, G2P.
+//
+// The particle state (x, v, C, and ids: slot -> particle id) is kept in the
+// cell order of the previous step, as the PBF engine keeps its particles:
+// prepare then reads it in order and most of a wavefront's lanes share a
+// cell (one atomic per cell and wavefront for the binning rank), and G2P
+// writes the new state to its own slot.  Readback and upload go through ids.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -56,17 +62,19 @@ __device__ __forceinline__ float pick3(int k, float a, float b, float c) { retur
 
 // Prepare: J (MPMShader.ts:101), base cell and fractional offset
 // (:103-105), stress and affine matrix (:112-113), p_mass v; the record
-// (fx, p_mass v, affine columns) in particle order and the binning rank.
-__global__ void __launch_bounds__(MBS) k_mpm_prepare(uint32_t n, const float4 *__restrict__ x, float4 *v,
-                                                     const float4 *__restrict__ C0, const float4 *__restrict__ C1,
-                                                     const float4 *__restrict__ C2, float4 *__restrict__ rec,
-                                                     uint32_t *__restrict__ cellkey, uint32_t *__restrict__ rank,
-                                                     uint32_t *__restrict__ counts, MpmK K) {
+// (fx, p_mass v, affine columns, and J for G2P) of slot p and its binning
+// rank, one atomic per distinct cell of the wavefront.
+__global__ void __launch_bounds__(MBS) k_mpm_prepare(uint32_t n, const float4 *__restrict__ x,
+                                                     const float4 *__restrict__ v, const float4 *__restrict__ C0,
+                                                     const float4 *__restrict__ C1, const float4 *__restrict__ C2,
+                                                     float4 *__restrict__ rec, uint32_t *__restrict__ cellkey,
+                                                     uint32_t *__restrict__ rank, uint32_t *__restrict__ counts,
+                                                     MpmK K) {
   const uint32_t p = blockIdx.x * MBS + threadIdx.x;
+  uint32_t key = 0;
   if (p < n) {
     const float4 xp = x[p], vp = v[p], c0 = C0[p], c1 = C1[p], c2 = C2[p];
     const float J = vp.w * (1.0f + K.dt * ((c0.x + c1.y) + c2.z));
-    v[p] = make_float4(vp.x, vp.y, vp.z, J);
     const float X0 = xp.x / K.dx, X1 = xp.y / K.dx, X2 = xp.z / K.dx;
     const uint32_t b0 = f2u_sat(X0 - 0.5f), b1 = f2u_sat(X1 - 0.5f), b2 = f2u_sat(X2 - 0.5f);
     const float f0 = X0 - (float)b0, f1 = X1 - (float)b1, f2 = X2 - (float)b2;
@@ -81,45 +89,53 @@ __global__ void __launch_bounds__(MBS) k_mpm_prepare(uint32_t n, const float4 *_
     rec[s] = make_float4(f0, f1, f2, m * vp.x);
     rec[s + 1] = make_float4(m * vp.y, m * vp.z, a00, a01);
     rec[s + 2] = make_float4(a02, a10, a11, a12);
-    rec[s + 3] = make_float4(a20, a21, a22, 0.0f);
-    const uint32_t key = (b0 < K.ng && b1 < K.ng && b2 < K.ng) ? (b0 * K.ng + b1) * K.ng + b2 : K.G;  // K.G: outside
+    rec[s + 3] = make_float4(a20, a21, a22, J);  // w: J, which P2G does not read
+    key = (b0 < K.ng && b1 < K.ng && b2 < K.ng) ? (b0 * K.ng + b1) * K.ng + b2 : K.G;  // K.G: outside
     cellkey[p] = key;
-    // one atomic per lane: the particles stay in the caller's order, so a
-    // wavefront's keys rarely repeat (the settle orders each cell by id)
-    rank[p] = atomicAdd(&counts[key], 1u);
   }
+  // the settle orders each cell by id, so the order of the ranks is free
+  const uint32_t r = wave_atomic_rank(counts, key, p < n);
+  if (p < n) rank[p] = r;
 }
 
+// Slot p's particle id and slot to its binned position.
 __global__ void __launch_bounds__(MBS) k_mpm_scatter(uint32_t n, const uint32_t *__restrict__ cellkey,
                                                      const uint32_t *__restrict__ rank,
                                                      const uint32_t *__restrict__ offsets,
-                                                     uint32_t *__restrict__ tmp_id) {
+                                                     const uint32_t *__restrict__ ids, uint32_t *__restrict__ tmp_id,
+                                                     uint32_t *__restrict__ tmp_slot) {
   const uint32_t p = blockIdx.x * MBS + threadIdx.x;
   if (p >= n) return;
-  tmp_id[offsets[cellkey[p]] + rank[p]] = p;
+  const uint32_t t = offsets[cellkey[p]] + rank[p];
+  tmp_id[t] = ids[p];
+  tmp_slot[t] = p;
 }
 
 // Records into cell order, ascending id inside a cell (the bin of
-// particles outside the grid is never read: left in scatter order).
+// particles outside the grid is never read: left in scatter order), with
+// each particle's position (xs) and id (ids_new) at its new slot.
 __global__ void __launch_bounds__(MBS) k_mpm_settle(uint32_t n, const uint32_t *__restrict__ tmp_id,
+                                                    const uint32_t *__restrict__ tmp_slot,
                                                     const uint32_t *__restrict__ cellkey,
                                                     const uint32_t *__restrict__ offsets,
-                                                    const float4 *__restrict__ rec, float4 *__restrict__ srec,
-                                                    uint32_t *__restrict__ order, uint32_t outside) {
+                                                    const float4 *__restrict__ rec, const float4 *__restrict__ x,
+                                                    float4 *__restrict__ srec, float4 *__restrict__ xs,
+                                                    uint32_t *__restrict__ ids_new, uint32_t outside) {
   const uint32_t s = blockIdx.x * MBS + threadIdx.x;
   if (s >= n) return;
-  const uint32_t p = tmp_id[s], c = cellkey[p];
+  const uint32_t id = tmp_id[s], q = tmp_slot[s], c = cellkey[q];
   uint32_t slot = s;
   if (c != outside) {
     const uint32_t o = offsets[c], e = offsets[c + 1];
     uint32_t r = 0;
-    for (uint32_t t = o; t < e; ++t) r += tmp_id[t] < p ? 1u : 0u;
+    for (uint32_t t = o; t < e; ++t) r += tmp_id[t] < id ? 1u : 0u;
     slot = o + r;
   }
-  const size_t f = (size_t)slot * kRec, q = (size_t)p * kRec;
+  const size_t f = (size_t)slot * kRec, g = (size_t)q * kRec;
 #pragma unroll
-  for (int k = 0; k < (int)kRec; ++k) srec[f + k] = rec[q + k];
-  order[slot] = p;
+  for (int k = 0; k < (int)kRec; ++k) srec[f + k] = rec[g + k];
+  xs[slot] = x[q];
+  ids_new[slot] = id;
 }
 
 // P2G as a gather (MPMShader.ts:4-62 summed per node) and the grid pass
@@ -270,17 +286,21 @@ __global__ void __launch_bounds__(kP2GBS) k_mpm_p2g_grid(const uint32_t *__restr
 }
 
 // G2P (MPMShader.ts:240-280): 27 nodes in the shader's order (i outer, k
-// inner); nodes outside the grid read as zero.  Thread s takes particle
-// order[s] (the cell order of the step's sort, so a wavefront's node
-// gathers fall on neighbouring nodes), or s when order is null.
-__global__ void __launch_bounds__(MBS) k_mpm_g2p(uint32_t n, float4 *x, float4 *v, float4 *__restrict__ C0,
-                                                 float4 *__restrict__ C1, float4 *__restrict__ C2,
-                                                 const float4 *__restrict__ grid, const uint32_t *__restrict__ order,
+// inner); nodes outside the grid read as zero.  STEP: slot s of the step's
+// cell order takes its position from xs and J from its record (both moved
+// there by the settle) and writes the new state, and its id, to slot s;
+// otherwise (a G2P on its own) slot s reads and writes its own state.  In
+// cell order a wavefront's node gathers fall on neighbouring nodes.
+template <bool STEP>
+__global__ void __launch_bounds__(MBS) k_mpm_g2p(uint32_t n, float4 *__restrict__ x, float4 *__restrict__ v,
+                                                 float4 *__restrict__ C0, float4 *__restrict__ C1,
+                                                 float4 *__restrict__ C2, const float4 *__restrict__ grid,
+                                                 const float4 *__restrict__ xs, const float4 *__restrict__ srec,
+                                                 const uint32_t *__restrict__ ids_new, uint32_t *__restrict__ ids,
                                                  MpmK K) {
   const uint32_t s = blockIdx.x * MBS + threadIdx.x;
   if (s >= n) return;
-  const uint32_t p = order ? order[s] : s;
-  const float4 xp = x[p];
+  const float4 xp = STEP ? xs[s] : x[s];
   const float X[3] = {xp.x / K.dx, xp.y / K.dx, xp.z / K.dx};
   uint32_t b[3];
   float f[3], w[3][3];
@@ -315,23 +335,26 @@ __global__ void __launch_bounds__(MBS) k_mpm_g2p(uint32_t n, float4 *x, float4 *
 #pragma unroll
           for (int r = 0; r < 3; ++r) nC[3 * c + r] = nC[3 * c + r] + s * (vel[c] * dpos[r]);
       }
-  const float J = v[p].w;
-  v[p] = make_float4(nv[0], nv[1], nv[2], J);
-  C0[p] = make_float4(nC[0], nC[1], nC[2], 0.0f);
-  C1[p] = make_float4(nC[3], nC[4], nC[5], 0.0f);
-  C2[p] = make_float4(nC[6], nC[7], nC[8], 0.0f);
-  x[p] = make_float4(xp.x + K.dt * nv[0], xp.y + K.dt * nv[1], xp.z + K.dt * nv[2], xp.w);
+  const float J = STEP ? srec[(size_t)s * kRec + 3].w : v[s].w;
+  v[s] = make_float4(nv[0], nv[1], nv[2], J);
+  C0[s] = make_float4(nC[0], nC[1], nC[2], 0.0f);
+  C1[s] = make_float4(nC[3], nC[4], nC[5], 0.0f);
+  C2[s] = make_float4(nC[6], nC[7], nC[8], 0.0f);
+  x[s] = make_float4(xp.x + K.dt * nv[0], xp.y + K.dt * nv[1], xp.z + K.dt * nv[2], xp.w);
+  if (STEP) ids[s] = ids_new[s];
 }
 
 __global__ void __launch_bounds__(MBS) k_mpm_init(uint32_t n, const float4 *__restrict__ in, float4 *__restrict__ x,
                                                   float4 *__restrict__ v, float4 *__restrict__ C0,
-                                                  float4 *__restrict__ C1, float4 *__restrict__ C2) {
+                                                  float4 *__restrict__ C1, float4 *__restrict__ C2,
+                                                  uint32_t *__restrict__ ids) {
   const uint32_t p = blockIdx.x * MBS + threadIdx.x;
   if (p >= n) return;
   const float4 a = in[p];
   x[p] = make_float4(a.x, a.y, a.z, 0.0f);
   v[p] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);  // J = 1 (MPM.ts:115-119)
   C0[p] = C1[p] = C2[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  ids[p] = p;  // caller order
 }
 
 inline uint32_t nblk(uint64_t n) { return (uint32_t)((n + MBS - 1) / MBS); }
@@ -353,7 +376,9 @@ struct pbf_mpm {
   float4 *x = nullptr, *v = nullptr, *C0 = nullptr, *C1 = nullptr, *C2 = nullptr, *grid = nullptr;
   float4 *rec = nullptr, *srec = nullptr, *stage = nullptr;
   uint32_t *cellkey = nullptr, *rank = nullptr, *counts = nullptr, *offsets = nullptr, *tmp_id = nullptr;
-  uint32_t *order = nullptr;  // slot (cell order of the step) -> particle
+  uint32_t *tmp_slot = nullptr;
+  float4 *xs = nullptr;                           // positions moved into the step's cell order
+  uint32_t *ids = nullptr, *ids_new = nullptr;  // slot -> particle id (state order; the step's new order)
   std::vector<uint32_t *> scan_sums, scan_prefix;
   hipGraphExec_t graph = nullptr;
   uint32_t graph_n = 0;
@@ -381,17 +406,18 @@ void enqueue_step(pbf_mpm *m) {
   // counts over G + 1 bins (the last: particles whose base cell is outside
   // the grid, which the reference's bounds test skips); zeroed by the scan
   launch_scan(st, m->counts, m->G + 1, m->offsets, 1, m->scan_sums.data(), m->scan_prefix.data(), 0);
-  hipLaunchKernelGGL(k_mpm_scatter, dim3(nblk(n)), dim3(MBS), 0, st, n, m->cellkey, m->rank, m->offsets, m->tmp_id);
-  hipLaunchKernelGGL(k_mpm_settle, dim3(nblk(n)), dim3(MBS), 0, st, n, m->tmp_id, m->cellkey, m->offsets, m->rec,
-                     m->srec, m->order, m->G);
+  hipLaunchKernelGGL(k_mpm_scatter, dim3(nblk(n)), dim3(MBS), 0, st, n, m->cellkey, m->rank, m->offsets, m->ids,
+                     m->tmp_id, m->tmp_slot);
+  hipLaunchKernelGGL(k_mpm_settle, dim3(nblk(n)), dim3(MBS), 0, st, n, m->tmp_id, m->tmp_slot, m->cellkey,
+                     m->offsets, m->rec, m->x, m->srec, m->xs, m->ids_new, m->G);
   if (m->G <= kSmallGrid)
     hipLaunchKernelGGL(k_mpm_p2g_grid<1>, dim3((m->G + 63u) / 64u), dim3(kP2GBS), 0, st, m->offsets, m->srec,
                        m->grid, m->K);
   else
     hipLaunchKernelGGL(k_mpm_p2g_grid<4>, dim3((m->G + 255u) / 256u), dim3(kP2GBS), 0, st, m->offsets, m->srec,
                        m->grid, m->K);
-  hipLaunchKernelGGL(k_mpm_g2p, dim3(nblk(n)), dim3(MBS), 0, st, n, m->x, m->v, m->C0, m->C1, m->C2, m->grid,
-                     (const uint32_t *)m->order, m->K);
+  hipLaunchKernelGGL(k_mpm_g2p<true>, dim3(nblk(n)), dim3(MBS), 0, st, n, m->x, m->v, m->C0, m->C1, m->C2, m->grid,
+                     (const float4 *)m->xs, (const float4 *)m->srec, (const uint32_t *)m->ids_new, m->ids, m->K);
 }
 
 void drop_graph(pbf_mpm *m) {
@@ -462,7 +488,8 @@ int pbf_mpm_create(const pbf_mpm_params *params, pbf_mpm **out) try {
   step(dalloc(&m->rec, kRec * N), "alloc"); step(dalloc(&m->srec, kRec * N), "alloc");
   step(dalloc(&m->stage, N), "alloc");
   step(dalloc(&m->cellkey, N), "alloc"); step(dalloc(&m->rank, N), "alloc"); step(dalloc(&m->tmp_id, N), "alloc");
-  step(dalloc(&m->order, N), "alloc");
+  step(dalloc(&m->tmp_slot, N), "alloc"); step(dalloc(&m->xs, N), "alloc");
+  step(dalloc(&m->ids, N), "alloc"); step(dalloc(&m->ids_new, N), "alloc");
   step(dalloc(&m->grid, (size_t)m->G), "alloc");
   step(dalloc(&m->counts, (size_t)m->G + 1), "alloc");
   step(dalloc(&m->offsets, (size_t)m->G + 2), "alloc");
@@ -499,7 +526,7 @@ int pbf_mpm_destroy(pbf_mpm *m) {
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   drop_graph(m);
   void *bufs[] = {m->x, m->v, m->C0, m->C1, m->C2, m->grid, m->rec, m->srec, m->stage, m->cellkey, m->rank,
-                  m->counts, m->offsets, m->tmp_id, m->order};
+                  m->counts, m->offsets, m->tmp_id, m->tmp_slot, m->xs, m->ids, m->ids_new};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto q : m->scan_sums)
@@ -521,7 +548,7 @@ int pbf_mpm_set_positions(pbf_mpm *m, const float *xyzw, uint32_t count) try {
   if (count) {
     MCHK(hipMemcpy(m->stage, xyzw, (size_t)count * 16, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_mpm_init, dim3(nblk(count)), dim3(MBS), 0, m->stream, count, m->stage, m->x, m->v, m->C0,
-                       m->C1, m->C2);
+                       m->C1, m->C2, m->ids);
     MCHK(hipGetLastError());
     MCHK(hipStreamSynchronize(m->stream));
   }
@@ -578,20 +605,22 @@ int pbf_mpm_read(pbf_mpm *m, int field, float *out, size_t count) try {
   }
   if (count != n) return mfail(m, PBF_E_LENGTH, "readback count must equal the particle count");
   if (!n) return PBF_OK;
-  if (field == PBF_MPM_POSITION) {
-    MCHK(hipMemcpy(out, m->x, n * 16, hipMemcpyDeviceToHost));
-  } else if (field == PBF_MPM_VELOCITY) {
-    MCHK(hipMemcpy(out, m->v, n * 16, hipMemcpyDeviceToHost));
-  } else if (field == PBF_MPM_AFFINE) {
-    std::vector<float> c(n * 4);
+  if (field != PBF_MPM_POSITION && field != PBF_MPM_VELOCITY && field != PBF_MPM_AFFINE)
+    return mfail(m, PBF_E_INVALID, "unknown MPM field");
+  // the state is in slot order: slot s holds particle ids[s]
+  std::vector<uint32_t> ids(n);
+  MCHK(hipMemcpy(ids.data(), m->ids, n * 4, hipMemcpyDeviceToHost));
+  std::vector<float> c(n * 4);
+  if (field == PBF_MPM_AFFINE) {
     float4 *cols[3] = {m->C0, m->C1, m->C2};
     for (int k = 0; k < 3; ++k) {
       MCHK(hipMemcpy(c.data(), cols[k], n * 16, hipMemcpyDeviceToHost));
-      for (size_t p = 0; p < n; ++p)
-        for (int r = 0; r < 3; ++r) out[9 * p + 3 * k + r] = c[4 * p + r];
+      for (size_t s = 0; s < n; ++s)
+        for (int r = 0; r < 3; ++r) out[9 * (size_t)ids[s] + 3 * k + r] = c[4 * s + r];
     }
   } else {
-    return mfail(m, PBF_E_INVALID, "unknown MPM field");
+    MCHK(hipMemcpy(c.data(), field == PBF_MPM_POSITION ? m->x : m->v, n * 16, hipMemcpyDeviceToHost));
+    for (size_t s = 0; s < n; ++s) std::memcpy(out + 4 * (size_t)ids[s], c.data() + 4 * s, 16);
   }
   return PBF_OK;
 }
@@ -609,20 +638,22 @@ int pbf_mpm_write(pbf_mpm *m, int field, const float *in, size_t count) try {
   }
   if (count != n) return mfail(m, PBF_E_LENGTH, "count must equal the particle count");
   if (!n) return PBF_OK;
-  if (field == PBF_MPM_POSITION) {
-    MCHK(hipMemcpy(m->x, in, n * 16, hipMemcpyHostToDevice));
-  } else if (field == PBF_MPM_VELOCITY) {
-    MCHK(hipMemcpy(m->v, in, n * 16, hipMemcpyHostToDevice));
-  } else if (field == PBF_MPM_AFFINE) {
-    std::vector<float> c(n * 4, 0.0f);
+  if (field != PBF_MPM_POSITION && field != PBF_MPM_VELOCITY && field != PBF_MPM_AFFINE)
+    return mfail(m, PBF_E_INVALID, "unknown MPM field");
+  // into slot order: slot s takes particle ids[s]
+  std::vector<uint32_t> ids(n);
+  MCHK(hipMemcpy(ids.data(), m->ids, n * 4, hipMemcpyDeviceToHost));
+  std::vector<float> c(n * 4, 0.0f);
+  if (field == PBF_MPM_AFFINE) {
     float4 *cols[3] = {m->C0, m->C1, m->C2};
     for (int k = 0; k < 3; ++k) {
-      for (size_t p = 0; p < n; ++p)
-        for (int r = 0; r < 3; ++r) c[4 * p + r] = in[9 * p + 3 * k + r];
+      for (size_t s = 0; s < n; ++s)
+        for (int r = 0; r < 3; ++r) c[4 * s + r] = in[9 * (size_t)ids[s] + 3 * k + r];
       MCHK(hipMemcpy(cols[k], c.data(), n * 16, hipMemcpyHostToDevice));
     }
   } else {
-    return mfail(m, PBF_E_INVALID, "unknown MPM field");
+    for (size_t s = 0; s < n; ++s) std::memcpy(c.data() + 4 * s, in + 4 * (size_t)ids[s], 16);
+    MCHK(hipMemcpy(field == PBF_MPM_POSITION ? m->x : m->v, c.data(), n * 16, hipMemcpyHostToDevice));
   }
   return PBF_OK;
 }
@@ -632,8 +663,9 @@ int pbf_mpm_g2p(pbf_mpm *m) try {
   if (!m) return mfail(m, PBF_E_INVALID, "null context");
   if (!m->n) return PBF_OK;
   MCHK(hipSetDevice(m->device));
-  hipLaunchKernelGGL(k_mpm_g2p, dim3(nblk(m->n)), dim3(MBS), 0, m->stream, m->n, m->x, m->v, m->C0, m->C1, m->C2,
-                     m->grid, (const uint32_t *)nullptr, m->K);
+  hipLaunchKernelGGL(k_mpm_g2p<false>, dim3(nblk(m->n)), dim3(MBS), 0, m->stream, m->n, m->x, m->v, m->C0, m->C1,
+                     m->C2, m->grid, (const float4 *)nullptr, (const float4 *)nullptr, (const uint32_t *)nullptr,
+                     (uint32_t *)nullptr, m->K);
   MCHK(hipGetLastError());
   return PBF_OK;
 }
