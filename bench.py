@@ -391,7 +391,7 @@ def warm_start_profile():
 # moves the record; the node gather reads it once; G2P reads x, v and writes
 # x, v, C) and per grid node (scan 12, the gather's cell offsets 8, grid write
 # 16, G2P's node read 16).
-MPM_PASS_BYTES = {"prepare": 184, "scatter": 16, "settle": 176, "p2g_grid": 80, "g2p": 112}
+MPM_PASS_BYTES = {"prepare": 152, "scatter": 24, "settle": 184, "p2g_grid": 80, "g2p": 120}
 MPM_NODE_BYTES = {"scan": 12, "p2g_grid": 24, "g2p": 16}
 
 
