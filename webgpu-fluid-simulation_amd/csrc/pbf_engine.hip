@@ -499,9 +499,10 @@ void copy_state(pbf_ctx *c, bool save, bool readable = true) {
 
 // One learned substep.  With a rejection threshold set, the check runs on the
 // device in the substep's own launch sequence (graph): the energy is summed,
-// the state saved, the learned substep runs, the FP64 mean density error, the
-// largest |v| dt / h and the energy are reduced, and k_fallback_gate writes
-// its decision to pinned host memory.  The host reads that one word once the
+// the state saved, the learned substep runs (its density pass also leaves
+// every particle's FP64 density error), one reduction takes the mean error,
+// the largest |v| dt / h and the energy, and k_fallback_gate writes its
+// decision to pinned host memory.  The host reads that one word once the
 // stream has drained and, for a rejected substep, launches the restore and
 // the plain re-solve (the collection mode: the plain solve that also keeps
 // the lambda sum the next learned substep reads).  Predicating the re-solve's
