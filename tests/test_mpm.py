@@ -67,6 +67,33 @@ def test_stressed_state_bit_exact(flags):
     sim.close()
 
 
+def test_state_round_trip_after_steps_keeps_caller_order():
+    """The engine keeps particles in the cell order of its last step; reading
+    every field after some steps and writing it back (through the slot -> id
+    map) must leave the run on the oracle's track, which never reorders."""
+    rng = np.random.default_rng(8)
+    n = 12000
+    x = np.zeros((n, 4), np.float32)
+    x[:, :3] = rng.uniform(0.2, 0.7, (n, 3))
+    x[:, 3] = rng.uniform(-1.0, 1.0, n)  # w is carried untouched
+    sim = MPM(n_particle=n, n_grid=48)
+    sim.setParticlePosition(x)
+    sim.write(MPM_POSITION, x)
+    o = oracle.MpmOracle(x, n_grid=48)
+    o.set(oracle.MPM_POSITION, x)
+    sim.run(6)
+    o.step(6)
+    compare(sim, o)
+    state = {f: sim.read(f) for f in (MPM_POSITION, MPM_VELOCITY, MPM_AFFINE)}
+    for f, a in state.items():
+        sim.write(f, a)
+    compare(sim, o)
+    sim.run(6)
+    o.step(6)
+    compare(sim, o)
+    sim.close()
+
+
 def test_g2p_reproduces_a_linear_field():
     n, ng = 3000, 32
     rng = np.random.default_rng(6)
