@@ -24,7 +24,7 @@ struct Phys {
   float vort, tension;
   uint32_t ld;   // leading dimension of the slot-major neighbour table
   uint32_t maxn; // neighbour capacity per particle
-  uint32_t *nbr16;  // 16-bit neighbour table (offset pairs), slot-major
+  uint32_t *nbr16;  // 16-bit neighbour table, 8-entry chunks, slot-major
   uint32_t *wfmt;   // per wavefront: 1 = 16-bit table, 0 = 32-bit table
   const uint32_t *cellsub;  // per cell: its particles below z-slabs 1, 2, 3 and its count (8 bits each)
   uint32_t slab_log2;       // log2 of the z-slabs per cell (0: one slab)

@@ -745,7 +745,7 @@ int create_ctx_body(const pbf_params *params, uint32_t nscenes, const pbf_scene_
   step(dalloc(&c->ids, N), "alloc"); step(dalloc(&c->cellkey, N), "alloc"); step(dalloc(&c->rank, N), "alloc");
   step(dalloc(&c->tmp_key, N), "alloc"); step(dalloc(&c->tmp_src, N), "alloc"); step(dalloc(&c->ncount, N), "alloc");
   step(dalloc(&c->nbr, (size_t)c->maxn * c->ld), "alloc");
-  step(dalloc(&c->nbr16, (size_t)((c->maxn + 1) / 2) * c->ld), "alloc");
+  step(dalloc(&c->nbr16, (size_t)((c->maxn + 7) / 8) * 4 * c->ld), "alloc");
   step(dalloc(&c->wfmt, (size_t)c->ncap / 64), "alloc");
   step(dalloc(&c->lam, N), "alloc"); step(dalloc(&c->lam_sorted, N), "alloc");
   step(dalloc(&c->counts, (size_t)c->ncells), "alloc");
@@ -1445,7 +1445,7 @@ int pbf_read_neighbors(pbf_ctx *c, uint32_t *nbr_off, uint32_t *nbr_ids, uint64_
   }
   uint32_t mx = 0;
   for (uint32_t i = 0; i < n; ++i) mx = std::max(mx, cnt[i]);
-  std::vector<uint32_t> table((size_t)mx * c->ld), table16((size_t)((mx + 1) / 2) * c->ld);
+  std::vector<uint32_t> table((size_t)mx * c->ld), table16((size_t)((mx + 7) / 8) * 4 * c->ld);
   std::vector<uint32_t> wfmt((size_t)c->ncap / 64);
   if (mx) {
     HIPCHK(hipMemcpy(table.data(), c->nbr, table.size() * 4, hipMemcpyDeviceToHost));
@@ -1455,7 +1455,7 @@ int pbf_read_neighbors(pbf_ctx *c, uint32_t *nbr_off, uint32_t *nbr_ids, uint64_
   // entry k of slot s in the wavefront's format (16-bit offsets or slots)
   auto entry = [&](uint32_t s, uint32_t k) -> uint32_t {
     if (!wfmt[s >> 6]) return table[(size_t)k * c->ld + s];
-    const uint32_t w = table16[(size_t)(k >> 1) * c->ld + s];
+    const uint32_t w = table16[((size_t)(k >> 3) * c->ld + s) * 4 + ((k & 7u) >> 1)];
     return s - 32768u + ((k & 1u) ? (w >> 16) : (w & 0xffffu));
   };
   std::vector<uint32_t> by_id(n ? n : 1), slot_of(n ? n : 1);

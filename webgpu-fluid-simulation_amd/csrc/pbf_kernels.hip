@@ -43,13 +43,19 @@ __device__ __forceinline__ uint32_t slot_index() {
 // The neighbour table is slot-major: entry k of particle i sits at
 // nbr[k * ld + i], so one load instruction of a wavefront reads 256
 // contiguous bytes.  A wavefront whose entries all lie within +-32767 slots
-// of their particles is stored instead as 16-bit offsets j - i + 32768,
-// entries 2p and 2p + 1 packed in word nbr16[p * ld + i] (wfmt[wave] = 1):
-// half the bytes for every walk.  Dense grids list neighbours in ascending
-// slot order, so the first and last entries bound the offsets; hashed grids
-// take the range over every entry.
+// of their particles is stored instead as 16-bit offsets j - i + 32768
+// (wfmt[wave] = 1): half the bytes for every walk.  Entries 8c..8c+7 of slot
+// i form one 16-byte chunk at chunk c * ld + i, so a lane reads eight
+// entries with one load and a wavefront's load covers 1 KB of consecutive
+// chunks.  Dense grids list neighbours in ascending slot order, so the first
+// and last entries bound the offsets; hashed grids take the range over every
+// entry.
 __device__ __forceinline__ bool wave16(const Phys &P, uint32_t i) {
   return __builtin_amdgcn_readfirstlane(P.wfmt[i >> 6]) != 0u;
+}
+// Word holding 16-bit entries k and k + 1 (k even) of slot i.
+__device__ __forceinline__ size_t chunk_word(uint32_t ld, uint32_t i, uint32_t k) {
+  return ((size_t)(k >> 3) * ld + i) * 4u + ((k & 7u) >> 1);
 }
 
 // Per-lane walk (learned-feature and diagnostic passes): visits the
@@ -59,15 +65,14 @@ __device__ __forceinline__ void for_each_neighbor(const uint32_t *__restrict__ n
                                                   uint32_t cnt, F &&f) {
   const uint32_t ld = P.ld;
   if (wave16(P, i)) {
-    const uint32_t *row = P.nbr16 + i;
     const uint32_t base = i - 32768u;
     uint32_t k = 0;
     for (; k + 2 <= cnt; k += 2) {
-      const uint32_t w = row[(size_t)(k >> 1) * ld];
+      const uint32_t w = P.nbr16[chunk_word(ld, i, k)];
       f(base + (w & 0xffffu));
       f(base + (w >> 16));
     }
-    if (k < cnt) f(base + (row[(size_t)(k >> 1) * ld] & 0xffffu));
+    if (k < cnt) f(base + (P.nbr16[chunk_word(ld, i, k)] & 0xffffu));
     return;
   }
   const uint32_t *row = nbr + i;
@@ -116,6 +121,8 @@ __device__ __forceinline__ uint32_t active_max(uint32_t v) {
 constexpr int kBufWord3 = 0x00020000;
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x3 __attribute__((ext_vector_type(3)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 struct Gather {
   __amdgpu_buffer_rsrc_t r;
   __device__ Gather(const float4 *p, uint32_t n)
@@ -148,18 +155,15 @@ __device__ __forceinline__ void for_each_neighbor_w(const uint32_t *__restrict__
   using T = decltype(fetch(0u));
   uint32_t k = 0;
   if (wave16(P, i)) {
-    const uint32_t base = i - 32768u;
-    auto pairs = [&](uint32_t k) {  // pair rows k/2.. of the 16-bit table as one buffer
-      return __builtin_amdgcn_make_buffer_rsrc(P.nbr16 + (size_t)(k >> 1) * ld, 0, -1, kBufWord3);
+    const uint32_t base = i - 32768u, col16 = i * 16u;
+    auto chunks = [&](uint32_t k) {  // chunk k/8 of the 16-bit table as one buffer
+      return __builtin_amdgcn_make_buffer_rsrc(P.nbr16 + (size_t)(k >> 3) * ld * 4u, 0, -1, kBufWord3);
     };
-    // each batch: the table loads, then every gather, then the bodies
+    // each batch: one table load, then every gather, then the bodies
     if constexpr (UNROLL >= 8) {
       for (; k + 8 <= kmax; k += 8) {
-        const auto r = pairs(k);
-        uint32_t w[4];
+        const u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(chunks(k), col16, 0, 0);
         T d[8];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) w[q] = __builtin_amdgcn_raw_buffer_load_b32(r, col, q * rstride, 0);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           d[2 * q] = fetch(base + (w[q] & 0xffffu));
@@ -169,12 +173,9 @@ __device__ __forceinline__ void for_each_neighbor_w(const uint32_t *__restrict__
         for (int q = 0; q < 8; ++q) use(d[q], k + q < cnt);
       }
     }
-    for (; k + 4 <= kmax; k += 4) {
-      const auto r = pairs(k);
-      uint32_t w[2];
+    for (; k + 4 <= kmax; k += 4) {  // k % 8 is 0 or 4
+      const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(chunks(k), col16, (k & 7u) * 2u, 0);
       T d[4];
-#pragma unroll
-      for (int q = 0; q < 2; ++q) w[q] = __builtin_amdgcn_raw_buffer_load_b32(r, col, q * rstride, 0);
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         d[2 * q] = fetch(base + (w[q] & 0xffffu));
@@ -184,12 +185,13 @@ __device__ __forceinline__ void for_each_neighbor_w(const uint32_t *__restrict__
       for (int q = 0; q < 4; ++q) use(d[q], k + q < cnt);
     }
     for (; k + 2 <= kmax; k += 2) {
-      const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(pairs(k), col, 0, 0);
+      const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(chunks(k), col16, (k & 7u) * 2u, 0);
       const T d0 = fetch(base + (w & 0xffffu)), d1 = fetch(base + (w >> 16));
       use(d0, k < cnt);
       use(d1, k + 1 < cnt);
     }
-    if (k < kmax) use(fetch(base + (__builtin_amdgcn_raw_buffer_load_b32(pairs(k), col, 0, 0) & 0xffffu)), k < cnt);
+    if (k < kmax)
+      use(fetch(base + (__builtin_amdgcn_raw_buffer_load_b32(chunks(k), col16, (k & 7u) * 2u, 0) & 0xffffu)), k < cnt);
     return;
   }
   auto rows = [&](uint32_t k) {  // rows k.. of the table as one buffer
@@ -232,18 +234,13 @@ __device__ __forceinline__ void for_each_neighbor_g(const uint32_t *__restrict__
   const uint32_t ld = P.ld;
   uint32_t k = 0;
   if (wave16(P, i)) {
-    const uint32_t *row = P.nbr16 + i;
-    const uint32_t base = i - 32768u;
-    const uint32_t col = i * 4u, rstride = ld * 4u;
-    auto pairs = [&](uint32_t k) {  // pair rows k/2.. of the 16-bit table as one buffer
-      return __builtin_amdgcn_make_buffer_rsrc(P.nbr16 + (size_t)(k >> 1) * ld, 0, -1, kBufWord3);
+    const uint32_t base = i - 32768u, col16 = i * 16u;
+    auto chunks = [&](uint32_t k) {  // chunk k/8 of the 16-bit table as one buffer
+      return __builtin_amdgcn_make_buffer_rsrc(P.nbr16 + (size_t)(k >> 3) * ld * 4u, 0, -1, kBufWord3);
     };
     if constexpr (UNROLL >= 8) {
       for (; k + 8 <= kmax; k += 8) {
-        const auto r = pairs(k);
-        uint32_t w[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) w[q] = __builtin_amdgcn_raw_buffer_load_b32(r, col, q * rstride, 0);
+        const u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(chunks(k), col16, 0, 0);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           f(base + (w[q] & 0xffffu), k + 2 * q < cnt);
@@ -251,11 +248,8 @@ __device__ __forceinline__ void for_each_neighbor_g(const uint32_t *__restrict__
         }
       }
     }
-    for (; k + 4 <= kmax; k += 4) {
-      const auto r = pairs(k);
-      uint32_t w[2];
-#pragma unroll
-      for (int q = 0; q < 2; ++q) w[q] = __builtin_amdgcn_raw_buffer_load_b32(r, col, q * rstride, 0);
+    for (; k + 4 <= kmax; k += 4) {  // k % 8 is 0 or 4
+      const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(chunks(k), col16, (k & 7u) * 2u, 0);
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         f(base + (w[q] & 0xffffu), k + 2 * q < cnt);
@@ -263,11 +257,11 @@ __device__ __forceinline__ void for_each_neighbor_g(const uint32_t *__restrict__
       }
     }
     for (; k + 2 <= kmax; k += 2) {
-      const uint32_t w = row[(size_t)(k >> 1) * ld];
+      const uint32_t w = P.nbr16[chunk_word(ld, i, k)];
       f(base + (w & 0xffffu), k < cnt);
       f(base + (w >> 16), k + 1 < cnt);
     }
-    if (k < kmax) f(base + (row[(size_t)(k >> 1) * ld] & 0xffffu), k < cnt);
+    if (k < kmax) f(base + (P.nbr16[chunk_word(ld, i, k)] & 0xffffu), k < cnt);
     return;
   }
   const uint32_t *row = nbr + i;
@@ -618,8 +612,9 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
   const uint32_t tid = threadIdx.x;
   const uint32_t base_slot = xcd_block(blockIdx.x, gridDim.x) * LIST_BS;
   const uint32_t i = base_slot + tid;
+  static_assert(LIST_R % 8 == 0, "the LDS rows fill whole 8-entry chunks");
   auto store16 = [&](uint32_t k, uint32_t j) {  // entry k as a 16-bit offset
-    reinterpret_cast<uint16_t *>(P.nbr16)[((size_t)(k >> 1) * P.ld + i) * 2 + (k & 1u)] =
+    reinterpret_cast<uint16_t *>(P.nbr16)[((size_t)(k >> 3) * P.ld + i) * 8u + (k & 7u)] =
         (uint16_t)(j - (i - 32768u));
   };
   uint32_t cnt = 0, last = i, smin = i, smax = i;
@@ -826,13 +821,20 @@ __global__ void __launch_bounds__(LIST_BS) k_build_list(uint32_t n, const float4
     const uint32_t c = cnt < P.maxn ? cnt : P.maxn;
     const uint32_t k1 = c > (uint32_t)LIST_R ? c : (uint32_t)LIST_R;
     if (w16) {
-      // whole pair words for the LDS rows (R is even), then half-words; the
+      // whole 8-entry chunks for the LDS rows (R is a multiple of 8; entries
+      // of the last chunk past wcap are never read), then half-words; the
       // walks read whole pairs, so padding runs to wcap rounded up to even
       const uint32_t base = i - 32768u;
-      for (uint32_t k = 0; k < lrows; k += 2) {
-        const uint32_t a = k < c ? tile[k * LIST_BS + tid] : i;
-        const uint32_t b = k + 1 < c ? tile[(k + 1) * LIST_BS + tid] : i;
-        P.nbr16[(size_t)(k >> 1) * P.ld + i] = (a - base) | ((b - base) << 16);
+      for (uint32_t k = 0; k < lrows; k += 8) {
+        uint32_t w[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t ka = k + 2 * q, kb = ka + 1;
+          const uint32_t a = ka < c ? tile[ka * LIST_BS + tid] : i;
+          const uint32_t b = kb < c ? tile[kb * LIST_BS + tid] : i;
+          w[q] = (a - base) | ((b - base) << 16);
+        }
+        reinterpret_cast<uint4 *>(P.nbr16)[(size_t)(k >> 3) * P.ld + i] = make_uint4(w[0], w[1], w[2], w[3]);
       }
       for (uint32_t k = k1; k < ((wcap + 1u) & ~1u); ++k) store16(k, i);
     } else {
@@ -887,7 +889,7 @@ __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__
   // of poly6() and spiky_f()); padding entries (j = i) have a zero gradient
   // and their poly6 term is not added
   const Gather GX(X, n);
-  for_each_neighbor_w<4>(nbr, i, P, ncount[i], [&](uint32_t j) { return GX.xyz(j); },
+  for_each_neighbor_w<8>(nbr, i, P, ncount[i], [&](uint32_t j) { return GX.xyz(j); },
                          [&](const float3 &xj, bool valid) {
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float d2 = dist2(dx, dy, dz);
