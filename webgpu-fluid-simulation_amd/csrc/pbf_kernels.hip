@@ -147,7 +147,7 @@ struct Gather {
 // nothing (d = 0: every gradient and difference term is 0; a poly6 sum takes
 // them with weight 0).  Valid entries are visited in walk order, so every sum
 // is the same as the per-lane loop's.
-template <int UNROLL, typename Fetch, typename Use>
+template <int UNROLL, int UNROLL32 = UNROLL, typename Fetch, typename Use>
 __device__ __forceinline__ void for_each_neighbor_w(const uint32_t *__restrict__ nbr, uint32_t i, const Phys &P,
                                                     uint32_t cnt, Fetch &&fetch, Use &&use) {
   const uint32_t kmax = active_max(cnt);
@@ -197,7 +197,7 @@ __device__ __forceinline__ void for_each_neighbor_w(const uint32_t *__restrict__
   auto rows = [&](uint32_t k) {  // rows k.. of the table as one buffer
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(nbr + (size_t)k * ld), 0, -1, kBufWord3);
   };
-  if constexpr (UNROLL >= 8) {
+  if constexpr (UNROLL32 >= 8) {
     for (; k + 8 <= kmax; k += 8) {
       const auto r = rows(k);
       uint32_t j[8];
@@ -889,7 +889,7 @@ __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__
   // of poly6() and spiky_f()); padding entries (j = i) have a zero gradient
   // and their poly6 term is not added
   const Gather GX(X, n);
-  for_each_neighbor_w<8>(nbr, i, P, ncount[i], [&](uint32_t j) { return GX.xyz(j); },
+  for_each_neighbor_w<8, 4>(nbr, i, P, ncount[i], [&](uint32_t j) { return GX.xyz(j); },
                          [&](const float3 &xj, bool valid) {
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float d2 = dist2(dx, dy, dz);
