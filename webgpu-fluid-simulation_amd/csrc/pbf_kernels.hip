@@ -963,7 +963,7 @@ __global__ void __launch_bounds__(BS) k_lambda_feat(uint32_t n, float4 *X, float
   // the previous velocity gathered too; padding entries (j = i) have a zero
   // gradient and velocity difference, and their poly6 term is not added
   const Gather GX(X, n), GV(lb.vel_prev, n);
-  for_each_neighbor_w<4>(nbr, i, P, cnt, [&](uint32_t j) { return FeatRec{GX.xyz(j), GV[j]}; },
+  for_each_neighbor_w<8, 4>(nbr, i, P, cnt, [&](uint32_t j) { return FeatRec{GX.xyz(j), GV[j]}; },
                          [&](const FeatRec &r, bool valid) {
     float dx = xi.x - r.x.x, dy = xi.y - r.x.y, dz = xi.z - r.x.z;
     float d2 = dist2(dx, dy, dz);
