@@ -1138,7 +1138,7 @@ __global__ void __launch_bounds__(BS) k_xsph(uint32_t n, const float4 *__restric
   const float rhoi = xi.w;
   float ex = 0.0f, ey = 0.0f, ez = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
   // padding entries (j = i): zero gradient, cohesion and normal difference
-  for_each_neighbor_g<4>(nbr, i, P, ncount[i], [&](uint32_t j, bool) {
+  for_each_neighbor_g<8>(nbr, i, P, ncount[i], [&](uint32_t j, bool) {
     const float4 xj = pos[j], nj = nrm[j];
     const float rhoj = xj.w;
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
