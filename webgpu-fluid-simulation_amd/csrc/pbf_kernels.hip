@@ -284,6 +284,13 @@ __device__ __forceinline__ void for_each_neighbor_g(const uint32_t *__restrict__
   for (; k < kmax; ++k) f(row[(size_t)k * ld], k < cnt);
 }
 
+// The gather walk with the split (fetch, use) interface of the buffer walk.
+template <int UNROLL, typename Fetch, typename Use>
+__device__ __forceinline__ void for_each_neighbor_gw(const uint32_t *__restrict__ nbr, uint32_t i, const Phys &P,
+                                                     uint32_t cnt, Fetch &&fetch, Use &&use) {
+  for_each_neighbor_g<UNROLL>(nbr, i, P, cnt, [&](uint32_t j, bool v) { use(fetch(j), v); });
+}
+
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -866,8 +873,12 @@ __global__ void __launch_bounds__(BS) k_boundary(uint32_t n, const float4 *__res
 // generation modes); 2 projected accumulation: the increment comes from the
 // unclamped constraint rho~ - 1, the substep's sum stays <= 0, and X[i].w
 // receives the change of the sum (an over-pushed warm start is pulled back,
-// never past a zero sum).
-template <bool MULTI, int FK, int ACC = 0>
+// never past a zero sum).  GATHER: wavefronts on the 32-bit table walk with
+// 64-bit global gathers (hashed grids, where most wavefronts take that
+// table: 292 -> 267 us per launch at the 4M obstacle drop; compiled into the
+// dense-grid kernel it costs 1 us there); the pairs and their order are the
+// same either way.
+template <bool MULTI, int FK, int ACC = 0, bool GATHER = false>
 __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__restrict__ B,
                                                const uint32_t *__restrict__ nbr, const uint32_t *__restrict__ ncount,
                                                const float *__restrict__ pool, Scene s0,
@@ -889,8 +900,7 @@ __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__
   // of poly6() and spiky_f()); padding entries (j = i) have a zero gradient
   // and their poly6 term is not added
   const Gather GX(X, n);
-  for_each_neighbor_w<8, 4>(nbr, i, P, ncount[i], [&](uint32_t j) { return GX.xyz(j); },
-                         [&](const float3 &xj, bool valid) {
+  auto pair = [&](const float3 &xj, bool valid) {
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float d2 = dist2(dx, dy, dz);
     const bool in = d2 <= P.h2;
@@ -904,7 +914,19 @@ __global__ void __launch_bounds__(BS) k_lambda(uint32_t n, float4 *X, float4 *__
     gx += ax; gy += ay; gz += az;
     s2 = fmaf(az, az, fmaf(ay, ay, fmaf(ax, ax, s2)));
     rho += (in && valid) ? w : 0.0f;
-  });
+  };
+  auto fetch = [&](uint32_t j) { return GX.xyz(j); };
+  if constexpr (GATHER) {
+    if (wave16(P, i))
+      for_each_neighbor_w<8>(nbr, i, P, ncount[i], fetch, pair);
+    else
+      for_each_neighbor_gw<8>(nbr, i, P, ncount[i], [&](uint32_t j) {
+        const float4 *q = X + j;
+        return make_float3(q->x, q->y, q->z);
+      }, pair);
+  } else {
+    for_each_neighbor_w<8, 4>(nbr, i, P, ncount[i], fetch, pair);
+  }
   rho *= P.V;
   gx *= P.V; gy *= P.V; gz *= P.V;
   s2 *= P.V2;
@@ -1547,6 +1569,7 @@ static void fk_dispatch(const LaunchCtx &L, hipEvent_t e0, hipEvent_t e1, dim3 g
   else go(L.stream, e0, e1, Sel<MULTI, 0>::fn(), g, dim3(L.bs), args...);
 }
 template <bool M, int F> struct SelLambda { static auto fn() { return k_lambda<M, F, 0>; } };
+template <bool M, int F> struct SelLambdaGather { static auto fn() { return k_lambda<M, F, 0, true>; } };
 template <bool M, int F> struct SelLambdaAcc { static auto fn() { return k_lambda<M, F, 1>; } };
 template <bool M, int F> struct SelLambdaProj { static auto fn() { return k_lambda<M, F, 2>; } };
 template <bool M, int F> struct SelLambdaFeat { static auto fn() { return k_lambda_feat<M, F>; } };
@@ -1571,6 +1594,9 @@ void launch_lambda(const LaunchCtx &L, float4 *X, float4 *B, const uint32_t *nbr
   } else if (lam_acc) {
     if (L.multi) fk_dispatch<true, SelLambdaAcc>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lam_acc);
     else fk_dispatch<false, SelLambdaAcc>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lam_acc);
+  } else if (L.hash) {
+    if (L.multi) fk_dispatch<true, SelLambdaGather>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lam_acc);
+    else fk_dispatch<false, SelLambdaGather>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lam_acc);
   } else {
     if (L.multi) fk_dispatch<true, SelLambda>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lam_acc);
     else fk_dispatch<false, SelLambda>(L, e0, L.ev1, g, L.n, X, B, nbr, ncount, L.pool, L.s0, L.scenes, L.sid, L.P, lam_acc);
