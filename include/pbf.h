@@ -34,7 +34,7 @@ enum {
   PBF_E_LENGTH = -3,    /* "Illegal length of Particle Position Buffer!" */
   PBF_E_FORMAT = -4,    /* "Invalid Boundary Model File!" */
   PBF_E_DEVICE = -5,    /* HIP runtime error */
-  PBF_E_OVERFLOW = -6,  /* a particle had more neighbours than max_neighbors */
+  PBF_E_OVERFLOW = -6,  /* a particle had more neighbours than max_neighbors, or a non-finite position */
   PBF_E_STATE = -7      /* call out of order (e.g. no particles yet) */
 };
 

@@ -272,6 +272,22 @@ def test_neighbour_overflow_is_reported():
     sim.close()
 
 
+def test_diverged_particle_is_reported():
+    """A non-finite predicted position (a NaN velocity here) is reported at
+    the next sync; the particle is binned by its slot rather than into one
+    shared cell, and the other particles' steps stay finite."""
+    sc = scenes.dam_break(8, 8, 8, map_res=16, name="diverged")
+    sim = scenes.make_engine(sc)
+    v = np.zeros((sc.n, 3), np.float32)
+    v[[3, 100, 511]] = np.nan
+    sim.write_field("velocity", v)
+    sim.step(sc.dt, 1, 1)
+    with pytest.raises(PBFError, match="non-finite particle position"):
+        sim.sync()
+    sim.sync()  # the flag was cleared when it was reported
+    sim.close()
+
+
 def test_generated_obstacle_cdm_runs_at_parity():
     """A reference-format field built by pbf_build_cubic_field (sphere + torus
     in the unit box, 24^3 elements, the reference's own volume definition),

@@ -93,7 +93,7 @@ struct pbf_ctx {
   uint32_t *cellsub = nullptr;  // per cell: z-slab boundaries and count (null with one slab per cell)
   uint32_t slab_log2 = 2;
   std::vector<uint32_t *> scan_sums, scan_prefix;
-  uint32_t *flags = nullptr;  // [0] overflow, [1] max neighbour count
+  uint32_t *flags = nullptr;  // [0] bit 0 neighbour overflow, bit 1 non-finite position; [1] max neighbour count
   // scenes on device
   Scene *d_scenes = nullptr;
   uint32_t *sid = nullptr, *ranges = nullptr;
@@ -320,7 +320,7 @@ void drain_events(pbf_ctx *c) {
 // Predict, bin, scan, reorder and neighbour lists.  dbg: keep cell counts.
 void enqueue_search(pbf_ctx *c, const LaunchCtx &L, PassTimer &T, bool dbg, uint32_t mode = M_PLAIN) {
   LaunchCtx t = T.begin(P_PREDICT, L);
-  launch_predict_bin(t, c->pos, c->vel, c->acc, c->XB, c->cellkey, c->rank, c->counts);
+  launch_predict_bin(t, c->pos, c->vel, c->acc, c->XB, c->cellkey, c->rank, c->counts, c->flags);
   T.end();
   if (dbg) (void)hipMemcpyAsync(c->dbg_counts, c->counts, (size_t)c->ncells * 4, hipMemcpyDeviceToDevice, c->stream);
   t = T.begin(P_SCAN, L);
@@ -556,6 +556,10 @@ int run_learned_substep(pbf_ctx *c, float dt, uint32_t iters) {
 int check_flags(pbf_ctx *c) {
   uint32_t f[2] = {0, 0};
   HIPCHK(hipMemcpy(f, c->flags, 8, hipMemcpyDeviceToHost));
+  if (f[0] & 2u) {
+    HIPCHK(hipMemset(c->flags, 0, 4));
+    return fail(c, PBF_E_OVERFLOW, "non-finite particle position: the simulation diverged");
+  }
   if (f[0]) {
     HIPCHK(hipMemset(c->flags, 0, 4));
     char buf[160];

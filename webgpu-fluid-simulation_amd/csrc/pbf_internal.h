@@ -47,8 +47,9 @@ struct LaunchCtx {
 constexpr uint32_t kListVariants = 4;
 constexpr uint32_t kMaxReduceBlocks = 64;  // blocks per scene of a per-scene reduction (at most one wavefront)
 
+// flags[0] bit 1: a predicted position was not finite
 void launch_predict_bin(const LaunchCtx &L, const float4 *pos, const float4 *vel, const float4 *acc, float4 *xs,
-                        uint32_t *cellkey, uint32_t *rank, uint32_t *counts);
+                        uint32_t *cellkey, uint32_t *rank, uint32_t *counts, uint32_t *flags);
 // zero_in: clear `in` (and `zero2`, when not null, over the same n words)
 void launch_scan(hipStream_t st, uint32_t *in, uint32_t n, uint32_t *out, int zero_in, uint32_t *const *sums,
                  uint32_t *const *prefix, int level, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,

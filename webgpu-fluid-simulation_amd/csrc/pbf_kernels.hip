@@ -308,7 +308,8 @@ __global__ void __launch_bounds__(BS) k_predict_bin(uint32_t n, const float4 *__
                                                     float4 *__restrict__ xs, uint32_t *__restrict__ cellkey,
                                                     uint32_t *__restrict__ rank, uint32_t *__restrict__ counts,
                                                     Scene s0, const Scene *__restrict__ scenes,
-                                                    const uint32_t *__restrict__ sid, Phys P) {
+                                                    const uint32_t *__restrict__ sid, Phys P,
+                                                    uint32_t *__restrict__ flags) {
   const uint32_t i = slot_index();
   const bool active = i < n;
   uint32_t key = 0, slab = 0;
@@ -321,10 +322,20 @@ __global__ void __launch_bounds__(BS) k_predict_bin(uint32_t n, const float4 *__
     float px = fmaf(P.dt, vx, x.x), py = fmaf(P.dt, vy, x.y), pz = fmaf(P.dt, vz, x.z);
     clamp_rand(sc, px, py, pz);
     xs[i] = make_float4(px, py, pz, 0.0f);
-    const int cz = cell_axis(sc, pz, 2);
-    key = sc.cell_base + cell_local(sc, cell_axis(sc, px, 0), cell_axis(sc, py, 1), cz);
+    if (__builtin_isfinite(px) && __builtin_isfinite(py) && __builtin_isfinite(pz)) {
+      const int cz = cell_axis(sc, pz, 2);
+      key = sc.cell_base + cell_local(sc, cell_axis(sc, px, 0), cell_axis(sc, py, 1), cz);
+      slab = P.slab_log2 ? z_slab(sc, P.slab_log2, pz, cz) : 0u;
+    } else {
+      // a diverged particle: binned by its slot, so such particles spread
+      // over the grid instead of all falling into one cell (whose sort and
+      // walks are quadratic in its population); its distances are NaN, so
+      // it has no neighbours and is nobody's.  Reported at the next sync.
+      const uint32_t D0 = (uint32_t)sc.dims[0], D1 = (uint32_t)sc.dims[1], D2 = (uint32_t)sc.dims[2];
+      key = sc.cell_base + cell_local(sc, (int)(i % D0), (int)((i / D0) % D1), (int)((i / (D0 * D1)) % D2));
+      atomicOr(&flags[0], 2u);
+    }
     cellkey[i] = key;
-    slab = P.slab_log2 ? z_slab(sc, P.slab_log2, pz, cz) : 0u;
   }
   uint32_t r = wave_atomic_rank(counts, key, active);
   if (active) rank[i] = r | (slab << kRankSlabShift);
@@ -1499,11 +1510,11 @@ static void go(hipStream_t st, hipEvent_t e0, hipEvent_t e1, K kernel, dim3 g, d
 }
 
 void launch_predict_bin(const LaunchCtx &L, const float4 *pos, const float4 *vel, const float4 *acc, float4 *xs,
-                        uint32_t *cellkey, uint32_t *rank, uint32_t *counts) {
+                        uint32_t *cellkey, uint32_t *rank, uint32_t *counts, uint32_t *flags) {
   if (!L.n) return;
   const dim3 g(nblocks(L.n, BS)), b(BS);
-  if (L.multi) go(L.stream, L.ev0, L.ev1, k_predict_bin<true>, g, b, L.n, pos, vel, acc, xs, cellkey, rank, counts, L.s0, L.scenes, L.sid, L.P);
-  else go(L.stream, L.ev0, L.ev1, k_predict_bin<false>, g, b, L.n, pos, vel, acc, xs, cellkey, rank, counts, L.s0, L.scenes, L.sid, L.P);
+  if (L.multi) go(L.stream, L.ev0, L.ev1, k_predict_bin<true>, g, b, L.n, pos, vel, acc, xs, cellkey, rank, counts, L.s0, L.scenes, L.sid, L.P, flags);
+  else go(L.stream, L.ev0, L.ev1, k_predict_bin<false>, g, b, L.n, pos, vel, acc, xs, cellkey, rank, counts, L.s0, L.scenes, L.sid, L.P, flags);
 }
 
 void launch_scan(hipStream_t st, uint32_t *in, uint32_t n, uint32_t *out, int zero_in, uint32_t *const *sums,
